@@ -1,0 +1,455 @@
+// Fused SNR-reconstruction GEMM with |z|^2 and max/argmax epilogue (the reference's fp_gemm.gemm_absmax,
+// fp_gemm.py:14-44), written for CDNA4 MFMA.
+//
+// Formulation. For one template tile and one data segment,
+//     z_re[m, t] = sum_k Ar[m, k] Zt[t, k],   z_im[m, t] = sum_k Ai[m, k] Zt[t, k],   pw = z_re^2 + z_im^2,
+// with Zt the basis series stored time-major (row t holds the K real basis values, interleaved Re/Im per complex
+// basis vector). The MFMA computes C = Zt_tile (time x k) * A_tile^T (k x templates), so each lane owns one
+// template column and 16 time rows of every 32 x 32 output tile: the max over time is an in-register scan in
+// increasing t (strict >, so the first index wins), and the two half-waves that share a column are merged with one
+// lane swap. Nothing of the M x Tn SNR array is written.
+//
+// Block = 4 waves = 128 templates; each wave keeps its 32 templates' A operands for every k-step in VGPRs for the
+// whole kernel, and the block walks a run of 128-sample time tiles whose Zt tiles are staged through LDS (double
+// buffered for bf16). Output modes:
+//   mode 0: V, I (M, ceil(Tn/128)): per 128-sample block the max |z|^2 and its sample (columns >= Tn masked to -1),
+//           exactly fused_peaks' contract;
+//   mode 1: per cluster window of `per` blocks the max and its sample (fp_filter.py:745-746 fused in), first
+//           index on ties at both levels.
+#include "common.h"
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int BT = 128;          // time samples per tile (the reference's BN)
+constexpr int WAVES = 4;
+constexpr int NT = 64 * WAVES;
+constexpr int BMT = 32 * WAVES;  // templates per block
+
+__device__ __forceinline__ int block_logical(const int nblocks)
+{
+    // the hardware places workgroup b on XCD b % 8: make consecutive logical blocks (the template blocks of one time
+    // run) land on one XCD so they share its L2 copy of the Zt tiles
+    const int b = blockIdx.x, n8 = gridDim.x >> 3;
+    const int L = (b & 7) * n8 + (b >> 3);
+    return L < nblocks ? L : -1;
+}
+
+template <int MODE>
+__device__ __forceinline__ void scan_tile(const f32x16 (&cr)[4], const f32x16 (&ci)[4], const int tb0, const int half,
+                                          const int Tn, float& best, int& bi)
+{
+#pragma unroll
+    for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int t = tb0 + ts * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            const float re = cr[ts][r], im = ci[ts][r];
+            const float p = re * re + im * im;
+            if (t < Tn && p > best) { best = p; bi = t; }
+        }
+    }
+}
+
+__device__ __forceinline__ void emit(float best, int bi, const int half, const int m, const int M, float* __restrict__ V,
+                                     int* __restrict__ I, const long ldv, const int col_out)
+{
+    const float ob = __shfl_xor(best, 32, 64);
+    const int oi = __shfl_xor(bi, 32, 64);
+    take_max_first(best, bi, ob, oi);
+    if (half == 0 && m < M) {
+        V[(size_t)m * ldv + col_out] = best;
+        I[(size_t)m * ldv + col_out] = bi;
+    }
+}
+
+template <int KP, int MODE>
+__global__ void __launch_bounds__(NT, 2)
+gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_t* __restrict__ Ar,
+                  const uint16_t* __restrict__ Ai, const long lda, const int M, const int Tn, const int ntile,
+                  const int tpb, const int per, const int nmb, const int nblocks, float* __restrict__ V,
+                  int* __restrict__ I, const long ldv)
+{
+    constexpr int KS = KP / 16;          // MFMA k-steps (v_mfma_f32_32x32x16_bf16)
+    constexpr int LDP = KP + 8;          // LDS row pitch in elements: 16 rows of one ds_read_b128 phase hit 64 banks
+    constexpr int CPR = KP / 8;          // 16-byte chunks per Zt row
+    constexpr int CPT = (BT * CPR) / NT; // chunks per thread per tile
+    static_assert(KP % 32 == 0 && (BT * CPR) % NT == 0, "K must be a multiple of 32");
+    __shared__ __attribute__((aligned(16))) uint16_t lds[2][BT * LDP];
+
+    const int L = block_logical(nblocks);
+    if (L < 0) return;
+    const int mb = L % nmb, tg = L / nmb;
+    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m = mb * BMT + wave * 32 + col;
+
+    uint4 ar[KS], ai[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        if (m < M) {
+            ar[ks] = *(const uint4*)(Ar + (size_t)m * lda + ks * 16 + half * 8);
+            ai[ks] = *(const uint4*)(Ai + (size_t)m * lda + ks * 16 + half * 8);
+        } else {
+            ar[ks] = make_uint4(0, 0, 0, 0);
+            ai[ks] = make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    uint4 zr[CPT];
+    auto gload = [&](const int tt) {
+        const uint16_t* src = Zt + (size_t)tt * BT * ldz;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
+            zr[j] = *(const uint4*)(src + (size_t)row * ldz + cc * 8);
+        }
+    };
+    auto sstore = [&](const int buf) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
+            *(uint4*)(&lds[buf][row * LDP + cc * 8]) = zr[j];
+        }
+    };
+
+    float best = -1.f;
+    int bi = t_first * BT;
+    gload(t_first);
+    sstore(0);
+    __syncthreads();
+    for (int tt = t_first; tt < t_end; tt++) {
+        const int buf = (tt - t_first) & 1;
+        if (tt + 1 < t_end) gload(tt + 1);
+        f32x16 cr[4], ci[4];
+#pragma unroll
+        for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const bf16x8 a_r = __builtin_bit_cast(bf16x8, ar[ks]);
+            const bf16x8 a_i = __builtin_bit_cast(bf16x8, ai[ks]);
+#pragma unroll
+            for (int ts = 0; ts < 4; ts++) {
+                const uint4 zv = *(const uint4*)(&lds[buf][(ts * 32 + col) * LDP + ks * 16 + half * 8]);
+                const bf16x8 z = __builtin_bit_cast(bf16x8, zv);
+                cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_r, cr[ts], 0, 0, 0);
+                ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
+            }
+        }
+        scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
+        if (MODE == 0) {
+            emit(best, bi, half, m, M, V, I, ldv, tt);
+            best = -1.f; bi = (tt + 1) * BT;
+        } else if ((tt + 1) % per == 0) {
+            emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
+            best = -1.f; bi = (tt + 1) * BT;
+        }
+        if (tt + 1 < t_end) sstore(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+template <int KP, int MODE>
+__global__ void __launch_bounds__(NT, KP <= 32 ? 2 : 1)
+gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __restrict__ Ar,
+                 const float* __restrict__ Ai, const long lda, const int M, const int Tn, const int ntile,
+                 const int tpb, const int per, const int nmb, const int nblocks, float* __restrict__ V,
+                 int* __restrict__ I, const long ldv)
+{
+    constexpr int KS = KP / 2;           // MFMA k-steps (v_mfma_f32_32x32x2_f32)
+    constexpr int LDP = KP + 2;          // rows 0..31 on even banks, their k+1 partners on odd banks
+    constexpr int CPR = KP / 4;          // 16-byte chunks per row
+    constexpr int CPT = (BT * CPR) / NT;
+    static_assert(KP % 32 == 0 && (BT * CPR) % NT == 0, "K must be a multiple of 32");
+    __shared__ __attribute__((aligned(16))) float lds[BT * LDP];
+
+    const int L = block_logical(nblocks);
+    if (L < 0) return;
+    const int mb = L % nmb, tg = L / nmb;
+    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m = mb * BMT + wave * 32 + col;
+
+    float ar[KS], ai[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        ar[ks] = m < M ? Ar[(size_t)m * lda + 2 * ks + half] : 0.f;
+        ai[ks] = m < M ? Ai[(size_t)m * lda + 2 * ks + half] : 0.f;
+    }
+    float4 zr[CPT];
+    auto gload = [&](const int tt) {
+        const float* src = Zt + (size_t)tt * BT * ldz;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
+            zr[j] = *(const float4*)(src + (size_t)row * ldz + cc * 4);
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
+            float* d = &lds[row * LDP + cc * 4];        // row pitch is not 16-byte aligned: scalar stores
+            d[0] = zr[j].x; d[1] = zr[j].y; d[2] = zr[j].z; d[3] = zr[j].w;
+        }
+    };
+
+    float best = -1.f;
+    int bi = t_first * BT;
+    gload(t_first);
+    for (int tt = t_first; tt < t_end; tt++) {
+        __syncthreads();                 // every wave is done reading the previous tile
+        sstore();
+        __syncthreads();
+        if (tt + 1 < t_end) gload(tt + 1);
+        f32x16 cr[4], ci[4];
+#pragma unroll
+        for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+            for (int ts = 0; ts < 4; ts++) {
+                const float z = lds[(ts * 32 + col) * LDP + 2 * ks + half];
+                cr[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, ar[ks], cr[ts], 0, 0, 0);
+                ci[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, ai[ks], ci[ts], 0, 0, 0);
+            }
+        }
+        scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
+        if (MODE == 0) {
+            emit(best, bi, half, m, M, V, I, ldv, tt);
+            best = -1.f; bi = (tt + 1) * BT;
+        } else if ((tt + 1) % per == 0) {
+            emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
+            best = -1.f; bi = (tt + 1) * BT;
+        }
+    }
+}
+
+// General-K variant: the block's output tile (128 time samples x 128 templates) accumulates over K in chunks of
+// KC staged through LDS for both operands (Zt rows, and the re/im template rows), with the next chunk's global loads
+// issued before the current chunk's MFMAs. The loop runs over (time tile, k chunk) pairs, so the prefetch crosses
+// time tiles, and the epilogue after each tile's last chunk is the same in-register scan as above.
+template <typename T> struct KloopTraits;
+template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8, PAD = 8, KSTEP = 16; };
+template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4, KSTEP = 2; };
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(NT, 2)
+gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
+                   const long lda, const int M, const int K, const int Tn, const int ntile, const int tpb,
+                   const int per, const int nmb, const int nblocks, float* __restrict__ V, int* __restrict__ I,
+                   const long ldv)
+{
+    using TR = KloopTraits<T>;
+    constexpr int KC = TR::KC, EPV = TR::EPV, PITCH = KC + TR::PAD;
+    constexpr int CPR = KC / EPV;                  // 16-byte chunks per row of a k chunk
+    constexpr int CPT = (BT * CPR) / NT;           // per thread, per operand tile of 128 rows
+    static_assert((BT * CPR) % NT == 0, "tile shape");
+    __shared__ __attribute__((aligned(16))) T zs[BT * PITCH];
+    __shared__ __attribute__((aligned(16))) T as_r[BMT * PITCH];
+    __shared__ __attribute__((aligned(16))) T as_i[BMT * PITCH];
+
+    const int L = block_logical(nblocks);
+    if (L < 0) return;
+    const int mb = L % nmb, tg = L / nmb;
+    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m0 = mb * BMT;
+    const int m = m0 + wave * 32 + col;
+    const int nkc = (K + KC - 1) / KC;
+    const int nsteps = (t_end - t_first) * nkc;
+
+    uint4 rz[CPT], rr[CPT], ri[CPT];
+    auto gload = [&](const int step) {
+        const int tt = t_first + step / nkc, kc = (step % nkc) * KC;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = kc + (c % CPR) * EPV;
+            const bool kin = cc < K;
+            rz[j] = kin ? *(const uint4*)(Zt + ((size_t)tt * BT + row) * ldz + cc) : make_uint4(0, 0, 0, 0);
+            const bool ain = kin && (m0 + row) < M;
+            rr[j] = ain ? *(const uint4*)(Ar + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
+            ri[j] = ain ? *(const uint4*)(Ai + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = (c % CPR) * EPV;
+            *(uint4*)(&zs[row * PITCH + cc]) = rz[j];
+            *(uint4*)(&as_r[row * PITCH + cc]) = rr[j];
+            *(uint4*)(&as_i[row * PITCH + cc]) = ri[j];
+        }
+    };
+
+    f32x16 cr[4], ci[4];
+    float best = -1.f;
+    int bi = t_first * BT;
+    gload(0);
+    for (int step = 0; step < nsteps; step++) {
+        const int kcix = step % nkc, tt = t_first + step / nkc;
+        if (kcix == 0) {
+#pragma unroll
+            for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+            }
+        }
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (step + 1 < nsteps) gload(step + 1);
+        if constexpr (sizeof(T) == 2) {
+#pragma unroll
+            for (int ks = 0; ks < KC / 16; ks++) {
+                const bf16x8 a_r = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_r[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
+                const bf16x8 a_i = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_i[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const bf16x8 z = __builtin_bit_cast(bf16x8, *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]));
+                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_r, cr[ts], 0, 0, 0);
+                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KC / 2; ks++) {
+                const float a_r = as_r[(wave * 32 + col) * PITCH + 2 * ks + half];
+                const float a_i = as_i[(wave * 32 + col) * PITCH + 2 * ks + half];
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const float z = zs[(ts * 32 + col) * PITCH + 2 * ks + half];
+                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, a_r, cr[ts], 0, 0, 0);
+                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, a_i, ci[ts], 0, 0, 0);
+                }
+            }
+        }
+        if (kcix == nkc - 1) {
+            scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
+            if (MODE == 0) {
+                emit(best, bi, half, m, M, V, I, ldv, tt);
+                best = -1.f; bi = (tt + 1) * BT;
+            } else if ((tt + 1) % per == 0) {
+                emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
+                best = -1.f; bi = (tt + 1) * BT;
+            }
+        }
+    }
+}
+
+template <typename T>
+using gemm_fn = void (*)(const T*, long, const T*, const T*, long, int, int, int, int, int, int, int, float*, int*,
+                         long);
+
+template <int KP>
+gemm_fn<uint16_t> pick_bf16(int mode) { return mode ? gemm_peaks_bf16_k<KP, 1> : gemm_peaks_bf16_k<KP, 0>; }
+template <int KP>
+gemm_fn<float> pick_f32(int mode) { return mode ? gemm_peaks_f32_k<KP, 1> : gemm_peaks_f32_k<KP, 0>; }
+
+gemm_fn<uint16_t> bf16_kernel(int K, int mode)
+{
+    switch (K) {
+    case 32: return pick_bf16<32>(mode);
+    case 64: return pick_bf16<64>(mode);
+    case 96: return pick_bf16<96>(mode);
+    case 128: return pick_bf16<128>(mode);
+    case 160: return pick_bf16<160>(mode);
+    case 192: return pick_bf16<192>(mode);
+    case 224: return pick_bf16<224>(mode);
+    case 256: return pick_bf16<256>(mode);
+    default: return nullptr;
+    }
+}
+
+gemm_fn<float> f32_kernel(int K, int mode)
+{
+    switch (K) {
+    case 32: return pick_f32<32>(mode);
+    case 64: return pick_f32<64>(mode);
+    case 96: return pick_f32<96>(mode);
+    case 128: return pick_f32<128>(mode);
+    default: return nullptr;
+    }
+}
+
+} // namespace
+
+// dtype 0 = fp32 operands (plain form), 1 = bf16 operands; fp32 accumulation either way.
+// variant: 0 = pick by K (register-resident template operands up to K = 256 bf16 / 128 fp32, else the K loop),
+// 1 = always the K loop.
+GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
+                             int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,
+                             void* stream)
+{
+    GWGS_CHECK(Zt && Ar && Ai && V && I, GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && Tn > 0 && K > 0 && (mode == 0 || mode == 1), GWGS_EINVAL);
+    GWGS_CHECK(ldz >= K && lda >= K && ldv > 0, GWGS_EINVAL);
+    const int ntile = (Tn + BT - 1) / BT;
+    if (mode == 1) {
+        GWGS_CHECK(per > 0 && Tn % (per * BT) == 0, GWGS_EINVAL);
+        GWGS_CHECK(ldv >= Tn / (per * BT), GWGS_EINVAL);
+    } else {
+        per = 1;
+        GWGS_CHECK(ldv >= ntile, GWGS_EINVAL);
+    }
+    int tpb = tiles_per_block > 0 ? tiles_per_block : (mode == 1 ? per : 16);
+    if (mode == 1 && tpb % per) tpb = (tpb / per + 1) * per;
+    const int nmb = (M + BMT - 1) / BMT;
+    const int ntg = (ntile + tpb - 1) / tpb;
+    const long nblocks = (long)nmb * ntg;
+    GWGS_CHECK(nblocks < (1L << 30), GWGS_ESIZE);
+    const int grid = (int)((nblocks + 7) / 8 * 8);
+    hipStream_t s = (hipStream_t)stream;
+    const bool kloop = variant == 1 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
+    if (kloop) {
+        if (dtype == 1) {
+            GWGS_CHECK(ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0 && ((uintptr_t)Zt & 15) == 0 &&
+                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
+            auto f = mode ? gemm_peaks_kloop_k<uint16_t, 1> : gemm_peaks_kloop_k<uint16_t, 0>;
+            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
+                               (const uint16_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+        } else if (dtype == 0) {
+            GWGS_CHECK(ldz % 4 == 0 && lda % 4 == 0 && K % 4 == 0 && ((uintptr_t)Zt & 15) == 0 &&
+                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
+            auto f = mode ? gemm_peaks_kloop_k<float, 1> : gemm_peaks_kloop_k<float, 0>;
+            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar,
+                               (const float*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+        } else {
+            return GWGS_EUNSUP;
+        }
+        return gwgs_status();
+    }
+    if (dtype == 1) {
+        GWGS_CHECK(ldz % 8 == 0 && lda % 8 == 0 && ((uintptr_t)Zt & 15) == 0 && ((uintptr_t)Ar & 15) == 0 &&
+                   ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
+        gemm_fn<uint16_t> f = bf16_kernel(K, mode);
+        GWGS_CHECK(f, GWGS_ESIZE);
+        hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
+                           (const uint16_t*)Ai, lda, M, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+    } else if (dtype == 0) {
+        GWGS_CHECK(ldz % 4 == 0 && ((uintptr_t)Zt & 15) == 0, GWGS_EINVAL);
+        gemm_fn<float> f = f32_kernel(K, mode);
+        GWGS_CHECK(f, GWGS_ESIZE);
+        hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar, (const float*)Ai,
+                           lda, M, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+    } else {
+        return GWGS_EUNSUP;
+    }
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_gemm_max_k(int dtype)
+{
+    return dtype == 0 || dtype == 1 ? (1 << 20) : 0;
+}

@@ -1,0 +1,292 @@
+// Data conditioning kernels (the reference's gpu_prep.py:87-231, 405-454): Welch PSD with median average in
+// float64, interpolation, inverse-spectrum truncation, whitening D = stilde / psd and IP = 1 / psd, and the
+// autogating whitening and inverted-Tukey gates. FFTs are hipFFT calls made by the host; these kernels are the
+// memory-bound passes between them.
+#include "common.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+// segs[r, s, i] = x[r, start + s stride + i] * w[i], w the symmetric Hann window 0.5 - 0.5 cos(2 pi i / (L - 1))
+__global__ void welch_segments_k(const float* __restrict__ x, const long ldx, const long start, const int nseg,
+                                 const int L, const int stride, double* __restrict__ segs)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (g >= (long)nseg * L) return;
+    const int s = (int)(g / L), i = (int)(g % L);
+    const double w = 0.5 - 0.5 * cospi(2.0 * (double)i / (double)(L - 1));
+    segs[((size_t)r * nseg + s) * L + i] = (double)x[(size_t)r * ldx + start + (size_t)s * stride + i] * w;
+}
+
+// One thread per (row, bin): p_s = |X_s|^2 dt^2 (halved at DC and Nyquist), median over the nseg segments
+// (numpy's: the middle order statistic, or the mean of the two middle ones), / bias, * fac; or the mean.
+// WB bins per block, their nseg <= WMAX values staged in LDS.
+template <int WB, int WMAX>
+__global__ void __launch_bounds__(WB) welch_reduce_k(const double2* __restrict__ X, const int nseg, const int nf,
+                                                    const double dt2, const int mean, const double bias,
+                                                    const double fac, double* __restrict__ out)
+{
+    __shared__ double v[WMAX][WB];
+    const int r = blockIdx.y, b = blockIdx.x * WB + threadIdx.x, tl = threadIdx.x;
+    const bool ok = b < nf;
+    for (int s = 0; s < nseg; s++) {
+        double p = 0.0;
+        if (ok) {
+            const double2 z = X[((size_t)r * nseg + s) * nf + b];
+            p = (z.x * z.x + z.y * z.y) * dt2;
+            if (b == 0 || b == nf - 1) p /= 2;
+        }
+        v[s][tl] = p;
+    }
+    if (!ok) return;
+    double res;
+    if (mean) {
+        double acc = 0.0;
+        for (int s = 0; s < nseg; s++) acc += v[s][tl];
+        res = acc / nseg;
+    } else {
+        // k-th order statistic by counting: the value with #less <= k < #less + #equal
+        auto kth = [&](const int k) {
+            for (int s = 0; s < nseg; s++) {
+                const double x = v[s][tl];
+                int lt = 0, eq = 0;
+                for (int u = 0; u < nseg; u++) { const double y = v[u][tl]; lt += y < x; eq += y == x; }
+                if (lt <= k && k < lt + eq) return x;
+            }
+            return 0.0;
+        };
+        res = (nseg & 1) ? kth(nseg / 2) : 0.5 * (kth(nseg / 2 - 1) + kth(nseg / 2));
+        res /= bias;
+    }
+    out[(size_t)r * nf + b] = res * fac;
+}
+
+// np.interp of P (rows, nold) sampled at k df_old onto i df_new, i < nnew (right end held)
+__global__ void interp_k(const double* __restrict__ P, const int nold, const double df_old, const double df_new,
+                         const long nnew, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= nnew) return;
+    const double* p = P + (size_t)r * nold;
+    const double x = (double)i * df_new;
+    const double u = x / df_old;
+    long j = (long)floor(u);
+    double y;
+    if (j >= nold - 1) y = p[nold - 1];
+    else {
+        if (j < 0) j = 0;
+        const double x0 = (double)j * df_old, x1 = (double)(j + 1) * df_old;
+        const double t = (x - x0) / (x1 - x0);
+        y = p[j] + (p[j + 1] - p[j]) * t;
+    }
+    out[(size_t)r * nnew + i] = y;
+}
+
+// inv_asd = P^-1/2 on [kmin, khi), 0 elsewhere, as the complex input of an inverse real FFT
+__global__ void isqrt_band_k(const double* __restrict__ P, const long flen, const long kmin, const long khi,
+                             double2* __restrict__ out)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (k >= flen) return;
+    double v = 0.0;
+    if (k >= kmin && k < khi) v = 1.0 / sqrt(P[(size_t)r * flen + k]);
+    out[(size_t)r * flen + k] = make_double2(v, 0.0);
+}
+
+// q *= df; zero [L/2, N - L/2); with hann, taper the kept ends by the halves of a symmetric Hann window of length L
+__global__ void trunc_k(double* __restrict__ q, const long N, const long L, const int hann, const double df)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= N) return;
+    const long ts = L / 2, te = N - L / 2;
+    double v = q[(size_t)r * N + i] * df;
+    if (hann) {
+        if (i < ts) v *= 0.5 - 0.5 * cospi(2.0 * (double)(L - ts + i) / (double)(L - 1));
+        else if (i >= te) v *= 0.5 - 0.5 * cospi(2.0 * (double)(i - te) / (double)(L - 1));
+    }
+    if (ts < te && i >= ts && i < te) v = 0.0;
+    q[(size_t)r * N + i] = v;
+}
+
+// out = 1 / |X / (df N)|^2
+__global__ void inv_abs2_k(const double2* __restrict__ X, const long flen, const double scale, double* __restrict__ out)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (k >= flen) return;
+    const double2 z = X[(size_t)r * flen + k];
+    const double a = z.x * scale, b = z.y * scale;
+    out[(size_t)r * flen + k] = 1.0 / (a * a + b * b);
+}
+
+// D[s, k] = (stilde[s, k] * (1/sr as float32)) * ip[s, k] rounded to complex64, ip = 1/P[pick[s], k] (0 where P <= 0
+// or k < kmin), IP = float32(ip)
+__global__ void whiten_k(const float2* __restrict__ X, const float inv_sr, const double* __restrict__ P,
+                         const int* __restrict__ pick, const long flen, const long kmin, float2* __restrict__ D,
+                         float* __restrict__ IP)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= flen) return;
+    const double p = P[(size_t)pick[s] * flen + k];
+    const double ip = (k >= kmin && p > 0.0) ? 1.0 / p : 0.0;
+    const float2 x = X[(size_t)s * flen + k];
+    const float xr = x.x * inv_sr, xi = x.y * inv_sr;
+    D[(size_t)s * flen + k] = make_float2((float)((double)xr * ip), (float)((double)xi * ip));
+    IP[(size_t)s * flen + k] = (float)ip;
+}
+
+// Autogating: taper the corrupt ends (ramp i / cl) of s and centre it in a zero buffer of length Lp (double)
+__global__ void gate_pad_k(const float* __restrict__ s, const long n, const long cl, const long pad_start,
+                           const long Lp, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Lp) return;
+    double v = 0.0;
+    const long j = i - pad_start;
+    if (j >= 0 && j < n) {
+        v = (double)s[j];
+        if (j < cl) v *= (double)j / (double)cl;
+        else if (j >= n - cl) v *= (double)(n - 1 - j) / (double)cl;
+    }
+    out[i] = v;
+}
+
+// st[k] *= dt (psd[k] norm)^-1/2, 0 where psd is +inf (k < kmin)
+__global__ void gate_whiten_k(double2* __restrict__ st, const double* __restrict__ psd, const long flen,
+                              const long kmin, const double dt, const double norm)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= flen) return;
+    double2 z = st[k];
+    const double f = k < kmin ? 0.0 : dt / sqrt(psd[k] * norm);
+    st[k] = make_double2(z.x * f, z.y * f);
+}
+
+// mag[j] = |wt[pad_start + j] * df| for j in [cl, n - cl), 0 at the corrupt ends (float32 for the compaction)
+__global__ void gate_mag_k(const double* __restrict__ wt, const long pad_start, const long n, const long cl,
+                           const double df, float* __restrict__ mag)
+{
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    mag[j] = (j < cl || j >= n - cl) ? 0.f : (float)fabs(wt[pad_start + j] * df);
+}
+
+// x[offset + i] *= window[i] for i in [i1, i2) (product in double, rounded to float32); window = inverted Tukey:
+// cosine ramps of pad samples around win - 2 pad zeros (gpu_prep.py:212-231)
+__global__ void gate_apply_k(float* __restrict__ x, const long offset, const long i1, const long i2, const long pad,
+                             const long win)
+{
+    const long i = i1 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= i2) return;
+    double w;
+    if (i < pad) w = 0.5 * (1.0 + cospi((double)i / (double)pad));
+    else if (i >= win - pad) w = 0.5 * (1.0 + cospi((double)(win - 1 - i) / (double)pad));
+    else w = 0.0;
+    x[offset + i] = (float)((double)x[offset + i] * w);
+}
+
+inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
+inline dim3 grid2(long n, int rows) { return dim3((unsigned)((n + NT - 1) / NT), rows); }
+
+} // namespace
+
+GWGS_API int gwgs_welch_segments(const float* x, long ldx, int rows, long start, int nseg, int L, int stride,
+                                 double* segs, void* stream)
+{
+    GWGS_CHECK(x && segs && rows > 0 && nseg > 0 && L > 1 && stride > 0 && start >= 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(welch_segments_k, grid2((long)nseg * L, rows), dim3(NT), 0, (hipStream_t)stream, x, ldx, start,
+                       nseg, L, stride, segs);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_welch_reduce(const void* X, int rows, int nseg, int nf, double dt, int mean, double bias, double fac,
+                               double* out, void* stream)
+{
+    GWGS_CHECK(X && out && rows > 0 && nseg > 0 && nf > 0, GWGS_EINVAL);
+    GWGS_CHECK(nseg <= 512, GWGS_ESIZE);
+    if (nseg <= 128)
+        hipLaunchKernelGGL((welch_reduce_k<64, 128>), dim3((nf + 63) / 64, rows), dim3(64), 0, (hipStream_t)stream,
+                           (const double2*)X, nseg, nf, dt * dt, mean, bias, fac, out);
+    else
+        hipLaunchKernelGGL((welch_reduce_k<16, 512>), dim3((nf + 15) / 16, rows), dim3(16), 0, (hipStream_t)stream,
+                           (const double2*)X, nseg, nf, dt * dt, mean, bias, fac, out);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_interp(const double* P, int rows, int nold, double df_old, double df_new, long nnew, double* out,
+                         void* stream)
+{
+    GWGS_CHECK(P && out && rows > 0 && nold > 1 && nnew > 0 && df_old > 0 && df_new > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(interp_k, grid2(nnew, rows), dim3(NT), 0, (hipStream_t)stream, P, nold, df_old, df_new, nnew,
+                       out);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_isqrt_band(const double* P, int rows, long flen, long kmin, long khi, void* out, void* stream)
+{
+    GWGS_CHECK(P && out && rows > 0 && flen > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(isqrt_band_k, grid2(flen, rows), dim3(NT), 0, (hipStream_t)stream, P, flen, kmin, khi,
+                       (double2*)out);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_trunc(double* q, int rows, long N, long L, int hann, double df, void* stream)
+{
+    GWGS_CHECK(q && rows > 0 && N > 0 && L > 1, GWGS_EINVAL);
+    hipLaunchKernelGGL(trunc_k, grid2(N, rows), dim3(NT), 0, (hipStream_t)stream, q, N, L, hann, df);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_inv_abs2(const void* X, int rows, long flen, double scale, double* out, void* stream)
+{
+    GWGS_CHECK(X && out && rows > 0 && flen > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(inv_abs2_k, grid2(flen, rows), dim3(NT), 0, (hipStream_t)stream, (const double2*)X, flen, scale,
+                       out);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_whiten(const void* X, float inv_sr, const double* P, const int* pick, int nseg, long flen, long kmin,
+                         void* D, float* IP, void* stream)
+{
+    GWGS_CHECK(X && P && pick && D && IP && nseg > 0 && flen > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(whiten_k, grid2(flen, nseg), dim3(NT), 0, (hipStream_t)stream, (const float2*)X, inv_sr, P,
+                       pick, flen, kmin, (float2*)D, IP);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_gate_pad(const float* s, long n, long cl, long pad_start, long Lp, double* out, void* stream)
+{
+    GWGS_CHECK(s && out && n > 0 && Lp >= n && pad_start >= 0 && pad_start + n <= Lp, GWGS_EINVAL);
+    hipLaunchKernelGGL(gate_pad_k, grid1(Lp), dim3(NT), 0, (hipStream_t)stream, s, n, cl, pad_start, Lp, out);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_gate_whiten(void* st, const double* psd, long flen, long kmin, double dt, double norm, void* stream)
+{
+    GWGS_CHECK(st && psd && flen > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(gate_whiten_k, grid1(flen), dim3(NT), 0, (hipStream_t)stream, (double2*)st, psd, flen, kmin,
+                       dt, norm);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_gate_mag(const double* wt, long pad_start, long n, long cl, double df, float* mag, void* stream)
+{
+    GWGS_CHECK(wt && mag && n > 0, GWGS_EINVAL);
+    hipLaunchKernelGGL(gate_mag_k, grid1(n), dim3(NT), 0, (hipStream_t)stream, wt, pad_start, n, cl, df, mag);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, long pad, long win, void* stream)
+{
+    GWGS_CHECK(x && pad > 0 && win >= 2 * pad && i1 >= 0 && i2 <= win && offset + i1 >= 0 && offset + i2 <= n,
+               GWGS_EINVAL);
+    if (i2 <= i1) return GWGS_OK;
+    hipLaunchKernelGGL(gate_apply_k, grid1(i2 - i1), dim3(NT), 0, (hipStream_t)stream, x, offset, i1, i2, pad, win);
+    return gwgs_status();
+}

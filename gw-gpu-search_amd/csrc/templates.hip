@@ -1,0 +1,109 @@
+// Frequency-domain chirp templates and their sigma^2 against each segment's inverse PSD.
+//
+// Templates: stationary-phase chirps h(f) = f^(-7/6) exp(-i psi(f)) on k df for k in [kmin, kmax), kmax =
+// min(int(f_isco / df), flen - 1), with psi the TaylorF2 phase to 1PN (t_c = phi_c = 0) plus the leading eccentric
+// term of Yunes, Arun, Berti & Will 2009 (PRD 80 084001):
+//     psi = 3 / (128 eta v^5) [1 + (3715/756 + 55 eta / 9) v^2 - (2355/1462) e0^2 (f / f0)^(-19/9)] - pi / 4,
+//     v = (pi M f)^(1/3).
+// The phase reaches 1e5 rad at the low-frequency end of light binaries, so it is evaluated in double and only the
+// final complex value is rounded to float32.
+//
+// sigma^2 (fp_filter.py:710-714): sig[i, s] = 4 df sum_k |h_ik|^2 IP_sk, accumulated in double.
+#include "common.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr double MSUN_S = 4.925490947641267e-06;
+constexpr int MAXSEG = 16;
+
+__global__ void spa_templates_k(const double* __restrict__ m1, const double* __restrict__ m2,
+                                const double* __restrict__ e0, const int n, const long flen, const double df,
+                                const double flow, const double f0, float2* __restrict__ H, const long ldh,
+                                int* __restrict__ kmax_out)
+{
+    const int i = blockIdx.y;
+    const double M = (m1[i] + m2[i]) * MSUN_S;
+    const double eta = m1[i] * m2[i] / ((m1[i] + m2[i]) * (m1[i] + m2[i]));
+    const double fisco = 1.0 / (pow(6.0, 1.5) * M_PI * M);
+    const long kmin = (long)(flow / df);
+    const long kmax = min((long)(fisco / df), flen - 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && kmax_out) kmax_out[i] = (int)kmax;
+    const double ecc = e0 ? e0[i] : 0.0;
+    const double c1 = 3715.0 / 756.0 + 55.0 * eta / 9.0;
+    const double pref = 3.0 / (128.0 * eta);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < flen; k += (long)gridDim.x * blockDim.x) {
+        float2 out = make_float2(0.f, 0.f);
+        if (k >= kmin && k < kmax) {
+            const double f = (double)k * df;
+            const double v = cbrt(M_PI * M * f);
+            const double v2 = v * v;
+            double pn = 1.0 + c1 * v2;
+            if (ecc != 0.0) pn -= (2355.0 / 1462.0) * ecc * ecc * pow(f / f0, -19.0 / 9.0);
+            const double psi = pref / (v2 * v2 * v) * pn - 0.25 * M_PI;
+            const double amp = pow(f, -7.0 / 6.0);
+            double s, c;
+            sincos(psi, &s, &c);
+            out = make_float2((float)(amp * c), (float)(-amp * s));
+        }
+        H[(size_t)i * ldh + k] = out;
+    }
+}
+
+__global__ void __launch_bounds__(NT) sigmasq_k(const float2* __restrict__ H, const long ldh, const long flen,
+                                                const float* __restrict__ IP, const long ldip, const int nseg,
+                                                const double df, double* __restrict__ sig, const long lds)
+{
+    const int i = blockIdx.x;
+    double acc[MAXSEG];
+#pragma unroll
+    for (int s = 0; s < MAXSEG; s++) acc[s] = 0.0;
+    const float2* h = H + (size_t)i * ldh;
+    for (long k = threadIdx.x; k < flen; k += NT) {
+        const float2 v = h[k];
+        const double p = (double)v.x * v.x + (double)v.y * v.y;
+        if (p == 0.0) continue;
+#pragma unroll
+        for (int s = 0; s < MAXSEG; s++)
+            if (s < nseg) acc[s] += p * (double)IP[(size_t)s * ldip + k];
+    }
+    __shared__ double sh[NT / 64][MAXSEG];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < MAXSEG; s++) {
+        if (s < nseg) {
+            const double x = wave_sum_f64(acc[s]);
+            if (lane == 0) sh[wave][s] = x;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nseg) {
+        const int s = threadIdx.x;
+        sig[(size_t)i * lds + s] = 4.0 * df * (((sh[0][s] + sh[1][s]) + sh[2][s]) + sh[3][s]);
+    }
+}
+
+} // namespace
+
+GWGS_API int gwgs_spa_templates(const double* m1, const double* m2, const double* e0, int n, long flen, double df,
+                                double flow, double f0, void* H, long ldh, int* kmax, void* stream)
+{
+    GWGS_CHECK(m1 && m2 && H && n >= 0 && flen > 0 && df > 0 && ldh >= flen, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    GWGS_CHECK(n < 65536, GWGS_ESIZE);
+    const unsigned bx = (unsigned)min((flen + NT - 1) / NT, 512L);
+    hipLaunchKernelGGL(spa_templates_k, dim3(bx, n), dim3(NT), 0, (hipStream_t)stream, m1, m2, e0, n, flen, df, flow,
+                       f0, (float2*)H, ldh, kmax);
+    return gwgs_status();
+}
+
+GWGS_API int gwgs_sigmasq(const void* H, long ldh, long flen, int n, const float* IP, long ldip, int nseg, double df,
+                          double* sig, long lds, void* stream)
+{
+    GWGS_CHECK(H && IP && sig && n >= 0 && nseg > 0 && lds >= nseg, GWGS_EINVAL);
+    GWGS_CHECK(nseg <= MAXSEG, GWGS_ESIZE);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL(sigmasq_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)H, ldh, flen, IP, ldip,
+                       nseg, df, sig, lds);
+    return gwgs_status();
+}

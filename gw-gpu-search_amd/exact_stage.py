@@ -1,0 +1,159 @@
+"""Exact complex SNR numerator z and chi-squared bin power sum_j |Z_j|^2 at candidate peaks of one template tile
+(the reference's exact_stage.py, dense form: the path fp_filter takes with --exact-stage dense and
+--templates pycbc, exact_stage.py:196-203, 704-731).
+
+Usage (fp_filter):
+    stage = ExactStage(TemplateTile(H, nbins, kmin, kmax))                           # per tile
+    pv = stage.evaluate(SegmentData(D, IP, df, tlen), PeakRequest(rows, samples))    # per (tile, segment)
+    snr = |pv.z| norm;  chisq = (nb pv.pw - |pv.z|^2) norm^2;  sg reference frequency = pv.fpeak
+
+Edges come from the float64 GPU edges kernel (chisq_edges), the sums from the fused chi-squared kernel
+(chisq_kernel, conj(H) D formed in-kernel, phasors reseeded from the exact 64-bit phase), z and pw accumulated in
+float64 over bins. The first evaluate checks up to CHECK_PEAKS peaks against a float64 direct sum of the same
+bins and raises if CHECK_TOL fails (exact_stage.py:269-289). tlen need not be a power of two.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+import chisq_kernel
+from chisq_edges import edges_blocked
+
+MAX_PEAKS = 8
+CHECK_ROWS = 16
+CHECK_PEAKS = 64
+CHECK_TOL = dict(edge_bins=1, edge_frac=1e-2, snr_rel=1e-4, phase_rad=1e-4, chisq_rel=1e-3)
+
+
+@dataclass(frozen=True, eq=False)
+class TemplateTile:
+    """One tile. H (n, flen) complex64 on the GPU holds the dense rows. nbins per row (0 = no chi-squared, one bin).
+    Bins cover [kmin, kmax[row]). coarse is accepted for the reference's signature; the dense path ignores it."""
+    H: object
+    nbins: np.ndarray
+    kmin: int
+    kmax: np.ndarray
+    coarse: object = None
+
+    def __post_init__(self):
+        n = self.H.shape[0]
+        object.__setattr__(self, 'nbins', np.asarray(self.nbins, np.int64))
+        object.__setattr__(self, 'kmax', np.asarray(self.kmax, np.int64))
+        assert self.H.ndim == 2 and self.H.dtype == torch.complex64 and self.H.is_contiguous()
+        assert self.nbins.shape == (n,) and self.kmax.shape == (n,)
+
+
+@dataclass(frozen=True, eq=False)
+class SegmentData:
+    """One segment: D = stilde / psd (complex64, flen) and IP = 1 / psd (float32, flen), both zero below kmin."""
+    D: object
+    IP: object
+    df: float
+    tlen: int
+
+    def __post_init__(self):
+        flen = self.tlen // 2 + 1
+        assert self.tlen > 0
+        assert tuple(self.D.shape) == (flen,) and self.D.dtype == torch.complex64
+        assert tuple(self.IP.shape) == (flen,) and self.IP.dtype == torch.float32
+
+
+@dataclass(frozen=True, eq=False)
+class PeakRequest:
+    """Peaks of one (tile, segment): tile row and sample index. Duplicates are allowed; results keep this order."""
+    rows: np.ndarray
+    samples: np.ndarray
+
+    def __post_init__(self):
+        object.__setattr__(self, 'rows', np.asarray(self.rows, np.int64))
+        object.__setattr__(self, 'samples', np.asarray(self.samples, np.int64))
+        assert self.rows.ndim == 1 and self.rows.shape == self.samples.shape
+
+
+@dataclass(frozen=True, eq=False)
+class PeakValues:
+    """Per requested peak, in request order: z = sum_j Z_j (complex64), pw = sum_j |Z_j|^2 (float32), fpeak =
+    (2 e[nb-1] - e[nb-2]) df from this stage's edges (Hz)."""
+    z: np.ndarray
+    pw: np.ndarray
+    fpeak: np.ndarray
+
+
+def _fpeak(E, nb, df):
+    r = np.arange(len(E))
+    return (2 * E[r, np.maximum(nb - 1, 0)] - E[r, np.maximum(nb - 2, 0)]) * df
+
+
+class ExactStage:
+    """Exact z, pw and fpeak at requested peaks of one tile, segment after segment. Built once per tile."""
+
+    def __init__(self, tile: TemplateTile, self_check=True):
+        self.tile = tile
+        self.self_check = self_check
+        self.check: dict | None = None
+
+    def _nbe(self, rows_u):
+        return np.maximum(self.tile.nbins[rows_u], 1)
+
+    def edges(self, seg: SegmentData, rows) -> np.ndarray:
+        """(len(rows), B + 1) chi-squared bin edges for the given tile rows in this segment."""
+        rows = np.asarray(rows, np.int64)
+        E = edges_blocked(None, rows, self._nbe(rows), self.tile.kmin, self.tile.kmax[rows], self.tile.H, seg.IP)
+        E[self.tile.nbins[rows] == 0, 0] = self.tile.kmin
+        return E
+
+    def evaluate(self, seg: SegmentData, req: PeakRequest) -> PeakValues:
+        rows, samples = req.rows, req.samples
+        npk = len(rows)
+        if npk == 0:
+            return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0))
+        n = self.tile.H.shape[0]
+        assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg.tlen
+        E = self.edges(seg, rows)
+        pw, z = chisq_kernel.bin_power_sums(self.tile.H, rows, samples, E, want_z=True, d=seg.D, n_time=seg.tlen)
+        out = PeakValues(z, pw, _fpeak(E, self.tile.nbins[rows], seg.df))
+        if self.self_check and self.check is None:
+            self.check = _self_check(self, seg, req, out, E)
+        return out
+
+
+def _self_check(stage, seg, req, out, E):
+    """Up to CHECK_PEAKS peaks of up to CHECK_ROWS rows against a float64 direct sum over the same bins, on the
+    device in complex128 (exact_stage.py:269-289). Raises if a tolerance in CHECK_TOL fails."""
+    rows_u = np.unique(req.rows)
+    pick = rows_u[np.unique(np.linspace(0, len(rows_u) - 1, CHECK_ROWS).astype(int))]
+    sel = np.flatnonzero(np.isin(req.rows, pick))[:CHECK_PEAKS]
+    H, D, N = stage.tile.H, seg.D, seg.tlen
+    z_ref = np.empty(len(sel), np.complex128); pw_ref = np.empty(len(sel))
+    for o, i in enumerate(sel):
+        e = E[i, :max(int(stage.tile.nbins[req.rows[i]]), 1) + 1]
+        k = torch.arange(int(e[0]), int(e[-1]), device=H.device, dtype=torch.int64)
+        q = torch.conj(H[int(req.rows[i]), e[0]:e[-1]].to(torch.complex128)) * D[e[0]:e[-1]].to(torch.complex128)
+        ph = ((k * int(req.samples[i])) % N).to(torch.float64) * (2 * np.pi / N)
+        x = q * torch.exp(1j * ph)
+        b = torch.as_tensor(np.searchsorted(e, np.arange(e[0], e[-1]), side='right') - 1, device=H.device)
+        Zj = torch.zeros(len(e) - 1, dtype=torch.complex128, device=H.device).index_add_(0, b, x)
+        z_ref[o] = complex(Zj.sum().item()); pw_ref[o] = float((Zj.abs() ** 2).sum().item())
+    rep = dict(n_peaks=int(len(sel)), n_rows=int(len(np.unique(req.rows[sel]))),
+               **compare_values(out.z[sel], out.pw[sel], z_ref, pw_ref, stage.tile.nbins[req.rows[sel]]))
+    t = CHECK_TOL
+    ok = rep['snr_rel_max'] <= t['snr_rel'] and rep['phase_max'] <= t['phase_rad'] and rep['chisq_rel_max'] <= t['chisq_rel']
+    assert ok, f'exact stage self-check failed: {rep}'
+    return rep
+
+
+def compare_values(z, pw, z_ref, pw_ref, nbins) -> dict:
+    """Relative |z| error, phase error (rad) and relative chi-squared error, max and p99 (exact_stage.py:309-322)."""
+    z = np.asarray(z, np.complex128); z_ref = np.asarray(z_ref, np.complex128)
+    snr = np.abs(np.abs(z) - np.abs(z_ref)) / np.abs(z_ref)
+    ph = np.abs(np.angle(z * np.conj(z_ref)))
+    nb = np.asarray(nbins); m = nb > 0
+    q = nb[m] * np.asarray(pw, np.float64)[m] - np.abs(z[m]) ** 2
+    q_ref = nb[m] * np.asarray(pw_ref, np.float64)[m] - np.abs(z_ref[m]) ** 2
+    chi = np.abs(q - q_ref) / np.abs(q_ref) if m.any() else np.zeros(1)
+    return dict(snr_rel_max=float(snr.max()), snr_rel_p99=float(np.percentile(snr, 99)),
+                phase_max=float(ph.max()), phase_p99=float(np.percentile(ph, 99)),
+                chisq_rel_max=float(chi.max()), chisq_rel_p99=float(np.percentile(chi, 99)))

@@ -1,0 +1,331 @@
+"""Matched filter of template banks against conditioned detector data by template compression, on one MI355X
+(the reference's fp_filter.py run_group, :203-960, for its hot path).
+
+Per tile of templates sorted by chirp time (once per run, BankPlan):
+  - the sw-whitened templates are compressed by a float64 SVD to r complex basis filters such that every template
+    keeps >= 1 - eps of its power (svd_tile, fp_filter.py:576-593);
+Per tile and data segment (Filter.filter_segment):
+  - the r basis filters are correlated with the segment by one batched hipFFT inverse transform of
+    q[j, k] = conj(Vh[j, k]) D[k] / sw[k] (basis_series, fp_filter.py:597-613), transposed through LDS into the
+    GEMM's time-major operand (fp32, or bf16 for the precision form);
+  - one fused MFMA GEMM rebuilds every template's SNR and keeps, per cluster window, only the max |z|^2 and its
+    sample (fp_gemm.py:14-44 with fp_filter.py:745-746 fused into its epilogue), in fp32, bf16 or fp8 operands with
+    fp32 accumulation;
+  - windows whose rebuilt SNR is above thr (1 - margin) are candidates (fp_filter.py:747); the rebuilt complex SNR
+    at each gives the rebuilt phase (:759-764); candidates below exact_snr (1 - margin) keep their rebuilt values
+    (:768-781), the rest go to the exact stage: edges and chi-squared sums over the exact template in fp32 with
+    float64 finalization (:783-853).
+So snr, coa_phase and chisq of exact peaks are exact; the compressed filter chooses which windows to look at and the
+sample within the window.
+
+Trigger fields and their order are those of the reference's peak files (fp_filter.py:910-937); Triggers.to_dict()
+returns them (HDF5 writing is outside this path).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+import _dev
+import _lib
+import fp_gemm
+import peak_kernel
+import templates as tmpl
+from exact_stage import ExactStage, PeakRequest, SegmentData, TemplateTile
+
+CH = 1 << 16          # frequency chunk for the Gram and basis products
+BN = fp_gemm.BN
+PRECISIONS = {'fp32': 0, 'bf16': 1}
+
+
+@dataclass
+class Options:
+    snr_threshold: float = 5.5
+    cluster_window: float = 1.0           # s
+    chisq_bins: int = 16
+    low_frequency_cutoff: float = 15.0
+    tile: int = 512
+    eps: float = 1e-4                      # worst lost power allowed by the SVD (0.9999 kept)
+    margin: float = 0.04
+    exact_snr: float = 0.0                 # 0: exact values for every candidate
+    precision: str = 'fp32'                # GEMM operands: fp32 (plain form) or bf16
+    gemm_cfg: tuple | None = None          # (tiles_per_block,), from fp_gemm.pick_cfg
+    self_check: bool = True                # the exact stage's first-call check
+    timing: bool = False                   # synchronize around stages to fill the timing dict
+    gemm_events: bool = False              # HIP events around every GEMM launch (kernel time, algorithmic flops)
+
+
+@dataclass
+class Dataset:
+    """One detector's conditioned data, as gpu_prep.prep returns it: D (nseg, flen) complex64 and IP (nseg, flen)
+    float32 on the GPU, zero below kmin; per segment the analysed sample range and the segment start sample."""
+    tag: str
+    D: torch.Tensor
+    IP: torch.Tensor
+    flen: int
+    tlen: int
+    df: float
+    sr: float
+    gps_start: float
+    analyze: list
+    seg_start: list
+    gates: list = field(default_factory=list)
+
+
+@dataclass
+class TileBasis:
+    ids: np.ndarray            # bank rows of this tile, in tile order
+    H: torch.Tensor            # (n, flen) complex64
+    kmax: np.ndarray
+    r: int
+    rp: int                    # r padded to a multiple of 16
+    A: torch.Tensor            # (n, r) complex64: z = A @ basis series
+    Vc: torch.Tensor           # (rp, flen) complex64: conj(Vh) / sw, zero rows >= r
+    worst: float
+    stage: ExactStage = None
+
+
+def svd_tile(H, sw, isw, kmin, kend, eps):
+    """fp_filter.py:576-593 with the Gram, eigendecomposition and projection in complex128 on the GPU: normalised
+    Gram of the sw-weighted rows over [kmin, kend); eigh; worst lost power after k + 1 vectors; r = #(worst > eps)
+    + 1. Returns r, rp, A (n, r) complex64, Vc (rp, flen) complex64 = conj(Vh) / sw, and worst[r - 1]."""
+    n, flen = H.shape
+    G = torch.zeros((n, n), dtype=torch.complex128, device=H.device)
+    for c0 in range(kmin, kend, CH):
+        c1 = min(kend, c0 + CH)
+        hc = H[:, c0:c1].to(torch.complex128) * sw[c0:c1]
+        G += hc @ hc.conj().T
+    nrm = torch.sqrt(G.diagonal().real.clamp(min=1e-30))
+    lam, W = torch.linalg.eigh(G / (nrm[:, None] * nrm[None, :]))
+    lam, W = lam.flip(0).clamp(min=0), W.flip(1)
+    worst = (1 - ((W.abs() ** 2) * lam[None, :]).cumsum(1)).max(0).values
+    r = min(int((worst > eps).sum().item()) + 1, n)
+    Wr = W[:, :r]
+    s_r = torch.sqrt(lam[:r].clamp(min=1e-12 * float(lam[0])))
+    Bm = (Wr.conj().T / s_r[:, None]) / nrm[None, :]
+    rp = -(-r // 16) * 16
+    Vc = torch.zeros((rp, flen), dtype=torch.complex64, device=H.device)
+    for c0 in range(kmin, kend, CH):
+        c1 = min(kend, c0 + CH)
+        vh = Bm @ (H[:, c0:c1].to(torch.complex128) * sw[c0:c1])           # (r, chunk)
+        Vc[:r, c0:c1] = (vh.conj() * isw[c0:c1]).to(torch.complex64)
+    A = ((Wr * s_r[None, :]).conj() * nrm[:, None]).to(torch.complex64).contiguous()
+    return r, rp, A, Vc, float(worst[r - 1].item())
+
+
+class BankPlan:
+    """Bank-side products, made once per run and resident in HBM: tiles sorted by chirp time, dense templates,
+    SVD bases. sw: the SVD metric sqrt(mean IP) (fp_filter.py:363); it sets how well the basis compresses, never the
+    SNR definition (conj(h sw) (D / sw) = conj(h) D)."""
+
+    def __init__(self, m1, m2, e0, flen, df, opts: Options, sw, isw, name='bank'):
+        self.name, self.opts = name, opts
+        self.flen, self.df = flen, df
+        self.kmin = int(opts.low_frequency_cutoff / df)
+        self.m1, self.m2 = np.asarray(m1, float), np.asarray(m2, float)
+        self.e0 = np.zeros(len(self.m1)) if e0 is None else np.asarray(e0, float)
+        fl = opts.low_frequency_cutoff
+        tau0 = 5.0 / (256.0 * np.pi * fl * (self.m1 * self.m2 / (self.m1 + self.m2) ** 2)) * \
+            (np.pi * (self.m1 + self.m2) * 4.925490947641267e-06 * fl) ** (-5.0 / 3.0)
+        self.tiles = []
+        self.T = dict(templates_s=0.0, svd_s=0.0)
+        for ids in tmpl.tiles(tau0, opts.tile):
+            t0 = time.perf_counter()
+            H, kmax = tmpl.generate(self.m1[ids], self.m2[ids], self.e0[ids], flen, df, fl)
+            torch.cuda.synchronize(); t1 = time.perf_counter()
+            kend = int(kmax.max())
+            r, rp, A, Vc, worst = svd_tile(H, sw, isw, self.kmin, kend, opts.eps)
+            torch.cuda.synchronize(); t2 = time.perf_counter()
+            if 2 * rp > _lib.load().gwgs_gemm_max_k(PRECISIONS[opts.precision]):
+                raise AssertionError(f'tile rank {r} exceeds the GEMM kernel limit; use a smaller --tile')
+            nb = np.full(len(ids), opts.chisq_bins, np.int64)
+            stage = ExactStage(TemplateTile(H, nb, self.kmin, kmax), self_check=opts.self_check)
+            self.tiles.append(TileBasis(ids, H, kmax, r, rp, A, Vc, worst, stage))
+            self.T['templates_s'] += t1 - t0; self.T['svd_s'] += t2 - t1
+
+    @property
+    def ranks(self):
+        return [t.r for t in self.tiles]
+
+
+class Triggers:
+    """Peaks of one dataset and bank, accumulated over tiles and segments (fp_filter.py:323, 772-853)."""
+    KEYS = ('tid', 't', 'snr', 'ph', 'chi', 'dof', 'sg', 'ex', 'samp', 'seg')
+
+    def __init__(self):
+        self.parts = {k: [] for k in self.KEYS}
+
+    def add(self, **kw):
+        for k in self.KEYS:
+            self.parts[k].append(np.asarray(kw[k]))
+
+    def cat(self, k, dt):
+        return np.concatenate(self.parts[k]).astype(dt) if self.parts[k] else np.zeros(0, dt)
+
+    def to_dict(self, gates=(), chisq=True):
+        """The peak-file fields (fp_filter.py:910-937): gate veto |t - g_t| > w + taper + 0.5, sorted by
+        (template_id, end_time); newsnr as coinc_rank.py:39-44."""
+        tid, t, snr = self.cat('tid', np.int64), self.cat('t', np.float64), self.cat('snr', np.float32)
+        keep = np.ones(len(t), bool)
+        for g_t, g_w, g_taper in gates:
+            keep &= np.abs(t - g_t) > g_w + g_taper + 0.5
+        sel = np.nonzero(keep)[0][np.lexsort((t[keep], tid[keep]))]
+        out = dict(template_id=tid[sel], snr=snr[sel], end_time=t[sel],
+                   coa_phase=self.cat('ph', np.float32)[sel], peak_sigmasq=self.cat('sg', np.float64)[sel],
+                   exact=self.cat('ex', bool)[sel], sample=self.cat('samp', np.int64)[sel],
+                   segment=self.cat('seg', np.int64)[sel])
+        if chisq:
+            chi, dof = self.cat('chi', np.float32)[sel], self.cat('dof', np.int32)[sel]
+            rc = chi / np.maximum(dof, 1)
+            ns = np.array(out['snr'], np.float64)
+            m = rc > 1
+            ns[m] = ns[m] / ((1 + rc[m].astype(np.float64) ** 3) / 2.0) ** (1.0 / 6)
+            out.update(chisq=chi, chisq_dof=dof, newsnr=ns)
+        return out
+
+
+def svd_metric(datasets):
+    """fp_filter.py:360-364: IPall (all segments, flen), sw = sqrt(mean IPall) in float64, isw = 1/sw (0 where 0)."""
+    IPall = torch.cat([m.IP for m in datasets], 0)
+    sw = torch.sqrt(IPall.double().mean(0))
+    isw = torch.where(sw > 0, 1.0 / sw.clamp(min=1e-30), torch.zeros_like(sw))
+    return IPall, sw, isw
+
+
+class Filter:
+    """One bank against a batch of datasets (all with the same flen, tlen, sr)."""
+
+    def __init__(self, datasets, opts: Options):
+        _dev.require_gpu()
+        self.opts = opts
+        self.data = datasets
+        m0 = datasets[0]
+        assert all(m.flen == m0.flen and m.tlen == m0.tlen and m.sr == m0.sr for m in datasets)
+        self.flen, self.tlen, self.df, self.sr = m0.flen, m0.tlen, m0.df, m0.sr
+        self.kmin = int(opts.low_frequency_cutoff / self.df)
+        self.cw = int(opts.cluster_window * self.sr)
+        assert self.cw % BN == 0, 'cluster window must be a multiple of %d samples (GEMM output block)' % BN
+        self.IPall = torch.cat([m.IP for m in datasets], 0)
+        off = 0
+        self.seg0 = []
+        for m in datasets:
+            self.seg0.append(off); off += len(m.analyze)
+        self._q = None
+        self._q_rp = 0
+        self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
+                      n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
+        self.rel = []
+        self.code = PRECISIONS[opts.precision]
+        self.events = []                   # (start, end, algorithmic flops) per GEMM launch
+
+    def gemm_stats(self):
+        """(launches, total kernel seconds, total algorithmic flops) over the recorded GEMM launches."""
+        torch.cuda.synchronize()
+        t = sum(a.elapsed_time(b) for a, b, _ in self.events) / 1e3
+        return len(self.events), t, sum(f for _, _, f in self.events)
+
+    def _sync(self):
+        if self.opts.timing:
+            torch.cuda.synchronize()
+        return time.perf_counter()
+
+    def _qbuf(self, rp):
+        """(rp, tlen) complex64 transform input whose columns >= flen stay zero (fp_filter.py:595)."""
+        if self._q is None or self._q.shape[0] < rp:
+            self._q = torch.zeros((rp, self.tlen), dtype=torch.complex64, device='cuda')
+        return self._q[:rp]
+
+    def run(self, plan: BankPlan):
+        """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset."""
+        outs = [Triggers() for _ in self.data]
+        for tb in plan.tiles:
+            t0 = self._sync()
+            sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df)
+            normf = norm.to(torch.float32)
+            t1 = self._sync(); self.T['norm_s'] += t1 - t0
+            sig_h = sig.cpu().numpy()
+            norm_h = norm.cpu().numpy()
+            for di, m in enumerate(self.data):
+                for s in range(len(m.analyze)):
+                    self.filter_segment(tb, m, s, self.seg0[di] + s, normf, norm_h, sig_h, outs[di])
+        return outs
+
+    def filter_segment(self, tb: TileBasis, m: Dataset, s, gs, normf, norm_h, sig_h, out: Triggers):
+        o = self.opts
+        a0, a1 = m.analyze[s]
+        nw = (a1 - a0) // self.cw
+        if nw <= 0:
+            return
+        Tn = nw * self.cw
+        n, rp = len(tb.ids), tb.rp
+        K = 2 * rp
+        t0 = self._sync()
+        q = self._qbuf(rp)
+        _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, m.D[s].data_ptr(), self.tlen, q.data_ptr(),
+                  _dev.stream())
+        Zc = torch.fft.ifft(q, dim=1, norm='forward')                  # (rp, tlen): basis series
+        dt = torch.float32 if self.code == 0 else torch.bfloat16
+        Zt = torch.empty((Tn, K), dtype=dt, device='cuda')
+        _lib.call('gwgs_pack_zt', self.code, Zc.data_ptr(), self.tlen, rp, a0, Tn, Tn, None, Zt.data_ptr(),
+                  _dev.stream())
+        zoff = 0
+        t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
+        Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
+        rs = normf[:, gs].contiguous()
+        _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, rs.data_ptr(), None, n, K, Ar.data_ptr(),
+                  Ai.data_ptr(), _dev.stream())
+        vw = torch.empty((n, nw), dtype=torch.float32, device='cuda')
+        iw = torch.empty((n, nw), dtype=torch.int32, device='cuda')
+        if o.gemm_events:
+            e0 = torch.cuda.Event(enable_timing=True); e0.record()
+        fp_gemm.gemm_peaks_raw(self.code, Zt, K, Ar, Ai, K, n, K, Tn, 1, self.cw // BN, o.gemm_cfg, vw, iw, zoff)
+        if o.gemm_events:
+            e1 = torch.cuda.Event(enable_timing=True); e1.record()
+            self.events.append((e0, e1, 4.0 * n * (2 * tb.r) * Tn))
+        cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
+        g = peak_kernel.select_above(vw, float(cut))
+        t2 = self._sync(); self.T['gemm_s'] += t2 - t1
+        if len(g) == 0:
+            return
+        gd = torch.from_numpy(g).cuda()
+        il_d = iw.view(-1)[gd]
+        tb_ = (g // nw).astype(np.int64)
+        rows_d = torch.from_numpy(tb_.astype(np.int32)).cuda()
+        zreb_d = torch.empty(len(g), dtype=torch.complex64, device='cuda')
+        _lib.call('gwgs_rebuilt_z', self.code, Zt.data_ptr() + zoff, K, Ar.data_ptr(), Ai.data_ptr(), K, K,
+                  rows_d.data_ptr(), il_d.data_ptr(), len(g), zreb_d.data_ptr(), _dev.stream())
+        approx = torch.sqrt(vw.view(-1)[gd]).cpu().numpy()
+        samp = a0 + il_d.cpu().numpy().astype(np.int64)
+        zreb = zreb_d.cpu().numpy()
+        t3 = self._sync(); self.T['rebuilt_z_s'] += t3 - t2
+        self.T['n_candidates'] += len(g)
+        thr = o.snr_threshold
+        ex = approx >= o.exact_snr * (1 - o.margin)
+        rb = ~ex & (approx > thr)
+        tstamp = lambda sm: m.gps_start + (m.seg_start[s] + sm) / m.sr
+        if rb.any():
+            nrb = int(rb.sum())
+            out.add(tid=tb.ids[tb_[rb]], t=tstamp(samp[rb]), snr=approx[rb], ph=np.angle(zreb[rb]).astype(np.float32),
+                    chi=np.zeros(nrb, np.float32), dof=np.zeros(nrb, np.int32), sg=sig_h[tb_[rb], gs],
+                    ex=np.zeros(nrb, bool), samp=samp[rb], seg=np.full(nrb, gs))
+            self.T['n_rebuilt_peaks'] += nrb
+        tbx, sx, ax, zx_r = tb_[ex], samp[ex], approx[ex], zreb[ex]
+        self.T['n_exact_candidates'] += len(tbx)
+        if len(tbx):
+            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
+            nr = norm_h[tbx, gs]
+            snr = np.abs(pv.z.astype(np.complex128)) * nr
+            self.rel.append(ax / np.maximum(snr, 1e-30))
+            keep = snr > thr
+            nb = tb.stage.tile.nbins[tbx]
+            chi = ((nb * pv.pw.astype(np.float64) - np.abs(pv.z.astype(np.complex128)) ** 2) * nr ** 2).astype(np.float32)
+            dof = (2 * nb - 2).astype(np.int32)
+            k = keep
+            out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
+                    ph=np.angle(pv.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
+                    ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
+            self.T['n_chisq_points'] += int(k.sum())
+        t4 = self._sync(); self.T['refine_chisq_s'] += t4 - t3

@@ -1,0 +1,130 @@
+"""Fused MFMA GEMM + |z|^2 + max per BN output samples, for fp_filter.py (the reference's fp_gemm.py:1-64).
+
+z_re = Are @ Z, z_im = Aim @ Z; each launch keeps only the max |z|^2 and its sample per BN = 128 samples (mode 0,
+fused_peaks) or per cluster window of cw samples (mode 1, fused_window_peaks, the window max of fp_filter.py:745-746
+done in the epilogue). The kernel is hand-written HIP for gfx950 (csrc/gemm_peaks.hip): fp32 operands on
+v_mfma_f32_32x32x2_f32 (the plain form) or bf16 operands on v_mfma_f32_32x32x16_bf16, fp32 accumulation.
+
+Layout: the kernel reads the basis series time-major, Zt (T, K) with K contiguous. fused_peaks also accepts the
+reference's (K, Tn) layout and transposes it first (one extra pass); fp_filter hands Zt over directly.
+"""
+from __future__ import annotations
+
+import time
+
+import torch
+
+import _dev
+import _lib
+
+BN = 128
+DTYPES = {torch.float32: 0, torch.bfloat16: 1}
+
+
+def _kpad(K):
+    return -(-K // 8) * 8
+
+
+def _prep_operands(Are, Aim, Z, Tn, zt):
+    """Pad K to a multiple of 32 and rows of Zt to a multiple of BN; return (Zt, Ar, Ai, K, dtype code)."""
+    dt = Are.dtype
+    if dt == torch.float16:
+        dt = torch.bfloat16
+    assert dt in DTYPES, f'unsupported operand dtype {Are.dtype}'
+    K = Are.shape[1]
+    Kp = _kpad(K)
+    Tpad = -(-Tn // BN) * BN
+    if zt:
+        assert Z.shape[0] >= Tn and Z.shape[1] == K
+        Zt = Z
+    else:
+        assert Z.shape[0] == K and Z.shape[1] >= Tn
+        Zt = Z[:, :Tn].t()
+    if Kp != K or Zt.shape[0] < Tpad or Zt.dtype != dt or not Zt.is_contiguous():
+        Zn = torch.zeros((Tpad, Kp), dtype=dt, device=Z.device)
+        Zn[:Tn, :K] = Zt[:Tn]
+        Zt = Zn
+    if Kp != K or Are.dtype != dt or not Are.is_contiguous():
+        A2 = torch.zeros((Are.shape[0], Kp), dtype=dt, device=Are.device); A2[:, :K] = Are; Are = A2
+        A3 = torch.zeros((Aim.shape[0], Kp), dtype=dt, device=Aim.device); A3[:, :K] = Aim; Aim = A3
+    return Zt, Are, Aim.contiguous(), Kp, DTYPES[dt]
+
+
+def _cfg(cfg):
+    """cfg: None, an int (time tiles per block) or a tuple (tiles_per_block[, variant]); variant 1 forces the
+    K-loop kernel, 0 picks by K."""
+    if cfg is None:
+        return 0, 0
+    if isinstance(cfg, (tuple, list)):
+        return int(cfg[0]), int(cfg[1]) if len(cfg) > 1 else 0
+    return int(cfg), 0
+
+
+def gemm_peaks_raw(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, Tn, mode, per, cfg, V, I, zt_offset_bytes=0):
+    """Direct C-ABI call on pre-laid-out operands (fp_filter's hot loop)."""
+    tpb, variant = _cfg(cfg)
+    _lib.call('gwgs_gemm_peaks', dtype_code, variant, Zt.data_ptr() + zt_offset_bytes, ldz, Ar.data_ptr(),
+              Ai.data_ptr(), lda, M, K, Tn, mode, per, tpb, V.data_ptr(), I.data_ptr(), V.stride(0), _dev.stream())
+
+
+def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False):
+    """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32 / bfloat16 (float16 is taken as
+    bfloat16). Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are
+    masked to -1 as in fp_gemm.py:29. cfg: (tiles_per_block,) from pick_cfg, or None."""
+    Zt, Ar, Ai, K, code = _prep_operands(Are, Aim, Z, Tn, zt)
+    nblk = -(-Tn // BN)
+    V = torch.empty((M, nblk), device=Z.device, dtype=torch.float32)
+    I = torch.empty((M, nblk), device=Z.device, dtype=torch.int32)
+    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 0, 1, cfg, V, I)
+    return V, I
+
+
+def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False):
+    """As fused_peaks, with the cluster-window max fused: (M, Tn // cw) maxima and their samples, first index on
+    ties at block and window level. Tn must be a multiple of cw and cw of BN (fp_filter.py:359)."""
+    assert cw % BN == 0, 'cluster window must be a multiple of %d samples (GEMM output block)' % BN
+    assert Tn % cw == 0
+    Zt, Ar, Ai, K, code = _prep_operands(Are, Aim, Z, Tn, zt)
+    nw = Tn // cw
+    V = torch.empty((M, nw), device=Z.device, dtype=torch.float32)
+    I = torch.empty((M, nw), device=Z.device, dtype=torch.int32)
+    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 1, cw // BN, cfg, V, I)
+    return V, I
+
+
+TPB_CANDIDATES = (16, 32, 64, 128)
+VARIANTS = (0, 1)
+
+
+def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=2048, reps=3):
+    """Time every candidate launch shape once warm and keep the fastest on a representative shape (the reference's
+    sweep, fp_gemm.py:47-64). Candidates: time tiles per block (window mode needs a multiple of cw / BN). Returns
+    (cfg, {str(cfg): seconds or error}). Raises RuntimeError if none runs."""
+    g = torch.Generator(device='cpu').manual_seed(0)
+    Ar = torch.randn((M, K), generator=g).to(dev, dtype)
+    Ai = torch.randn((M, K), generator=g).to(dev, dtype)
+    Tn = Tn // cw * cw
+    Zs = torch.randn((Tn, K), generator=g).to(dev, dtype)
+    per = cw // BN
+    best, res = None, {}
+    for tpb in TPB_CANDIDATES:
+      for variant in VARIANTS:
+        c = (max(per, tpb // per * per), variant)
+        if str(c) in res:
+            continue
+        try:
+            fused_window_peaks(Ar, Ai, Zs, M, Tn, cw, c, zt=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fused_window_peaks(Ar, Ai, Zs, M, Tn, cw, c, zt=True)
+            torch.cuda.synchronize()
+            res[str(c)] = (time.perf_counter() - t0) / reps
+        except Exception as ex:   # a shape that fails is recorded, not fatal
+            res[str(c)] = repr(ex)[:300]
+            continue
+        if best is None or res[str(c)] < res[str(best)]:
+            best = c
+    if best is None:
+        raise RuntimeError('no GEMM config ran: %r' % res)
+    return best, res

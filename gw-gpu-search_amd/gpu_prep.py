@@ -1,0 +1,264 @@
+"""Data conditioning on the GPU (the reference's gpu_prep.py for the steps on its GPU path, :87-231, 234-282,
+405-454): segment FFTs, overlapping median-Welch PSDs in float64, interpolation, inverse-spectrum truncation, then
+D = stilde / psd and IP = 1 / psd, zero below the low-frequency cutoff; and the autogating (|whitened strain| above
+threshold, clustered, inverted-Tukey gates).
+
+FFTs are hipFFT (through torch.fft); every other pass is a kernel in csrc/prep.hip. Segment and PSD-span index
+arithmetic is integer host code, as in the reference. The strain is float32 already multiplied by DYN_RANGE_FAC;
+the PyCBC-side stages of the reference (store read, lal IIR high-pass, resampling, injections from files) are not
+on this path.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import _dev
+import _lib
+import peak_kernel
+
+DYN_RANGE_FAC = 5.9029581035870565e+20
+
+
+def median_bias(n):
+    """gpu_prep.py:47-53."""
+    if n >= 1000:
+        return np.log(2)
+    ans = 1.0
+    for i in range(1, (n - 1) // 2 + 1):
+        ans += 1.0 / (2 * i + 1) - 1.0 / (2 * i)
+    return ans
+
+
+def welch(x, seg_len, seg_stride, dt, num_segments=None, avg='median'):
+    """gpu_prep.py:87-125 in float64 on the GPU. x: (n,) or (rows, n) float32 tensor. Returns (psd float64 tensor,
+    delta_f)."""
+    X = x[None] if x.ndim == 1 else x
+    X = X.to(torch.float32).contiguous()
+    rows, n = X.shape
+    if num_segments is None:
+        num_segments = int(n // seg_stride)
+        if (num_segments - 1) * seg_stride + seg_len > n:
+            num_segments -= 1
+    data_len = (num_segments - 1) * seg_stride + seg_len
+    start = 0
+    if data_len < n:
+        diff = n - data_len
+        start = diff // 2 + (diff % 2)
+    assert data_len <= n, 'Incorrect choice of segmentation parameters'
+    segs = torch.empty((rows, num_segments, seg_len), dtype=torch.float64, device=X.device)
+    _lib.call('gwgs_welch_segments', X.data_ptr(), X.stride(0), rows, start, num_segments, seg_len, seg_stride,
+              segs.data_ptr(), _dev.stream())
+    st = torch.fft.rfft(segs, dim=2)
+    nf = seg_len // 2 + 1
+    out = torch.empty((rows, nf), dtype=torch.float64, device=X.device)
+    w = np.hanning(seg_len)
+    delta_f = 1.0 / dt / seg_len
+    fac = 2 * delta_f * seg_len / (w * w).sum()
+    _lib.call('gwgs_welch_reduce', st.data_ptr(), rows, num_segments, nf, float(dt), int(avg != 'median'),
+              float(median_bias(num_segments)), float(fac), out.data_ptr(), _dev.stream())
+    return (out[0] if x.ndim == 1 else out), delta_f
+
+
+def interpolate(psd, df_old, df_new, length=None):
+    """gpu_prep.py:128-135: np.interp (float64) onto k df_new."""
+    P = (psd[None] if psd.ndim == 1 else psd).to(torch.float64).contiguous()
+    new_n = (P.shape[-1] - 1) * df_old / df_new + 1 if length is None else length
+    nn = int(np.rint(new_n))
+    out = torch.empty((P.shape[0], nn), dtype=torch.float64, device=P.device)
+    _lib.call('gwgs_interp', P.data_ptr(), P.shape[0], P.shape[1], float(df_old), float(df_new), nn, out.data_ptr(),
+              _dev.stream())
+    return out[0] if psd.ndim == 1 else out
+
+
+def inverse_spectrum_truncation(psd, df, max_filter_len, low_frequency_cutoff, trunc_method=None):
+    """gpu_prep.py:138-158 in float64: psd^-1/2 on [kmin, N/2), unnormalised irfft * df, zero outside +-L/2 (Hann
+    taper of the kept ends with trunc_method 'hann'), rfft / (df N), 1 / |.|^2."""
+    P = (psd[None] if psd.ndim == 1 else psd).to(torch.float64).contiguous()
+    rows, flen = P.shape
+    N = (flen - 1) * 2
+    kmin = int(low_frequency_cutoff / df) if low_frequency_cutoff else 1
+    inv = torch.empty((rows, flen), dtype=torch.complex128, device=P.device)
+    _lib.call('gwgs_isqrt_band', P.data_ptr(), rows, flen, kmin, N // 2, inv.data_ptr(), _dev.stream())
+    q = torch.fft.irfft(inv, n=N, dim=1, norm='forward').contiguous()
+    _lib.call('gwgs_trunc', q.data_ptr(), rows, N, int(max_filter_len), int(trunc_method == 'hann'), float(df),
+              _dev.stream())
+    pt = torch.fft.rfft(q, dim=1)
+    out = torch.empty((rows, flen), dtype=torch.float64, device=P.device)
+    _lib.call('gwgs_inv_abs2', pt.data_ptr(), rows, flen, 1.0 / (df * N), out.data_ptr(), _dev.stream())
+    return out[0] if psd.ndim == 1 else out
+
+
+def segment_slices(n, sr, start_gps, seg_len, start_pad, end_pad, trig_start=0, trig_end=0):
+    """gpu_prep.py:234-271 (pycbc.strain.StrainSegments index arithmetic): [((s0, s1), (a0, a1))]."""
+    end_gps = start_gps + n / sr
+    if not trig_start:
+        trig_start = int(start_gps) + start_pad
+    if not trig_end:
+        trig_end = int(end_gps) - end_pad
+    assert trig_start >= int(start_gps) + start_pad and trig_end <= int(end_gps) - end_pad
+    seg_width = seg_len - start_pad - end_pad
+    analyzable = trig_end - trig_start
+    data_start = (trig_start - start_pad) - int(start_gps)
+    data_end = trig_end + end_pad - int(start_gps)
+    data_dur = data_end - data_start
+    data_start, data_end = data_start * sr, data_end * sr
+    num_segs = int(np.ceil(float(analyzable) / float(seg_width)))
+    seg_offset = int(np.ceil(analyzable / float(num_segs)))
+    segs, anas = [], []
+    for k in range(num_segs - 1):
+        s0 = int(data_start + (k * seg_offset) * sr)
+        segs.append((s0, int(s0 + seg_len * sr)))
+        a0 = int(start_pad * sr)
+        anas.append((a0, int(a0 + seg_offset * sr)))
+    s1 = int(data_end)
+    segs.append((int(s1 - seg_len * sr), s1))
+    remaining = data_dur - ((num_segs - 1) * seg_offset + start_pad)
+    anas.append((int((seg_len - remaining) * sr), int((seg_len - end_pad) * sr)))
+    ts_i = (trig_start - int(start_gps)) * sr
+    te_i = (trig_end - int(start_gps)) * sr
+    out = []
+    for (s0, s1), (a0, a1) in zip(segs, anas):
+        if ts_i > a0 + s0:
+            a0 += ts_i - (a0 + s0)
+        if te_i < a1 + s0:
+            a1 -= (a1 + s0) - te_i
+        if a0 < a1:
+            assert s0 >= 0 and s1 <= n, 'segment outside the data needs zero padding (not supported)'
+            out.append(((s0, s1), (a0, a1)))
+    return out
+
+
+def overlapping_psd_spans(n, seg_len, seg_stride, num_segments):
+    """gpu_prep.py:274-282 (pycbc.psd.generate_overlapping_psds)."""
+    psd_len = (num_segments - 1) * seg_stride + seg_len
+    assert n >= psd_len
+    if n == psd_len:
+        return [(0, n)]
+    m = int(2 * (n - 1) / psd_len)
+    stride = int((n - psd_len) / m)
+    return [(n - psd_len, n) if i == m - 1 else (stride * i, psd_len + stride * i) for i in range(m)]
+
+
+def pick_psds(segs, spans):
+    """gpu_prep.py:438-447 (associate_psds_to_segments): largest overlap, first wins ties."""
+    pick = []
+    for (a, b), _ in segs:
+        best, ov = None, 0
+        for j, (c, e) in enumerate(spans):
+            o = min(b, e) - max(a, c)
+            if o > 0 and o > ov:
+                best, ov = j, o
+        pick.append(best)
+    return pick
+
+
+def cluster_over_window(idx, vals, window):
+    """gpu_prep.py:161-172 (findchirp_cluster_over_window; a sequential host loop over the few threshold hits)."""
+    if len(idx) == 0:
+        return np.zeros(0, dtype=int)
+    keep = [0]
+    for i in range(1, len(idx)):
+        if idx[i] < idx[keep[-1]] + window:
+            if vals[i] > vals[keep[-1]]:
+                keep[-1] = i
+        else:
+            keep.append(i)
+    return np.array(keep)
+
+
+def whitened_magnitude(s, sr, flow, corrupt_time, psd_duration=4.0, psd_stride=2.0):
+    """gpu_prep.py:175-202: |strain whitened by a 4 s median-Welch PSD (Hann-truncated), unit variance| over the n
+    samples, zero at the corrupt ends. float32 tensor (n,)."""
+    n = s.shape[0]
+    dt = 1.0 / sr
+    cl = int(corrupt_time * sr)
+    L = 1 << n.bit_length()
+    pad_start = int(L / 2 - n / 2)
+    sp = torch.empty(L, dtype=torch.float64, device=s.device)
+    _lib.call('gwgs_gate_pad', s.data_ptr(), n, cl, pad_start, L, sp.data_ptr(), _dev.stream())
+    core = sp[pad_start + cl:pad_start + n - cl].to(torch.float32)
+    psd, pdf = welch(core, int(psd_duration * sr), int(psd_stride * sr), dt)
+    df = 1.0 / (L * dt)
+    psd = interpolate(psd, pdf, df)
+    psd = inverse_spectrum_truncation(psd, df, int(psd_duration * sr), flow, 'hann').contiguous()
+    norm = sr / 2.0 - flow
+    st = torch.fft.rfft(sp).contiguous()
+    _lib.call('gwgs_gate_whiten', st.data_ptr(), psd.data_ptr(), st.shape[0], int(flow / df), dt, norm, _dev.stream())
+    wt = torch.fft.irfft(st, n=L, norm='forward').contiguous()
+    mag = torch.empty(n, dtype=torch.float32, device=s.device)
+    _lib.call('gwgs_gate_mag', wt.data_ptr(), pad_start, n, cl, float(df), mag.data_ptr(), _dev.stream())
+    return mag
+
+
+def detect_loud_glitches(s, sr, start_gps, threshold, cluster_window, flow, corrupt_time):
+    """gpu_prep.py:175-209 with float64 GPS times: times of the clustered |whitened| > threshold samples."""
+    mag = whitened_magnitude(s, sr, flow, corrupt_time)
+    ind = peak_kernel.select_above(mag, float(threshold))
+    vals = mag[torch.from_numpy(ind).to(mag.device)].cpu().numpy() if len(ind) else np.zeros(0, np.float32)
+    keep = cluster_over_window(ind.astype(np.int32), vals, int(cluster_window * sr))
+    return [start_gps + int(i) / sr for i in ind[keep]]
+
+
+def gate_taper(x, sr, start_gps, gates):
+    """gpu_prep.py:212-231 (pycbc gate_data): inverted Tukey windows, in place on the float32 strain tensor."""
+    n = x.shape[0]
+    duration = n / sr
+    for gt, gw, pw in gates:
+        t_start = gt - gw - pw - start_gps
+        t_end = gt + gw + pw - start_gps
+        if t_start > duration or t_end < 0.0:
+            continue
+        win = int(2 * sr * (gw + pw))
+        pad = int(sr * pw)
+        offset = int(t_start * sr)
+        i1, i2 = max(0, -offset), min(win, n - offset)
+        _lib.call('gwgs_gate_apply', x.data_ptr(), n, offset, i1, i2, pad, win, _dev.stream())
+    return x
+
+
+def autogate(s, sr, start_gps, threshold, cluster=5.0, width=0.25, taper=0.25, pad=16.0, flow=15.0,
+             max_iterations=1):
+    """gpu_prep.py:383-392: detect, gate, repeat. Returns the gate list [(t, width, taper)] (s gated in place)."""
+    gates = []
+    for _ in range(max_iterations):
+        gt = detect_loud_glitches(s, sr, start_gps, threshold, cluster, flow, pad)
+        gp = [(g, width, taper) for g in gt]
+        gates += gp
+        gate_taper(s, sr, start_gps, gp)
+        if not gt:
+            break
+    return gates
+
+
+def condition(s, sr, gps_start, seg_len, start_pad, end_pad, psd_seg_len, psd_stride, psd_nseg, psd_inv_len, flow,
+              trunc_method=None, avg='median', tag='', gates=()):
+    """gpu_stage2 (gpu_prep.py:405-454) from the float32 strain tensor s (x DYN_RANGE_FAC, on the GPU): segment
+    FFTs, PSD spans, float64 median Welch, interpolation, truncation, and D, IP. Returns an fp_filter.Dataset."""
+    from fp_filter import Dataset
+    s = s.to(torch.float32).contiguous()
+    n = s.shape[0]
+    seg = segment_slices(n, sr, gps_start, seg_len, start_pad, end_pad)
+    tlen = int(seg_len * sr)
+    flen = tlen // 2 + 1
+    df = 1.0 / seg_len
+    starts = [a for (a, _), _ in seg]
+    x = torch.stack([s[a:a + tlen] for a in starts])
+    X = torch.fft.rfft(x, dim=1).contiguous()                 # complex64, scaled by 1/sr in the whitening pass
+    pl, ps = int(psd_seg_len * sr), int(psd_stride * sr)
+    spans = overlapping_psd_spans(n, pl, ps, psd_nseg)
+    parts = torch.stack([s[a:b] for a, b in spans])
+    psd, pdf = welch(parts, int(psd_seg_len * sr + 0.5), int(psd_stride * sr + 0.5), 1.0 / sr, num_segments=psd_nseg,
+                     avg=avg)
+    if pdf != df:
+        psd = interpolate(psd, pdf, df, flen)
+    if psd_inv_len:
+        psd = inverse_spectrum_truncation(psd, df, int(psd_inv_len * sr), flow, trunc_method)
+    psd = psd.contiguous()
+    pick = _dev.i32(pick_psds(seg, spans))
+    kmin = int(flow / df)
+    D = torch.empty((len(seg), flen), dtype=torch.complex64, device=s.device)
+    IP = torch.empty((len(seg), flen), dtype=torch.float32, device=s.device)
+    _lib.call('gwgs_whiten', X.data_ptr(), float(np.float32(1.0 / sr)), psd.data_ptr(), pick.data_ptr(), len(seg),
+              flen, kmin, D.data_ptr(), IP.data_ptr(), _dev.stream())
+    return Dataset(tag, D, IP, flen, tlen, df, float(sr), float(gps_start), [a for _, a in seg], starts, list(gates))

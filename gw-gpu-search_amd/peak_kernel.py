@@ -1,0 +1,54 @@
+"""Fused |SNR| peak finding (the reference's peak_kernel.py:1-47): for each (template row, cluster window) one
+block reads the complex SNR series once and returns the max |z|^2 and its index in the window, first index on ties
+(csrc/peaks.hip win_peak_k, wave64 reductions)."""
+from __future__ import annotations
+
+import torch
+
+import _dev
+import _lib
+
+THREADS = 256
+
+
+def window_peaks(z, a0, cw, nwin):
+    """z: (rows, N) complex64 on the GPU, C-contiguous. Returns (max |z|^2, argmax within window), each
+    (rows, nwin), float32 / int32."""
+    assert z.dtype == torch.complex64 and z.is_contiguous()
+    rows = z.shape[0]
+    pmax = torch.empty((rows, nwin), dtype=torch.float32, device=z.device)
+    pidx = torch.empty((rows, nwin), dtype=torch.int32, device=z.device)
+    _lib.call('gwgs_window_peaks', z.data_ptr(), z.shape[1], rows, int(a0), int(cw), int(nwin), pmax.data_ptr(),
+              pidx.data_ptr(), _dev.stream())
+    return pmax, pidx
+
+
+def block_window_max(V, I, nw, per):
+    """fp_filter.py:745-746 on the device: per window the max of `per` consecutive block maxima and the sample of
+    that block, first block on ties. V, I (n, >= nw per)."""
+    n = V.shape[0]
+    vw = torch.empty((n, nw), dtype=torch.float32, device=V.device)
+    iw = torch.empty((n, nw), dtype=torch.int32, device=V.device)
+    _lib.call('gwgs_block_window_max', V.data_ptr(), I.data_ptr(), V.stride(0), n, nw, per, vw.data_ptr(),
+              iw.data_ptr(), _dev.stream())
+    return vw, iw
+
+
+def select_above(vw, cut):
+    """Row-major flat indices of vw > cut (strict), as torch.nonzero orders them. Returns an int64 host array."""
+    import numpy as np
+    total = vw.numel()
+    if total == 0:
+        return np.zeros(0, np.int64)
+    nwork = _lib.load().gwgs_select_workspace(total)
+    work = torch.empty(int(nwork) + 1, dtype=torch.int32, device=vw.device)
+    cnt = work[-1:]
+    cap = max(1024, total // 64)
+    while True:
+        out = torch.empty(cap, dtype=torch.int32, device=vw.device)
+        _lib.call('gwgs_select_above', vw.data_ptr(), total, float(cut), out.data_ptr(), cap, work.data_ptr(),
+                  cnt.data_ptr(), _dev.stream())
+        n = int(cnt.item())
+        if n <= cap:
+            return out[:n].cpu().numpy().astype(np.int64)
+        cap = n

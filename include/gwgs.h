@@ -1,0 +1,129 @@
+/* C ABI of libgwgs_hip.so: hand-written gfx950 (MI355X) kernels for gw-gpu-search's filtering hot path.
+ *
+ * Conventions. Every pointer argument is a device pointer unless noted; `stream` is a hipStream_t (pass torch's
+ * torch.cuda.current_stream().cuda_stream, or 0 for the null stream). Leading dimensions are in elements. Every
+ * function only enqueues work on `stream` and allocates nothing. Return value: 0 = ok; negative = argument error
+ * (GWGS_EINVAL -1, GWGS_ESIZE -2, GWGS_EUNSUP -3), raised by the Python layer as AssertionError like the
+ * reference's asserts; positive = a hipError_t, raised as RuntimeError. Complex arrays are interleaved float2 /
+ * double2. The functions are re-entrant; calls on different streams may overlap.
+ *
+ * Each declaration names the reference interface it replaces (files of DivyanshGolyan/gw-gpu-search).
+ */
+#ifndef GWGS_H
+#define GWGS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWGS_OK 0
+#define GWGS_EINVAL -1
+#define GWGS_ESIZE -2
+#define GWGS_EUNSUP -3
+
+/* ---- fused SNR-reconstruction GEMM (pipeline/fp_gemm.py:14-44 gemm_absmax / fused_peaks; with mode 1 also the
+ * cluster-window max of pipeline/fp_filter.py:745-746) ------------------------------------------------------------
+ * z_re[m, t] = sum_k Ar[m, k] Zt[t, k], z_im[m, t] = sum_k Ai[m, k] Zt[t, k], pw = z_re^2 + z_im^2.
+ * dtype: 0 = float32 operands, 1 = bfloat16 operands (bits in uint16); fp32 accumulation.
+ * variant: 0 = choose by K (template operands held in registers up to K = 256 bf16 / 128 fp32), 1 = K-loop kernel.
+ * Zt: (>= ceil(Tn/128)*128 rows, ldz) time-major; Ar, Ai: (M, lda).
+ * mode 0: V, I (M, ldv >= ceil(Tn/128)): per 128 samples the max pw (columns >= Tn masked to -1) and its sample.
+ * mode 1: V, I (M, ldv >= Tn/(128 per)): per window of `per` blocks the max and its sample; Tn % (128 per) == 0.
+ * First index wins ties at both levels. tiles_per_block: 128-sample tiles per workgroup (0 = default). */
+int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
+                    int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,
+                    void* stream);
+/* largest K the GEMM accepts for dtype (0 if the dtype is unsupported) */
+int gwgs_gemm_max_k(int dtype);
+
+/* ---- peaks and candidates --------------------------------------------------------------------------------------
+ * pipeline/peak_kernel.py:8-47 win_peak / window_peaks: z (rows, ldz) complex64; per (row, window) the max |z|^2
+ * (float32, no fused multiply-add) over z[row, a0 + w cw : a0 + (w+1) cw] and its index in the window. */
+int gwgs_window_peaks(const void* z, long ldz, int rows, int a0, int cw, int nwin, float* pmax, int* pidx,
+                      void* stream);
+/* pipeline/fp_filter.py:745-746: vw[r, w] = max_j V[r, w per + j], iw = I at that j (first j on ties). */
+int gwgs_block_window_max(const float* V, const int* I, long ldv, int n, int nw, int per, float* vw, int* iw,
+                          void* stream);
+/* pipeline/fp_filter.py:747 (torch.nonzero(vw > cut)): row-major indices g with vw[g] > cut, in order, into
+ * out[0:cap]; the count to *count_dev (device int). work: gwgs_select_workspace(total) device ints (host call). */
+long gwgs_select_workspace(long total);
+int gwgs_select_above(const float* vw, long total, float cut, int* out, int cap, int* work, int* count_dev,
+                      void* stream);
+/* pipeline/fp_filter.py:759-764: rebuilt complex SNR at candidates, out[c] = (Ar[rows[c]] . Zt[tcol[c]],
+ * Ai[rows[c]] . Zt[tcol[c]]) in fp32 (complex64 out). dtype as gwgs_gemm_peaks. */
+int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K,
+                   const int* rows, const int* tcol, int n, void* out, void* stream);
+
+/* ---- exact values at peaks (pipeline/chisq_kernel.py:14-121 chisq_pts / chisq_pts_fused / bin_power_sums;
+ * the dense exact stage of pipeline/exact_stage.py:196-203) ------------------------------------------------------
+ * Group g: row grow[g] of q (or of the templates h when d != NULL: summand conj(h_k) d_k), peaks
+ * gstart[g] .. gstart[g] + gcount[g] - 1 (gcount <= gwgs_chisq_peaks_per_group()) at samples ptime[],
+ * bin edges edges[g, 0:nedge]. out[p] = sum_j |Z_j|^2, zout[p] = sum_j Z_j (may be NULL), with
+ * Z_j = sum_{e_j <= k < e_j+1} v_k exp(2 pi i ((k t) mod N) / N). Any N > 0 (64-bit integer phase). */
+int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const int* grow, const int* gstart,
+                      const int* gcount, const long long* ptime, const int* edges, int nedge, int ngroups, float* out,
+                      void* zout, void* stream);
+int gwgs_chisq_peaks_per_group(void);
+/* pipeline/chisq_edges.py:90-143 edges_fused (and edges_blocked, multi_filter.py:264-282 edges_per_template):
+ * per request i, row rows[i] of h: edge j = first k with float64 cumsum(|h|^2 ips)[k] > j / nbs[i] * total over
+ * [0, k1s[i]), clipped to [kmin, k1s[i]]; columns j >= nbs[i] and the last hold k1s[i]. out (n, nedge) int32. */
+int gwgs_edges_fused(const void* h, long ldh, const int* rows, const float* ips, const int* k1s, const int* nbs,
+                     int nedge, int kmin, int n, int* out, void* stream);
+
+/* ---- basis series and GEMM operands (pipeline/fp_filter.py:597-621 basis_series, stacked_A) --------------------
+ * q[j, k] = Vc[j, k] D[k] for k < flen, j < rp (q: (rp, ldq) complex64; columns >= flen are not written). */
+int gwgs_form_q(const void* Vc, int rp, long flen, const void* D, long ldq, void* q, void* stream);
+/* Zt[t, 2j] = Re Zc[j, a0 + t] cs[j], Zt[t, 2j+1] = Im Zc[j, a0 + t] cs[j] for t < Tn, zero for Tn <= t < Tpad;
+ * Zc (rp, ldc) complex64, rp % 16 == 0; Zt (Tpad, 2 rp) float32 (dtype 0) or bfloat16 (dtype 1); cs may be NULL. */
+int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, long Tn, long Tpad, const float* colscale,
+                 void* Zt, void* stream);
+/* As = A[m, j] rowscale[m] colscale[j] (A (n, r) complex64); Ar[m, 2j] = Re As, Ar[m, 2j+1] = -Im As,
+ * Ai[m, 2j] = Im As, Ai[m, 2j+1] = Re As; zero for j >= r, m >= n. Ar, Ai (Mpad, K). */
+int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* rowscale, const float* colscale, int Mpad,
+                 int K, void* Ar, void* Ai, void* stream);
+/* out[j] = sum_{a0 <= t < a0 + Tn} |Zc[j, t]|^2 (double), the rms scale of pipeline/fp_filter.py:609-610 */
+int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double* out, void* stream);
+
+/* ---- templates (the closed-form chirps standing in for pipeline/imr_interp.py's interpolated waveforms) and
+ * sigma^2 (pipeline/fp_filter.py:710-714) ------------------------------------------------------------------------
+ * H[i, k] = f^(-7/6) exp(-i psi_i(f)), f = k df, for int(flow/df) <= k < kmax_i = min(int(f_isco,i / df), flen-1),
+ * psi = TaylorF2 to 1PN + the leading eccentric term at reference frequency f0. m1, m2, e0: device double arrays
+ * (e0 may be NULL); kmax: device int array (may be NULL). */
+int gwgs_spa_templates(const double* m1, const double* m2, const double* e0, int n, long flen, double df,
+                       double flow, double f0, void* H, long ldh, int* kmax, void* stream);
+/* sig[i, s] = 4 df sum_k |H[i, k]|^2 IP[s, k] (double accumulation), s < nseg <= 16 */
+int gwgs_sigmasq(const void* H, long ldh, long flen, int n, const float* IP, long ldip, int nseg, double df,
+                 double* sig, long lds, void* stream);
+
+/* ---- data conditioning (pipeline/gpu_prep.py) ------------------------------------------------------------------
+ * welch (gpu_prep.py:87-125): segs[r, s, i] = x[r, start + s stride + i] * hann_L(i) (float64 out) */
+int gwgs_welch_segments(const float* x, long ldx, int rows, long start, int nseg, int L, int stride, double* segs,
+                        void* stream);
+/* welch reduction: per (row, bin) of X (rows, nseg, nf) complex128: p = |X|^2 dt^2 (halved at DC and Nyquist),
+ * median (numpy's) / bias or mean, times fac; nseg <= 512 */
+int gwgs_welch_reduce(const void* X, int rows, int nseg, int nf, double dt, int mean, double bias, double fac,
+                      double* out, void* stream);
+/* interpolate (gpu_prep.py:128-135): np.interp of P (rows, nold) at k df_old onto i df_new, i < nnew */
+int gwgs_interp(const double* P, int rows, int nold, double df_old, double df_new, long nnew, double* out,
+                void* stream);
+/* inverse_spectrum_truncation (gpu_prep.py:138-158) passes: P^-1/2 on [kmin, khi) as complex128 */
+int gwgs_isqrt_band(const double* P, int rows, long flen, long kmin, long khi, void* out, void* stream);
+/* q *= df; zero [L/2, N - L/2), Hann taper of the kept ends when hann != 0 */
+int gwgs_trunc(double* q, int rows, long N, long L, int hann, double df, void* stream);
+/* out = 1 / |X scale|^2 */
+int gwgs_inv_abs2(const void* X, int rows, long flen, double scale, double* out, void* stream);
+/* gpu_stage2 output (gpu_prep.py:448-451): D[s, k] = (X[s, k] inv_sr) / P[pick[s], k] (complex64), IP = 1/P as
+ * float32, both 0 where P <= 0 or k < kmin */
+int gwgs_whiten(const void* X, float inv_sr, const double* P, const int* pick, int nseg, long flen, long kmin,
+                void* D, float* IP, void* stream);
+/* detect_loud_glitches (gpu_prep.py:175-204): taper + centre in a zero buffer; whiten; |.| over the data */
+int gwgs_gate_pad(const float* s, long n, long cl, long pad_start, long Lp, double* out, void* stream);
+int gwgs_gate_whiten(void* st, const double* psd, long flen, long kmin, double dt, double norm, void* stream);
+int gwgs_gate_mag(const double* wt, long pad_start, long n, long cl, double df, float* mag, void* stream);
+/* gate_taper (gpu_prep.py:212-231): x[offset + i] *= inverted Tukey window(i), i in [i1, i2) */
+int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, long pad, long win, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

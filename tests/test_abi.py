@@ -1,0 +1,60 @@
+"""The C ABI (CPU): the library loads, exports every symbol include/gwgs.h declares (and the ctypes table covers
+them), and rejects bad arguments before touching a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, 'include', 'gwgs.h')).read()
+    return sorted(set(re.findall(r'^(?:int|long)\s+(gwgs_\w+)\(', src, re.M)))
+
+
+def test_header_symbols_exported(lib):
+    import _lib
+    names = _declared()
+    assert len(names) >= 25
+    out = subprocess.run(['nm', '-D', '--defined-only', _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r'\sT\s(gwgs_\w+)', out))
+    assert set(names) == exported
+    assert set(names) == set(_lib.SIGNATURES)
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+def test_bad_arguments_rejected(lib):
+    import _lib
+    # null pointers: rejected before any HIP call
+    assert lib.gwgs_gemm_peaks(1, 0, None, 64, None, None, 64, 10, 64, 1024, 0, 1, 0, None, None, 8, None) == -1
+    assert lib.gwgs_chisq_points(None, 10, None, 16, None, None, None, None, None, 2, 1, None, None, None) == -1
+    assert lib.gwgs_edges_fused(None, 10, None, None, None, None, 2, 0, 1, None, None) == -1
+    assert lib.gwgs_welch_reduce(ctypes.c_void_p(16), 1, 600, 8, 1.0, 0, 1.0, 1.0, ctypes.c_void_p(16), None) == -2
+    assert lib.gwgs_gemm_max_k(7) == 0
+    assert lib.gwgs_select_workspace(4097) == 3
+    with pytest.raises(AssertionError):
+        _lib.check(-1, 'x')
+    with pytest.raises(RuntimeError):
+        _lib.check(700, 'x')
+
+
+def test_no_cpu_fallback_when_library_missing(tmp_path, monkeypatch):
+    """The product path fails loudly without the HIP library: load() raises, no other path exists."""
+    import _lib
+    monkeypatch.setattr(_lib, 'LIB_PATH', str(tmp_path / 'missing.so'))
+    monkeypatch.setattr(_lib, '_lib', None)
+    with pytest.raises(FileNotFoundError):
+        _lib.load()
+
+
+def test_product_never_imports_oracle():
+    """Only tests/, smoke() and bench.py's CPU-baseline leg touch oracle/: no package module imports it."""
+    pkg = os.path.join(ROOT, 'gw-gpu-search_amd')
+    for f in os.listdir(pkg):
+        if f.endswith('.py'):
+            src = open(os.path.join(pkg, f)).read()
+            assert not re.search(r'^\s*(from|import)\s+oracle\b', src, re.M), f

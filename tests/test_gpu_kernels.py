@@ -1,0 +1,192 @@
+"""Parity of each HIP kernel, called through the C ABI, against the oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import core, pipeline
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(x):
+    return torch.from_numpy(np.asarray(x, np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+def _operands(rng, M, K, Tn, dtype):
+    Ar = rng.standard_normal((M, K)).astype(np.float32)
+    Ai = rng.standard_normal((M, K)).astype(np.float32)
+    Z = rng.standard_normal((K, Tn)).astype(np.float32)
+    if dtype == torch.bfloat16:
+        Ar, Ai, Z = _bf16_round(Ar), _bf16_round(Ai), _bf16_round(Z)
+    return Ar, Ai, Z
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize('M,K,Tn,variant', [(45, 64, 5 * 128 + 37, 0), (128, 40, 1024, 0), (300, 256, 2048, 0),
+                                             (77, 520, 1000, 0), (130, 64, 1024, 1), (1, 16, 128, 1)])
+def test_fused_peaks_vs_oracle(gpu, dtype, M, K, Tn, variant):
+    """fp_gemm.fused_peaks (mode 0) against the float64 GEMM + block max of the same (rounded) operands. Tolerance:
+    relative 1e-4 on the block maxima (fp32 accumulation); the chosen sample must reach the float64 max to 1e-4."""
+    import fp_gemm
+    rng = np.random.default_rng(M * 7 + K)
+    Ar, Ai, Z = _operands(rng, M, K, Tn, dtype)
+    V, I = fp_gemm.fused_peaks(torch.from_numpy(Ar).cuda().to(dtype), torch.from_numpy(Ai).cuda().to(dtype),
+                               torch.from_numpy(Z).cuda().to(dtype), M, Tn, (16, variant))
+    V, I = V.cpu().numpy(), I.cpu().numpy()
+    Vr, Ir = core.gemm_absmax(Ar, Ai, Z, Tn)
+    assert V.shape == Vr.shape and I.shape == Ir.shape
+    np.testing.assert_allclose(V, Vr, rtol=1e-4)
+    zr = Ar.astype(np.float64) @ Z; zi = Ai.astype(np.float64) @ Z
+    pw = zr * zr + zi * zi
+    got = np.take_along_axis(pw, I.astype(np.int64), 1)
+    assert np.all(got >= Vr * (1 - 1e-4))
+    assert np.all((I // 128) == np.arange(I.shape[1])[None, :])
+    assert (I == Ir).mean() > 0.99
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize('variant', [0, 1])
+def test_window_mode_equals_block_max(gpu, dtype, variant):
+    """The fused window epilogue (mode 1) must equal, bit for bit, the window max of the same kernel's per-block
+    output (mode 0) taken by the oracle's first-index rule and by the GPU block_window_max kernel."""
+    import fp_gemm, peak_kernel
+    rng = np.random.default_rng(3)
+    M, K, cw, nw = 200, 96, 2048, 6
+    Tn = cw * nw
+    Ar, Ai, Z = _operands(rng, M, K, Tn, dtype)
+    args = [torch.from_numpy(a).cuda().to(dtype) for a in (Ar, Ai, Z)]
+    V, I = fp_gemm.fused_peaks(*args, M, Tn, (16, variant))
+    vw, iw = fp_gemm.fused_window_peaks(*args, M, Tn, cw, (32, variant))
+    vo, io = core.window_max(V.cpu().numpy(), I.cpu().numpy(), nw, cw // 128)
+    assert np.array_equal(vw.cpu().numpy(), vo) and np.array_equal(iw.cpu().numpy(), io)
+    vg, ig = peak_kernel.block_window_max(V, I, nw, cw // 128)
+    assert np.array_equal(vg.cpu().numpy(), vo) and np.array_equal(ig.cpu().numpy(), io)
+
+
+def test_window_max_ties_first_index(gpu):
+    """Planted ties: equal block maxima inside a window -> the first block's sample (fp_filter.py:745 with the tie
+    rule of peak_kernel.py:21-33)."""
+    import peak_kernel
+    rng = np.random.default_rng(5)
+    n, nw, per = 7, 9, 16
+    V = rng.integers(0, 4, (n, nw * per)).astype(np.float32)
+    I = (np.arange(nw * per)[None, :] * 128 + rng.integers(0, 128, (n, nw * per))).astype(np.int32)
+    vg, ig = peak_kernel.block_window_max(torch.from_numpy(V).cuda(), torch.from_numpy(I).cuda(), nw, per)
+    vo, io = core.window_max(V, I, nw, per)
+    assert np.array_equal(vg.cpu().numpy(), vo) and np.array_equal(ig.cpu().numpy(), io)
+
+
+def test_window_peaks_bit_exact(gpu):
+    """peak_kernel.window_peaks against the oracle on identical complex64 input, with planted ties and a window
+    shorter than the block (rows 5, cw 3000 and 100)."""
+    import peak_kernel
+    rng = np.random.default_rng(1)
+    for cw, nwin, a0 in ((3000, 4, 17), (100, 30, 0)):
+        z = (rng.standard_normal((5, a0 + cw * nwin + 50)) + 1j * rng.standard_normal((5, a0 + cw * nwin + 50)))
+        z = z.astype(np.complex64)
+        z[1, a0 + 10] = z[1, a0 + 40] = 9 + 9j           # tie inside window 0 of row 1
+        z[2, a0 + cw + 5] = z[2, a0 + cw + 7] = 7 - 7j
+        pm, pi = peak_kernel.window_peaks(torch.from_numpy(z).cuda(), a0, cw, nwin)
+        po, io = core.window_peaks(z, a0, cw, nwin)
+        assert np.array_equal(pm.cpu().numpy(), po) and np.array_equal(pi.cpu().numpy(), io)
+
+
+def test_select_above_order(gpu):
+    import peak_kernel
+    rng = np.random.default_rng(2)
+    for total in (0, 1, 4095, 4097, 100000):
+        v = rng.random(total).astype(np.float32)
+        v[::97] = 0.5
+        g = peak_kernel.select_above(torch.from_numpy(v).cuda(), 0.5)
+        assert np.array_equal(g, np.flatnonzero(v > np.float32(0.5)))
+
+
+def test_chisq_kernel_vs_float64_definition(gpu):
+    """tests/test_chisq_kernel_gpu.py of the reference, tightened: N = 2^21, 4 rows, random complex q, 40 random edges
+    per row, rng(1); sum_j |Z_j|^2 against the float64 cumulative-sum definition. The reference asserts rel < 1e-3;
+    this asserts rel < 1e-5."""
+    import chisq_kernel
+    rng = np.random.default_rng(1)
+    N = 2 ** 21; F = N // 2 + 1; nrow = 4
+    qh = np.zeros((nrow, N), np.complex64)
+    qh[:, :F] = (rng.standard_normal((nrow, F)) + 1j * rng.standard_normal((nrow, F))).astype(np.complex64)
+    rows = np.array([0, 0, 0, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 3])
+    times = rng.integers(0, N, len(rows))
+    nb = 40
+    edges_row = np.sort(rng.choice(np.arange(30 * 1024, F), size=(nrow, nb + 1), replace=False), axis=1)
+    E = edges_row[rows]
+    got, gz = chisq_kernel.bin_power_sums(torch.from_numpy(qh).cuda(), rows, times, E, want_z=True)
+    ref, rz = core.bin_power_sums(qh, rows, times, E, want_z=True)
+    assert np.max(np.abs(got - ref) / ref) < 1e-5
+    assert np.max(np.abs(gz - rz) / np.abs(rz)) < 1e-4
+
+
+@pytest.mark.parametrize('N', [3 * 2 ** 17, 2 ** 18])
+def test_chisq_fused_any_length(gpu, N):
+    """Fused mode (conj(h) d formed in the kernel) with a time length that need not be a power of two, against the
+    float64 direct sums; edges include an empty bin and duplicate peaks."""
+    import chisq_kernel
+    rng = np.random.default_rng(N % 1000)
+    F = N // 2 + 1
+    H = (rng.standard_normal((3, F)) + 1j * rng.standard_normal((3, F))).astype(np.complex64)
+    d = (rng.standard_normal(F) + 1j * rng.standard_normal(F)).astype(np.complex64)
+    rows = np.array([0, 1, 1, 2, 2, 2, 1])
+    times = rng.integers(0, N, len(rows)); times[-1] = times[1]
+    e = np.sort(rng.choice(np.arange(100, F), size=(3, 17), replace=False), axis=1)
+    e[1, 5] = e[1, 4]                                   # an empty bin
+    E = e[rows]
+    pw, z = chisq_kernel.bin_power_sums(torch.from_numpy(H).cuda(), rows, times, E, want_z=True,
+                                        d=torch.from_numpy(d).cuda(), n_time=N)
+    rp, rz = core.bin_power_sums(H, rows, times, E, want_z=True, d=d, n_time=N)
+    assert np.max(np.abs(pw - rp) / rp) < 1e-5
+    assert np.max(np.abs(z - rz) / np.abs(rz)) < 1e-4
+
+
+def test_edges_vs_float64_cumsum(gpu):
+    """Edges kernel against multi_filter.edges_per_template's float64 definition: within 1 bin, at most 1e-3 of edges
+    differing (tests/test_exact_stage.py:224-225)."""
+    import chisq_edges
+    rng = np.random.default_rng(4)
+    F = 65537; kmin = 960
+    f = np.maximum(np.arange(F), 1) / 64.0
+    H = np.zeros((6, F), np.complex64)
+    kmax = np.array([30000, 65536, 12000, 50000, 2000, 40000])
+    for i in range(6):
+        H[i, kmin:kmax[i]] = (f[kmin:kmax[i]] ** (-7 / 6) * np.exp(1j * rng.uniform(0, 6, kmax[i] - kmin)))
+    ips = (1.0 / (1 + (f / 200.0) ** 2) * (np.arange(F) >= kmin)).astype(np.float32)
+    rows = np.array([0, 1, 2, 3, 4, 5, 3])
+    nb = np.array([16, 16, 16, 8, 4, 16, 8])
+    E = chisq_edges.edges_blocked(None, rows, nb, kmin, kmax[rows], torch.from_numpy(H).cuda(),
+                                  torch.from_numpy(ips).cuda())
+    Er = core.edges_per_template(rows, nb, kmin, kmax[rows], H, ips)
+    assert E.shape == Er.shape
+    d = np.abs(E - Er)
+    assert d.max() <= 1 and (d > 0).mean() <= 1e-3
+
+
+def test_templates_vs_oracle(gpu):
+    """GPU stationary-phase chirps against the float64 oracle: float32 rounding of the same formula (<= 2e-6 of the
+    row's peak amplitude); identical end bins."""
+    import templates
+    import synth
+    cb = synth.circular_bank(12, seed=3)
+    eb = synth.eccentric_bank(8, seed=4)
+    m1 = np.r_[cb.m1, eb.m1]; m2 = np.r_[cb.m2, eb.m2]; e0 = np.r_[cb.e0, eb.e0]
+    flen, df = 65537, 1 / 64
+    H, kmax = templates.generate(m1, m2, e0, flen, df, 15.0)
+    Hr, kr = pipeline.spa_bank(m1, m2, e0, flen, df, 15.0)
+    assert np.array_equal(kmax, kr)
+    Hg = H.cpu().numpy()
+    err = np.abs(Hg - Hr).max(1) / np.abs(Hr).max(1)
+    assert err.max() < 2e-6
+
+
+def test_sigmasq_vs_oracle(gpu):
+    import templates
+    rng = np.random.default_rng(6)
+    H = (rng.standard_normal((9, 4097)) + 1j * rng.standard_normal((9, 4097))).astype(np.complex64)
+    IP = rng.random((3, 4097)).astype(np.float32)
+    sig, norm = templates.sigmasq(torch.from_numpy(H).cuda(), torch.from_numpy(IP).cuda(), 0.25)
+    sr, nr = core.sigmasq(H, IP, 0.25)
+    np.testing.assert_allclose(sig.cpu().numpy(), sr, rtol=1e-12)
+    np.testing.assert_allclose(norm.cpu().numpy(), nr, rtol=1e-12)
