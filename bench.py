@@ -53,22 +53,22 @@ def direct_snr_check(datasets, plans, outs, max_templates=24):
                 continue
             tids = tr['template_id'][ex]; samp = tr['sample'][ex]; snr = tr['snr'][ex]; seg = tr['segment'][ex]
             order = np.argsort(-snr)
-            done = []
+            series = {}
             for i in order:
                 t = int(tids[i])
-                if t not in done:
-                    if len(done) >= max_templates:
-                        continue
-                    done.append(t)
-                tb, r = tile_of[t]
                 s = int(seg[i]) - sum(len(x.analyze) for x in datasets[:di])
-                h = tb.H[r].to(torch.complex128)
-                D = d.D[s].to(torch.complex128); ip = d.IP[s].to(torch.float64)
-                sig = 4 * d.df * float((h.abs() ** 2 * ip).sum())
-                q = torch.zeros(d.tlen, dtype=torch.complex128, device='cuda')
-                q[:d.flen] = torch.conj(h) * D
-                z = torch.fft.ifft(q, norm='forward')[int(samp[i])]
-                ref = abs(complex(z)) * 4 * d.df / np.sqrt(sig)
+                if (t, s) not in series:
+                    if len(series) >= max_templates:
+                        continue
+                    tb, r = tile_of[t]
+                    h = tb.H[r].to(torch.complex128)
+                    D = d.D[s].to(torch.complex128); ip = d.IP[s].to(torch.float64)
+                    sig = 4 * d.df * float((h.abs() ** 2 * ip).sum())
+                    q = torch.zeros(d.tlen, dtype=torch.complex128, device='cuda')
+                    q[:d.flen] = torch.conj(h) * D
+                    series[(t, s)] = ((torch.fft.ifft(q, norm='forward').abs() * (4 * d.df / np.sqrt(sig)))
+                                      .cpu().numpy())
+                ref = series[(t, s)][int(samp[i])]
                 errs.append(abs(float(snr[i]) - ref) / ref)
     return (float(max(errs)) if errs else None), len(errs)
 

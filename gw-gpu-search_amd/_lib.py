@@ -27,19 +27,20 @@ _ll = _C.c_longlong
 
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
-    'gwgs_gemm_peaks': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _l, _p],
+    'gwgs_gemm_peaks': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _l, _p, _p],
     'gwgs_gemm_max_k': [_i],
     'gwgs_window_peaks': [_p, _l, _i, _i, _i, _i, _p, _p, _p],
     'gwgs_block_window_max': [_p, _p, _l, _i, _i, _i, _p, _p, _p],
     'gwgs_select_workspace': [_l],
     'gwgs_select_above': [_p, _l, _f, _p, _i, _p, _p, _p],
-    'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _i, _p, _p],
+    'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _i, _p, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_chisq_peaks_per_group': [],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_form_q': [_p, _i, _l, _p, _l, _p, _p],
     'gwgs_pack_zt': [_i, _p, _l, _i, _l, _l, _l, _p, _p, _p],
     'gwgs_stack_a': [_i, _p, _i, _i, _p, _p, _i, _i, _p, _p, _p],
+    'gwgs_stack_a_fp8': [_p, _i, _i, _p, _p, _i, _p, _p, _p, _p],
     'gwgs_basis_power': [_p, _l, _i, _l, _l, _p, _p],
     'gwgs_spa_templates': [_p, _p, _p, _i, _l, _d, _d, _d, _p, _l, _p, _p],
     'gwgs_sigmasq': [_p, _l, _l, _i, _p, _l, _i, _d, _p, _l, _p],

@@ -32,6 +32,8 @@ template <>
 __device__ __forceinline__ float cvt_out<float>(const float x) { return x; }
 template <>
 __device__ __forceinline__ uint16_t cvt_out<uint16_t>(const float x) { return f32_to_bf16(x); }
+template <>
+__device__ __forceinline__ uint8_t cvt_out<uint8_t>(const float x) { return f32_to_fp8(x); }
 
 template <typename T>
 __global__ void __launch_bounds__(NT) pack_zt_k(const float2* __restrict__ Zc, const long ldc, const int rp,
@@ -93,6 +95,43 @@ __global__ void stack_a_k(const float2* __restrict__ A, const int n, const int r
     Ai[(size_t)m * K + 2 * j + 1] = from_f32<T>(re);
 }
 
+// fp8 operands for one template row per block: values As = A[m, j] rs[m] cs[j] (Re/Im stacked as stack_a_k), a row
+// scale ra = 224 / max |As| so the row spans half the e4m3 range, bytes fp8(As ra), and pw_scale[m] = 1 / ra^2,
+// which the GEMM applies to the row's |z|^2.
+__global__ void __launch_bounds__(NT) stack_a_fp8_k(const float2* __restrict__ A, const int n, const int r,
+                                                    const float* __restrict__ rs, const float* __restrict__ cs,
+                                                    const int K, uint8_t* __restrict__ Ar, uint8_t* __restrict__ Ai,
+                                                    float* __restrict__ pw_scale)
+{
+    const int m = blockIdx.x;
+    float mx = 0.f;
+    for (int j = threadIdx.x; j < r && m < n; j += NT) {
+        const float2 a = A[(size_t)m * r + j];
+        const float s = rs[m] * (cs ? cs[j] : 1.f);
+        mx = fmaxf(mx, fmaxf(fabsf(a.x * s), fabsf(a.y * s)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    __shared__ float sh[NT / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    const float ra = mx > 0.f ? 224.f / mx : 1.f;
+    if (threadIdx.x == 0) pw_scale[m] = 1.f / (ra * ra);
+    for (int j = threadIdx.x; j < K / 2; j += NT) {
+        float re = 0.f, im = 0.f;
+        if (m < n && j < r) {
+            const float2 a = A[(size_t)m * r + j];
+            const float s = rs[m] * (cs ? cs[j] : 1.f) * ra;
+            re = a.x * s; im = a.y * s;
+        }
+        Ar[(size_t)m * K + 2 * j] = f32_to_fp8(re);
+        Ar[(size_t)m * K + 2 * j + 1] = f32_to_fp8(-im);
+        Ai[(size_t)m * K + 2 * j] = f32_to_fp8(im);
+        Ai[(size_t)m * K + 2 * j + 1] = f32_to_fp8(re);
+    }
+}
+
 // Per basis vector j: sum over t in [a0, a0 + Tn) of |Zc[j, t]|^2 in double (the rms scale of fp_filter.py:609-610
 // and per-basis scales). One block per basis vector.
 __global__ void __launch_bounds__(NT) basis_power_k(const float2* __restrict__ Zc, const long ldc, const long a0,
@@ -121,7 +160,8 @@ GWGS_API int gwgs_form_q(const void* Vc, int rp, long flen, const void* D, long 
     return gwgs_status();
 }
 
-// dtype 0: float32 Zt, 1: bfloat16 Zt. Zc (rp, ldc) complex64; Zt (Tpad, 2 rp). colscale: per basis vector or null.
+// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (values clamped to +-448). Zc (rp, ldc) complex64; Zt (Tpad, 2 rp).
+// colscale: per basis vector or null.
 GWGS_API int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, long Tn, long Tpad,
                           const float* colscale, void* Zt, void* stream)
 {
@@ -129,7 +169,10 @@ GWGS_API int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, 
                GWGS_EINVAL);
     if (Tpad == 0) return GWGS_OK;
     const dim3 grid((unsigned)((Tpad + 63) / 64), rp / 16);
-    if (dtype == 1)
+    if (dtype == 2)
+        hipLaunchKernelGGL(pack_zt_k<uint8_t>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)Zc, ldc, rp, a0,
+                           Tn, Tpad, colscale, (uint8_t*)Zt);
+    else if (dtype == 1)
         hipLaunchKernelGGL(pack_zt_k<uint16_t>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)Zc, ldc, rp, a0,
                            Tn, Tpad, colscale, (uint16_t*)Zt);
     else if (dtype == 0)
@@ -155,6 +198,17 @@ GWGS_API int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* r
                            colscale, Mpad, K, (float*)Ar, (float*)Ai);
     else
         return GWGS_EUNSUP;
+    return gwgs_status();
+}
+
+// fp8 operands: Ar, Ai (n, K) bytes and pw_scale (n floats); see stack_a_fp8_k.
+GWGS_API int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const float* colscale, int K,
+                              void* Ar, void* Ai, float* pw_scale, void* stream)
+{
+    GWGS_CHECK(A && rowscale && Ar && Ai && pw_scale && n >= 0 && r > 0 && K >= 2 * r && K % 2 == 0, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL(stack_a_fp8_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)A, n, r, rowscale,
+                       colscale, K, (uint8_t*)Ar, (uint8_t*)Ai, pw_scale);
     return gwgs_status();
 }
 

@@ -57,3 +57,15 @@ __device__ __forceinline__ uint16_t f32_to_bf16(const float x)
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
+
+// float -> fp8 e4m3 (OCP, gfx950 v_cvt_pk_fp8_f32: round to nearest even), saturated to +-448 first
+__device__ __forceinline__ uint8_t f32_to_fp8(float x)
+{
+    x = fminf(fmaxf(x, -448.f), 448.f);
+    return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false) & 0xff);
+}
+
+__device__ __forceinline__ float fp8_to_f32(const uint8_t b)
+{
+    return __builtin_amdgcn_cvt_f32_fp8((int)b, 0);
+}

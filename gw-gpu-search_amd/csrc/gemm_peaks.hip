@@ -20,6 +20,15 @@
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ i32x8 join8(const uint4 a, const uint4 b)
+{
+    i32x8 r;
+    r[0] = (int)a.x; r[1] = (int)a.y; r[2] = (int)a.z; r[3] = (int)a.w;
+    r[4] = (int)b.x; r[5] = (int)b.y; r[6] = (int)b.z; r[7] = (int)b.w;
+    return r;
+}
 
 namespace {
 
@@ -242,13 +251,14 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
 template <typename T> struct KloopTraits;
 template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8, PAD = 8, KSTEP = 16; };
 template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4, KSTEP = 2; };
+template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16, KSTEP = 64; };
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(NT, 2)
 gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
                    const long lda, const int M, const int K, const int Tn, const int ntile, const int tpb,
                    const int per, const int nmb, const int nblocks, float* __restrict__ V, int* __restrict__ I,
-                   const long ldv)
+                   const long ldv, const float* __restrict__ rowscale)
 {
     using TR = KloopTraits<T>;
     constexpr int KC = TR::KC, EPV = TR::EPV, PITCH = KC + TR::PAD;
@@ -310,7 +320,24 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
         sstore();
         __syncthreads();
         if (step + 1 < nsteps) gload(step + 1);
-        if constexpr (sizeof(T) == 2) {
+        if constexpr (sizeof(T) == 1) {
+            // v_mfma_scale_f32_32x32x64_f8f6f4 with unit (E8M0 127) block scales: fp8 e4m3 operands, 32 bytes per
+            // lane; lanes 0-31 and 32-63 hold the two 32-byte halves of the 64-wide k step for their row/column
+#pragma unroll
+            for (int ks = 0; ks < KC / 64; ks++) {
+                const T* pa = &as_r[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
+                const T* pi = &as_i[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
+                const i32x8 a_r = join8(*(const uint4*)pa, *(const uint4*)(pa + 16));
+                const i32x8 a_i = join8(*(const uint4*)pi, *(const uint4*)(pi + 16));
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const T* pz = &zs[(ts * 32 + col) * PITCH + ks * 64 + half * 32];
+                    const i32x8 z = join8(*(const uint4*)pz, *(const uint4*)(pz + 16));
+                    cr[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(z, a_r, cr[ts], 0, 0, 0, 127, 0, 127);
+                    ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(z, a_i, ci[ts], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        } else if constexpr (sizeof(T) == 2) {
 #pragma unroll
             for (int ks = 0; ks < KC / 16; ks++) {
                 const bf16x8 a_r = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_r[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
@@ -337,11 +364,12 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
         }
         if (kcix == nkc - 1) {
             scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
+            const float sc = (rowscale && m < M) ? rowscale[m] : 1.f;   // per-template scale of quantized operands
             if (MODE == 0) {
-                emit(best, bi, half, m, M, V, I, ldv, tt);
+                emit(best * sc, bi, half, m, M, V, I, ldv, tt);
                 best = -1.f; bi = (tt + 1) * BT;
             } else if ((tt + 1) % per == 0) {
-                emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
+                emit(best * sc, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
                 best = -1.f; bi = (tt + 1) * BT;
             }
         }
@@ -386,11 +414,12 @@ gemm_fn<float> f32_kernel(int K, int mode)
 } // namespace
 
 // dtype 0 = fp32 operands (plain form), 1 = bf16 operands; fp32 accumulation either way.
-// variant: 0 = pick by K (register-resident template operands up to K = 256 bf16 / 128 fp32, else the K loop),
-// 1 = always the K loop.
+// dtype 0 = fp32, 1 = bf16, 2 = fp8 e4m3 (OCP) operands. variant: 0 = pick by K (register-resident template operands
+// up to K = 256 bf16 / 128 fp32, else the K loop), 1 = always the K loop (fp8 always). rowscale (M floats, may be
+// null; needs the K loop): multiplies each template's reported |z|^2, undoing per-template operand scaling.
 GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                              int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,
-                             void* stream)
+                             const float* rowscale, void* stream)
 {
     GWGS_CHECK(Zt && Ar && Ai && V && I, GWGS_EINVAL);
     GWGS_CHECK(M > 0 && Tn > 0 && K > 0 && (mode == 0 || mode == 1), GWGS_EINVAL);
@@ -411,20 +440,29 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
     GWGS_CHECK(nblocks < (1L << 30), GWGS_ESIZE);
     const int grid = (int)((nblocks + 7) / 8 * 8);
     hipStream_t s = (hipStream_t)stream;
-    const bool kloop = variant == 1 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
+    const bool kloop = variant == 1 || dtype == 2 || rowscale || (dtype == 1 ? K > 256 : K > 128) || K % 32;
     if (kloop) {
         if (dtype == 1) {
             GWGS_CHECK(ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0 && ((uintptr_t)Zt & 15) == 0 &&
                        ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
             auto f = mode ? gemm_peaks_kloop_k<uint16_t, 1> : gemm_peaks_kloop_k<uint16_t, 0>;
             hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
-                               (const uint16_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+                               (const uint16_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
+                               rowscale);
         } else if (dtype == 0) {
             GWGS_CHECK(ldz % 4 == 0 && lda % 4 == 0 && K % 4 == 0 && ((uintptr_t)Zt & 15) == 0 &&
                        ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
             auto f = mode ? gemm_peaks_kloop_k<float, 1> : gemm_peaks_kloop_k<float, 0>;
             hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar,
-                               (const float*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
+                               (const float*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
+                               rowscale);
+        } else if (dtype == 2) {
+            GWGS_CHECK(ldz % 16 == 0 && lda % 16 == 0 && K % 16 == 0 && ((uintptr_t)Zt & 15) == 0 &&
+                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
+            auto f = mode ? gemm_peaks_kloop_k<uint8_t, 1> : gemm_peaks_kloop_k<uint8_t, 0>;
+            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
+                               (const uint8_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
+                               rowscale);
         } else {
             return GWGS_EUNSUP;
         }
@@ -451,5 +489,5 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
 
 GWGS_API int gwgs_gemm_max_k(int dtype)
 {
-    return dtype == 0 || dtype == 1 ? (1 << 20) : 0;
+    return dtype >= 0 && dtype <= 2 ? (1 << 20) : 0;
 }

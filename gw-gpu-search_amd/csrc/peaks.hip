@@ -127,12 +127,15 @@ template <>
 __device__ __forceinline__ float to_f32<float>(const float x) { return x; }
 template <>
 __device__ __forceinline__ float to_f32<uint16_t>(const uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
+template <>
+__device__ __forceinline__ float to_f32<uint8_t>(const uint8_t x) { return fp8_to_f32(x); }
 
 template <typename T>
 __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar,
                                                   const T* __restrict__ Ai, const long lda, const int K,
                                                   const int* __restrict__ rows, const int* __restrict__ tcol,
-                                                  const int n, float2* __restrict__ out)
+                                                  const int n, const float* __restrict__ pw_scale,
+                                                  float2* __restrict__ out)
 {
     const int c = blockIdx.x * (NT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (c >= n) return;
@@ -147,7 +150,8 @@ __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, cons
     }
     sr = wave_sum_f32(sr);
     si = wave_sum_f32(si);
-    if (lane == 0) out[c] = make_float2(sr, si);
+    const float sc = pw_scale ? sqrtf(pw_scale[rows[c]]) : 1.f;
+    if (lane == 0) out[c] = make_float2(sr * sc, si * sc);
 }
 
 } // namespace
@@ -194,17 +198,21 @@ GWGS_API int gwgs_select_above(const float* vw, long total, float cut, int* out,
 }
 
 GWGS_API int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K,
-                            const int* rows, const int* tcol, int n, void* out, void* stream)
+                            const int* rows, const int* tcol, int n, const float* pw_scale, void* out, void* stream)
 {
     GWGS_CHECK(Zt && Ar && Ai && rows && tcol && out && K > 0 && n >= 0, GWGS_EINVAL);
     if (n == 0) return GWGS_OK;
     const dim3 grid((n + NT / 64 - 1) / (NT / 64));
-    if (dtype == 1)
-        hipLaunchKernelGGL(rebuilt_z_k<uint16_t>, grid, dim3(NT), 0, (hipStream_t)stream, (const uint16_t*)Zt, ldz,
-                           (const uint16_t*)Ar, (const uint16_t*)Ai, lda, K, rows, tcol, n, (float2*)out);
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 2)
+        hipLaunchKernelGGL(rebuilt_z_k<uint8_t>, grid, dim3(NT), 0, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
+                           (const uint8_t*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
+    else if (dtype == 1)
+        hipLaunchKernelGGL(rebuilt_z_k<uint16_t>, grid, dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
+                           (const uint16_t*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
     else if (dtype == 0)
-        hipLaunchKernelGGL(rebuilt_z_k<float>, grid, dim3(NT), 0, (hipStream_t)stream, (const float*)Zt, ldz,
-                           (const float*)Ar, (const float*)Ai, lda, K, rows, tcol, n, (float2*)out);
+        hipLaunchKernelGGL(rebuilt_z_k<float>, grid, dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar,
+                           (const float*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
     else
         return GWGS_EUNSUP;
     return gwgs_status();

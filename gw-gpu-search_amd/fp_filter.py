@@ -38,7 +38,8 @@ from exact_stage import ExactStage, PeakRequest, SegmentData, TemplateTile
 
 CH = 1 << 16          # frequency chunk for the Gram and basis products
 BN = fp_gemm.BN
-PRECISIONS = {'fp32': 0, 'bf16': 1}
+PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp8': 2}
+OPERAND_DTYPES = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8}
 
 
 @dataclass
@@ -51,7 +52,7 @@ class Options:
     eps: float = 1e-4                      # worst lost power allowed by the SVD (0.9999 kept)
     margin: float = 0.04
     exact_snr: float = 0.0                 # 0: exact values for every candidate
-    precision: str = 'fp32'                # GEMM operands: fp32 (plain form) or bf16
+    precision: str = 'fp32'                # GEMM operands: fp32 (plain form), bf16, or fp8 e4m3 with scales
     gemm_cfg: tuple | None = None          # (tiles_per_block,), from fp_gemm.pick_cfg
     self_check: bool = True                # the exact stage's first-call check
     timing: bool = False                   # synchronize around stages to fill the timing dict
@@ -267,21 +268,36 @@ class Filter:
         _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, m.D[s].data_ptr(), self.tlen, q.data_ptr(),
                   _dev.stream())
         Zc = torch.fft.ifft(q, dim=1, norm='forward')                  # (rp, tlen): basis series
-        dt = torch.float32 if self.code == 0 else torch.bfloat16
+        dt = OPERAND_DTYPES[self.code]
+        zcs = acs = pws = None
+        if self.code == 2:
+            # per-basis scales for fp8: mean |z_j|^2 over the segment = sum_k |q_jk|^2 (Parseval, unnormalised
+            # inverse); Z is divided by its rms, A multiplied by it, and each template row of A gets its own scale
+            pw = torch.empty(rp, dtype=torch.float64, device='cuda')
+            _lib.call('gwgs_basis_power', q.data_ptr(), self.tlen, rp, 0, self.flen, pw.data_ptr(), _dev.stream())
+            rms = torch.sqrt(pw).to(torch.float32)
+            zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
+            acs = rms.contiguous()
+            pws = torch.empty(n, dtype=torch.float32, device='cuda')
         Zt = torch.empty((Tn, K), dtype=dt, device='cuda')
-        _lib.call('gwgs_pack_zt', self.code, Zc.data_ptr(), self.tlen, rp, a0, Tn, Tn, None, Zt.data_ptr(),
+        _lib.call('gwgs_pack_zt', self.code, Zc.data_ptr(), self.tlen, rp, a0, Tn, Tn, _dev.ptr(zcs), Zt.data_ptr(),
                   _dev.stream())
         zoff = 0
         t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
         Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
         rs = normf[:, gs].contiguous()
-        _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, rs.data_ptr(), None, n, K, Ar.data_ptr(),
-                  Ai.data_ptr(), _dev.stream())
+        if self.code == 2:
+            _lib.call('gwgs_stack_a_fp8', tb.A.data_ptr(), n, tb.r, rs.data_ptr(), acs.data_ptr(), K, Ar.data_ptr(),
+                      Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+        else:
+            _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, rs.data_ptr(), None, n, K, Ar.data_ptr(),
+                      Ai.data_ptr(), _dev.stream())
         vw = torch.empty((n, nw), dtype=torch.float32, device='cuda')
         iw = torch.empty((n, nw), dtype=torch.int32, device='cuda')
         if o.gemm_events:
             e0 = torch.cuda.Event(enable_timing=True); e0.record()
-        fp_gemm.gemm_peaks_raw(self.code, Zt, K, Ar, Ai, K, n, K, Tn, 1, self.cw // BN, o.gemm_cfg, vw, iw, zoff)
+        fp_gemm.gemm_peaks_raw(self.code, Zt, K, Ar, Ai, K, n, K, Tn, 1, self.cw // BN, o.gemm_cfg, vw, iw, zoff,
+                               rowscale=pws)
         if o.gemm_events:
             e1 = torch.cuda.Event(enable_timing=True); e1.record()
             self.events.append((e0, e1, 4.0 * n * (2 * tb.r) * Tn))
@@ -296,7 +312,7 @@ class Filter:
         rows_d = torch.from_numpy(tb_.astype(np.int32)).cuda()
         zreb_d = torch.empty(len(g), dtype=torch.complex64, device='cuda')
         _lib.call('gwgs_rebuilt_z', self.code, Zt.data_ptr() + zoff, K, Ar.data_ptr(), Ai.data_ptr(), K, K,
-                  rows_d.data_ptr(), il_d.data_ptr(), len(g), zreb_d.data_ptr(), _dev.stream())
+                  rows_d.data_ptr(), il_d.data_ptr(), len(g), _dev.ptr(pws), zreb_d.data_ptr(), _dev.stream())
         approx = torch.sqrt(vw.view(-1)[gd]).cpu().numpy()
         samp = a0 + il_d.cpu().numpy().astype(np.int64)
         zreb = zreb_d.cpu().numpy()

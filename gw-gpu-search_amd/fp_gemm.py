@@ -3,7 +3,9 @@
 z_re = Are @ Z, z_im = Aim @ Z; each launch keeps only the max |z|^2 and its sample per BN = 128 samples (mode 0,
 fused_peaks) or per cluster window of cw samples (mode 1, fused_window_peaks, the window max of fp_filter.py:745-746
 done in the epilogue). The kernel is hand-written HIP for gfx950 (csrc/gemm_peaks.hip): fp32 operands on
-v_mfma_f32_32x32x2_f32 (the plain form) or bf16 operands on v_mfma_f32_32x32x16_bf16, fp32 accumulation.
+v_mfma_f32_32x32x2_f32 (the plain form), bf16 operands on v_mfma_f32_32x32x16_bf16, or fp8 e4m3 operands on
+v_mfma_scale_f32_32x32x64_f8f6f4 (unit block scales; per-template factors applied to |z|^2 in the epilogue), fp32
+accumulation.
 
 Layout: the kernel reads the basis series time-major, Zt (T, K) with K contiguous. fused_peaks also accepts the
 reference's (K, Tn) layout and transposes it first (one extra pass); fp_filter hands Zt over directly.
@@ -18,15 +20,16 @@ import _dev
 import _lib
 
 BN = 128
-DTYPES = {torch.float32: 0, torch.bfloat16: 1}
+DTYPES = {torch.float32: 0, torch.bfloat16: 1, torch.float8_e4m3fn: 2}
 
 
 def _kpad(K):
-    return -(-K // 8) * 8
+    return -(-K // 16) * 16
 
 
 def _prep_operands(Are, Aim, Z, Tn, zt):
-    """Pad K to a multiple of 32 and rows of Zt to a multiple of BN; return (Zt, Ar, Ai, K, dtype code)."""
+    """Pad K to a multiple of 16 and rows of Zt to a multiple of BN; return (Zt, Ar, Ai, K, dtype code). float8_e4m3fn
+    operands are passed to the kernel as their bytes."""
     dt = Are.dtype
     if dt == torch.float16:
         dt = torch.bfloat16
@@ -47,7 +50,10 @@ def _prep_operands(Are, Aim, Z, Tn, zt):
     if Kp != K or Are.dtype != dt or not Are.is_contiguous():
         A2 = torch.zeros((Are.shape[0], Kp), dtype=dt, device=Are.device); A2[:, :K] = Are; Are = A2
         A3 = torch.zeros((Aim.shape[0], Kp), dtype=dt, device=Aim.device); A3[:, :K] = Aim; Aim = A3
-    return Zt, Are, Aim.contiguous(), Kp, DTYPES[dt]
+    code = DTYPES[dt]
+    if code == 2:
+        Zt, Are, Aim = Zt.view(torch.uint8), Are.view(torch.uint8), Aim.contiguous().view(torch.uint8)
+    return Zt, Are, Aim.contiguous(), Kp, code
 
 
 def _cfg(cfg):
@@ -60,14 +66,17 @@ def _cfg(cfg):
     return int(cfg), 0
 
 
-def gemm_peaks_raw(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, Tn, mode, per, cfg, V, I, zt_offset_bytes=0):
-    """Direct C-ABI call on pre-laid-out operands (fp_filter's hot loop)."""
+def gemm_peaks_raw(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, Tn, mode, per, cfg, V, I, zt_offset_bytes=0,
+                   rowscale=None):
+    """Direct C-ABI call on pre-laid-out operands (fp_filter's hot loop). dtype_code 0 fp32, 1 bf16, 2 fp8 e4m3
+    (torch.uint8 bytes); rowscale: per-template factor on |z|^2 (fp8 row scales)."""
     tpb, variant = _cfg(cfg)
     _lib.call('gwgs_gemm_peaks', dtype_code, variant, Zt.data_ptr() + zt_offset_bytes, ldz, Ar.data_ptr(),
-              Ai.data_ptr(), lda, M, K, Tn, mode, per, tpb, V.data_ptr(), I.data_ptr(), V.stride(0), _dev.stream())
+              Ai.data_ptr(), lda, M, K, Tn, mode, per, tpb, V.data_ptr(), I.data_ptr(), V.stride(0),
+              _dev.ptr(rowscale), _dev.stream())
 
 
-def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False):
+def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False, rowscale=None):
     """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32 / bfloat16 (float16 is taken as
     bfloat16). Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are
     masked to -1 as in fp_gemm.py:29. cfg: (tiles_per_block,) from pick_cfg, or None."""
@@ -75,11 +84,11 @@ def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False):
     nblk = -(-Tn // BN)
     V = torch.empty((M, nblk), device=Z.device, dtype=torch.float32)
     I = torch.empty((M, nblk), device=Z.device, dtype=torch.int32)
-    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 0, 1, cfg, V, I)
+    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 0, 1, cfg, V, I, rowscale=rowscale)
     return V, I
 
 
-def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False):
+def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False, rowscale=None):
     """As fused_peaks, with the cluster-window max fused: (M, Tn // cw) maxima and their samples, first index on
     ties at block and window level. Tn must be a multiple of cw and cw of BN (fp_filter.py:359)."""
     assert cw % BN == 0, 'cluster window must be a multiple of %d samples (GEMM output block)' % BN
@@ -88,7 +97,8 @@ def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False):
     nw = Tn // cw
     V = torch.empty((M, nw), device=Z.device, dtype=torch.float32)
     I = torch.empty((M, nw), device=Z.device, dtype=torch.int32)
-    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 1, cw // BN, cfg, V, I)
+    gemm_peaks_raw(code, Zt, Zt.stride(0), Ar, Ai, Ar.stride(0), M, K, Tn, 1, cw // BN, cfg, V, I,
+                   rowscale=rowscale)
     return V, I
 
 

@@ -1,0 +1,45 @@
+"""Overlap-save chunks of a conditioned segment (the reference's window_filter.py:1-47, imported by fp_filter for
+--chunk / --chunk2): chunk starts whose valid ranges [s + hist, s + Lc - fut) tile the analysed span, the range each
+chunk alone reports, and the over-whitened chunks Dc = dt rfft(d[s : s + Lc]) with 1/psd sampled at the chunk
+resolution. A template supported on [-hist, fut] gives, on a chunk's valid samples, the same SNR as on the whole
+segment. Index arithmetic is host integer code; the transforms are hipFFT (torch.fft) on the device."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def chunk_starts(tlen, Lc, hist, fut, lo=None, hi=None):
+    """Chunk starts whose valid ranges tile [lo, hi) (default: every sample with hist of history and fut of future);
+    the last chunk is moved back to end at the segment end (window_filter.py:16-26)."""
+    lo = hist if lo is None else max(lo, hist)
+    hi = tlen - fut if hi is None else min(hi, tlen - fut)
+    V = Lc - hist - fut
+    assert V > 0 and hi > lo
+    s = np.arange(lo - hist, hi - hist, V)
+    s[-1] = min(s[-1], tlen - Lc)
+    return s
+
+
+def valid(starts, Lc, hist, fut, hi=None):
+    """Per chunk the half-open range [a, b) it alone reports: no overlaps, nothing at or after hi
+    (window_filter.py:29-36)."""
+    a = starts + hist
+    b = np.append(a[1:], starts[-1] + Lc - fut)
+    if hi is not None:
+        b = np.minimum(b, hi)
+    return a, np.maximum(b, a)
+
+
+def chunk_segment(D, IP, tlen, dt, starts, Lc, xp=None):
+    """(nch, Lc//2 + 1) over-whitened chunks Dc = dt rfft(d[s : s + Lc]) (d = irfft(D) / dt) and IP sampled every
+    tlen / Lc bins (window_filter.py:39-47). D, IP: torch tensors on the GPU; xp is accepted for the reference's
+    signature."""
+    step = tlen // Lc
+    assert step * Lc == tlen, 'chunk length must divide the segment length'
+    d = torch.fft.irfft(D.to(torch.complex64), n=tlen) / dt
+    st = torch.as_tensor(np.asarray(starts), device=d.device)
+    idx = st[:, None] + torch.arange(Lc, device=d.device)[None, :]
+    Dc = (dt * torch.fft.rfft(d[idx], dim=1)).to(torch.complex64)
+    IPc = IP[::step][:Lc // 2 + 1].expand(Dc.shape[0], Lc // 2 + 1)
+    return Dc, IPc

@@ -40,7 +40,7 @@ def _direct_snr(bank, ds, tids, samps):
     return out
 
 
-@pytest.mark.parametrize('precision', ['fp32', 'bf16'])
+@pytest.mark.parametrize('precision', ['fp32', 'bf16', 'fp8'])
 def test_c1_triggers_vs_oracle(c1_data, precision):
     """Same above-threshold (template, window) list as the float64 oracle, the same sample in each window, and exact
     SNR within 1e-4 of the float64 direct filter (exact_stage.py:35 snr_rel), chi-squared within 1e-3."""

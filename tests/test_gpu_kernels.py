@@ -190,3 +190,66 @@ def test_sigmasq_vs_oracle(gpu):
     sr, nr = core.sigmasq(H, IP, 0.25)
     np.testing.assert_allclose(sig.cpu().numpy(), sr, rtol=1e-12)
     np.testing.assert_allclose(norm.cpu().numpy(), nr, rtol=1e-12)
+
+
+def _fp8(x):
+    """Host e4m3 (OCP) rounding of float32 values already within +-448, as torch rounds (nearest even)."""
+    return torch.from_numpy(np.asarray(x, np.float32)).to(torch.float8_e4m3fn)
+
+
+@pytest.mark.parametrize('M,K,Tn,mode', [(77, 128, 1024, 0), (130, 784, 2048, 1), (5, 48, 300, 0)])
+def test_fp8_gemm_vs_oracle(gpu, M, K, Tn, mode):
+    """fp8 e4m3 operands (bytes rounded on the host) against the float64 GEMM of the same dequantized values, with a
+    per-template row scale applied to |z|^2. Products of e4m3 values are exact in fp32: relative 1e-4."""
+    import fp_gemm
+    rng = np.random.default_rng(K)
+    Ar, Ai = (rng.standard_normal((M, K)) * 30).astype(np.float32), (rng.standard_normal((M, K)) * 30).astype(np.float32)
+    Z = (rng.standard_normal((K, Tn)) * 2).astype(np.float32)
+    qa, qi, qz = _fp8(Ar), _fp8(Ai), _fp8(Z)
+    da, di, dz = (q.to(torch.float32).numpy().astype(np.float64) for q in (qa, qi, qz))
+    rs = (rng.random(M) + 0.5).astype(np.float32)
+    if mode == 0:
+        V, I = fp_gemm.fused_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, (16, 1), rowscale=torch.from_numpy(rs).cuda())
+        Vr, Ir = core.gemm_absmax(da, di, dz, Tn)
+    else:
+        V, I = fp_gemm.fused_window_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, 2048, (16, 1),
+                                          rowscale=torch.from_numpy(rs).cuda())
+        V0, I0 = core.gemm_absmax(da, di, dz, Tn)
+        Vr, Ir = core.window_max(V0, I0, Tn // 2048, 16)
+    np.testing.assert_allclose(V.cpu().numpy(), Vr * rs[:, None], rtol=1e-4)
+    pw = (da @ dz) ** 2 + (di @ dz) ** 2
+    got = np.take_along_axis(pw, I.cpu().numpy().astype(np.int64), 1)
+    assert np.all(got >= Vr * (1 - 1e-4))
+
+
+def test_fp8_operand_conversion(gpu):
+    """Device e4m3 rounding (pack_zt, stack_a_fp8) equals torch's host rounding, with saturation at +-448 and the
+    row scale 224 / max |row|."""
+    import _dev, _lib
+    rng = np.random.default_rng(11)
+    rp, tlen, a0, Tn = 16, 4096, 100, 1000
+    Zc = (rng.standard_normal((rp, tlen)) + 1j * rng.standard_normal((rp, tlen))).astype(np.complex64) * 50
+    Zc[3, a0 + 7] = 900 + 1j                           # saturates
+    cs = (rng.random(rp) + 0.5).astype(np.float32)
+    Zt = torch.empty((Tn, 2 * rp), dtype=torch.uint8, device='cuda')
+    csd = torch.from_numpy(cs).cuda()
+    _lib.call('gwgs_pack_zt', 2, torch.from_numpy(Zc).cuda().data_ptr(), tlen, rp, a0, Tn, Tn, csd.data_ptr(),
+              Zt.data_ptr(), _dev.stream())
+    ref = Zc[:, a0:a0 + Tn] * cs[:, None]
+    ref2 = np.empty((Tn, 2 * rp), np.float32); ref2[:, 0::2] = ref.real.T; ref2[:, 1::2] = ref.imag.T
+    exp = _fp8(np.clip(ref2, -448, 448)).view(torch.uint8).numpy()
+    assert np.array_equal(Zt.cpu().numpy(), exp)
+    n, r, K = 9, 13, 32
+    A = (rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r))).astype(np.complex64)
+    rs = (rng.random(n) + 0.1).astype(np.float32)
+    Ar = torch.empty((n, K), dtype=torch.uint8, device='cuda'); Ai = torch.empty_like(Ar)
+    pws = torch.empty(n, dtype=torch.float32, device='cuda')
+    _lib.call('gwgs_stack_a_fp8', torch.from_numpy(A).cuda().data_ptr(), n, r, torch.from_numpy(rs).cuda().data_ptr(),
+              None, K, Ar.data_ptr(), Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+    As = A * rs[:, None]
+    ra = 224.0 / np.maximum(np.abs(As.real), np.abs(As.imag)).max(1)
+    np.testing.assert_allclose(pws.cpu().numpy(), 1 / ra ** 2, rtol=1e-6)
+    Xr = np.zeros((n, K), np.float32); Xr[:, 0:2 * r:2] = As.real * ra[:, None]; Xr[:, 1:2 * r:2] = -As.imag * ra[:, None]
+    got = Ar.cpu().view(torch.float8_e4m3fn).to(torch.float32).numpy()
+    exp = _fp8(Xr).to(torch.float32).numpy()
+    assert np.abs(got - exp).max() <= 1e-6 * 448
