@@ -29,11 +29,13 @@ _ll = _C.c_longlong
 SIGNATURES = {
     'gwgs_gemm_peaks': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _l, _p, _p],
     'gwgs_gemm_max_k': [_i],
+    'gwgs_gemm_peaks_multi': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _p, _i, _i, _i, _p, _p, _l, _p, _p],
+    'gwgs_gemm_item_bytes': [],
     'gwgs_window_peaks': [_p, _l, _i, _i, _i, _i, _p, _p, _p],
     'gwgs_block_window_max': [_p, _p, _l, _i, _i, _i, _p, _p, _p],
     'gwgs_select_workspace': [_l],
     'gwgs_select_above': [_p, _l, _f, _p, _i, _p, _p, _p],
-    'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _i, _p, _p, _p],
+    'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_chisq_peaks_per_group': [],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],

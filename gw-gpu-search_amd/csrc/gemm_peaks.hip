@@ -9,18 +9,42 @@
 // increasing t (strict >, so the first index wins), and the two half-waves that share a column are merged with one
 // lane swap. Nothing of the M x Tn SNR array is written.
 //
-// Block = 4 waves = 128 templates; each wave keeps its 32 templates' A operands for every k-step in VGPRs for the
-// whole kernel, and the block walks a run of 128-sample time tiles whose Zt tiles are staged through LDS (double
-// buffered for bf16). Output modes:
-//   mode 0: V, I (M, ceil(Tn/128)): per 128-sample block the max |z|^2 and its sample (columns >= Tn masked to -1),
+// Work decomposition. A launch covers one or more segments whose Zt rows lie in one buffer. It is cut into items:
+// a run of 128-sample tiles of one segment. Workgroup = 4 waves = 128 templates x one item; the grid is
+// (template blocks) x (items), ordered so that the template blocks of one item are dispatched to one XCD and share
+// its L2 copy of the Zt tiles. Output modes:
+//   mode 0: V, I per 128-sample block: the max |z|^2 and its sample (samples >= the segment's Tn masked to -1),
 //           exactly fused_peaks' contract;
 //   mode 1: per cluster window of `per` blocks the max and its sample (fp_filter.py:745-746 fused in), first
 //           index on ties at both levels.
+// Samples are relative to the segment's first Zt row; segment s writes columns col0(s) + its block / window index.
+// An optional row scale rowscale[seg M + m] multiplies the reported |z|^2 (template norms per segment, fp8 scales).
+//
+// Kernels: register-resident template operands for K <= 256 (bf16) / 128 (fp32), Zt tiles double buffered through
+// LDS; and a K-loop kernel (both operands staged through LDS in K chunks) for any K and for fp8 e4m3 operands.
 #include "common.h"
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+// One item of a launch: tiles [t_first, t_end) of one segment (tile = 128 samples).
+struct GemmItem {
+    long long zrow0;   // the segment's first row in Zt
+    int t_first;
+    int t_end;
+    int col0;          // the segment's first output column
+    int seg;           // row-scale set
+    int Tn;            // the segment's samples (mode 0 masks t >= Tn)
+    int pad;
+};
+
+namespace {
+
+constexpr int BT = 128;          // time samples per tile (the reference's BN)
+constexpr int WAVES = 4;
+constexpr int NT = 64 * WAVES;
+constexpr int BMT = 32 * WAVES;  // templates per block
 
 __device__ __forceinline__ i32x8 join8(const uint4 a, const uint4 b)
 {
@@ -30,23 +54,31 @@ __device__ __forceinline__ i32x8 join8(const uint4 a, const uint4 b)
     return r;
 }
 
-namespace {
+struct Job {
+    const GemmItem* items;   // null: one segment at row 0, items derived from tpb, ntile, Tn
+    int nitems, nmb, tpb, ntile, Tn, M, per;
+    const float* rowscale;
+};
 
-constexpr int BT = 128;          // time samples per tile (the reference's BN)
-constexpr int WAVES = 4;
-constexpr int NT = 64 * WAVES;
-constexpr int BMT = 32 * WAVES;  // templates per block
-
-__device__ __forceinline__ int block_logical(const int nblocks)
+// logical block -> (template block, item); false for the grid's padding blocks
+__device__ __forceinline__ bool block_item(const Job& J, int& mb, GemmItem& it)
 {
-    // the hardware places workgroup b on XCD b % 8: make consecutive logical blocks (the template blocks of one time
-    // run) land on one XCD so they share its L2 copy of the Zt tiles
+    // the hardware places workgroup b on XCD b % 8: consecutive logical blocks (the template blocks of one item)
+    // land on one XCD
     const int b = blockIdx.x, n8 = gridDim.x >> 3;
     const int L = (b & 7) * n8 + (b >> 3);
-    return L < nblocks ? L : -1;
+    if (L >= J.nitems * J.nmb) return false;
+    mb = L % J.nmb;
+    const int ix = L / J.nmb;
+    if (J.items) {
+        it = J.items[ix];
+    } else {
+        it.zrow0 = 0; it.t_first = ix * J.tpb; it.t_end = min(it.t_first + J.tpb, J.ntile);
+        it.col0 = 0; it.seg = 0; it.Tn = J.Tn; it.pad = 0;
+    }
+    return true;
 }
 
-template <int MODE>
 __device__ __forceinline__ void scan_tile(const f32x16 (&cr)[4], const f32x16 (&ci)[4], const int tb0, const int half,
                                           const int Tn, float& best, int& bi)
 {
@@ -62,23 +94,36 @@ __device__ __forceinline__ void scan_tile(const f32x16 (&cr)[4], const f32x16 (&
     }
 }
 
-__device__ __forceinline__ void emit(float best, int bi, const int half, const int m, const int M, float* __restrict__ V,
-                                     int* __restrict__ I, const long ldv, const int col_out)
+__device__ __forceinline__ void emit(float best, int bi, const int half, const int m, const int M, const float sc,
+                                     float* __restrict__ V, int* __restrict__ I, const long ldv, const int col_out)
 {
     const float ob = __shfl_xor(best, 32, 64);
     const int oi = __shfl_xor(bi, 32, 64);
     take_max_first(best, bi, ob, oi);
     if (half == 0 && m < M) {
-        V[(size_t)m * ldv + col_out] = best;
+        V[(size_t)m * ldv + col_out] = best * sc;
         I[(size_t)m * ldv + col_out] = bi;
     }
+}
+
+// after tile tt (segment-relative) of item `it`: emit per block (mode 0) or at each window end (mode 1)
+template <int MODE>
+__device__ __forceinline__ void tile_done(const Job& J, const GemmItem& it, const int tt, const int half, const int m,
+                                          float& best, int& bi, float* __restrict__ V, int* __restrict__ I,
+                                          const long ldv)
+{
+    const bool out = MODE == 0 || (tt + 1) % J.per == 0;
+    if (!out) return;
+    const float sc = (J.rowscale && m < J.M) ? J.rowscale[(size_t)it.seg * J.M + m] : 1.f;
+    emit(best, bi, half, m, J.M, sc, V, I, ldv, it.col0 + (MODE == 0 ? tt : (tt + 1) / J.per - 1));
+    best = -1.f;
+    bi = (tt + 1) * BT;
 }
 
 template <int KP, int MODE>
 __global__ void __launch_bounds__(NT, 2)
 gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_t* __restrict__ Ar,
-                  const uint16_t* __restrict__ Ai, const long lda, const int M, const int Tn, const int ntile,
-                  const int tpb, const int per, const int nmb, const int nblocks, float* __restrict__ V,
+                  const uint16_t* __restrict__ Ai, const long lda, const Job J, float* __restrict__ V,
                   int* __restrict__ I, const long ldv)
 {
     constexpr int KS = KP / 16;          // MFMA k-steps (v_mfma_f32_32x32x16_bf16)
@@ -88,13 +133,13 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
     static_assert(KP % 32 == 0 && (BT * CPR) % NT == 0, "K must be a multiple of 32");
     __shared__ __attribute__((aligned(16))) uint16_t lds[2][BT * LDP];
 
-    const int L = block_logical(nblocks);
-    if (L < 0) return;
-    const int mb = L % nmb, tg = L / nmb;
-    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, half = lane >> 5;
     const int m = mb * BMT + wave * 32 + col;
+    const int M = J.M;
 
     uint4 ar[KS], ai[KS];
 #pragma unroll
@@ -110,7 +155,7 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
 
     uint4 zr[CPT];
     auto gload = [&](const int tt) {
-        const uint16_t* src = Zt + (size_t)tt * BT * ldz;
+        const uint16_t* src = Zt + ((size_t)it.zrow0 + (size_t)tt * BT) * ldz;
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
             const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
@@ -126,13 +171,13 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
     };
 
     float best = -1.f;
-    int bi = t_first * BT;
-    gload(t_first);
+    int bi = it.t_first * BT;
+    gload(it.t_first);
     sstore(0);
     __syncthreads();
-    for (int tt = t_first; tt < t_end; tt++) {
-        const int buf = (tt - t_first) & 1;
-        if (tt + 1 < t_end) gload(tt + 1);
+    for (int tt = it.t_first; tt < it.t_end; tt++) {
+        const int buf = (tt - it.t_first) & 1;
+        if (tt + 1 < it.t_end) gload(tt + 1);
         f32x16 cr[4], ci[4];
 #pragma unroll
         for (int ts = 0; ts < 4; ts++) {
@@ -151,15 +196,9 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
                 ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
             }
         }
-        scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
-        if (MODE == 0) {
-            emit(best, bi, half, m, M, V, I, ldv, tt);
-            best = -1.f; bi = (tt + 1) * BT;
-        } else if ((tt + 1) % per == 0) {
-            emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
-            best = -1.f; bi = (tt + 1) * BT;
-        }
-        if (tt + 1 < t_end) sstore(buf ^ 1);
+        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+        tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+        if (tt + 1 < it.t_end) sstore(buf ^ 1);
         __syncthreads();
     }
 }
@@ -167,8 +206,7 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
 template <int KP, int MODE>
 __global__ void __launch_bounds__(NT, KP <= 32 ? 2 : 1)
 gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __restrict__ Ar,
-                 const float* __restrict__ Ai, const long lda, const int M, const int Tn, const int ntile,
-                 const int tpb, const int per, const int nmb, const int nblocks, float* __restrict__ V,
+                 const float* __restrict__ Ai, const long lda, const Job J, float* __restrict__ V,
                  int* __restrict__ I, const long ldv)
 {
     constexpr int KS = KP / 2;           // MFMA k-steps (v_mfma_f32_32x32x2_f32)
@@ -178,13 +216,13 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
     static_assert(KP % 32 == 0 && (BT * CPR) % NT == 0, "K must be a multiple of 32");
     __shared__ __attribute__((aligned(16))) float lds[BT * LDP];
 
-    const int L = block_logical(nblocks);
-    if (L < 0) return;
-    const int mb = L % nmb, tg = L / nmb;
-    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, half = lane >> 5;
     const int m = mb * BMT + wave * 32 + col;
+    const int M = J.M;
 
     float ar[KS], ai[KS];
 #pragma unroll
@@ -194,7 +232,7 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
     }
     float4 zr[CPT];
     auto gload = [&](const int tt) {
-        const float* src = Zt + (size_t)tt * BT * ldz;
+        const float* src = Zt + ((size_t)it.zrow0 + (size_t)tt * BT) * ldz;
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
             const int c = tid + j * NT, row = c / CPR, cc = c % CPR;
@@ -211,13 +249,13 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
     };
 
     float best = -1.f;
-    int bi = t_first * BT;
-    gload(t_first);
-    for (int tt = t_first; tt < t_end; tt++) {
+    int bi = it.t_first * BT;
+    gload(it.t_first);
+    for (int tt = it.t_first; tt < it.t_end; tt++) {
         __syncthreads();                 // every wave is done reading the previous tile
         sstore();
         __syncthreads();
-        if (tt + 1 < t_end) gload(tt + 1);
+        if (tt + 1 < it.t_end) gload(tt + 1);
         f32x16 cr[4], ci[4];
 #pragma unroll
         for (int ts = 0; ts < 4; ts++) {
@@ -233,14 +271,8 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
                 ci[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, ai[ks], ci[ts], 0, 0, 0);
             }
         }
-        scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
-        if (MODE == 0) {
-            emit(best, bi, half, m, M, V, I, ldv, tt);
-            best = -1.f; bi = (tt + 1) * BT;
-        } else if ((tt + 1) % per == 0) {
-            emit(best, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
-            best = -1.f; bi = (tt + 1) * BT;
-        }
+        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+        tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
     }
 }
 
@@ -249,16 +281,15 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
 // issued before the current chunk's MFMAs. The loop runs over (time tile, k chunk) pairs, so the prefetch crosses
 // time tiles, and the epilogue after each tile's last chunk is the same in-register scan as above.
 template <typename T> struct KloopTraits;
-template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8, PAD = 8, KSTEP = 16; };
-template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4, KSTEP = 2; };
-template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16, KSTEP = 64; };
+template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8, PAD = 8; };
+template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4; };
+template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16; };
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(NT, 2)
 gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
-                   const long lda, const int M, const int K, const int Tn, const int ntile, const int tpb,
-                   const int per, const int nmb, const int nblocks, float* __restrict__ V, int* __restrict__ I,
-                   const long ldv, const float* __restrict__ rowscale)
+                   const long lda, const int K, const Job J, float* __restrict__ V, int* __restrict__ I,
+                   const long ldv)
 {
     using TR = KloopTraits<T>;
     constexpr int KC = TR::KC, EPV = TR::EPV, PITCH = KC + TR::PAD;
@@ -269,25 +300,26 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
     __shared__ __attribute__((aligned(16))) T as_r[BMT * PITCH];
     __shared__ __attribute__((aligned(16))) T as_i[BMT * PITCH];
 
-    const int L = block_logical(nblocks);
-    if (L < 0) return;
-    const int mb = L % nmb, tg = L / nmb;
-    const int t_first = tg * tpb, t_end = min(t_first + tpb, ntile);
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, half = lane >> 5;
     const int m0 = mb * BMT;
     const int m = m0 + wave * 32 + col;
+    const int M = J.M;
     const int nkc = (K + KC - 1) / KC;
-    const int nsteps = (t_end - t_first) * nkc;
+    const int nsteps = (it.t_end - it.t_first) * nkc;
 
     uint4 rz[CPT], rr[CPT], ri[CPT];
     auto gload = [&](const int step) {
-        const int tt = t_first + step / nkc, kc = (step % nkc) * KC;
+        const int tt = it.t_first + step / nkc, kc = (step % nkc) * KC;
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
             const int c = tid + j * NT, row = c / CPR, cc = kc + (c % CPR) * EPV;
             const bool kin = cc < K;
-            rz[j] = kin ? *(const uint4*)(Zt + ((size_t)tt * BT + row) * ldz + cc) : make_uint4(0, 0, 0, 0);
+            rz[j] = kin ? *(const uint4*)(Zt + ((size_t)it.zrow0 + (size_t)tt * BT + row) * ldz + cc)
+                        : make_uint4(0, 0, 0, 0);
             const bool ain = kin && (m0 + row) < M;
             rr[j] = ain ? *(const uint4*)(Ar + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
             ri[j] = ain ? *(const uint4*)(Ai + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
@@ -305,10 +337,10 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
 
     f32x16 cr[4], ci[4];
     float best = -1.f;
-    int bi = t_first * BT;
+    int bi = it.t_first * BT;
     gload(0);
     for (int step = 0; step < nsteps; step++) {
-        const int kcix = step % nkc, tt = t_first + step / nkc;
+        const int kcix = step % nkc, tt = it.t_first + step / nkc;
         if (kcix == 0) {
 #pragma unroll
             for (int ts = 0; ts < 4; ts++) {
@@ -363,29 +395,21 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
             }
         }
         if (kcix == nkc - 1) {
-            scan_tile<MODE>(cr, ci, tt * BT, half, Tn, best, bi);
-            const float sc = (rowscale && m < M) ? rowscale[m] : 1.f;   // per-template scale of quantized operands
-            if (MODE == 0) {
-                emit(best * sc, bi, half, m, M, V, I, ldv, tt);
-                best = -1.f; bi = (tt + 1) * BT;
-            } else if ((tt + 1) % per == 0) {
-                emit(best * sc, bi, half, m, M, V, I, ldv, (tt + 1) / per - 1);
-                best = -1.f; bi = (tt + 1) * BT;
-            }
+            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
         }
     }
 }
 
 template <typename T>
-using gemm_fn = void (*)(const T*, long, const T*, const T*, long, int, int, int, int, int, int, int, float*, int*,
-                         long);
+using resident_fn = void (*)(const T*, long, const T*, const T*, long, const Job, float*, int*, long);
 
 template <int KP>
-gemm_fn<uint16_t> pick_bf16(int mode) { return mode ? gemm_peaks_bf16_k<KP, 1> : gemm_peaks_bf16_k<KP, 0>; }
+resident_fn<uint16_t> pick_bf16(int mode) { return mode ? gemm_peaks_bf16_k<KP, 1> : gemm_peaks_bf16_k<KP, 0>; }
 template <int KP>
-gemm_fn<float> pick_f32(int mode) { return mode ? gemm_peaks_f32_k<KP, 1> : gemm_peaks_f32_k<KP, 0>; }
+resident_fn<float> pick_f32(int mode) { return mode ? gemm_peaks_f32_k<KP, 1> : gemm_peaks_f32_k<KP, 0>; }
 
-gemm_fn<uint16_t> bf16_kernel(int K, int mode)
+resident_fn<uint16_t> bf16_kernel(int K, int mode)
 {
     switch (K) {
     case 32: return pick_bf16<32>(mode);
@@ -400,7 +424,7 @@ gemm_fn<uint16_t> bf16_kernel(int K, int mode)
     }
 }
 
-gemm_fn<float> f32_kernel(int K, int mode)
+resident_fn<float> f32_kernel(int K, int mode)
 {
     switch (K) {
     case 32: return pick_f32<32>(mode);
@@ -411,12 +435,47 @@ gemm_fn<float> f32_kernel(int K, int mode)
     }
 }
 
+int launch(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K, int mode,
+           const Job& J, float* V, int* I, long ldv, hipStream_t s)
+{
+    const long nblocks = (long)J.nitems * J.nmb;
+    GWGS_CHECK(nblocks < (1L << 30), GWGS_ESIZE);
+    if (nblocks == 0) return GWGS_OK;
+    const dim3 grid((unsigned)((nblocks + 7) / 8 * 8)), block(NT);
+    const bool aligned = ((uintptr_t)Zt & 15) == 0 && ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0;
+    const bool kloop = variant == 1 || dtype == 2 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
+    if (dtype == 1) {
+        GWGS_CHECK(aligned && ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0, GWGS_EINVAL);
+        if (kloop)
+            hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<uint16_t, 1> : gemm_peaks_kloop_k<uint16_t, 0>), grid, block,
+                               0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar, (const uint16_t*)Ai, lda, K, J, V,
+                               I, ldv);
+        else
+            hipLaunchKernelGGL(bf16_kernel(K, mode), grid, block, 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
+                               (const uint16_t*)Ai, lda, J, V, I, ldv);
+    } else if (dtype == 0) {
+        GWGS_CHECK(aligned && ldz % 4 == 0 && lda % 4 == 0 && K % 4 == 0, GWGS_EINVAL);
+        if (kloop)
+            hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<float, 1> : gemm_peaks_kloop_k<float, 0>), grid, block, 0, s,
+                               (const float*)Zt, ldz, (const float*)Ar, (const float*)Ai, lda, K, J, V, I, ldv);
+        else
+            hipLaunchKernelGGL(f32_kernel(K, mode), grid, block, 0, s, (const float*)Zt, ldz, (const float*)Ar,
+                               (const float*)Ai, lda, J, V, I, ldv);
+    } else if (dtype == 2) {
+        GWGS_CHECK(aligned && ldz % 16 == 0 && lda % 16 == 0 && K % 16 == 0, GWGS_EINVAL);
+        hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<uint8_t, 1> : gemm_peaks_kloop_k<uint8_t, 0>), grid, block, 0, s,
+                           (const uint8_t*)Zt, ldz, (const uint8_t*)Ar, (const uint8_t*)Ai, lda, K, J, V, I, ldv);
+    } else {
+        return GWGS_EUNSUP;
+    }
+    return gwgs_status();
+}
+
 } // namespace
 
-// dtype 0 = fp32 operands (plain form), 1 = bf16 operands; fp32 accumulation either way.
-// dtype 0 = fp32, 1 = bf16, 2 = fp8 e4m3 (OCP) operands. variant: 0 = pick by K (register-resident template operands
-// up to K = 256 bf16 / 128 fp32, else the K loop), 1 = always the K loop (fp8 always). rowscale (M floats, may be
-// null; needs the K loop): multiplies each template's reported |z|^2, undoing per-template operand scaling.
+// One segment. dtype 0 = fp32, 1 = bf16, 2 = fp8 e4m3 (OCP) operands. variant: 0 = pick by K (register-resident
+// template operands up to K = 256 bf16 / 128 fp32, else the K loop), 1 = always the K loop (fp8 always).
+// rowscale (M floats, may be null) multiplies each template's reported |z|^2.
 GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                              int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,
                              const float* rowscale, void* stream)
@@ -434,58 +493,29 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
     }
     int tpb = tiles_per_block > 0 ? tiles_per_block : (mode == 1 ? per : 16);
     if (mode == 1 && tpb % per) tpb = (tpb / per + 1) * per;
-    const int nmb = (M + BMT - 1) / BMT;
-    const int ntg = (ntile + tpb - 1) / tpb;
-    const long nblocks = (long)nmb * ntg;
-    GWGS_CHECK(nblocks < (1L << 30), GWGS_ESIZE);
-    const int grid = (int)((nblocks + 7) / 8 * 8);
-    hipStream_t s = (hipStream_t)stream;
-    const bool kloop = variant == 1 || dtype == 2 || rowscale || (dtype == 1 ? K > 256 : K > 128) || K % 32;
-    if (kloop) {
-        if (dtype == 1) {
-            GWGS_CHECK(ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0 && ((uintptr_t)Zt & 15) == 0 &&
-                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
-            auto f = mode ? gemm_peaks_kloop_k<uint16_t, 1> : gemm_peaks_kloop_k<uint16_t, 0>;
-            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
-                               (const uint16_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
-                               rowscale);
-        } else if (dtype == 0) {
-            GWGS_CHECK(ldz % 4 == 0 && lda % 4 == 0 && K % 4 == 0 && ((uintptr_t)Zt & 15) == 0 &&
-                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
-            auto f = mode ? gemm_peaks_kloop_k<float, 1> : gemm_peaks_kloop_k<float, 0>;
-            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar,
-                               (const float*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
-                               rowscale);
-        } else if (dtype == 2) {
-            GWGS_CHECK(ldz % 16 == 0 && lda % 16 == 0 && K % 16 == 0 && ((uintptr_t)Zt & 15) == 0 &&
-                       ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
-            auto f = mode ? gemm_peaks_kloop_k<uint8_t, 1> : gemm_peaks_kloop_k<uint8_t, 0>;
-            hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
-                               (const uint8_t*)Ai, lda, M, K, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv,
-                               rowscale);
-        } else {
-            return GWGS_EUNSUP;
-        }
-        return gwgs_status();
-    }
-    if (dtype == 1) {
-        GWGS_CHECK(ldz % 8 == 0 && lda % 8 == 0 && ((uintptr_t)Zt & 15) == 0 && ((uintptr_t)Ar & 15) == 0 &&
-                   ((uintptr_t)Ai & 15) == 0, GWGS_EINVAL);
-        gemm_fn<uint16_t> f = bf16_kernel(K, mode);
-        GWGS_CHECK(f, GWGS_ESIZE);
-        hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
-                           (const uint16_t*)Ai, lda, M, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
-    } else if (dtype == 0) {
-        GWGS_CHECK(ldz % 4 == 0 && ((uintptr_t)Zt & 15) == 0, GWGS_EINVAL);
-        gemm_fn<float> f = f32_kernel(K, mode);
-        GWGS_CHECK(f, GWGS_ESIZE);
-        hipLaunchKernelGGL(f, dim3(grid), dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar, (const float*)Ai,
-                           lda, M, Tn, ntile, tpb, per, nmb, (int)nblocks, V, I, ldv);
-    } else {
-        return GWGS_EUNSUP;
-    }
-    return gwgs_status();
+    Job J;
+    J.items = nullptr; J.nmb = (M + BMT - 1) / BMT; J.tpb = tpb; J.ntile = ntile; J.Tn = Tn; J.M = M; J.per = per;
+    J.nitems = (ntile + tpb - 1) / tpb; J.rowscale = rowscale;
+    return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, mode, J, V, I, ldv, (hipStream_t)stream);
 }
+
+// Several segments in one launch. items: device array of nitems GemmItem (32 bytes each: zrow0 int64, t_first,
+// t_end, col0, seg, Tn, pad int32), with t_first a multiple of `per` in mode 1. rowscale: (nseg, M) floats or null.
+GWGS_API int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai,
+                                   long lda, int M, int K, const void* items, int nitems, int mode, int per, float* V,
+                                   int* I, long ldv, const float* rowscale, void* stream)
+{
+    GWGS_CHECK(Zt && Ar && Ai && V && I && items && nitems >= 0, GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && K > 0 && (mode == 0 || mode == 1) && ldz >= K && lda >= K && ldv > 0, GWGS_EINVAL);
+    GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
+    Job J;
+    J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
+    J.Tn = 0; J.M = M; J.per = mode == 1 ? per : 1; J.rowscale = rowscale;
+    GWGS_CHECK(J.per > 0, GWGS_EINVAL);
+    return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, mode, J, V, I, ldv, (hipStream_t)stream);
+}
+
+GWGS_API int gwgs_gemm_item_bytes(void) { return (int)sizeof(GemmItem); }
 
 GWGS_API int gwgs_gemm_max_k(int dtype)
 {

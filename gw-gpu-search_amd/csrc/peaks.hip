@@ -120,7 +120,8 @@ __global__ void __launch_bounds__(NT) write_above_k(const float* __restrict__ vw
 }
 
 // Rebuilt complex SNR numerator at candidates (fp_filter.py:759-764): zr = Ar[tb] . Zt[t], zi = Ai[tb] . Zt[t] in
-// fp32 from the GEMM's own operands. One wave per candidate.
+// fp32 from the GEMM's own operands, times sqrt(pw_scale[sidx]) (the GEMM's row scale of that template and segment).
+// One wave per candidate.
 template <typename T>
 __device__ __forceinline__ float to_f32(const T x);
 template <>
@@ -134,8 +135,8 @@ template <typename T>
 __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar,
                                                   const T* __restrict__ Ai, const long lda, const int K,
                                                   const int* __restrict__ rows, const int* __restrict__ tcol,
-                                                  const int n, const float* __restrict__ pw_scale,
-                                                  float2* __restrict__ out)
+                                                  const int* __restrict__ sidx, const int n,
+                                                  const float* __restrict__ pw_scale, float2* __restrict__ out)
 {
     const int c = blockIdx.x * (NT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (c >= n) return;
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, cons
     }
     sr = wave_sum_f32(sr);
     si = wave_sum_f32(si);
-    const float sc = pw_scale ? sqrtf(pw_scale[rows[c]]) : 1.f;
+    const float sc = pw_scale ? sqrtf(pw_scale[sidx ? sidx[c] : rows[c]]) : 1.f;
     if (lane == 0) out[c] = make_float2(sr * sc, si * sc);
 }
 
@@ -198,7 +199,8 @@ GWGS_API int gwgs_select_above(const float* vw, long total, float cut, int* out,
 }
 
 GWGS_API int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K,
-                            const int* rows, const int* tcol, int n, const float* pw_scale, void* out, void* stream)
+                            const int* rows, const int* tcol, const int* sidx, int n, const float* pw_scale, void* out,
+                            void* stream)
 {
     GWGS_CHECK(Zt && Ar && Ai && rows && tcol && out && K > 0 && n >= 0, GWGS_EINVAL);
     if (n == 0) return GWGS_OK;
@@ -206,13 +208,13 @@ GWGS_API int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar,
     hipStream_t s = (hipStream_t)stream;
     if (dtype == 2)
         hipLaunchKernelGGL(rebuilt_z_k<uint8_t>, grid, dim3(NT), 0, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
-                           (const uint8_t*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
+                           (const uint8_t*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
     else if (dtype == 1)
         hipLaunchKernelGGL(rebuilt_z_k<uint16_t>, grid, dim3(NT), 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
-                           (const uint16_t*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
+                           (const uint16_t*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
     else if (dtype == 0)
         hipLaunchKernelGGL(rebuilt_z_k<float>, grid, dim3(NT), 0, s, (const float*)Zt, ldz, (const float*)Ar,
-                           (const float*)Ai, lda, K, rows, tcol, n, pw_scale, (float2*)out);
+                           (const float*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
     else
         return GWGS_EUNSUP;
     return gwgs_status();

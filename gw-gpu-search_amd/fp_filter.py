@@ -57,6 +57,7 @@ class Options:
     self_check: bool = True                # the exact stage's first-call check
     timing: bool = False                   # synchronize around stages to fill the timing dict
     gemm_events: bool = False              # HIP events around every GEMM launch (kernel time, algorithmic flops)
+    segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
 
 
 @dataclass
@@ -233,74 +234,94 @@ class Filter:
             torch.cuda.synchronize()
         return time.perf_counter()
 
-    def _qbuf(self, rp):
-        """(rp, tlen) complex64 transform input whose columns >= flen stay zero (fp_filter.py:595)."""
-        if self._q is None or self._q.shape[0] < rp:
-            self._q = torch.zeros((rp, self.tlen), dtype=torch.complex64, device='cuda')
-        return self._q[:rp]
+    def _qbuf(self, nb, rp):
+        """(nb, rp, tlen) complex64 transform input whose columns >= flen stay zero (fp_filter.py:595): allocated
+        zeroed once and only its [:flen] columns are ever written."""
+        if self._q is None or self._q.shape[0] < nb or self._q.shape[1] < rp:
+            self._q = None
+            self._q = torch.zeros((nb, rp, self.tlen), dtype=torch.complex64, device='cuda')
+        return self._q[:nb, :rp]
 
     def run(self, plan: BankPlan):
         """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset."""
         outs = [Triggers() for _ in self.data]
+        segs = []
+        for di, m in enumerate(self.data):
+            for s, (a0, a1) in enumerate(m.analyze):
+                nw = (a1 - a0) // self.cw
+                if nw > 0:
+                    segs.append((di, s, self.seg0[di] + s, a0, nw))
+        B = max(1, self.opts.segment_batch)
         for tb in plan.tiles:
             t0 = self._sync()
             sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df)
-            normf = norm.to(torch.float32)
             t1 = self._sync(); self.T['norm_s'] += t1 - t0
             sig_h = sig.cpu().numpy()
             norm_h = norm.cpu().numpy()
-            for di, m in enumerate(self.data):
-                for s in range(len(m.analyze)):
-                    self.filter_segment(tb, m, s, self.seg0[di] + s, normf, norm_h, sig_h, outs[di])
+            for b0 in range(0, len(segs), B):
+                self.filter_batch(tb, segs[b0:b0 + B], norm, norm_h, sig_h, outs)
         return outs
 
-    def filter_segment(self, tb: TileBasis, m: Dataset, s, gs, normf, norm_h, sig_h, out: Triggers):
+    def filter_batch(self, tb: TileBasis, segs, norm, norm_h, sig_h, outs):
+        """One tile against a batch of segments: the basis transforms of every segment in one batched hipFFT call,
+        and one GEMM launch over all their windows (an item table of (segment, run of tiles)); the template norm of
+        each segment is applied to |z|^2 in the GEMM epilogue as a per-(segment, template) row scale."""
         o = self.opts
-        a0, a1 = m.analyze[s]
-        nw = (a1 - a0) // self.cw
-        if nw <= 0:
-            return
-        Tn = nw * self.cw
         n, rp = len(tb.ids), tb.rp
         K = 2 * rp
-        t0 = self._sync()
-        q = self._qbuf(rp)
-        _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, m.D[s].data_ptr(), self.tlen, q.data_ptr(),
-                  _dev.stream())
-        Zc = torch.fft.ifft(q, dim=1, norm='forward')                  # (rp, tlen): basis series
+        nb = len(segs)
         dt = OPERAND_DTYPES[self.code]
-        zcs = acs = pws = None
+        t0 = self._sync()
+        q = self._qbuf(nb, rp)
+        for i, (di, s, gs, a0, nw) in enumerate(segs):
+            _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, self.data[di].D[s].data_ptr(), self.tlen,
+                      q[i].data_ptr(), _dev.stream())
+        Zc = torch.fft.ifft(q, dim=2, norm='forward')                 # (nb, rp, tlen): basis series
+        Tns = [nw * self.cw for (_, _, _, _, nw) in segs]
+        row0 = np.concatenate([[0], np.cumsum(Tns)]).astype(np.int64)
+        zcs = acs = None
         if self.code == 2:
-            # per-basis scales for fp8: mean |z_j|^2 over the segment = sum_k |q_jk|^2 (Parseval, unnormalised
-            # inverse); Z is divided by its rms, A multiplied by it, and each template row of A gets its own scale
-            pw = torch.empty(rp, dtype=torch.float64, device='cuda')
-            _lib.call('gwgs_basis_power', q.data_ptr(), self.tlen, rp, 0, self.flen, pw.data_ptr(), _dev.stream())
-            rms = torch.sqrt(pw).to(torch.float32)
+            # fp8: per-basis scales from the batch's mean |z_j|^2 = mean over segments of sum_k |q_jk|^2
+            # (Parseval, unnormalised inverse); Z is divided by the rms, A multiplied by it
+            pw = torch.empty((nb, rp), dtype=torch.float64, device='cuda')
+            for i in range(nb):
+                _lib.call('gwgs_basis_power', q[i].data_ptr(), self.tlen, rp, 0, self.flen, pw[i].data_ptr(),
+                          _dev.stream())
+            rms = torch.sqrt(pw.mean(0)).to(torch.float32)
             zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
             acs = rms.contiguous()
-            pws = torch.empty(n, dtype=torch.float32, device='cuda')
-        Zt = torch.empty((Tn, K), dtype=dt, device='cuda')
-        _lib.call('gwgs_pack_zt', self.code, Zc.data_ptr(), self.tlen, rp, a0, Tn, Tn, _dev.ptr(zcs), Zt.data_ptr(),
-                  _dev.stream())
-        zoff = 0
+        Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
+        for i, (di, s, gs, a0, nw) in enumerate(segs):
+            _lib.call('gwgs_pack_zt', self.code, Zc[i].data_ptr(), self.tlen, rp, a0, Tns[i], Tns[i], _dev.ptr(zcs),
+                      Zt[row0[i]].data_ptr(), _dev.stream())
+        del Zc
         t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
         Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
-        rs = normf[:, gs].contiguous()
+        ones = torch.ones(n, dtype=torch.float32, device='cuda')
+        gsl = [gs for (_, _, gs, _, _) in segs]
+        rows2 = (norm[:, gsl] ** 2).T.to(torch.float32)               # (nb, n): norm^2 per (segment, template)
         if self.code == 2:
-            _lib.call('gwgs_stack_a_fp8', tb.A.data_ptr(), n, tb.r, rs.data_ptr(), acs.data_ptr(), K, Ar.data_ptr(),
-                      Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+            pws = torch.empty(n, dtype=torch.float32, device='cuda')
+            _lib.call('gwgs_stack_a_fp8', tb.A.data_ptr(), n, tb.r, ones.data_ptr(), acs.data_ptr(), K,
+                      Ar.data_ptr(), Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+            rows2 = rows2 * pws[None, :]
         else:
-            _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, rs.data_ptr(), None, n, K, Ar.data_ptr(),
-                      Ai.data_ptr(), _dev.stream())
-        vw = torch.empty((n, nw), dtype=torch.float32, device='cuda')
-        iw = torch.empty((n, nw), dtype=torch.int32, device='cuda')
+            _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, ones.data_ptr(), None, n, K,
+                      Ar.data_ptr(), Ai.data_ptr(), _dev.stream())
+        rows2 = rows2.contiguous()
+        per = self.cw // BN
+        nwsum = sum(nw for (_, _, _, _, nw) in segs)
+        col0 = np.concatenate([[0], np.cumsum([nw for (_, _, _, _, nw) in segs])]).astype(np.int64)
+        items = fp_gemm.items_table([(int(row0[i]), Tns[i] // BN, int(col0[i]), i, Tns[i]) for i in range(nb)],
+                                    fp_gemm.tiles_per_item(o.gemm_cfg, per))
+        vw = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')
+        iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
         if o.gemm_events:
             e0 = torch.cuda.Event(enable_timing=True); e0.record()
-        fp_gemm.gemm_peaks_raw(self.code, Zt, K, Ar, Ai, K, n, K, Tn, 1, self.cw // BN, o.gemm_cfg, vw, iw, zoff,
-                               rowscale=pws)
+        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, items, 1, per, o.gemm_cfg, vw, iw, rows2)
         if o.gemm_events:
             e1 = torch.cuda.Event(enable_timing=True); e1.record()
-            self.events.append((e0, e1, 4.0 * n * (2 * tb.r) * Tn))
+            self.events.append((e0, e1, 4.0 * n * (2 * tb.r) * float(sum(Tns))))
         cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
         g = peak_kernel.select_above(vw, float(cut))
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
@@ -308,16 +329,32 @@ class Filter:
             return
         gd = torch.from_numpy(g).cuda()
         il_d = iw.view(-1)[gd]
-        tb_ = (g // nw).astype(np.int64)
-        rows_d = torch.from_numpy(tb_.astype(np.int32)).cuda()
+        m_ = (g // nwsum).astype(np.int64)
+        c_ = g % nwsum
+        si = np.searchsorted(col0, c_, side='right') - 1                # batch-local segment of each candidate
+        il = il_d.cpu().numpy().astype(np.int64)
+        tcol = torch.from_numpy((row0[si] + il).astype(np.int32)).cuda()
+        sidx = torch.from_numpy((si * n + m_).astype(np.int32)).cuda()
+        rows_d = torch.from_numpy(m_.astype(np.int32)).cuda()
         zreb_d = torch.empty(len(g), dtype=torch.complex64, device='cuda')
-        _lib.call('gwgs_rebuilt_z', self.code, Zt.data_ptr() + zoff, K, Ar.data_ptr(), Ai.data_ptr(), K, K,
-                  rows_d.data_ptr(), il_d.data_ptr(), len(g), _dev.ptr(pws), zreb_d.data_ptr(), _dev.stream())
+        _lib.call('gwgs_rebuilt_z', self.code, Zt.data_ptr(), K, Ar.data_ptr(), Ai.data_ptr(), K, K,
+                  rows_d.data_ptr(), tcol.data_ptr(), sidx.data_ptr(), len(g), rows2.data_ptr(), zreb_d.data_ptr(),
+                  _dev.stream())
         approx = torch.sqrt(vw.view(-1)[gd]).cpu().numpy()
-        samp = a0 + il_d.cpu().numpy().astype(np.int64)
         zreb = zreb_d.cpu().numpy()
         t3 = self._sync(); self.T['rebuilt_z_s'] += t3 - t2
         self.T['n_candidates'] += len(g)
+        for i, (di, s, gs, a0, nw) in enumerate(segs):
+            sel = si == i
+            if sel.any():
+                self._finish_segment(tb, self.data[di], s, gs, m_[sel], a0 + il[sel], approx[sel], zreb[sel],
+                                     norm_h, sig_h, outs[di])
+        t4 = self._sync(); self.T['refine_chisq_s'] += t4 - t3
+
+    def _finish_segment(self, tb, m, s, gs, tb_, samp, approx, zreb, norm_h, sig_h, out: Triggers):
+        """fp_filter.py:768-853 for one segment's candidates: the rebuilt band keeps rebuilt values; the exact band
+        goes through the exact stage and the final threshold."""
+        o = self.opts
         thr = o.snr_threshold
         ex = approx >= o.exact_snr * (1 - o.margin)
         rb = ~ex & (approx > thr)
@@ -328,20 +365,20 @@ class Filter:
                     chi=np.zeros(nrb, np.float32), dof=np.zeros(nrb, np.int32), sg=sig_h[tb_[rb], gs],
                     ex=np.zeros(nrb, bool), samp=samp[rb], seg=np.full(nrb, gs))
             self.T['n_rebuilt_peaks'] += nrb
-        tbx, sx, ax, zx_r = tb_[ex], samp[ex], approx[ex], zreb[ex]
+        tbx, sx, ax = tb_[ex], samp[ex], approx[ex]
         self.T['n_exact_candidates'] += len(tbx)
-        if len(tbx):
-            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
-            nr = norm_h[tbx, gs]
-            snr = np.abs(pv.z.astype(np.complex128)) * nr
-            self.rel.append(ax / np.maximum(snr, 1e-30))
-            keep = snr > thr
-            nb = tb.stage.tile.nbins[tbx]
-            chi = ((nb * pv.pw.astype(np.float64) - np.abs(pv.z.astype(np.complex128)) ** 2) * nr ** 2).astype(np.float32)
-            dof = (2 * nb - 2).astype(np.int32)
-            k = keep
-            out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
-                    ph=np.angle(pv.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
-                    ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
-            self.T['n_chisq_points'] += int(k.sum())
-        t4 = self._sync(); self.T['refine_chisq_s'] += t4 - t3
+        if not len(tbx):
+            return
+        pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
+        nr = norm_h[tbx, gs]
+        zx = pv.z.astype(np.complex128)
+        snr = np.abs(zx) * nr
+        self.rel.append(ax / np.maximum(snr, 1e-30))
+        k = snr > thr
+        nb = tb.stage.tile.nbins[tbx]
+        chi = ((nb * pv.pw.astype(np.float64) - np.abs(zx) ** 2) * nr ** 2).astype(np.float32)
+        dof = (2 * nb - 2).astype(np.int32)
+        out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
+                ph=np.angle(pv.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
+                ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
+        self.T['n_chisq_points'] += int(k.sum())

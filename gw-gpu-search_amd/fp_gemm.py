@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import time
 
+import numpy as np
 import torch
 
 import _dev
@@ -74,6 +75,37 @@ def gemm_peaks_raw(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, Tn, mode, per, cfg, V
     _lib.call('gwgs_gemm_peaks', dtype_code, variant, Zt.data_ptr() + zt_offset_bytes, ldz, Ar.data_ptr(),
               Ai.data_ptr(), lda, M, K, Tn, mode, per, tpb, V.data_ptr(), I.data_ptr(), V.stride(0),
               _dev.ptr(rowscale), _dev.stream())
+
+
+ITEM = np.dtype([('zrow0', '<i8'), ('t_first', '<i4'), ('t_end', '<i4'), ('col0', '<i4'), ('seg', '<i4'),
+                 ('Tn', '<i4'), ('pad', '<i4')])
+
+
+def tiles_per_item(cfg, per):
+    """128-sample tiles per GEMM work item: cfg's tiles_per_block rounded up to whole windows (default 2 windows)."""
+    tpb, _ = _cfg(cfg)
+    tpb = tpb or 2 * per
+    return -(-tpb // per) * per
+
+
+def items_table(segments, tpb):
+    """Device item table for gemm_peaks_multi. segments: (zrow0, ntile, col0, seg, Tn) per segment; each is cut into
+    runs of tpb tiles (csrc/gemm_peaks.hip GemmItem)."""
+    rows = []
+    for zrow0, ntile, col0, seg, Tn in segments:
+        for t in range(0, ntile, tpb):
+            rows.append((zrow0, t, min(t + tpb, ntile), col0, seg, Tn, 0))
+    tab = np.array(rows, dtype=ITEM)
+    return torch.from_numpy(tab.view(np.uint8)).cuda(), len(rows)
+
+
+def gemm_peaks_multi(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, mode, per, cfg, V, I, rowscale=None):
+    """One launch over several segments (item table from items_table); rowscale (nseg, M) or None."""
+    _, variant = _cfg(cfg)
+    tab, nitems = items
+    _lib.call('gwgs_gemm_peaks_multi', dtype_code, variant, Zt.data_ptr(), ldz, Ar.data_ptr(), Ai.data_ptr(), lda,
+              M, K, tab.data_ptr(), nitems, mode, per, V.data_ptr(), I.data_ptr(), V.stride(0), _dev.ptr(rowscale),
+              _dev.stream())
 
 
 def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False, rowscale=None):

@@ -24,7 +24,8 @@ extern "C" {
 /* ---- fused SNR-reconstruction GEMM (pipeline/fp_gemm.py:14-44 gemm_absmax / fused_peaks; with mode 1 also the
  * cluster-window max of pipeline/fp_filter.py:745-746) ------------------------------------------------------------
  * z_re[m, t] = sum_k Ar[m, k] Zt[t, k], z_im[m, t] = sum_k Ai[m, k] Zt[t, k], pw = z_re^2 + z_im^2.
- * dtype: 0 = float32 operands, 1 = bfloat16 operands (bits in uint16); fp32 accumulation.
+ * dtype: 0 = float32 operands, 1 = bfloat16 (bits in uint16), 2 = fp8 e4m3 bytes; fp32 accumulation.
+ * rowscale (M floats or NULL) multiplies template m's reported |z|^2.
  * variant: 0 = choose by K (template operands held in registers up to K = 256 bf16 / 128 fp32), 1 = K-loop kernel.
  * Zt: (>= ceil(Tn/128)*128 rows, ldz) time-major; Ar, Ai: (M, lda).
  * mode 0: V, I (M, ldv >= ceil(Tn/128)): per 128 samples the max pw (columns >= Tn masked to -1) and its sample.
@@ -32,7 +33,16 @@ extern "C" {
  * First index wins ties at both levels. tiles_per_block: 128-sample tiles per workgroup (0 = default). */
 int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                     int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,
-                    void* stream);
+                    const float* rowscale, void* stream);
+/* Several segments in one launch (the batched form): items is a device array of nitems work items, each
+ * { int64 zrow0; int32 t_first, t_end, col0, seg, Tn, pad } (gwgs_gemm_item_bytes() = 32 bytes): 128-sample tiles
+ * [t_first, t_end) of the segment whose Zt rows start at zrow0; its outputs go to columns col0 + block (mode 0) or
+ * col0 + window (mode 1, t_first a multiple of per); rowscale (nseg, M) or NULL multiplies |z|^2 of template m in
+ * segment seg (the segment's template norm^2, times fp8 row scales). dtype 2 = fp8 e4m3 (OCP) operands. */
+int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
+                          int M, int K, const void* items, int nitems, int mode, int per, float* V, int* I, long ldv,
+                          const float* rowscale, void* stream);
+int gwgs_gemm_item_bytes(void);
 /* largest K the GEMM accepts for dtype (0 if the dtype is unsupported) */
 int gwgs_gemm_max_k(int dtype);
 
@@ -50,9 +60,11 @@ long gwgs_select_workspace(long total);
 int gwgs_select_above(const float* vw, long total, float cut, int* out, int cap, int* work, int* count_dev,
                       void* stream);
 /* pipeline/fp_filter.py:759-764: rebuilt complex SNR at candidates, out[c] = (Ar[rows[c]] . Zt[tcol[c]],
- * Ai[rows[c]] . Zt[tcol[c]]) in fp32 (complex64 out). dtype as gwgs_gemm_peaks. */
+ * Ai[rows[c]] . Zt[tcol[c]]) sqrt(pw_scale[sidx[c]]) in fp32 (complex64 out); sidx NULL indexes pw_scale by rows[c];
+ * pw_scale NULL: no scale. dtype as gwgs_gemm_peaks. */
 int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K,
-                   const int* rows, const int* tcol, int n, void* out, void* stream);
+                   const int* rows, const int* tcol, const int* sidx, int n, const float* pw_scale, void* out,
+                   void* stream);
 
 /* ---- exact values at peaks (pipeline/chisq_kernel.py:14-121 chisq_pts / chisq_pts_fused / bin_power_sums;
  * the dense exact stage of pipeline/exact_stage.py:196-203) ------------------------------------------------------
@@ -74,13 +86,18 @@ int gwgs_edges_fused(const void* h, long ldh, const int* rows, const float* ips,
  * q[j, k] = Vc[j, k] D[k] for k < flen, j < rp (q: (rp, ldq) complex64; columns >= flen are not written). */
 int gwgs_form_q(const void* Vc, int rp, long flen, const void* D, long ldq, void* q, void* stream);
 /* Zt[t, 2j] = Re Zc[j, a0 + t] cs[j], Zt[t, 2j+1] = Im Zc[j, a0 + t] cs[j] for t < Tn, zero for Tn <= t < Tpad;
- * Zc (rp, ldc) complex64, rp % 16 == 0; Zt (Tpad, 2 rp) float32 (dtype 0) or bfloat16 (dtype 1); cs may be NULL. */
+ * Zc (rp, ldc) complex64, rp % 16 == 0; Zt (Tpad, 2 rp) float32 (dtype 0), bfloat16 (1) or fp8 e4m3 bytes (2,
+ * saturated to +-448); cs may be NULL. */
 int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, long Tn, long Tpad, const float* colscale,
                  void* Zt, void* stream);
 /* As = A[m, j] rowscale[m] colscale[j] (A (n, r) complex64); Ar[m, 2j] = Re As, Ar[m, 2j+1] = -Im As,
  * Ai[m, 2j] = Im As, Ai[m, 2j+1] = Re As; zero for j >= r, m >= n. Ar, Ai (Mpad, K). */
 int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* rowscale, const float* colscale, int Mpad,
                  int K, void* Ar, void* Ai, void* stream);
+/* fp8 operands: per template row the scale ra = 224 / max |As| (As as gwgs_stack_a), Ar/Ai = e4m3(As ra) bytes
+ * (n, K), pw_scale[m] = 1 / ra^2 (to multiply |z|^2). */
+int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const float* colscale, int K, void* Ar,
+                     void* Ai, float* pw_scale, void* stream);
 /* out[j] = sum_{a0 <= t < a0 + Tn} |Zc[j, t]|^2 (double), the rms scale of pipeline/fp_filter.py:609-610 */
 int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double* out, void* stream);
 

@@ -253,3 +253,30 @@ def test_fp8_operand_conversion(gpu):
     got = Ar.cpu().view(torch.float8_e4m3fn).to(torch.float32).numpy()
     exp = _fp8(Xr).to(torch.float32).numpy()
     assert np.abs(got - exp).max() <= 1e-6 * 448
+
+
+@pytest.mark.parametrize('dtype,variant', [(torch.bfloat16, 0), (torch.bfloat16, 1), (torch.float32, 0)])
+def test_multi_segment_launch_equals_single(gpu, dtype, variant):
+    """One launch over three segments of different lengths (item table, per-segment row scales) equals the
+    single-segment launches, bit for bit."""
+    import fp_gemm
+    rng = np.random.default_rng(9)
+    M, K, cw = 150, 64, 2048
+    nws = [3, 1, 5]
+    Tns = [nw * cw for nw in nws]
+    Ar, Ai, _ = _operands(rng, M, K, 8, dtype)
+    Zs = [_operands(rng, 1, K, T, dtype)[2].T.copy() for T in Tns]
+    scales = (rng.random((3, M)) + 0.5).astype(np.float32)
+    A_r, A_i = torch.from_numpy(Ar).cuda().to(dtype), torch.from_numpy(Ai).cuda().to(dtype)
+    Zt = torch.from_numpy(np.concatenate(Zs)).cuda().to(dtype)
+    row0 = np.r_[0, np.cumsum(Tns)]; col0 = np.r_[0, np.cumsum(nws)]
+    code = fp_gemm.DTYPES[dtype]
+    items = fp_gemm.items_table([(int(row0[i]), Tns[i] // 128, int(col0[i]), i, Tns[i]) for i in range(3)], 32)
+    V = torch.empty((M, sum(nws)), device='cuda'); I = torch.empty((M, sum(nws)), dtype=torch.int32, device='cuda')
+    fp_gemm.gemm_peaks_multi(code, Zt, K, A_r, A_i, K, M, K, items, 1, 16, (32, variant), V, I,
+                             torch.from_numpy(scales).cuda())
+    for i in range(3):
+        v, ii = fp_gemm.fused_window_peaks(A_r, A_i, Zt[row0[i]:row0[i + 1]], M, Tns[i], cw, (32, variant), zt=True,
+                                           rowscale=torch.from_numpy(scales[i].copy()).cuda())
+        assert np.array_equal(V[:, col0[i]:col0[i + 1]].cpu().numpy(), v.cpu().numpy())
+        assert np.array_equal(I[:, col0[i]:col0[i + 1]].cpu().numpy(), ii.cpu().numpy())
