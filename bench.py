@@ -116,6 +116,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-check', action='store_true', help='skip the float64 direct-filter SNR check')
     ap.add_argument('--gemm-cfg', default=None, help='tiles_per_block[,variant]')
+    ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
+    ap.add_argument('--timing', action='store_true', help='one extra step with stage timing (synchronised) to stderr')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); the '
@@ -135,8 +137,8 @@ def main():
     ds0 = workloads.condition_streams(streams, cond, [x.clone() for x in x_dev])
     IPall, sw, isw = fp_filter.svd_metric(ds0)
     cfg = tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None
-    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, gemm_events=True)
-            for p in ('bf16', 'fp32')}
+    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, gemm_events=True,
+                                 refine=a.refine) for p in ('bf16', 'fp32')}
     plans = []
     for bank in banks:
         plans.append(fp_filter.BankPlan(bank.m1, bank.m2, bank.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
@@ -162,6 +164,15 @@ def main():
     log('warmup')
     for _ in range(a.warmup):
         ds, _, _ = step('bf16', check=True)
+    if a.timing:
+        for p in ('bf16', 'fp32'):
+            opts[p].timing = True
+            t0 = time.perf_counter()
+            _, fs, _ = step(p)
+            torch.cuda.synchronize()
+            log(f'timing {p}: step {time.perf_counter() - t0:.3f} s', json.dumps(
+                {k: round(sum(f.T[k] for f in fs), 4) for k in fs[0].T}))
+            opts[p].timing = False
     an_s = workloads.analysed_seconds(ds, cw)
     ntemp = sum(len(b) for b in banks)
     torch.cuda.synchronize()

@@ -68,9 +68,14 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
 
 
+# per-file flags: the peak kernels compute |z|^2 without fused multiply-add so that they are bit-exact against
+# numpy's float32 arithmetic on the same input
+FILE_FLAGS = {'peaks.hip': ['-ffp-contract=off']}
+
+
 def _compile(src):
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + '.o')
-    cmd = [HIPCC] + HIP_FLAGS + ['-c', src, '-o', obj]
+    cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ['-c', src, '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f'hipcc failed on {src}:\n{r.stderr[-4000:]}')

@@ -22,13 +22,14 @@ def plan_groups(rows, times):
     npk = len(rows)
     o = np.lexsort((times, rows))
     rs = rows[o]
-    brk = np.r_[0, np.flatnonzero(np.diff(rs)) + 1, npk]
-    starts, counts = [], []
-    for s0, s1 in zip(brk[:-1], brk[1:]):
-        for s in range(s0, s1, P):
-            starts.append(s); counts.append(min(P, s1 - s))
-    starts = np.asarray(starts, np.int64); counts = np.asarray(counts, np.int64)
-    return o, rs[starts] if npk else np.zeros(0, np.int64), starts, counts
+    if npk == 0:
+        z = np.zeros(0, np.int64)
+        return o, z, z, z
+    run0 = np.r_[0, np.flatnonzero(np.diff(rs)) + 1]
+    pos = np.arange(npk) - np.repeat(run0, np.diff(np.r_[run0, npk]))       # position inside the row's run
+    starts = np.flatnonzero(pos % P == 0)
+    counts = np.diff(np.r_[starts, npk])
+    return o, rs[starts], starts.astype(np.int64), counts.astype(np.int64)
 
 
 def bin_power_sums(q, rows, times, edges_per_peak, want_z=False, d=None, n_time=None):

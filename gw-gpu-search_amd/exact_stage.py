@@ -76,10 +76,12 @@ class PeakRequest:
 @dataclass(frozen=True, eq=False)
 class PeakValues:
     """Per requested peak, in request order: z = sum_j Z_j (complex64), pw = sum_j |Z_j|^2 (float32), fpeak =
-    (2 e[nb-1] - e[nb-2]) df from this stage's edges (Hz)."""
+    (2 e[nb-1] - e[nb-2]) df from this stage's edges (Hz); samples: where they were evaluated (the requested ones,
+    or the refined ones with evaluate(refine > 0))."""
     z: np.ndarray
     pw: np.ndarray
     fpeak: np.ndarray
+    samples: np.ndarray = None
 
 
 def _fpeak(E, nb, df):
@@ -105,18 +107,37 @@ class ExactStage:
         E[self.tile.nbins[rows] == 0, 0] = self.tile.kmin
         return E
 
-    def evaluate(self, seg: SegmentData, req: PeakRequest) -> PeakValues:
+    def evaluate(self, seg: SegmentData, req: PeakRequest, refine=0, lo=None, hi=None) -> PeakValues:
+        """Exact values at the requested peaks. refine = w > 0: each peak is re-centred on the largest exact |z| among
+        the samples t - w .. t + w that lie in [lo, hi) (per-peak bounds, e.g. its cluster window; first sample on
+        ties), all candidates evaluated in the same pass; PeakValues.samples holds the chosen samples. refine = 0 is
+        the reference's rule: values at the requested sample."""
         rows, samples = req.rows, req.samples
         npk = len(rows)
         if npk == 0:
-            return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0))
+            return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0), np.zeros(0, np.int64))
         n = self.tile.H.shape[0]
         assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg.tlen
         E = self.edges(seg, rows)
-        pw, z = chisq_kernel.bin_power_sums(self.tile.H, rows, samples, E, want_z=True, d=seg.D, n_time=seg.tlen)
-        out = PeakValues(z, pw, _fpeak(E, self.tile.nbins[rows], seg.df))
+        if refine > 0:
+            lo = np.zeros(npk, np.int64) if lo is None else np.asarray(lo, np.int64)
+            hi = np.full(npk, seg.tlen, np.int64) if hi is None else np.asarray(hi, np.int64)
+            off = np.arange(-refine, refine + 1)
+            cand = samples[:, None] + off[None, :]
+            cand = np.clip(cand, lo[:, None], hi[:, None] - 1)       # repeats at the bounds, removed by argmax order
+            rr = np.repeat(rows, len(off))
+            pw_all, z_all = chisq_kernel.bin_power_sums(self.tile.H, rr, cand.ravel(), np.repeat(E, len(off), 0),
+                                                        want_z=True, d=seg.D, n_time=seg.tlen)
+            mag = np.abs(z_all.astype(np.complex128)).reshape(npk, len(off))
+            j = np.argmax(mag, axis=1)                                # first index: the earliest sample on ties
+            pick = np.arange(npk) * len(off) + j
+            samples = cand.ravel()[pick]
+            pw, z = pw_all[pick], z_all[pick]
+        else:
+            pw, z = chisq_kernel.bin_power_sums(self.tile.H, rows, samples, E, want_z=True, d=seg.D, n_time=seg.tlen)
+        out = PeakValues(z, pw, _fpeak(E, self.tile.nbins[rows], seg.df), samples)
         if self.self_check and self.check is None:
-            self.check = _self_check(self, seg, req, out, E)
+            self.check = _self_check(self, seg, PeakRequest(rows, samples), out, E)
         return out
 
 

@@ -58,6 +58,8 @@ class Options:
     timing: bool = False                   # synchronize around stages to fill the timing dict
     gemm_events: bool = False              # HIP events around every GEMM launch (kernel time, algorithmic flops)
     segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
+    refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
+                                           # samples inside its window (0: the reference's rebuilt-argmax sample)
 
 
 @dataclass
@@ -369,7 +371,15 @@ class Filter:
         self.T['n_exact_candidates'] += len(tbx)
         if not len(tbx):
             return
-        pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
+        if o.refine > 0:
+            # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
+            a0 = m.analyze[s][0]
+            wlo = a0 + (sx - a0) // self.cw * self.cw
+            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx), refine=o.refine,
+                                   lo=wlo, hi=wlo + self.cw)
+            sx = pv.samples
+        else:
+            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
         nr = norm_h[tbx, gs]
         zx = pv.z.astype(np.complex128)
         snr = np.abs(zx) * nr

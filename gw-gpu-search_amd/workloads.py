@@ -53,6 +53,38 @@ def production(duration=512, n_circ=4612, n_ecc=1268, n_inj=20, glitch_snr=300.0
     return streams, banks, Conditioning(duration, 48, 16, 16, 8, 63, 16, autogating_threshold=50.0)
 
 
+def chunk_conditioning(L):
+    """Segmenting of one uneven chunk of L seconds (the batched form): 512 s segments when L >= 576 s, 128 s
+    segments from 128 s up, with the production pads (48 s / 16 s); the 16 s / 8 s median-Welch PSD uses as many
+    segments as the chunk holds, up to 63; chunks under 128 s leave nothing to analyse (None)."""
+    if L < 128:
+        return None
+    seg = 512 if L >= 576 else 128
+    nseg = int(min(63, (L - 16) // 8 + 1))
+    return Conditioning(seg, 48, 16, 16, 8, nseg, 16)
+
+
+def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4612, n_ecc=1268):
+    """The batched form: per detector n_chunks chunks with lengths lmin + step U{0 .. (lmax - lmin) / step} s (segment
+    gaps), independent seeded noise with the production lines; both banks. Returns (chunks, banks): chunks is a list
+    of (Stream, Conditioning or None)."""
+    rng = np.random.default_rng(seed)
+    lens = lmin + step * rng.integers(0, (lmax - lmin) // step + 1, (ndet, n_chunks))
+    lines = synth.default_lines()
+    chunks = []
+    t0 = 1.0e9
+    for c in range(n_chunks):
+        for d, ifo in enumerate(('H1', 'L1')[:ndet]):
+            L = int(lens[d, c])
+            st = synth.make_stream(ifo, L, SR, seed=1000 * (d + 1) + c, lines=lines, gps_start=t0)
+            chunks.append((st, chunk_conditioning(L)))
+        t0 += int(lens[:, c].max()) + 8
+    banks = [synth.circular_bank(n_circ, seed=SEEDS['bank'])]
+    if n_ecc:
+        banks.append(synth.eccentric_bank(n_ecc, seed=SEEDS['bank_ecc']))
+    return chunks, banks
+
+
 def condition_streams(streams, cond: Conditioning, x_dev=None):
     """GPU conditioning of each stream (autogating first when configured). x_dev: optional list of float32 strain
     tensors already on the GPU (they are gated in place)."""

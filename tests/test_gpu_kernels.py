@@ -233,8 +233,8 @@ def test_fp8_operand_conversion(gpu):
     cs = (rng.random(rp) + 0.5).astype(np.float32)
     Zt = torch.empty((Tn, 2 * rp), dtype=torch.uint8, device='cuda')
     csd = torch.from_numpy(cs).cuda()
-    _lib.call('gwgs_pack_zt', 2, torch.from_numpy(Zc).cuda().data_ptr(), tlen, rp, a0, Tn, Tn, csd.data_ptr(),
-              Zt.data_ptr(), _dev.stream())
+    Zcd = torch.from_numpy(Zc).cuda()                  # kept alive until the kernel has run
+    _lib.call('gwgs_pack_zt', 2, Zcd.data_ptr(), tlen, rp, a0, Tn, Tn, csd.data_ptr(), Zt.data_ptr(), _dev.stream())
     ref = Zc[:, a0:a0 + Tn] * cs[:, None]
     ref2 = np.empty((Tn, 2 * rp), np.float32); ref2[:, 0::2] = ref.real.T; ref2[:, 1::2] = ref.imag.T
     exp = _fp8(np.clip(ref2, -448, 448)).view(torch.uint8).numpy()
@@ -244,8 +244,9 @@ def test_fp8_operand_conversion(gpu):
     rs = (rng.random(n) + 0.1).astype(np.float32)
     Ar = torch.empty((n, K), dtype=torch.uint8, device='cuda'); Ai = torch.empty_like(Ar)
     pws = torch.empty(n, dtype=torch.float32, device='cuda')
-    _lib.call('gwgs_stack_a_fp8', torch.from_numpy(A).cuda().data_ptr(), n, r, torch.from_numpy(rs).cuda().data_ptr(),
-              None, K, Ar.data_ptr(), Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+    Ad, rsd = torch.from_numpy(A).cuda(), torch.from_numpy(rs).cuda()
+    _lib.call('gwgs_stack_a_fp8', Ad.data_ptr(), n, r, rsd.data_ptr(), None, K, Ar.data_ptr(), Ai.data_ptr(),
+              pws.data_ptr(), _dev.stream())
     As = A * rs[:, None]
     ra = 224.0 / np.maximum(np.abs(As.real), np.abs(As.imag)).max(1)
     np.testing.assert_allclose(pws.cpu().numpy(), 1 / ra ** 2, rtol=1e-6)
