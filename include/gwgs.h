@@ -140,6 +140,18 @@ int gwgs_gate_mag(const double* wt, long pad_start, long n, long cl, double df, 
 /* gate_taper (gpu_prep.py:212-231): x[offset + i] *= inverted Tukey window(i), i in [i1, i2) */
 int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, long pad, long win, void* stream);
 
+/* ---- time slides (pipeline/coinc_rank.py:100-138 find_pairs) -------------------------------------------------
+ * L triggers sorted by key lkey = tid * big + t (with copies at t - P and t + P), lt their unwrapped times, li their
+ * indices; per H trigger (key hkey, time hx) the pairs with |tH - tL - k step| <= window, |k| <= K, within
+ * [hkey - reach, hkey + reach]. count: counts (nh + 1 int64 workspace) become exclusive offsets, *total the number
+ * of pairs; write: (H index, L index, k, dt = k step - (tH - tL)) at those offsets. */
+int gwgs_slide_count(const double* hkey, const double* hx, int nh, const double* lkey, const double* lt,
+                     const int* li, long nl, double reach, double step, int K, double window, long long* counts,
+                     long long* total, void* stream);
+int gwgs_slide_write(const double* hkey, const double* hx, int nh, const double* lkey, const double* lt,
+                     const int* li, long nl, double reach, double step, int K, double window,
+                     const long long* offsets, int* oh, int* ol, int* ok, double* odt, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

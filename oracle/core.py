@@ -480,3 +480,26 @@ def direct_z(H, D, N):
     q = np.zeros((H.shape[0], N), np.complex128)
     q[:, :H.shape[1]] = np.conj(np.asarray(H, np.complex128)) * np.asarray(D, np.complex128)[None, :]
     return np.fft.ifft(q, axis=1) * N
+
+
+def find_pairs(H, L, P, step, K, window):
+    """coinc_rank.py:100-138 (find_pairs) by brute force: every (h, l, k) of the same template with
+    |tH - (tL + k step) mod P| <= window, |k| <= K, taking the unwrapped L time among tL - P, tL, tL + P within
+    K step + window of tH. Returns (h, l, k, dt = k step - (tH - tL)) sorted by (h, unwrapped tL)."""
+    out = []
+    reach = K * step + window
+    for h, (th, tidh) in enumerate(zip(H['x'], H['tid'])):
+        cand = []
+        for l, (tl, tidl) in enumerate(zip(L['x'], L['tid'])):
+            if tidl != tidh:
+                continue
+            for tu in (tl - P, tl, tl + P):
+                if abs(th - tu) <= reach:
+                    dt = th - tu
+                    k = np.rint(dt / step)
+                    if abs(dt - k * step) <= window and abs(k) <= K:
+                        cand.append((tu, l, int(k), k * step - dt))
+        cand.sort(key=lambda c: c[0])
+        out += [(h, l, k, d) for _, l, k, d in cand]
+    a = np.array(out, dtype=float).reshape(-1, 4)
+    return a[:, 0].astype(int), a[:, 1].astype(int), a[:, 2].astype(int), a[:, 3]

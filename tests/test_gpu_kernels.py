@@ -281,3 +281,20 @@ def test_multi_segment_launch_equals_single(gpu, dtype, variant):
                                            rowscale=torch.from_numpy(scales[i].copy()).cuda())
         assert np.array_equal(V[:, col0[i]:col0[i + 1]].cpu().numpy(), v.cpu().numpy())
         assert np.array_equal(I[:, col0[i]:col0[i + 1]].cpu().numpy(), ii.cpu().numpy())
+
+
+def test_time_slides_vs_brute_force(gpu):
+    """GPU slide coincidences against the brute-force restatement of coinc_rank.find_pairs: same pairs, order,
+    slides and dt, including wrap-around at the span ends."""
+    import slides
+    rng = np.random.default_rng(12)
+    P, step, K, window = 1000.0, 0.1, 100, 0.015
+    H = dict(x=rng.uniform(0, P, 400), tid=rng.integers(0, 5, 400))
+    L = dict(x=rng.uniform(0, P, 300), tid=rng.integers(0, 5, 300))
+    L['x'][:5] = H['x'][:5] + 0.3                      # planted coincidences at slide 3
+    L['x'][5] = P - 0.01; H['x'][5] = 0.04; L['tid'][5] = H['tid'][5]     # across the wrap
+    h, l, k, dt = slides.find_pairs(H, L, P, step, K, window)
+    hr, lr, kr, dr = core.find_pairs(H, L, P, step, K, window)
+    assert len(h) == len(hr) > 5
+    assert np.array_equal(h, hr) and np.array_equal(l, lr) and np.array_equal(k, kr)
+    np.testing.assert_allclose(dt, dr, atol=1e-12)

@@ -253,3 +253,13 @@ def test_planted_mistake_is_caught():
     good = core.direct_z(H, D, N)
     bad = np.fft.ifft(np.pad(H.astype(np.complex128) * D, ((0, 0), (0, N - H.shape[1]))), axis=1) * N
     assert np.abs(bad - good).max() > 1e-3 * np.abs(good).max()
+
+
+def test_find_pairs_brute_force_small():
+    """The brute-force slide search on a hand-built case: a pair 0.3 s apart is found at slide -3 with dt 0 and a
+    pair across the span end is found through the wrap."""
+    H = dict(x=np.array([10.0, 0.005]), tid=np.array([1, 2]))
+    L = dict(x=np.array([10.3, 99.995, 50.0]), tid=np.array([1, 2, 1]))
+    h, l, k, dt = core.find_pairs(H, L, 100.0, 0.1, 10, 0.015)
+    assert list(zip(h, l, k)) == [(0, 0, -3), (1, 1, 0)]
+    assert abs(dt[0]) < 1e-12 and abs(dt[1] + 0.01) < 1e-9
