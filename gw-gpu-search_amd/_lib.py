@@ -44,6 +44,9 @@ SIGNATURES = {
     'gwgs_stack_a': [_i, _p, _i, _i, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_stack_a_fp8': [_p, _i, _i, _p, _p, _i, _p, _p, _p, _p],
     'gwgs_basis_power': [_p, _l, _i, _l, _l, _p, _p],
+    'gwgs_basis_series_workspace': [_i, _l],
+    'gwgs_basis_series': [_i, _p, _l, _i, _i, _p, _l, _l, _l, _l, _l, _p, _p, _p, _l, _p],
+    'gwgs_vd_power': [_p, _l, _i, _p, _l, _l, _p, _p],
     'gwgs_spa_templates': [_p, _p, _p, _i, _l, _d, _d, _d, _p, _l, _p, _p],
     'gwgs_sigmasq': [_p, _l, _l, _i, _p, _l, _i, _d, _p, _l, _p],
     'gwgs_welch_segments': [_p, _l, _i, _l, _i, _i, _i, _p, _p],
@@ -60,7 +63,7 @@ SIGNATURES = {
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],
     'gwgs_slide_write': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p, _p, _p, _p],
 }
-_RESTYPE = {'gwgs_select_workspace': _l}
+_RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l}
 
 _lib = None
 _lock = threading.Lock()

@@ -1,0 +1,416 @@
+// Basis series of one tile against one segment by a fused four-step inverse FFT, written straight into the GEMM's
+// time-major operand (the reference's fp_filter.py basis_series :597-613: q = conj(Vh) D / sw zero-padded to the
+// segment length, z = ifft(q) * n, then the time-major fp16/fp32 stack the GEMM reads).
+//
+// n = n1 n2 (powers of two, 256..2048 each). With k = k1 + n1 k2 and t = t2 + n2 t1:
+//   z_j[t] = sum_k1 w_n^(k1 t2) w_n1^(k1 t1) sum_k2 x_j[k1 + n1 k2] w_n2^(k2 t2),   w_m = e^(+2 pi i / m),
+//   x_j[k] = Vc[j, k] D[k] on [kmin, kend) and 0 elsewhere.
+// cols_k: per basis row j and 16 (or 8) columns k1, the length-n2 transform over k2 (form of x fused into the load,
+// zeros above kend never read), times w_n^(k1 t2), to Y[j, t2, k1] (complex64, r x n).
+// rows_k: per t2 and 16 (or 8) basis rows, the length-n1 transform over k1 of Y[j, t2, :], written as
+// Zt[t - a0, 2j] = Re z_j[t] cs_j, Zt[t - a0, 2j + 1] = Im z_j[t] cs_j for t in [a0, a0 + Tn) (fp32, bf16 or fp8), and
+// zeros for rows r <= j < rp.
+// Each length-L line transform runs on P threads holding Q = L / P values: a Q-point DFT in registers per thread,
+// the twiddle w_L^(p m2), an exchange through LDS (real then imaginary parts, so two blocks fit a CU), and
+// Q / P P-point DFTs per thread. Twiddles of the in-register DFTs are compile-time constants; the others come from
+// float64 sincospi seeds advanced by float64 products (rounded once to float32).
+// HBM traffic per (tile, segment): 8 r (kend - kmin) bytes read of Vc, 8 r n written and read of Y, 2 Tn rp
+// elements of the output type written: two passes over Y instead of the batched FFT's several plus its input and
+// transpose passes.
+#include "common.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// e^(+2 pi i k / E) for k < E in float, evaluated by the compiler (Taylor series on the first quadrant in double,
+// rotated by whole quadrants)
+template <int E>
+struct Twiddles {
+    float c[E], s[E];
+    constexpr Twiddles() : c(), s()
+    {
+        for (int k = 0; k < E; k++) {
+            const int quad = (4 * k) / E;
+            const double x = 0.5 * kPi * (double)(4 * k - quad * E) / E;
+            double t = x, sn = x, u = 1, cs = 1;
+            for (int i = 1; i < 14; i++) {
+                t *= -x * x / ((2.0 * i) * (2.0 * i + 1));
+                sn += t;
+                u *= -x * x / ((2.0 * i - 1) * (2.0 * i));
+                cs += u;
+            }
+            for (int i = 0; i < quad; i++) { const double w = cs; cs = -sn; sn = w; }
+            c[k] = (float)cs;
+            s[k] = (float)sn;
+        }
+    }
+};
+template <int E>
+__device__ constexpr Twiddles<E> kTw{};
+
+constexpr int ilog2(int e) { int l = 0; while ((1 << l) < e) l++; return l; }
+constexpr int brev(int i, int lg) { int r = 0; for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b); return r; }
+
+// v[m] <- sum_n v[n] e^(+2 pi i n m / E), radix-2 decimation in time, every index a compile-time constant
+template <int E>
+__device__ __forceinline__ void dft_reg(float2 (&v)[E])
+{
+    constexpr int LG = ilog2(E);
+    float2 w[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) w[brev(i, LG)] = v[i];
+#pragma unroll
+    for (int h = 1; h < E; h <<= 1)
+#pragma unroll
+        for (int k = 0; k < h; k++) {
+            const int ti = k * (E / (2 * h));
+#pragma unroll
+            for (int i = k; i < E; i += 2 * h) {
+                const float2 a = w[i], b0 = w[i + h];
+                float2 b;
+                if (ti == 0) b = b0;
+                else if (4 * ti == E) b = make_float2(-b0.y, b0.x);
+                else {
+                    const float c = kTw<E>.c[ti], s = kTw<E>.s[ti];
+                    b = make_float2(b0.x * c - b0.y * s, b0.x * s + b0.y * c);
+                }
+                w[i] = make_float2(a.x + b.x, a.y + b.y);
+                w[i + h] = make_float2(a.x - b.x, a.y - b.y);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = w[i];
+}
+
+// e^(+2 pi i e / m) for an integer e reduced mod m (m a power of two), in double
+__device__ __forceinline__ void unit(const long e, const long m, double& c, double& s)
+{
+    const long r = e & (m - 1);
+    sincospi(2.0 * (double)r / (double)m, &s, &c);
+}
+
+__device__ __forceinline__ float2 cmulf(const float2 a, const double c, const double s)
+{
+    const float fc = (float)c, fs = (float)s;
+    return make_float2(a.x * fc - a.y * fs, a.x * fs + a.y * fc);
+}
+
+__device__ __forceinline__ void advance(double& c, double& s, const double dc, const double ds)
+{
+    const double t = c * dc - s * ds;
+    s = c * ds + s * dc;
+    c = t;
+}
+
+// LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
+constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
+
+// Column transforms: block (k1 block, j); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1
+template <int P, int Q, int C>
+__global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, const long ldv,
+                                                const float2* __restrict__ D, const long kmin, const long kend,
+                                                const int n1, const int lgn, float2* __restrict__ Y)
+{
+    constexpr int L = P * Q, ROW = pitch(P * C);
+    __shared__ float ex[Q * ROW];
+    const int tid = threadIdx.x;
+    const int c = tid % C, p = tid / C;
+    const int j = blockIdx.y;
+    const long n = 1L << lgn;
+    const int k1 = blockIdx.x * C + c;
+    const float2* __restrict__ vr = Vc + (size_t)j * ldv;
+    // per-lane 32-bit offset from wave-uniform bases (k < n <= 2^22)
+    const unsigned off = (unsigned)(k1 + n1 * p);
+
+    float2 v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const long kq = (long)n1 * P * q;
+        const long k = kq + off;
+        float2 x = make_float2(0.f, 0.f);
+        if (k >= kmin && k < kend) {
+            const float2 a = (vr + kq)[off], d = (D + kq)[off];
+            x = make_float2(a.x * d.x - a.y * d.y, a.x * d.y + a.y * d.x);
+        }
+        v[q] = x;
+    }
+    dft_reg<Q>(v);
+    {
+        double dc, ds, wc = 1.0, ws = 0.0;
+        unit(p, L, dc, ds);
+#pragma unroll
+        for (int m2 = 1; m2 < Q; m2++) {
+            advance(wc, ws, dc, ds);
+            v[m2] = cmulf(v[m2], wc, ws);
+        }
+    }
+    // exchange: U(c, p, m2) at m2 ROW + p C + c; step 2 takes tasks (c, m2 = tid / C + P i)
+    const int mb = tid / C;
+    float2 u[Q / P][P];
+#pragma unroll
+    for (int part = 0; part < 2; part++) {
+        if (part) __syncthreads();
+#pragma unroll
+        for (int m2 = 0; m2 < Q; m2++) ex[m2 * ROW + p * C + c] = part ? v[m2].y : v[m2].x;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < Q / P; i++)
+#pragma unroll
+            for (int pp = 0; pp < P; pp++) {
+                const float x = ex[(mb + P * i) * ROW + pp * C + c];
+                if (part) u[i][pp].y = x; else u[i][pp].x = x;
+            }
+    }
+    float2* __restrict__ yb = Y + (size_t)j * n;
+#pragma unroll
+    for (int i = 0; i < Q / P; i++) {
+        const int m2 = mb + P * i;
+        dft_reg<P>(u[i]);
+        // w_n^(k1 t2), t2 = m2 + Q m1: Y[j, t2, k1] at the uniform base Q m1 n1 plus the lane offset m2 n1 + k1
+        double wc, ws, dc, ds;
+        unit((long)k1 * m2, n, wc, ws);
+        unit((long)k1 * Q, n, dc, ds);
+        const unsigned yo = (unsigned)(m2 * n1 + k1);
+#pragma unroll
+        for (int m1 = 0; m1 < P; m1++) {
+            if (m1) advance(wc, ws, dc, ds);
+            (yb + (size_t)Q * m1 * n1)[yo] = cmulf(u[i][m1], wc, ws);
+        }
+    }
+}
+
+// One (Re, Im) pair of the output type as a single buffer store; stores whose offset is at or past the buffer's
+// size are dropped by the hardware range check, so rows outside the analysed span need no branch.
+template <typename T>
+struct Out;
+template <>
+struct Out<float> {
+    static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
+                                                 const float im)
+    {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 v;
+        v.x = __float_as_uint(re);
+        v.y = __float_as_uint(im);
+        __builtin_amdgcn_raw_buffer_store_b64(v, rs, (int)off, 0, 0);
+    }
+};
+template <>
+struct Out<uint16_t> {
+    static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
+                                                 const float im)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)f32_to_bf16(re) | ((uint32_t)f32_to_bf16(im) << 16), rs,
+                                              (int)off, 0, 0);
+    }
+};
+template <>
+struct Out<uint8_t> {
+    static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
+                                                 const float im)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)((uint32_t)f32_to_fp8(re) | ((uint32_t)f32_to_fp8(im) << 8)),
+                                              rs, (int)off, 0, 0);
+    }
+};
+
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
+
+// Row transforms: block (t2, row block); threads JB x P, thread (p, jr) = (tid % P, tid / P) in step 1
+template <int P, int Q, int JB, typename T>
+__global__ void __launch_bounds__(JB * P) rows_k(const float2* __restrict__ Y, const int r, const int rp,
+                                                 const int lgn1, const int lgn, const long a0, const long Tn,
+                                                 const float* __restrict__ cs, T* __restrict__ Zt, const long ldz)
+{
+    constexpr int L = P * Q, ROW = pitch(P * (JB + 1));
+    __shared__ float ex[Q * ROW];
+    const int tid = threadIdx.x;
+    const int p = tid % P, jr = tid / P;
+    const long n = 1L << lgn, n1 = 1L << lgn1, n2 = n >> lgn1;
+    const long t2 = blockIdx.x;
+    const int j0 = blockIdx.y * JB;
+
+    float2 v[Q];
+    {
+        // Y[j0 + jr, t2, p + P q]: uniform base plus a 32-bit lane offset (jr n + p < 2^26)
+        const bool live = j0 + jr < r;
+        const float2* __restrict__ yb = Y + (size_t)j0 * n + t2 * n1;
+        const unsigned yo = (unsigned)(jr * n + p);
+#pragma unroll
+        for (int q = 0; q < Q; q++) v[q] = live ? (yb + P * q)[yo] : make_float2(0.f, 0.f);
+    }
+    dft_reg<Q>(v);
+    {
+        double dc, ds, wc = 1.0, ws = 0.0;
+        unit(p, L, dc, ds);
+#pragma unroll
+        for (int m2 = 1; m2 < Q; m2++) {
+            advance(wc, ws, dc, ds);
+            v[m2] = cmulf(v[m2], wc, ws);
+        }
+    }
+    // exchange: U(jr, p, m2) at m2 ROW + p (JB + 1) + jr; step 2 takes tasks (jr2 = tid % JB, m2 = tid / JB + P i)
+    const int jr2 = tid % JB, mb = tid / JB;
+    float2 u[Q / P][P];
+#pragma unroll
+    for (int part = 0; part < 2; part++) {
+        if (part) __syncthreads();
+#pragma unroll
+        for (int m2 = 0; m2 < Q; m2++) ex[m2 * ROW + p * (JB + 1) + jr] = part ? v[m2].y : v[m2].x;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < Q / P; i++)
+#pragma unroll
+            for (int pp = 0; pp < P; pp++) {
+                const float x = ex[(mb + P * i) * ROW + pp * (JB + 1) + jr2];
+                if (part) u[i][pp].y = x; else u[i][pp].x = x;
+            }
+    }
+    const int j = j0 + jr2;
+    if (j >= rp) return;
+    const float sc = (cs && j < r) ? cs[j] : 1.f;
+    // Zt as a buffer of Tn rows: row t - a0 = tl + n2 Q m1; rows outside [0, Tn) go to offset Tn ldz (dropped)
+    constexpr unsigned SZ = 2 * sizeof(T);
+    const unsigned nbytes = (unsigned)(Tn * ldz * sizeof(T));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(Zt, 0, (int)nbytes, kRsrcWord3);
+    const unsigned rowb = (unsigned)(ldz * sizeof(T));
+#pragma unroll
+    for (int i = 0; i < Q / P; i++) {
+        const int m2 = mb + P * i;
+        dft_reg<P>(u[i]);
+        const int tl = (int)(t2 - a0 + n2 * m2);           // |tl| < n <= 2^22
+#pragma unroll
+        for (int m1 = 0; m1 < P; m1++) {
+            const int t = tl + (int)n2 * Q * m1;
+            const unsigned row = (unsigned)t < (unsigned)Tn ? (unsigned)t : (unsigned)Tn;
+            const float2 z = u[i][m1];
+            Out<T>::store(rs, row * rowb + (unsigned)j * SZ, z.x * sc, z.y * sc);
+        }
+    }
+}
+
+// sum_k |Vc[j, k] D[k]|^2 over [kmin, kend) in double (1 / n of sum_t |z_j[t]|^2 by Parseval); one block per row
+__global__ void __launch_bounds__(256) vd_power_k(const float2* __restrict__ Vc, const long ldv,
+                                                  const float2* __restrict__ D, const long kmin, const long kend,
+                                                  double* __restrict__ out)
+{
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (long k = kmin + threadIdx.x; k < kend; k += 256) {
+        const float2 a = Vc[(size_t)j * ldv + k], d = D[k];
+        const double m = ((double)a.x * a.x + (double)a.y * a.y) * ((double)d.x * d.x + (double)d.y * d.y);
+        s += m;
+    }
+    s = wave_sum_f64(s);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[j] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// line length L -> threads per line P (Q = L / P values each)
+bool split(const int lgl, int& P)
+{
+    if (lgl < 8 || lgl > 11) return false;
+    P = lgl <= 9 ? 16 : 32;
+    return true;
+}
+
+template <int P, int Q>
+void launch_cols(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r, float2* Y,
+                 hipStream_t s)
+{
+    constexpr int C = Q <= 32 ? 16 : 8;
+    hipLaunchKernelGGL((cols_k<P, Q, C>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin, kend, n1,
+                       lgn, Y);
+}
+
+template <int P, int Q, typename T>
+void launch_rows(const float2* Y, int r, int rp, int lgn1, int lgn, long a0, long Tn, const float* cs, T* Zt, long ldz,
+                 hipStream_t s)
+{
+    constexpr int JB = Q <= 32 ? 16 : 8;
+    const long n2 = 1L << (lgn - lgn1);
+    hipLaunchKernelGGL((rows_k<P, Q, JB, T>), dim3((unsigned)n2, (rp + JB - 1) / JB), dim3(JB * P), 0, s, Y, r, rp,
+                       lgn1, lgn, a0, Tn, cs, Zt, ldz);
+}
+
+template <typename T>
+int rows_dispatch(const int lgn1, const float2* Y, int r, int rp, int lgn, long a0, long Tn, const float* cs, T* Zt,
+                  long ldz, hipStream_t s)
+{
+    switch (lgn1) {
+    case 8: launch_rows<16, 16, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
+    case 9: launch_rows<16, 32, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
+    case 10: launch_rows<32, 32, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
+    case 11: launch_rows<32, 64, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
+    default: return GWGS_EUNSUP;
+    }
+    return GWGS_OK;
+}
+
+int log2_exact(const long n)
+{
+    if (n <= 0 || (n & (n - 1))) return -1;
+    int l = 0;
+    while ((1L << l) < n) l++;
+    return l;
+}
+
+} // namespace
+
+// Bytes of the intermediate Y for r basis rows and transform length n; 0 when n is not a supported length
+// (a power of two from 2^16 to 2^22).
+GWGS_API long gwgs_basis_series_workspace(int r, long n)
+{
+    const int lg = log2_exact(n);
+    if (lg < 16 || lg > 22 || r <= 0) return 0;
+    return (long)r * n * 8;
+}
+
+// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (clamped to +-448). Vc (>= r rows, row pitch ldv >= kend) and D
+// (>= kend) complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt (Tn rows, ldz >= 2 rp elements, even);
+// colscale: per basis row, or null.
+GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend,
+                               long n, long a0, long Tn, const float* colscale, void* work, void* Zt, long ldz,
+                               void* stream)
+{
+    GWGS_CHECK(Vc && D && work && Zt && r > 0 && rp >= r && kmin >= 0 && kmin <= kend && kend <= ldv && kend <= n &&
+               a0 >= 0 && Tn >= 0 && a0 + Tn <= n && ldz >= 2 * (long)rp && ldz % 2 == 0, GWGS_EINVAL);
+    const int lg = log2_exact(n);
+    if (lg < 16 || lg > 22) return GWGS_EUNSUP;
+    // the row kernel addresses Zt as one buffer with 32-bit byte offsets up to (Tn + 1) rows
+    const double esz = dtype == 0 ? 4.0 : dtype == 1 ? 2.0 : 1.0;
+    if ((double)(Tn + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;
+    if (dtype < 0 || dtype > 2) return GWGS_EUNSUP;
+    hipStream_t s = (hipStream_t)stream;
+    const int lgn1 = lg / 2, lgn2 = lg - lg / 2;
+    const int n1 = 1 << lgn1;
+    const float2* vc = (const float2*)Vc;
+    const float2* d = (const float2*)D;
+    float2* Y = (float2*)work;
+    switch (lgn2) {
+    case 8: launch_cols<16, 16>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
+    case 9: launch_cols<16, 32>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
+    case 10: launch_cols<32, 32>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
+    case 11: launch_cols<32, 64>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
+    default: return GWGS_EUNSUP;
+    }
+    int e;
+    if (dtype == 0) e = rows_dispatch<float>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (float*)Zt, ldz, s);
+    else if (dtype == 1) e = rows_dispatch<uint16_t>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (uint16_t*)Zt, ldz, s);
+    else e = rows_dispatch<uint8_t>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (uint8_t*)Zt, ldz, s);
+    if (e) return e;
+    return gwgs_status();
+}
+
+// out[j] = sum over [kmin, kend) of |Vc[j, k] D[k]|^2 in double, j < r.
+GWGS_API int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, long kend, double* out,
+                           void* stream)
+{
+    GWGS_CHECK(Vc && D && out && r > 0 && kmin >= 0 && kmin <= kend && kend <= ldv, GWGS_EINVAL);
+    hipLaunchKernelGGL(vd_power_k, dim3(r), dim3(256), 0, (hipStream_t)stream, (const float2*)Vc, ldv,
+                       (const float2*)D, kmin, kend, out);
+    return gwgs_status();
+}

@@ -52,10 +52,10 @@ __device__ __forceinline__ float wave_sum_f32(float x)
 // round-to-nearest-even float -> bfloat16 bits (NaN kept quiet)
 __device__ __forceinline__ uint16_t f32_to_bf16(const float x)
 {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const uint32_t u = __float_as_uint(x);
+    const uint32_t rn = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    const bool nan = (u & 0x7fffffffu) > 0x7f800000u;
+    return (uint16_t)(nan ? ((u >> 16) | 0x40u) : rn);     // a select, not a branch
 }
 
 // float -> fp8 e4m3 (OCP, gfx950 v_cvt_pk_fp8_f32: round to nearest even), saturated to +-448 first

@@ -4,10 +4,10 @@
 Per tile of templates sorted by chirp time (once per run, BankPlan):
   - the sw-whitened templates are compressed by a float64 SVD to r complex basis filters such that every template
     keeps >= 1 - eps of its power (svd_tile, fp_filter.py:576-593);
-Per tile and data segment (Filter.filter_segment):
-  - the r basis filters are correlated with the segment by one batched hipFFT inverse transform of
-    q[j, k] = conj(Vh[j, k]) D[k] / sw[k] (basis_series, fp_filter.py:597-613), transposed through LDS into the
-    GEMM's time-major operand (fp32, or bf16 for the precision form);
+Per tile and data segment (Filter.filter_batch):
+  - the r basis filters are correlated with the segment by the inverse transform of
+    q[j, k] = conj(Vh[j, k]) D[k] / sw[k] (basis_series, fp_filter.py:597-613): a fused four-step FFT of two HIP
+    passes that forms q on load and writes the GEMM's time-major operand (fp32, bf16 or fp8) directly;
   - one fused MFMA GEMM rebuilds every template's SNR and keeps, per cluster window, only the max |z|^2 and its
     sample (fp_gemm.py:14-44 with fp_filter.py:745-746 fused into its epilogue), in fp32, bf16 or fp8 operands with
     fp32 accumulation;
@@ -87,9 +87,10 @@ class TileBasis:
     r: int
     rp: int                    # r padded to a multiple of 16
     A: torch.Tensor            # (n, r) complex64: z = A @ basis series
-    Vc: torch.Tensor           # (rp, flen) complex64: conj(Vh) / sw, zero rows >= r
+    Vc: torch.Tensor           # (rp, flen) complex64: conj(Vh) / sw, zero rows >= r and columns outside [kmin, kend)
     worst: float
     stage: ExactStage = None
+    kend: int = 0              # max template kmax of the tile: Vc is zero from here on
 
 
 def svd_tile(H, sw, isw, kmin, kend, eps):
@@ -147,7 +148,7 @@ class BankPlan:
                 raise AssertionError(f'tile rank {r} exceeds the GEMM kernel limit; use a smaller --tile')
             nb = np.full(len(ids), opts.chisq_bins, np.int64)
             stage = ExactStage(TemplateTile(H, nb, self.kmin, kmax), self_check=opts.self_check)
-            self.tiles.append(TileBasis(ids, H, kmax, r, rp, A, Vc, worst, stage))
+            self.tiles.append(TileBasis(ids, H, kmax, r, rp, A, Vc, worst, stage, kend))
             self.T['templates_s'] += t1 - t0; self.T['svd_s'] += t2 - t1
 
     @property
@@ -191,6 +192,14 @@ class Triggers:
         return out
 
 
+def _fp8_scales(pw):
+    """Per-basis fp8 scales from (segments, rp) powers sum_k |q_jk|^2: Z is multiplied by 1 / rms and A by rms, with
+    rms = sqrt of the mean over segments (0 rows stay 0)."""
+    rms = torch.sqrt(pw.mean(0)).to(torch.float32)
+    zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
+    return zcs, rms.contiguous()
+
+
 def svd_metric(datasets):
     """fp_filter.py:360-364: IPall (all segments, flen), sw = sqrt(mean IPall) in float64, isw = 1/sw (0 where 0)."""
     IPall = torch.cat([m.IP for m in datasets], 0)
@@ -218,7 +227,7 @@ class Filter:
         for m in datasets:
             self.seg0.append(off); off += len(m.analyze)
         self._q = None
-        self._q_rp = 0
+        self._work = None
         self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
         self.rel = []
@@ -244,6 +253,57 @@ class Filter:
             self._q = torch.zeros((nb, rp, self.tlen), dtype=torch.complex64, device='cuda')
         return self._q[:nb, :rp]
 
+    def _workbuf(self, nbytes):
+        """Workspace of the fused basis transform (the column-pass output), grown on demand and reused."""
+        if self._work is None or self._work.numel() < nbytes:
+            self._work = None
+            self._work = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
+        return self._work
+
+    def basis_operand(self, tb: TileBasis, segs, Tns, row0):
+        """Zt (sum Tn, 2 rp) in the GEMM's operand type: the basis series of every segment of the batch over its
+        analysed span, stacked by rows (fp_filter.py:597-613). Power-of-two segment lengths 2^16..2^22 take the
+        fused four-step transform (csrc/basis_fft.hip): no transform input is materialised and the output is written
+        time-major directly; other lengths take one batched hipFFT transform and the LDS transpose (pack_zt).
+        For fp8, the per-basis scale 1 / rms (rms from the batch's mean sum_k |q_jk|^2, Parseval) is applied to Z
+        and returned for A."""
+        rp, K = tb.rp, 2 * tb.rp
+        nb = len(segs)
+        dt = OPERAND_DTYPES[self.code]
+        lib = _lib.load()
+        ws = lib.gwgs_basis_series_workspace(tb.r, self.tlen)
+        zcs = acs = None
+        Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
+        if ws > 0:
+            if self.code == 2:
+                pw = torch.zeros((nb, rp), dtype=torch.float64, device='cuda')
+                for i, (di, s, gs, a0, nw) in enumerate(segs):
+                    _lib.call('gwgs_vd_power', tb.Vc.data_ptr(), self.flen, tb.r, self.data[di].D[s].data_ptr(),
+                              self.kmin, tb.kend, pw[i].data_ptr(), _dev.stream())
+                zcs, acs = _fp8_scales(pw)
+            work = self._workbuf(ws)
+            for i, (di, s, gs, a0, nw) in enumerate(segs):
+                _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
+                          self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
+                          work.data_ptr(), Zt[row0[i]].data_ptr(), K, _dev.stream())
+            return Zt, acs
+        q = self._qbuf(nb, rp)
+        for i, (di, s, gs, a0, nw) in enumerate(segs):
+            _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, self.data[di].D[s].data_ptr(), self.tlen,
+                      q[i].data_ptr(), _dev.stream())
+        Zc = torch.fft.ifft(q, dim=2, norm='forward')                 # (nb, rp, tlen): basis series
+        if self.code == 2:
+            pw = torch.empty((nb, rp), dtype=torch.float64, device='cuda')
+            for i in range(nb):
+                _lib.call('gwgs_basis_power', q[i].data_ptr(), self.tlen, rp, 0, self.flen, pw[i].data_ptr(),
+                          _dev.stream())
+            zcs, acs = _fp8_scales(pw)
+        for i, (di, s, gs, a0, nw) in enumerate(segs):
+            _lib.call('gwgs_pack_zt', self.code, Zc[i].data_ptr(), self.tlen, rp, a0, Tns[i], Tns[i], _dev.ptr(zcs),
+                      Zt[row0[i]].data_ptr(), _dev.stream())
+        del Zc
+        return Zt, acs
+
     def run(self, plan: BankPlan):
         """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset."""
         outs = [Triggers() for _ in self.data]
@@ -265,38 +325,18 @@ class Filter:
         return outs
 
     def filter_batch(self, tb: TileBasis, segs, norm, norm_h, sig_h, outs):
-        """One tile against a batch of segments: the basis transforms of every segment in one batched hipFFT call,
-        and one GEMM launch over all their windows (an item table of (segment, run of tiles)); the template norm of
-        each segment is applied to |z|^2 in the GEMM epilogue as a per-(segment, template) row scale."""
+        """One tile against a batch of segments: the basis series of every segment (basis_operand), and one GEMM
+        launch over all their windows (an item table of (segment, run of tiles)); the template norm of each segment
+        is applied to |z|^2 in the GEMM epilogue as a per-(segment, template) row scale."""
         o = self.opts
         n, rp = len(tb.ids), tb.rp
         K = 2 * rp
         nb = len(segs)
         dt = OPERAND_DTYPES[self.code]
         t0 = self._sync()
-        q = self._qbuf(nb, rp)
-        for i, (di, s, gs, a0, nw) in enumerate(segs):
-            _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, self.data[di].D[s].data_ptr(), self.tlen,
-                      q[i].data_ptr(), _dev.stream())
-        Zc = torch.fft.ifft(q, dim=2, norm='forward')                 # (nb, rp, tlen): basis series
         Tns = [nw * self.cw for (_, _, _, _, nw) in segs]
         row0 = np.concatenate([[0], np.cumsum(Tns)]).astype(np.int64)
-        zcs = acs = None
-        if self.code == 2:
-            # fp8: per-basis scales from the batch's mean |z_j|^2 = mean over segments of sum_k |q_jk|^2
-            # (Parseval, unnormalised inverse); Z is divided by the rms, A multiplied by it
-            pw = torch.empty((nb, rp), dtype=torch.float64, device='cuda')
-            for i in range(nb):
-                _lib.call('gwgs_basis_power', q[i].data_ptr(), self.tlen, rp, 0, self.flen, pw[i].data_ptr(),
-                          _dev.stream())
-            rms = torch.sqrt(pw.mean(0)).to(torch.float32)
-            zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
-            acs = rms.contiguous()
-        Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
-        for i, (di, s, gs, a0, nw) in enumerate(segs):
-            _lib.call('gwgs_pack_zt', self.code, Zc[i].data_ptr(), self.tlen, rp, a0, Tns[i], Tns[i], _dev.ptr(zcs),
-                      Zt[row0[i]].data_ptr(), _dev.stream())
-        del Zc
+        Zt, acs = self.basis_operand(tb, segs, Tns, row0)
         t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
         Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
         ones = torch.ones(n, dtype=torch.float32, device='cuda')

@@ -101,6 +101,20 @@ int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const f
 /* out[j] = sum_{a0 <= t < a0 + Tn} |Zc[j, t]|^2 (double), the rms scale of pipeline/fp_filter.py:609-610 */
 int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double* out, void* stream);
 
+/* Fused basis series (replaces pipeline/fp_filter.py:597-613 basis_series: the zero-padded q = conj(Vh) D / sw,
+ * torch.fft.ifft(q) * n and the fp16 stacked Z), as a four-step FFT of two passes for power-of-two n from 2^16 to
+ * 2^22: x_j[k] = Vc[j, k] D[k] on [kmin, kend) (0 elsewhere), z_j[t] = sum_k x_j[k] e^(2 pi i k t / n), written as
+ * Zt[t - a0, 2j] = Re z_j[t] cs[j], Zt[t - a0, 2j + 1] = Im z_j[t] cs[j] for a0 <= t < a0 + Tn, j < r, and zeros
+ * for r <= j < rp. dtype 0 float32, 1 bfloat16 (nearest even), 2 fp8 e4m3 (saturated to +-448); colscale may be
+ * NULL. Vc: >= r rows of pitch ldv complex64; D complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt: Tn
+ * rows of ldz >= 2 rp elements (ldz even). Returns GWGS_EUNSUP for other n, GWGS_ESIZE when Zt exceeds 4 GiB. */
+long gwgs_basis_series_workspace(int r, long n);
+int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend, long n,
+                      long a0, long Tn, const float* colscale, void* work, void* Zt, long ldz, void* stream);
+/* out[j] = sum over kmin <= k < kend of |Vc[j, k] D[k]|^2 (double), j < r: the transform input's power by rows, from
+ * which pipeline/fp_filter.py:609-610's rms scale follows by Parseval without forming q */
+int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, long kend, double* out, void* stream);
+
 /* ---- templates (the closed-form chirps standing in for pipeline/imr_interp.py's interpolated waveforms) and
  * sigma^2 (pipeline/fp_filter.py:710-714) ------------------------------------------------------------------------
  * H[i, k] = f^(-7/6) exp(-i psi_i(f)), f = k df, for int(flow/df) <= k < kmax_i = min(int(f_isco,i / df), flen-1),

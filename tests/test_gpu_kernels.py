@@ -298,3 +298,63 @@ def test_time_slides_vs_brute_force(gpu):
     assert len(h) == len(hr) > 5
     assert np.array_equal(h, hr) and np.array_equal(l, lr) and np.array_equal(k, kr)
     np.testing.assert_allclose(dt, dr, atol=1e-12)
+
+
+@pytest.mark.parametrize('n,r,kmin,kend,a0,Tn', [
+    (1 << 16, 37, 5, 20000, 300, 1 << 15),            # 256 x 256
+    (1 << 17, 16, 0, (1 << 16) + 1, 0, 1 << 17),      # 256 x 512, whole series
+    (1 << 19, 9, 3840, 120000, 4096, 65536),          # 512 x 1024
+    (1 << 20, 21, 7680, 230000, 98304, 917504),       # 1024 x 1024, a 512 s segment at 2048 Hz
+    (1 << 21, 3, 100, 500000, 5, 1000),               # 1024 x 2048
+    (1 << 22, 2, 0, 1 << 21, (1 << 22) - 4096, 4096),  # 2048 x 2048, the last rows
+])
+def test_basis_series_vs_oracle(gpu, n, r, kmin, kend, a0, Tn):
+    """The fused four-step transform (gwgs_basis_series) against the oracle's float64 basis series of the same Vc and
+    D (fp_filter.py:597-613): fp32 output within 2e-6 rms and 2e-5 max of the series' rms; zero columns for rows
+    r..rp; the bf16 and fp8 (with per-basis scales) outputs are the device's own fp32 values rounded to nearest even,
+    bit for bit."""
+    import _dev, _lib
+    rng = np.random.default_rng(n + r)
+    flen = n // 2 + 1
+    rp = -(-r // 16) * 16
+    K = 2 * rp
+    Vc = np.zeros((rp, flen), np.complex64)
+    Vc[:r, kmin:kend] = (rng.standard_normal((r, kend - kmin)) + 1j * rng.standard_normal((r, kend - kmin)))
+    D = (rng.standard_normal(flen) + 1j * rng.standard_normal(flen)).astype(np.complex64)
+    ref = core.basis_series(np.conj(Vc[:r].astype(np.complex128)), D.astype(np.complex128), a0, Tn, n).T  # (Tn, r)
+    lib = _lib.load()
+    ws = lib.gwgs_basis_series_workspace(r, n)
+    assert ws == r * n * 8
+    work = torch.empty(ws, dtype=torch.uint8, device='cuda')
+    Vd, Dd = torch.from_numpy(Vc).cuda(), torch.from_numpy(D).cuda()
+    cs = (rng.random(rp) * 0.02 + 0.01).astype(np.float32)
+    csd = torch.from_numpy(cs).cuda()
+    outs = {}
+    for code, dt in ((0, torch.float32), (1, torch.bfloat16), (2, torch.uint8)):
+        Zt = torch.full((Tn, K), 7, dtype=dt, device='cuda')
+        _lib.call('gwgs_basis_series', code, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kend, n, a0, Tn,
+                  csd.data_ptr() if code == 2 else None, work.data_ptr(), Zt.data_ptr(), K, _dev.stream())
+        outs[code] = Zt.cpu()
+    z32 = outs[0].numpy()
+    got = z32[:, 0:2 * r:2].astype(np.float64) + 1j * z32[:, 1:2 * r:2]
+    rms = np.sqrt(np.mean(np.abs(ref) ** 2))
+    err = np.abs(got - ref)
+    assert np.sqrt(np.mean(err ** 2)) < 2e-6 * rms and err.max() < 2e-5 * rms, (err.max() / rms)
+    assert np.all(z32[:, 2 * r:] == 0)
+    assert torch.equal(outs[1], outs[0].to(torch.bfloat16))
+    scaled = np.clip(z32 * np.repeat(cs, 2)[None, :], -448, 448)
+    assert np.array_equal(outs[2].numpy(), _fp8(scaled).view(torch.uint8).numpy())
+
+
+def test_vd_power_vs_float64(gpu):
+    """gwgs_vd_power: sum over [kmin, kend) of |Vc D|^2 per row, against float64 (relative 1e-12)."""
+    import _dev, _lib
+    rng = np.random.default_rng(5)
+    r, flen, kmin, kend = 7, 5000, 30, 4100
+    Vc = (rng.standard_normal((r, flen)) + 1j * rng.standard_normal((r, flen))).astype(np.complex64)
+    D = (rng.standard_normal(flen) + 1j * rng.standard_normal(flen)).astype(np.complex64)
+    out = torch.empty(r, dtype=torch.float64, device='cuda')
+    Vd, Dd = torch.from_numpy(Vc).cuda(), torch.from_numpy(D).cuda()
+    _lib.call('gwgs_vd_power', Vd.data_ptr(), flen, r, Dd.data_ptr(), kmin, kend, out.data_ptr(), _dev.stream())
+    exp = (np.abs(Vc[:, kmin:kend].astype(np.complex128) * D[kmin:kend]) ** 2).sum(1)
+    np.testing.assert_allclose(out.cpu().numpy(), exp, rtol=1e-12)
