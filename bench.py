@@ -74,11 +74,23 @@ def direct_snr_check(datasets, plans, outs, max_templates=24):
 
 
 def compare_triggers(a, b):
-    """Above-threshold trigger lists of two forms: (template, sample) sets of exact triggers, and SNR agreement."""
+    """Exact above-threshold trigger lists of two forms: (template, sample) sets; (template, 1 s window) sets; for
+    windows in both, the fraction at the same sample, the largest sample offset and the largest relative SNR gap."""
     import numpy as np
-    ka = set(zip(a['template_id'][a['exact']].tolist(), a['sample'][a['exact']].tolist()))
-    kb = set(zip(b['template_id'][b['exact']].tolist(), b['sample'][b['exact']].tolist()))
-    return dict(n_a=len(ka), n_b=len(kb), only_a=len(ka - kb), only_b=len(kb - ka))
+    ea, eb = a['exact'], b['exact']
+    ka = set(zip(a['template_id'][ea].tolist(), a['sample'][ea].tolist()))
+    kb = set(zip(b['template_id'][eb].tolist(), b['sample'][eb].tolist()))
+    wa = {(int(t), int(s) // 2048): (int(s), float(v)) for t, s, v in
+          zip(a['template_id'][ea], a['sample'][ea], a['snr'][ea])}
+    wb = {(int(t), int(s) // 2048): (int(s), float(v)) for t, s, v in
+          zip(b['template_id'][eb], b['sample'][eb], b['snr'][eb])}
+    both = set(wa) & set(wb)
+    moved = [k for k in both if wa[k][0] != wb[k][0]]
+    return dict(n_a=len(ka), n_b=len(kb), only_a=len(ka - kb), only_b=len(kb - ka),
+                windows_only_a=len(set(wa) - set(wb)), windows_only_b=len(set(wb) - set(wa)),
+                same_sample=(len(both) - len(moved)) / max(len(both), 1),
+                max_sample_offset=max([abs(wa[k][0] - wb[k][0]) for k in moved], default=0),
+                max_snr_rel_gap_moved=max([abs(wa[k][1] - wb[k][1]) / wa[k][1] for k in moved], default=0.0))
 
 
 def cpu_baseline(streams, banks, cond, ntemp=128):
@@ -128,7 +140,7 @@ def main():
     import _lib
     if not os.path.exists(_lib.LIB_PATH):
         _lib.build()
-    import fp_filter, workloads
+    import _dev, fp_filter, workloads
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     t_setup = time.perf_counter()
     streams, banks, cond = workloads.production()
@@ -137,7 +149,7 @@ def main():
     ds0 = workloads.condition_streams(streams, cond, [x.clone() for x in x_dev])
     IPall, sw, isw = fp_filter.svd_metric(ds0)
     cfg = tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None
-    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, gemm_events=True,
+    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
                                  refine=a.refine) for p in ('bf16', 'fp32')}
     plans = []
     for bank in banks:
@@ -175,33 +187,36 @@ def main():
             opts[p].timing = False
     an_s = workloads.analysed_seconds(ds, cw)
     ntemp = sum(len(b) for b in banks)
+    meter = _dev.METER
+    meter.reset(); meter.on = True
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    gl = []
     for k in range(a.steps):
         ds, filters, outs_bf = step('bf16')
-        gl.append(filters)
         log(f'step {k + 1}/{a.steps}')
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
-    n_l, t_k, fl = 0, 0.0, 0.0
-    for fs in gl:
-        for f in fs:
-            n, t, ff = f.gemm_stats()
-            n_l += n; t_k += t; fl += ff
+    ms = meter.stats()
+    n_l, t_k, fl = ms['gemm_bf16']
     value = ntemp * an_s / dt
     log('fp32 form')
+    meter.on = False
     step('fp32')
+    meter.reset(); meter.on = True
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(a.fp32_steps):
         _, f32f, outs_fp = step('fp32')
     torch.cuda.synchronize()
     dt32 = (time.perf_counter() - t1) / a.fp32_steps
-    n32, t32, fl32 = 0, 0.0, 0.0
-    for f in f32f:
-        n, t, ff = f.gemm_stats()
-        n32 += n; t32 += t; fl32 += ff
+    n32, t32, fl32 = meter.stats()['gemm_fp32']
+    meter.on = False
+    hbm = {}
+    for name, (n_, t_, by) in ms.items():
+        if not name.startswith('gemm'):
+            hbm[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 1e9 / 8000.0,
+                             launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
+                             share_of_step=t_ / a.steps / dt)
     diffs = [compare_triggers(outs_bf[b][d], outs_fp[b][d]) for b in range(len(plans)) for d in range(len(ds))]
     err = err32 = None
     nchk = 0
@@ -223,10 +238,10 @@ def main():
                       avg_launch_ms=t_k / max(n_l, 1) * 1e3, gemm_share_of_step=t_k / a.steps / dt),
         fp32=dict(value=ntemp * an_s / dt32, ms_per_step=dt32 * 1e3, gemm_tflops=fl32 / t32 / 1e12,
                   gemm_frac=fl32 / t32 / 1e12 / PEAK['fp32'], gemm_share_of_step=t32 / dt32),
+        hbm_kernels=hbm,
         speedup_bf16_vs_fp32=dt32 / dt,
         snr_rel_err_vs_fp64=dict(bf16=err, fp32=err32, n_checked=nchk),
         trigger_diff_bf16_vs_fp32=diffs,
-        timing={k: v for k, v in filters[0].T.items()},
     )
     if not a.no_cpu:
         log('cpu baseline')

@@ -41,9 +41,98 @@ def host(t):
     return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
 
 
+_SIDE = None
+
+
+def side_stream():
+    """The stream that device-to-host copies run on (default priority), so a copy waits only for the work it
+    depends on, not for everything queued on the compute stream."""
+    global _SIDE
+    if _SIDE is None:
+        _SIDE = torch.cuda.Stream()
+    return _SIDE
+
+
+def to_dev(a, dtype=None):
+    """Host array -> device tensor without a host-side wait: staged in pinned memory and copied on the current
+    stream (the pinned block is kept by torch's host allocator until the copy has run)."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.pin_memory().to(DEV, non_blocking=True)
+
+
+def mark():
+    """An event at the current point of the current stream."""
+    e = torch.cuda.Event()
+    e.record()
+    return e
+
+
+class Fetch:
+    """Device -> host copies of tensors on the side stream, ordered after `after` (an event; default: everything
+    enqueued so far on the current stream). get() waits for these copies only and returns numpy arrays."""
+
+    def __init__(self, *tensors, after=None):
+        ev = after if after is not None else mark()
+        s = side_stream()
+        s.wait_event(ev)
+        self.host = []
+        with torch.cuda.stream(s):
+            for t in tensors:
+                if t is None:
+                    self.host.append(None)
+                    continue
+                h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                h.copy_(t, non_blocking=True)
+                t.record_stream(s)
+                self.host.append(h)
+        self.done = torch.cuda.Event()
+        self.done.record(s)
+
+    def get(self):
+        self.done.synchronize()
+        return [None if h is None else h.numpy() for h in self.host]
+
+
 def i32(x):
     return as_dev(np.asarray(x, np.int32))
 
 
 def i64(x):
     return as_dev(np.asarray(x, np.int64))
+
+
+class Meter:
+    """HIP events on the current stream around launches of named kernels, each with its algorithmic work (bytes
+    for HBM-bound kernels, flops for the GEMM). Off unless `on` is set; stats() synchronises."""
+
+    def __init__(self):
+        self.on = False
+        self.rec = {}
+
+    def start(self):
+        if not self.on:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def stop(self, name, e0, work):
+        if e0 is None:
+            return
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.rec.setdefault(name, []).append((e0, e1, float(work)))
+
+    def reset(self):
+        self.rec = {}
+
+    def stats(self):
+        """name -> (launches, kernel seconds, work)."""
+        torch.cuda.synchronize()
+        return {k: (len(v), sum(a.elapsed_time(b) for a, b, _ in v) / 1e3, sum(w for _, _, w in v))
+                for k, v in self.rec.items()}
+
+
+METER = Meter()

@@ -35,6 +35,7 @@ SIGNATURES = {
     'gwgs_block_window_max': [_p, _p, _l, _i, _i, _i, _p, _p, _p],
     'gwgs_select_workspace': [_l],
     'gwgs_select_above': [_p, _l, _f, _p, _i, _p, _p, _p],
+    'gwgs_select_peaks': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_chisq_peaks_per_group': [],
@@ -48,7 +49,8 @@ SIGNATURES = {
     'gwgs_basis_series': [_i, _p, _l, _i, _i, _p, _l, _l, _l, _l, _l, _p, _p, _p, _l, _p],
     'gwgs_vd_power': [_p, _l, _i, _p, _l, _l, _p, _p],
     'gwgs_spa_templates': [_p, _p, _p, _i, _l, _d, _d, _d, _p, _l, _p, _p],
-    'gwgs_sigmasq': [_p, _l, _l, _i, _p, _l, _i, _d, _p, _l, _p],
+    'gwgs_sigmasq_workspace': [_i, _l, _l, _i],
+    'gwgs_sigmasq': [_p, _l, _l, _l, _i, _p, _l, _i, _d, _p, _p, _l, _p],
     'gwgs_welch_segments': [_p, _l, _i, _l, _i, _i, _i, _p, _p],
     'gwgs_welch_reduce': [_p, _i, _i, _i, _d, _i, _d, _d, _p, _p],
     'gwgs_interp': [_p, _i, _i, _d, _d, _l, _p, _p],
@@ -63,7 +65,7 @@ SIGNATURES = {
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],
     'gwgs_slide_write': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p, _p, _p, _p],
 }
-_RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l}
+_RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l, 'gwgs_sigmasq_workspace': _l}
 
 _lib = None
 _lock = threading.Lock()

@@ -17,17 +17,25 @@ import _lib
 BL = 256
 
 
-def _edges_rows(rows_dev_idx, nb_, kmin, k1, Hd, ips):
+def edges_rows_dev(rows_dev_idx, nb_, kmin, k1, Hd, ips):
+    """Enqueue the edges of the given rows of Hd; returns the (n, B + 1) int32 device tensor without waiting."""
     n = len(rows_dev_idx)
     B = int(np.max(nb_)) if n else 0
     out = torch.empty((n, B + 1), dtype=torch.int32, device=Hd.device)
     if n == 0:
-        return np.zeros((0, B + 1), np.int64)
+        return out
     assert Hd.dtype == torch.complex64 and Hd.is_contiguous()
     ipd = _dev.as_dev(ips, torch.float32)
-    r, kk, nb = _dev.i32(rows_dev_idx), _dev.i32(k1), _dev.i32(nb_)
+    r, kk, nb = (_dev.to_dev(np.asarray(x, np.int32)) for x in (rows_dev_idx, k1, nb_))
+    e0 = _dev.METER.start()
     _lib.call('gwgs_edges_fused', Hd.data_ptr(), Hd.shape[1], r.data_ptr(), ipd.data_ptr(), kk.data_ptr(),
               nb.data_ptr(), B + 1, int(kmin), n, out.data_ptr(), _dev.stream())
+    _dev.METER.stop('edges', e0, 12.0 * float(np.asarray(k1, np.int64).sum()))      # h (8 B) and ips (4 B) per bin
+    return out
+
+
+def _edges_rows(rows_dev_idx, nb_, kmin, k1, Hd, ips):
+    out = edges_rows_dev(rows_dev_idx, nb_, kmin, k1, Hd, ips)
     return out.cpu().numpy().astype(np.int64)
 
 

@@ -32,8 +32,43 @@ def plan_groups(rows, times):
     return o, rs[starts], starts.astype(np.int64), counts.astype(np.int64)
 
 
+def _launch(q, times_sorted, grow, gstart, gcount, gedges_dev, want_z, d, N, span):
+    """Enqueue gwgs_chisq_points for planned groups; gedges_dev (ngroups, nedge) int32 on the device. Returns the
+    device outputs in sorted peak order. span: summed bin span of the groups (for the kernel meter's bytes)."""
+    npk = len(times_sorted)
+    tdev = _dev.to_dev(np.asarray(times_sorted, np.int64))
+    out = torch.empty(npk, dtype=torch.float32, device=q.device)
+    zo = torch.empty(npk, dtype=torch.complex64, device=q.device) if want_z else None
+    dd = None
+    if d is not None:
+        dd = _dev.as_dev(d, torch.complex64)
+    g_row, g_start, g_cnt = (_dev.to_dev(np.asarray(x, np.int32)) for x in (grow, gstart, gcount))
+    ge = gedges_dev.contiguous()
+    e0 = _dev.METER.start()
+    _lib.call('gwgs_chisq_points', q.data_ptr(), q.shape[1], _dev.ptr(dd), int(N), g_row.data_ptr(),
+              g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(), ge.shape[1], len(gstart),
+              out.data_ptr(), _dev.ptr(zo), _dev.stream())
+    # HBM bytes: per group the row (and the data) over its bins, 8 B each
+    _dev.METER.stop('chisq', e0, span * (16 if d is not None else 8))
+    return out, zo
+
+
+def launch_sums(q, rows, times, E_dev, e_index, want_z=False, d=None, n_time=None, row_span=None):
+    """Enqueue the sums without waiting: edges come as device rows E_dev (nu, nedge) int32 with e_index[p] the row
+    of peak p. Returns (order, out, zo): out / zo in the order `order` of the peaks (device tensors). row_span: bins
+    from the first to the last edge per row of q (for the kernel meter's bytes; optional)."""
+    N = q.shape[1] if d is None else n_time
+    assert N is not None and N > 0
+    assert q.dtype == torch.complex64 and q.is_contiguous()
+    o, grow, gstart, gcount = plan_groups(rows, times)
+    gi = _dev.to_dev(np.asarray(e_index, np.int64)[o][gstart])
+    span = 0.0 if row_span is None else float(np.asarray(row_span)[grow].sum())
+    out, zo = _launch(q, np.asarray(times, np.int64)[o], grow, gstart, gcount, E_dev[gi], want_z, d, N, span)
+    return o, out, zo
+
+
 def bin_power_sums(q, rows, times, edges_per_peak, want_z=False, d=None, n_time=None):
-    """sum_j |Z_j(t)|^2 for each peak; q is (nrows, >= N or flen) complex64 on the GPU. rows, times: int arrays per
+    """sum_j |Z_j|^2 for each peak; q is (nrows, >= N or flen) complex64 on the GPU. rows, times: int arrays per
     peak; edges_per_peak: (npeaks, nedge) int bin edges into q's row (bins with e_j >= e_j+1 are empty).
     want_z: also return z(t) = sum_j Z_j. d, n_time: fused mode, q holds the template rows H (nrows, flen), d the
     data (flen,), the kernel forms conj(H) d itself and n_time is the time length N of the phase factor.
@@ -48,20 +83,13 @@ def bin_power_sums(q, rows, times, edges_per_peak, want_z=False, d=None, n_time=
     E = np.asarray(edges_per_peak)
     o, grow, gstart, gcount = plan_groups(rows, times)
     gedges = np.ascontiguousarray(E[o][gstart].astype(np.int32))     # edges depend on the row only
-    tdev = _dev.i64(times[o])
-    out = torch.zeros(npk, dtype=torch.float32, device=q.device)
-    zo = torch.zeros(npk, dtype=torch.complex64, device=q.device) if want_z else None
-    dd = None
-    if d is not None:
-        dd = _dev.as_dev(d, torch.complex64)
-    g_row, g_start, g_cnt, g_e = _dev.i32(grow), _dev.i32(gstart), _dev.i32(gcount), _dev.i32(gedges)
-    _lib.call('gwgs_chisq_points', q.data_ptr(), q.shape[1], _dev.ptr(dd), int(N), g_row.data_ptr(),
-              g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), g_e.data_ptr(), gedges.shape[1], len(gstart),
-              out.data_ptr(), _dev.ptr(zo), _dev.stream())
+    span = float((gedges[:, -1] - gedges[:, 0]).clip(min=0).sum())
+    out, zo = _launch(q, times[o], grow, gstart, gcount, _dev.to_dev(gedges), want_z, d, N, span)
+    ho, hz = _dev.Fetch(out, zo).get()
     res = np.empty(npk, np.float32)
-    res[o] = out.cpu().numpy()
+    res[o] = ho
     if not want_z:
         return res
     zr = np.empty(npk, np.complex64)
-    zr[o] = zo.cpu().numpy()
+    zr[o] = hz
     return res, zr

@@ -7,7 +7,7 @@
 //   x_j[k] = Vc[j, k] D[k] on [kmin, kend) and 0 elsewhere.
 // cols_k: per basis row j and 16 (or 8) columns k1, the length-n2 transform over k2 (form of x fused into the load,
 // zeros above kend never read), times w_n^(k1 t2), to Y[j, t2, k1] (complex64, r x n).
-// rows_k: per t2 and 16 (or 8) basis rows, the length-n1 transform over k1 of Y[j, t2, :], written as
+// rows_k: per t2 and 8 to 32 basis rows, the length-n1 transform over k1 of Y[j, t2, :], written as
 // Zt[t - a0, 2j] = Re z_j[t] cs_j, Zt[t - a0, 2j + 1] = Im z_j[t] cs_j for t in [a0, a0 + Tn) (fp32, bf16 or fp8), and
 // zeros for rows r <= j < rp.
 // Each length-L line transform runs on P threads holding Q = L / P values: a Q-point DFT in registers per thread,
@@ -330,7 +330,8 @@ template <int P, int Q, typename T>
 void launch_rows(const float2* Y, int r, int rp, int lgn1, int lgn, long a0, long Tn, const float* cs, T* Zt, long ldz,
                  hipStream_t s)
 {
-    constexpr int JB = Q <= 32 ? 16 : 8;
+    // rows per block: 16 float32 pairs or 32 bf16 / fp8 pairs give 128- / 128- / 64-byte runs per output row
+    constexpr int JB = Q > 32 ? 8 : sizeof(T) == 4 ? 16 : 32;
     const long n2 = 1L << (lgn - lgn1);
     hipLaunchKernelGGL((rows_k<P, Q, JB, T>), dim3((unsigned)n2, (rp + JB - 1) / JB), dim3(JB * P), 0, s, Y, r, rp,
                        lgn1, lgn, a0, Tn, cs, Zt, ldz);

@@ -97,8 +97,11 @@ __global__ void __launch_bounds__(1024) scan_counts_k(int* __restrict__ counts, 
     if (t == 1023) *total = part[1023];
 }
 
+// out[pos] = g for each vw[g] > cut in increasing g; with iw, also out_i[pos] = iw[g] and out_v[pos] = vw[g]
 __global__ void __launch_bounds__(NT) write_above_k(const float* __restrict__ vw, const long total, const float cut,
-                                                    const int* __restrict__ offs, int* __restrict__ out, const int cap)
+                                                    const int* __restrict__ offs, int* __restrict__ out, const int cap,
+                                                    const int* __restrict__ iw, int* __restrict__ out_i,
+                                                    float* __restrict__ out_v)
 {
     const long c0 = (long)blockIdx.x * CH;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -113,7 +116,10 @@ __global__ void __launch_bounds__(NT) write_above_k(const float* __restrict__ vw
         int before = 0;
         for (int w = 0; w < wave; w++) before += wcount[w];
         const int pos = base + before + __popcll(m & ((1ull << lane) - 1ull));
-        if (hit && pos < cap) out[pos] = (int)g;
+        if (hit && pos < cap) {
+            out[pos] = (int)g;
+            if (iw) { out_i[pos] = iw[g]; out_v[pos] = vw[g]; }
+        }
         base += wcount[0] + wcount[1] + wcount[2] + wcount[3];
         __syncthreads();
     }
@@ -194,7 +200,27 @@ GWGS_API int gwgs_select_above(const float* vw, long total, float cut, int* out,
     GWGS_CHECK(nc < (1L << 24), GWGS_ESIZE);
     if (nc) hipLaunchKernelGGL(count_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work);
     hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, s, work, (int)nc, count_dev);   // nc = 0: count 0
-    if (nc) hipLaunchKernelGGL(write_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work, out, cap);
+    if (nc)
+        hipLaunchKernelGGL(write_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work, out, cap, nullptr,
+                           nullptr, nullptr);
+    return gwgs_status();
+}
+
+// As gwgs_select_above, and for each selected g also its window sample iw[g] into out_iw and its value vw[g] into
+// out_vw (the candidate list of fp_filter.py:747-757 in one pass, no host round trip).
+GWGS_API int gwgs_select_peaks(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw,
+                               float* out_vw, int cap, int* work, int* count_dev, void* stream)
+{
+    GWGS_CHECK((vw || total == 0) && (iw || total == 0) && ((out && out_iw && out_vw) || cap == 0) && work &&
+               count_dev && total >= 0 && cap >= 0, GWGS_EINVAL);
+    hipStream_t s = (hipStream_t)stream;
+    const long nc = (total + CH - 1) / CH;
+    GWGS_CHECK(nc < (1L << 24), GWGS_ESIZE);
+    if (nc) hipLaunchKernelGGL(count_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work);
+    hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, s, work, (int)nc, count_dev);
+    if (nc)
+        hipLaunchKernelGGL(write_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work, out, cap, iw,
+                           out_iw, out_vw);
     return gwgs_status();
 }
 

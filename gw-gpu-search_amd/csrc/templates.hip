@@ -50,19 +50,24 @@ __global__ void spa_templates_k(const double* __restrict__ m1, const double* __r
     }
 }
 
-__global__ void __launch_bounds__(NT) sigmasq_k(const float2* __restrict__ H, const long ldh, const long flen,
-                                                const float* __restrict__ IP, const long ldip, const int nseg,
-                                                const double df, double* __restrict__ sig, const long lds)
+// Partial sums over chunks of CHK bins: block (row i, chunk c) -> part[(c n + i) nseg + s] = sum over its bins of
+// |h_ik|^2 IP_sk in double. Rows vary fastest in the grid so the blocks in flight share the IP chunk in L2.
+constexpr int CHK = 8192;
+
+__global__ void __launch_bounds__(NT) sigmasq_part_k(const float2* __restrict__ H, const long ldh, const long kmin,
+                                                     const long kend, const float* __restrict__ IP, const long ldip,
+                                                     const int nseg, double* __restrict__ part)
 {
-    const int i = blockIdx.x;
+    const int i = blockIdx.x, c = blockIdx.y, n = gridDim.x;
+    const long k0 = kmin + (long)c * CHK, k1 = min(kend, k0 + CHK);
     double acc[MAXSEG];
 #pragma unroll
     for (int s = 0; s < MAXSEG; s++) acc[s] = 0.0;
-    const float2* h = H + (size_t)i * ldh;
-    for (long k = threadIdx.x; k < flen; k += NT) {
+    const float2* __restrict__ h = H + (size_t)i * ldh;
+#pragma unroll 4
+    for (long k = k0 + threadIdx.x; k < k1; k += NT) {
         const float2 v = h[k];
         const double p = (double)v.x * v.x + (double)v.y * v.y;
-        if (p == 0.0) continue;
 #pragma unroll
         for (int s = 0; s < MAXSEG; s++)
             if (s < nseg) acc[s] += p * (double)IP[(size_t)s * ldip + k];
@@ -79,8 +84,20 @@ __global__ void __launch_bounds__(NT) sigmasq_k(const float2* __restrict__ H, co
     __syncthreads();
     if (threadIdx.x < nseg) {
         const int s = threadIdx.x;
-        sig[(size_t)i * lds + s] = 4.0 * df * (((sh[0][s] + sh[1][s]) + sh[2][s]) + sh[3][s]);
+        part[((size_t)c * n + i) * nseg + s] = ((sh[0][s] + sh[1][s]) + sh[2][s]) + sh[3][s];
     }
+}
+
+// sig[i, s] = 4 df sum over chunks in order (deterministic)
+__global__ void sigmasq_sum_k(const double* __restrict__ part, const int n, const int nch, const int nseg,
+                              const double df, double* __restrict__ sig, const long lds)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)n * nseg) return;
+    const int i = (int)(g / nseg), s = (int)(g % nseg);
+    double t = 0.0;
+    for (int c = 0; c < nch; c++) t += part[((size_t)c * n + i) * nseg + s];
+    sig[(size_t)i * lds + s] = 4.0 * df * t;
 }
 
 } // namespace
@@ -97,13 +114,32 @@ GWGS_API int gwgs_spa_templates(const double* m1, const double* m2, const double
     return gwgs_status();
 }
 
-GWGS_API int gwgs_sigmasq(const void* H, long ldh, long flen, int n, const float* IP, long ldip, int nseg, double df,
-                          double* sig, long lds, void* stream)
+// Workspace doubles for gwgs_sigmasq over [kmin, kend) with n rows and nseg segments.
+GWGS_API long gwgs_sigmasq_workspace(int n, long kmin, long kend, int nseg)
 {
-    GWGS_CHECK(H && IP && sig && n >= 0 && nseg > 0 && lds >= nseg, GWGS_EINVAL);
+    if (n <= 0 || kend <= kmin || nseg <= 0) return 0;
+    return (long)n * ((kend - kmin + CHK - 1) / CHK) * nseg;
+}
+
+// sig[i, s] = 4 df sum_{kmin <= k < kend} |H[i, k]|^2 IP[s, k] (double); H is zero outside each row's band, so
+// [kmin, kend) covering every row's band gives the full-row sum. work: gwgs_sigmasq_workspace doubles.
+GWGS_API int gwgs_sigmasq(const void* H, long ldh, long kmin, long kend, int n, const float* IP, long ldip, int nseg,
+                          double df, double* work, double* sig, long lds, void* stream)
+{
+    GWGS_CHECK(H && IP && sig && n >= 0 && nseg > 0 && lds >= nseg && kmin >= 0 && kend >= kmin && kend <= ldh &&
+               kend <= ldip, GWGS_EINVAL);
     GWGS_CHECK(nseg <= MAXSEG, GWGS_ESIZE);
     if (n == 0) return GWGS_OK;
-    hipLaunchKernelGGL(sigmasq_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)H, ldh, flen, IP, ldip,
-                       nseg, df, sig, lds);
+    hipStream_t st = (hipStream_t)stream;
+    const int nch = (int)((kend - kmin + CHK - 1) / CHK);
+    if (nch > 0) {
+        GWGS_CHECK(work, GWGS_EINVAL);
+        GWGS_CHECK(nch < 65536 && n < (1 << 30), GWGS_ESIZE);
+        hipLaunchKernelGGL(sigmasq_part_k, dim3(n, nch), dim3(NT), 0, st, (const float2*)H, ldh, kmin, kend, IP, ldip,
+                           nseg, work);
+    }
+    const long tot = (long)n * nseg;
+    hipLaunchKernelGGL(sigmasq_sum_k, dim3((unsigned)((tot + NT - 1) / NT)), dim3(NT), 0, st, work, n, nch, nseg, df,
+                       sig, lds);
     return gwgs_status();
 }

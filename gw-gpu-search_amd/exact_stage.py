@@ -19,8 +19,9 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+import _dev
 import chisq_kernel
-from chisq_edges import edges_blocked
+from chisq_edges import edges_blocked, edges_rows_dev
 
 MAX_PEAKS = 8
 CHECK_ROWS = 16
@@ -112,32 +113,65 @@ class ExactStage:
         the samples t - w .. t + w that lie in [lo, hi) (per-peak bounds, e.g. its cluster window; first sample on
         ties), all candidates evaluated in the same pass; PeakValues.samples holds the chosen samples. refine = 0 is
         the reference's rule: values at the requested sample."""
+        return self.evaluate_async(seg, req, refine, lo, hi).result()
+
+    def evaluate_async(self, seg: SegmentData, req: PeakRequest, refine=0, lo=None, hi=None) -> 'PendingValues':
+        """evaluate() without waiting: edges and sums are enqueued on the current stream and their copies to the
+        host on the side stream; PendingValues.result() waits for them and returns the PeakValues."""
         rows, samples = req.rows, req.samples
         npk = len(rows)
         if npk == 0:
-            return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0), np.zeros(0, np.int64))
+            return PendingValues(self, seg, rows, samples, 1, None, None, None)
         n = self.tile.H.shape[0]
         assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg.tlen
-        E = self.edges(seg, rows)
+        ur, inv = np.unique(rows, return_inverse=True)
+        t = self.tile
+        E_dev = edges_rows_dev(ur, self._nbe(ur), t.kmin, t.kmax[ur], t.H, seg.IP)
+        zero = np.flatnonzero(t.nbins[ur] == 0)
+        if len(zero):
+            E_dev[_dev.to_dev(zero), 0] = t.kmin
         if refine > 0:
             lo = np.zeros(npk, np.int64) if lo is None else np.asarray(lo, np.int64)
             hi = np.full(npk, seg.tlen, np.int64) if hi is None else np.asarray(hi, np.int64)
             off = np.arange(-refine, refine + 1)
-            cand = samples[:, None] + off[None, :]
-            cand = np.clip(cand, lo[:, None], hi[:, None] - 1)       # repeats at the bounds, removed by argmax order
-            rr = np.repeat(rows, len(off))
-            pw_all, z_all = chisq_kernel.bin_power_sums(self.tile.H, rr, cand.ravel(), np.repeat(E, len(off), 0),
-                                                        want_z=True, d=seg.D, n_time=seg.tlen)
-            mag = np.abs(z_all.astype(np.complex128)).reshape(npk, len(off))
-            j = np.argmax(mag, axis=1)                                # first index: the earliest sample on ties
-            pick = np.arange(npk) * len(off) + j
-            samples = cand.ravel()[pick]
-            pw, z = pw_all[pick], z_all[pick]
+            cand = np.clip(samples[:, None] + off[None, :], lo[:, None], hi[:, None] - 1)   # repeats at the bounds,
+            noff = len(off)                                                                # removed by argmax order
+            rr, ei, times = np.repeat(rows, noff), np.repeat(inv, noff), cand.ravel()
         else:
-            pw, z = chisq_kernel.bin_power_sums(self.tile.H, rows, samples, E, want_z=True, d=seg.D, n_time=seg.tlen)
-        out = PeakValues(z, pw, _fpeak(E, self.tile.nbins[rows], seg.df), samples)
-        if self.self_check and self.check is None:
-            self.check = _self_check(self, seg, PeakRequest(rows, samples), out, E)
+            cand, noff, rr, ei, times = samples[:, None], 1, rows, inv, samples
+        o, out, zo = chisq_kernel.launch_sums(t.H, rr, times, E_dev, ei, want_z=True, d=seg.D, n_time=seg.tlen,
+                                              row_span=np.maximum(t.kmax - t.kmin, 0))
+        return PendingValues(self, seg, rows, cand, noff, o, _dev.Fetch(out, zo, E_dev), inv)
+
+
+class PendingValues:
+    """Exact values of one evaluate_async call; result() waits for the device work and builds the PeakValues (the
+    refine choice, fpeak from the edges, and the first-call self-check)."""
+
+    def __init__(self, stage, seg, rows, cand, noff, order, fetch, inv):
+        self.stage, self.seg, self.rows, self.cand, self.noff = stage, seg, rows, cand, noff
+        self.order, self.fetch, self.inv = order, fetch, inv
+
+    def result(self) -> PeakValues:
+        rows = self.rows
+        npk = len(rows)
+        if npk == 0:
+            return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0), np.zeros(0, np.int64))
+        st = self.stage
+        ho, hz, Eu = self.fetch.get()
+        nall = npk * self.noff
+        pw_all = np.empty(nall, np.float32); pw_all[self.order] = ho
+        z_all = np.empty(nall, np.complex64); z_all[self.order] = hz
+        if self.noff > 1:
+            mag = np.abs(z_all.astype(np.complex128)).reshape(npk, self.noff)
+            pick = np.arange(npk) * self.noff + np.argmax(mag, axis=1)   # first index: the earliest sample on ties
+            samples, pw, z = self.cand.ravel()[pick], pw_all[pick], z_all[pick]
+        else:
+            samples, pw, z = self.cand.ravel(), pw_all, z_all
+        E = Eu.astype(np.int64)[self.inv]
+        out = PeakValues(z, pw, _fpeak(E, st.tile.nbins[rows], self.seg.df), samples)
+        if st.self_check and st.check is None:
+            st.check = _self_check(st, self.seg, PeakRequest(rows, samples), out, E)
         return out
 
 

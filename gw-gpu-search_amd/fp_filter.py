@@ -4,7 +4,7 @@
 Per tile of templates sorted by chirp time (once per run, BankPlan):
   - the sw-whitened templates are compressed by a float64 SVD to r complex basis filters such that every template
     keeps >= 1 - eps of its power (svd_tile, fp_filter.py:576-593);
-Per tile and data segment (Filter.filter_batch):
+Per tile and data segment (Filter.run: launch_batch, decode_batch, collect_batch):
   - the r basis filters are correlated with the segment by the inverse transform of
     q[j, k] = conj(Vh[j, k]) D[k] / sw[k] (basis_series, fp_filter.py:597-613): a fused four-step FFT of two HIP
     passes that forms q on load and writes the GEMM's time-major operand (fp32, bf16 or fp8) directly;
@@ -24,6 +24,7 @@ returns them (HDF5 writing is outside this path).
 from __future__ import annotations
 
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -56,7 +57,6 @@ class Options:
     gemm_cfg: tuple | None = None          # (tiles_per_block,), from fp_gemm.pick_cfg
     self_check: bool = True                # the exact stage's first-call check
     timing: bool = False                   # synchronize around stages to fill the timing dict
-    gemm_events: bool = False              # HIP events around every GEMM launch (kernel time, algorithmic flops)
     segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
     refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
                                            # samples inside its window (0: the reference's rebuilt-argmax sample)
@@ -232,13 +232,6 @@ class Filter:
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
         self.rel = []
         self.code = PRECISIONS[opts.precision]
-        self.events = []                   # (start, end, algorithmic flops) per GEMM launch
-
-    def gemm_stats(self):
-        """(launches, total kernel seconds, total algorithmic flops) over the recorded GEMM launches."""
-        torch.cuda.synchronize()
-        t = sum(a.elapsed_time(b) for a, b, _ in self.events) / 1e3
-        return len(self.events), t, sum(f for _, _, f in self.events)
 
     def _sync(self):
         if self.opts.timing:
@@ -282,10 +275,15 @@ class Filter:
                               self.kmin, tb.kend, pw[i].data_ptr(), _dev.stream())
                 zcs, acs = _fp8_scales(pw)
             work = self._workbuf(ws)
+            esz = Zt.element_size()
             for i, (di, s, gs, a0, nw) in enumerate(segs):
+                e0 = _dev.METER.start()
                 _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
                           self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
                           work.data_ptr(), Zt[row0[i]].data_ptr(), K, _dev.stream())
+                # HBM bytes: Vc rows and D over [kmin, kend), Y written and read, Zt written
+                _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin) + 8 * (tb.kend - self.kmin)
+                                + 16 * tb.r * self.tlen + Tns[i] * K * esz)
             return Zt, acs
         q = self._qbuf(nb, rp)
         for i, (di, s, gs, a0, nw) in enumerate(segs):
@@ -305,7 +303,12 @@ class Filter:
         return Zt, acs
 
     def run(self, plan: BankPlan):
-        """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset."""
+        """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset.
+
+        Software pipeline on one stream: batch i + 1's transforms, GEMM and candidate selection are enqueued before
+        batch i's candidates are read back, and batch i's exact stage is enqueued before batch i - 1's results are
+        collected, so the GPU always has queued work while the host decodes and finalises. Copies to the host run on
+        a side stream after the events they depend on. With opts.timing every stage is synchronised instead."""
         outs = [Triggers() for _ in self.data]
         segs = []
         for di, m in enumerate(self.data):
@@ -314,20 +317,38 @@ class Filter:
                 if nw > 0:
                     segs.append((di, s, self.seg0[di] + s, a0, nw))
         B = max(1, self.opts.segment_batch)
+        prev, fin = None, deque()
         for tb in plan.tiles:
             t0 = self._sync()
-            sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df)
+            e0 = _dev.METER.start()
+            sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
+            _dev.METER.stop('sigmasq', e0, 8 * len(tb.ids) * (tb.kend - self.kmin)
+                            + 4 * self.IPall.shape[0] * (tb.kend - self.kmin))
+            tile_host = _dev.Fetch(sig, norm)
             t1 = self._sync(); self.T['norm_s'] += t1 - t0
-            sig_h = sig.cpu().numpy()
-            norm_h = norm.cpu().numpy()
             for b0 in range(0, len(segs), B):
-                self.filter_batch(tb, segs[b0:b0 + B], norm, norm_h, sig_h, outs)
+                cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host)
+                if self.opts.timing:
+                    self.decode_batch(cur); self.collect_batch(cur, outs)
+                    continue
+                if prev is not None:
+                    self.decode_batch(prev)
+                    fin.append(prev)
+                while len(fin) > 1:
+                    self.collect_batch(fin.popleft(), outs)
+                prev = cur
+        if prev is not None:
+            self.decode_batch(prev)
+            fin.append(prev)
+        while fin:
+            self.collect_batch(fin.popleft(), outs)
         return outs
 
-    def filter_batch(self, tb: TileBasis, segs, norm, norm_h, sig_h, outs):
-        """One tile against a batch of segments: the basis series of every segment (basis_operand), and one GEMM
-        launch over all their windows (an item table of (segment, run of tiles)); the template norm of each segment
-        is applied to |z|^2 in the GEMM epilogue as a per-(segment, template) row scale."""
+    def launch_batch(self, tb: TileBasis, segs, norm, tile_host):
+        """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
+        (basis_operand), one GEMM launch over all their windows (an item table of (segment, run of tiles)) with each
+        segment's template norm^2 applied to |z|^2 as a per-(segment, template) row scale, and the candidate list
+        (fp_filter.py:741-757) compacted on the device. Returns the batch state."""
         o = self.opts
         n, rp = len(tb.ids), tb.rp
         K = 2 * rp
@@ -352,83 +373,108 @@ class Filter:
                       Ar.data_ptr(), Ai.data_ptr(), _dev.stream())
         rows2 = rows2.contiguous()
         per = self.cw // BN
-        nwsum = sum(nw for (_, _, _, _, nw) in segs)
-        col0 = np.concatenate([[0], np.cumsum([nw for (_, _, _, _, nw) in segs])]).astype(np.int64)
+        nws = [nw for (_, _, _, _, nw) in segs]
+        nwsum = sum(nws)
+        col0 = np.concatenate([[0], np.cumsum(nws)]).astype(np.int64)
         items = fp_gemm.items_table([(int(row0[i]), Tns[i] // BN, int(col0[i]), i, Tns[i]) for i in range(nb)],
                                     fp_gemm.tiles_per_item(o.gemm_cfg, per))
         vw = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')
         iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
-        if o.gemm_events:
-            e0 = torch.cuda.Event(enable_timing=True); e0.record()
+        e0 = _dev.METER.start()
         fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, items, 1, per, o.gemm_cfg, vw, iw, rows2)
-        if o.gemm_events:
-            e1 = torch.cuda.Event(enable_timing=True); e1.record()
-            self.events.append((e0, e1, 4.0 * n * (2 * tb.r) * float(sum(Tns))))
+        # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
+        _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(Tns)))
         cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
-        g = peak_kernel.select_above(vw, float(cut))
+        sel = peak_kernel.select_peaks(vw, iw, float(cut))
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
+        return dict(tb=tb, segs=segs, Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, nwsum=nwsum, col0=col0, row0=row0, K=K,
+                    sel=sel, tile_host=tile_host, t=t2)
+
+    def decode_batch(self, st):
+        """Stage B1: read the batch's candidate list back (waiting only for its own stage A), split it by segment,
+        and enqueue the rebuilt phases (fp_filter.py:759-764) and each segment's exact stage."""
+        t0 = time.perf_counter()
+        g, il, val = st['sel'].result()
+        st['pend'] = []
+        self.T['n_candidates'] += len(g)
         if len(g) == 0:
             return
-        gd = torch.from_numpy(g).cuda()
-        il_d = iw.view(-1)[gd]
+        tb, segs, nwsum, col0, row0, K = st['tb'], st['segs'], st['nwsum'], st['col0'], st['row0'], st['K']
+        n = len(tb.ids)
         m_ = (g // nwsum).astype(np.int64)
         c_ = g % nwsum
         si = np.searchsorted(col0, c_, side='right') - 1                # batch-local segment of each candidate
-        il = il_d.cpu().numpy().astype(np.int64)
-        tcol = torch.from_numpy((row0[si] + il).astype(np.int32)).cuda()
-        sidx = torch.from_numpy((si * n + m_).astype(np.int32)).cuda()
-        rows_d = torch.from_numpy(m_.astype(np.int32)).cuda()
+        il = il.astype(np.int64)
+        approx = np.sqrt(val)
+        rows_d = _dev.to_dev(m_.astype(np.int32))
+        tcol = _dev.to_dev((row0[si] + il).astype(np.int32))
+        sidx = _dev.to_dev((si * n + m_).astype(np.int32))
         zreb_d = torch.empty(len(g), dtype=torch.complex64, device='cuda')
-        _lib.call('gwgs_rebuilt_z', self.code, Zt.data_ptr(), K, Ar.data_ptr(), Ai.data_ptr(), K, K,
-                  rows_d.data_ptr(), tcol.data_ptr(), sidx.data_ptr(), len(g), rows2.data_ptr(), zreb_d.data_ptr(),
-                  _dev.stream())
-        approx = torch.sqrt(vw.view(-1)[gd]).cpu().numpy()
-        zreb = zreb_d.cpu().numpy()
-        t3 = self._sync(); self.T['rebuilt_z_s'] += t3 - t2
-        self.T['n_candidates'] += len(g)
+        _lib.call('gwgs_rebuilt_z', self.code, st['Zt'].data_ptr(), K, st['Ar'].data_ptr(), st['Ai'].data_ptr(), K,
+                  K, rows_d.data_ptr(), tcol.data_ptr(), sidx.data_ptr(), len(g), st['rows2'].data_ptr(),
+                  zreb_d.data_ptr(), _dev.stream())
+        st['zreb'] = _dev.Fetch(zreb_d)
+        o = self.opts
         for i, (di, s, gs, a0, nw) in enumerate(segs):
-            sel = si == i
-            if sel.any():
-                self._finish_segment(tb, self.data[di], s, gs, m_[sel], a0 + il[sel], approx[sel], zreb[sel],
-                                     norm_h, sig_h, outs[di])
-        t4 = self._sync(); self.T['refine_chisq_s'] += t4 - t3
+            sel = np.flatnonzero(si == i)
+            if not len(sel):
+                continue
+            m = self.data[di]
+            tb_, samp, ax = m_[sel], a0 + il[sel], approx[sel]
+            ex = ax >= o.exact_snr * (1 - o.margin)
+            pv = None
+            if ex.any():
+                tbx, sx = tb_[ex], samp[ex]
+                seg = SegmentData(m.D[s], m.IP[s], m.df, m.tlen)
+                if o.refine > 0:
+                    # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
+                    wlo = a0 + (sx - a0) // self.cw * self.cw
+                    pv = tb.stage.evaluate_async(seg, PeakRequest(tbx, sx), refine=o.refine, lo=wlo,
+                                                 hi=wlo + self.cw)
+                else:
+                    pv = tb.stage.evaluate_async(seg, PeakRequest(tbx, sx))
+            st['pend'].append((di, s, gs, sel, tb_, samp, ax, ex, pv))
+        # the batch's device buffers stay referenced by st until collect_batch
+        self.T['rebuilt_z_s'] += time.perf_counter() - t0
 
-    def _finish_segment(self, tb, m, s, gs, tb_, samp, approx, zreb, norm_h, sig_h, out: Triggers):
-        """fp_filter.py:768-853 for one segment's candidates: the rebuilt band keeps rebuilt values; the exact band
-        goes through the exact stage and the final threshold."""
+    def collect_batch(self, st, outs):
+        """Stage B2: wait for the batch's exact values and finalise its triggers (fp_filter.py:768-853)."""
+        t0 = time.perf_counter()
+        if not st['pend']:
+            return
+        zreb, = st['zreb'].get()
+        sig_h, norm_h = st['tile_host'].get()
+        tb = st['tb']
         o = self.opts
         thr = o.snr_threshold
-        ex = approx >= o.exact_snr * (1 - o.margin)
-        rb = ~ex & (approx > thr)
-        tstamp = lambda sm: m.gps_start + (m.seg_start[s] + sm) / m.sr
-        if rb.any():
-            nrb = int(rb.sum())
-            out.add(tid=tb.ids[tb_[rb]], t=tstamp(samp[rb]), snr=approx[rb], ph=np.angle(zreb[rb]).astype(np.float32),
-                    chi=np.zeros(nrb, np.float32), dof=np.zeros(nrb, np.int32), sg=sig_h[tb_[rb], gs],
-                    ex=np.zeros(nrb, bool), samp=samp[rb], seg=np.full(nrb, gs))
-            self.T['n_rebuilt_peaks'] += nrb
-        tbx, sx, ax = tb_[ex], samp[ex], approx[ex]
-        self.T['n_exact_candidates'] += len(tbx)
-        if not len(tbx):
-            return
-        if o.refine > 0:
-            # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
-            a0 = m.analyze[s][0]
-            wlo = a0 + (sx - a0) // self.cw * self.cw
-            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx), refine=o.refine,
-                                   lo=wlo, hi=wlo + self.cw)
-            sx = pv.samples
-        else:
-            pv = tb.stage.evaluate(SegmentData(m.D[s], m.IP[s], m.df, m.tlen), PeakRequest(tbx, sx))
-        nr = norm_h[tbx, gs]
-        zx = pv.z.astype(np.complex128)
-        snr = np.abs(zx) * nr
-        self.rel.append(ax / np.maximum(snr, 1e-30))
-        k = snr > thr
-        nb = tb.stage.tile.nbins[tbx]
-        chi = ((nb * pv.pw.astype(np.float64) - np.abs(zx) ** 2) * nr ** 2).astype(np.float32)
-        dof = (2 * nb - 2).astype(np.int32)
-        out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
-                ph=np.angle(pv.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
-                ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
-        self.T['n_chisq_points'] += int(k.sum())
+        for di, s, gs, sel, tb_, samp, ax, ex, pv in st['pend']:
+            m = self.data[di]
+            out = outs[di]
+            tstamp = lambda sm: m.gps_start + (m.seg_start[s] + sm) / m.sr
+            rb = ~ex & (ax > thr)
+            if rb.any():
+                nrb = int(rb.sum())
+                out.add(tid=tb.ids[tb_[rb]], t=tstamp(samp[rb]), snr=ax[rb],
+                        ph=np.angle(zreb[sel][rb]).astype(np.float32), chi=np.zeros(nrb, np.float32),
+                        dof=np.zeros(nrb, np.int32), sg=sig_h[tb_[rb], gs], ex=np.zeros(nrb, bool), samp=samp[rb],
+                        seg=np.full(nrb, gs))
+                self.T['n_rebuilt_peaks'] += nrb
+            tbx = tb_[ex]
+            self.T['n_exact_candidates'] += len(tbx)
+            if pv is None:
+                continue
+            v = pv.result()
+            sx = v.samples
+            nr = norm_h[tbx, gs]
+            zx = v.z.astype(np.complex128)
+            snr = np.abs(zx) * nr
+            self.rel.append(ax[ex] / np.maximum(snr, 1e-30))
+            k = snr > thr
+            nbin = tb.stage.tile.nbins[tbx]
+            chi = ((nbin * v.pw.astype(np.float64) - np.abs(zx) ** 2) * nr ** 2).astype(np.float32)
+            dof = (2 * nbin - 2).astype(np.int32)
+            out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
+                    ph=np.angle(v.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
+                    ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
+            self.T['n_chisq_points'] += int(k.sum())
+        self.T['refine_chisq_s'] += time.perf_counter() - t0

@@ -52,3 +52,37 @@ def select_above(vw, cut):
         if n <= cap:
             return out[:n].cpu().numpy().astype(np.int64)
         cap = n
+
+
+class Selection:
+    """Candidate list being compacted on the device (select_peaks); result() reads it back."""
+
+    def __init__(self, idx, samp, val, count, after):
+        self.idx, self.samp, self.val, self.after = idx, samp, val, after
+        self.count = _dev.Fetch(count, after=after)
+
+    def result(self):
+        """(flat index g, window sample iw[g], vw[g]) of every vw > cut, in increasing g, as numpy arrays. Waits only
+        for the selection's own work (and the copies), not for work enqueued after it."""
+        import numpy as np
+        n = int(self.count.get()[0][0])
+        if n == 0:
+            return np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32)
+        g, s, v = _dev.Fetch(self.idx[:n], self.samp[:n], self.val[:n], after=self.after).get()
+        return g.astype(np.int64), s, v
+
+
+def select_peaks(vw, iw, cut):
+    """fp_filter.py:747-757 on the device: entries of vw > cut (strict, row-major order) with their window samples
+    iw and values, compacted in one pass (capacity: every entry). Returns a Selection without waiting."""
+    total = vw.numel()
+    nwork = _lib.load().gwgs_select_workspace(total)
+    work = torch.empty(int(nwork) + 1, dtype=torch.int32, device=vw.device)
+    cnt = work[-1:]
+    cap = max(total, 1)
+    idx = torch.empty(cap, dtype=torch.int32, device=vw.device)
+    samp = torch.empty(cap, dtype=torch.int32, device=vw.device)
+    val = torch.empty(cap, dtype=torch.float32, device=vw.device)
+    _lib.call('gwgs_select_peaks', vw.data_ptr(), iw.data_ptr(), total, float(cut), idx.data_ptr(), samp.data_ptr(),
+              val.data_ptr(), cap, work.data_ptr(), cnt.data_ptr(), _dev.stream())
+    return Selection(idx, samp, val, cnt, _dev.mark())

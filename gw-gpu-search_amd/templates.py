@@ -24,17 +24,21 @@ def generate(m1, m2, e0, flen, df, flow, out=None):
     return H, km.cpu().numpy().astype(np.int64)
 
 
-def sigmasq(H, IPall, df):
-    """sig (n, nseg) float64 = 4 df |H|^2 @ IPall^T, accumulated in double; norm = 4 df / sqrt(sig)."""
+def sigmasq(H, IPall, df, kmin=0, kend=None):
+    """sig (n, nseg) float64 = 4 df |H|^2 @ IPall^T, accumulated in double over [kmin, kend) (default: all of H's
+    columns; H must be zero outside it); norm = 4 df / sqrt(sig)."""
     n = H.shape[0]
     nseg = IPall.shape[0]
+    kend = H.shape[1] if kend is None else int(kend)
+    lib = _lib.load()
     sig = torch.empty((n, nseg), dtype=torch.float64, device=H.device)
+    work = torch.empty(max(lib.gwgs_sigmasq_workspace(n, int(kmin), kend, min(nseg, 16)), 1), dtype=torch.float64,
+                       device=H.device)
     for s0 in range(0, nseg, 16):
         s1 = min(nseg, s0 + 16)
-        sub = torch.empty((n, s1 - s0), dtype=torch.float64, device=H.device)
-        _lib.call('gwgs_sigmasq', H.data_ptr(), H.stride(0), H.shape[1], n, IPall[s0].data_ptr(), IPall.stride(0),
-                  s1 - s0, float(df), sub.data_ptr(), sub.stride(0), _dev.stream())
-        sig[:, s0:s1] = sub
+        _lib.call('gwgs_sigmasq', H.data_ptr(), H.stride(0), int(kmin), kend, n, IPall[s0].data_ptr(),
+                  IPall.stride(0), s1 - s0, float(df), work.data_ptr(), sig[:, s0:].data_ptr(), sig.stride(0),
+                  _dev.stream())
     return sig, (4.0 * df) / torch.sqrt(sig)
 
 

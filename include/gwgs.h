@@ -59,6 +59,10 @@ int gwgs_block_window_max(const float* V, const int* I, long ldv, int n, int nw,
 long gwgs_select_workspace(long total);
 int gwgs_select_above(const float* vw, long total, float cut, int* out, int cap, int* work, int* count_dev,
                       void* stream);
+/* pipeline/fp_filter.py:747-757 in one pass: as gwgs_select_above, plus out_iw[i] = iw[out[i]] (the window's sample)
+ * and out_vw[i] = vw[out[i]] for the selected entries, so the candidate list needs no host round trip. */
+int gwgs_select_peaks(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw, float* out_vw,
+                      int cap, int* work, int* count_dev, void* stream);
 /* pipeline/fp_filter.py:759-764: rebuilt complex SNR at candidates, out[c] = (Ar[rows[c]] . Zt[tcol[c]],
  * Ai[rows[c]] . Zt[tcol[c]]) sqrt(pw_scale[sidx[c]]) in fp32 (complex64 out); sidx NULL indexes pw_scale by rows[c];
  * pw_scale NULL: no scale. dtype as gwgs_gemm_peaks. */
@@ -122,9 +126,11 @@ int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, lon
  * (e0 may be NULL); kmax: device int array (may be NULL). */
 int gwgs_spa_templates(const double* m1, const double* m2, const double* e0, int n, long flen, double df,
                        double flow, double f0, void* H, long ldh, int* kmax, void* stream);
-/* sig[i, s] = 4 df sum_k |H[i, k]|^2 IP[s, k] (double accumulation), s < nseg <= 16 */
-int gwgs_sigmasq(const void* H, long ldh, long flen, int n, const float* IP, long ldip, int nseg, double df,
-                 double* sig, long lds, void* stream);
+/* sig[i, s] = 4 df sum_{kmin <= k < kend} |H[i, k]|^2 IP[s, k] (double accumulation, chunk partials summed in a
+ * fixed order), s < nseg <= 16; work: gwgs_sigmasq_workspace(n, kmin, kend, nseg) doubles */
+long gwgs_sigmasq_workspace(int n, long kmin, long kend, int nseg);
+int gwgs_sigmasq(const void* H, long ldh, long kmin, long kend, int n, const float* IP, long ldip, int nseg,
+                 double df, double* work, double* sig, long lds, void* stream);
 
 /* ---- data conditioning (pipeline/gpu_prep.py) ------------------------------------------------------------------
  * welch (gpu_prep.py:87-125): segs[r, s, i] = x[r, start + s stride + i] * hann_L(i) (float64 out) */

@@ -99,6 +99,11 @@ def test_select_above_order(gpu):
         v[::97] = 0.5
         g = peak_kernel.select_above(torch.from_numpy(v).cuda(), 0.5)
         assert np.array_equal(g, np.flatnonzero(v > np.float32(0.5)))
+        # the one-pass candidate list: indices, their window samples and values
+        iw = rng.integers(0, 2048, total).astype(np.int32)
+        g2, s2, v2 = peak_kernel.select_peaks(torch.from_numpy(v).cuda(), torch.from_numpy(iw).cuda(), 0.5).result()
+        exp = np.flatnonzero(v > np.float32(0.5))
+        assert np.array_equal(g2, exp) and np.array_equal(s2, iw[exp]) and np.array_equal(v2, v[exp])
 
 
 def test_chisq_kernel_vs_float64_definition(gpu):
@@ -190,6 +195,12 @@ def test_sigmasq_vs_oracle(gpu):
     sr, nr = core.sigmasq(H, IP, 0.25)
     np.testing.assert_allclose(sig.cpu().numpy(), sr, rtol=1e-12)
     np.testing.assert_allclose(norm.cpu().numpy(), nr, rtol=1e-12)
+    # rows zero outside [kmin, kend): the range-limited sum over several chunks equals the full-row oracle
+    H2 = (rng.standard_normal((600, 40000)) + 1j * rng.standard_normal((600, 40000))).astype(np.complex64)
+    H2[:, :123] = 0; H2[:, 31000:] = 0
+    IP2 = rng.random((17, 40000)).astype(np.float32)
+    sig2, _ = templates.sigmasq(torch.from_numpy(H2).cuda(), torch.from_numpy(IP2).cuda(), 0.5, 123, 31000)
+    np.testing.assert_allclose(sig2.cpu().numpy(), core.sigmasq(H2, IP2, 0.5)[0], rtol=1e-12)
 
 
 def _fp8(x):
