@@ -163,15 +163,12 @@ def main():
     def step(prec, check=False):
         xs = [x.clone() for x in x_dev]
         ds = workloads.condition_streams(streams, cond, xs)
-        filters, outs = [], []
-        for i, plan in enumerate(plans):
-            o = opts[prec]
-            if check and i == 0:
-                o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
-            f = fp_filter.Filter(ds, o)
-            outs.append([t.to_dict(gates=d.gates) for t, d in zip(f.run(plan), ds)])
-            filters.append(f)
-        return ds, filters, outs
+        o = opts[prec]
+        if check:
+            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
+        f = fp_filter.Filter(ds, o)
+        outs = [[t.to_dict(gates=d.gates) for t, d in zip(tr, ds)] for tr in f.run_banks(plans)]
+        return ds, [f], outs
 
     log('warmup')
     for _ in range(a.warmup):

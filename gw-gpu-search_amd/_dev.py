@@ -62,6 +62,30 @@ def to_dev(a, dtype=None):
     return t.pin_memory().to(DEV, non_blocking=True)
 
 
+def to_dev_many(*arrays):
+    """Several host arrays -> device tensors with one pinned staging buffer and one copy on the current stream
+    (each array 16-byte aligned in the buffer). Returns the device tensors in order, with their shapes."""
+    arrs = [np.ascontiguousarray(a) for a in arrays]
+    offs, total = [], 0
+    for a in arrs:
+        offs.append(total)
+        total += (a.nbytes + 15) // 16 * 16
+    host = torch.empty(max(total, 16), dtype=torch.uint8, pin_memory=True)
+    hn = host.numpy()
+    for a, o in zip(arrs, offs):
+        hn[o:o + a.nbytes] = a.view(np.uint8).ravel()
+    dev = host.to(DEV, non_blocking=True)
+    out = []
+    for a, o in zip(arrs, offs):
+        t = dev[o:o + a.nbytes].view(_TORCH_DTYPE[a.dtype.str[1:]])
+        out.append(t.view(a.shape))
+    return out
+
+
+_TORCH_DTYPE = {'i4': torch.int32, 'i8': torch.int64, 'f4': torch.float32, 'f8': torch.float64,
+                'c8': torch.complex64, 'u1': torch.uint8}
+
+
 def mark():
     """An event at the current point of the current stream."""
     e = torch.cuda.Event()

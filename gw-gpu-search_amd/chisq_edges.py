@@ -26,7 +26,7 @@ def edges_rows_dev(rows_dev_idx, nb_, kmin, k1, Hd, ips):
         return out
     assert Hd.dtype == torch.complex64 and Hd.is_contiguous()
     ipd = _dev.as_dev(ips, torch.float32)
-    r, kk, nb = (_dev.to_dev(np.asarray(x, np.int32)) for x in (rows_dev_idx, k1, nb_))
+    r, kk, nb = _dev.to_dev_many(*(np.asarray(x, np.int32) for x in (rows_dev_idx, k1, nb_)))
     e0 = _dev.METER.start()
     _lib.call('gwgs_edges_fused', Hd.data_ptr(), Hd.shape[1], r.data_ptr(), ipd.data_ptr(), kk.data_ptr(),
               nb.data_ptr(), B + 1, int(kmin), n, out.data_ptr(), _dev.stream())

@@ -15,35 +15,43 @@ P = 8
 THREADS = 256
 
 
+def groups_of_sorted(rs):
+    """Groups of <= P consecutive peaks within each run of equal rows of a row-sorted peak list: (row, start,
+    count) per group."""
+    npk = len(rs)
+    if npk == 0:
+        z = np.zeros(0, np.int64)
+        return z, z, z
+    run0 = np.r_[0, np.flatnonzero(np.diff(rs)) + 1]
+    pos = np.arange(npk) - np.repeat(run0, np.diff(np.r_[run0, npk]))       # position inside the row's run
+    starts = np.flatnonzero(pos % P == 0)
+    counts = np.diff(np.r_[starts, npk])
+    return rs[starts], starts.astype(np.int64), counts.astype(np.int64)
+
+
 def plan_groups(rows, times):
     """chisq_kernel.py:96-107: sort peaks by (row, time), cut each row's run into groups of <= P. Returns the sort
     order and per-group (row, start, count) over the sorted peaks."""
     rows = np.asarray(rows, np.int64); times = np.asarray(times, np.int64)
     npk = len(rows)
     o = np.lexsort((times, rows))
-    rs = rows[o]
-    if npk == 0:
-        z = np.zeros(0, np.int64)
-        return o, z, z, z
-    run0 = np.r_[0, np.flatnonzero(np.diff(rs)) + 1]
-    pos = np.arange(npk) - np.repeat(run0, np.diff(np.r_[run0, npk]))       # position inside the row's run
-    starts = np.flatnonzero(pos % P == 0)
-    counts = np.diff(np.r_[starts, npk])
-    return o, rs[starts], starts.astype(np.int64), counts.astype(np.int64)
+    grow, gstart, gcount = groups_of_sorted(rows[o])
+    return o, grow, gstart, gcount
 
 
 def _launch(q, times_sorted, grow, gstart, gcount, gedges_dev, want_z, d, N, span):
     """Enqueue gwgs_chisq_points for planned groups; gedges_dev (ngroups, nedge) int32 on the device. Returns the
     device outputs in sorted peak order. span: summed bin span of the groups (for the kernel meter's bytes)."""
     npk = len(times_sorted)
-    tdev = _dev.to_dev(np.asarray(times_sorted, np.int64))
     out = torch.empty(npk, dtype=torch.float32, device=q.device)
     zo = torch.empty(npk, dtype=torch.complex64, device=q.device) if want_z else None
     dd = None
     if d is not None:
         dd = _dev.as_dev(d, torch.complex64)
-    g_row, g_start, g_cnt = (_dev.to_dev(np.asarray(x, np.int32)) for x in (grow, gstart, gcount))
-    ge = gedges_dev.contiguous()
+    tdev, g_row, g_start, g_cnt = _dev.to_dev_many(np.asarray(times_sorted, np.int64), *(np.asarray(x, np.int32)
+                                                                                         for x in (grow, gstart, gcount)))
+    ge = gedges_dev if isinstance(gedges_dev, torch.Tensor) else _dev.to_dev(gedges_dev)
+    ge = ge.contiguous()
     e0 = _dev.METER.start()
     _lib.call('gwgs_chisq_points', q.data_ptr(), q.shape[1], _dev.ptr(dd), int(N), g_row.data_ptr(),
               g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(), ge.shape[1], len(gstart),
@@ -53,18 +61,18 @@ def _launch(q, times_sorted, grow, gstart, gcount, gedges_dev, want_z, d, N, spa
     return out, zo
 
 
-def launch_sums(q, rows, times, E_dev, e_index, want_z=False, d=None, n_time=None, row_span=None):
-    """Enqueue the sums without waiting: edges come as device rows E_dev (nu, nedge) int32 with e_index[p] the row
-    of peak p. Returns (order, out, zo): out / zo in the order `order` of the peaks (device tensors). row_span: bins
-    from the first to the last edge per row of q (for the kernel meter's bytes; optional)."""
+def launch_sorted(q, rows, times, E_dev, e_index, want_z=False, d=None, n_time=None, row_span=None):
+    """Enqueue the sums without waiting, for peaks already sorted by row (any time order within a row): edges come as
+    device rows E_dev (nu, nedge) int32 with e_index[p] the row of peak p. Returns device (out, zo) in the given
+    peak order. row_span: bins from the first to the last edge per row of q (for the kernel meter; optional)."""
     N = q.shape[1] if d is None else n_time
     assert N is not None and N > 0
     assert q.dtype == torch.complex64 and q.is_contiguous()
-    o, grow, gstart, gcount = plan_groups(rows, times)
-    gi = _dev.to_dev(np.asarray(e_index, np.int64)[o][gstart])
+    rows = np.asarray(rows, np.int64)
+    grow, gstart, gcount = groups_of_sorted(rows)
+    gi = _dev.to_dev(np.asarray(e_index, np.int64)[gstart])
     span = 0.0 if row_span is None else float(np.asarray(row_span)[grow].sum())
-    out, zo = _launch(q, np.asarray(times, np.int64)[o], grow, gstart, gcount, E_dev[gi], want_z, d, N, span)
-    return o, out, zo
+    return _launch(q, np.asarray(times, np.int64), grow, gstart, gcount, E_dev[gi], want_z, d, N, span)
 
 
 def bin_power_sums(q, rows, times, edges_per_peak, want_z=False, d=None, n_time=None):

@@ -54,6 +54,27 @@ __device__ __forceinline__ i32x8 join8(const uint4 a, const uint4 b)
     return r;
 }
 
+// The imaginary part without a second template operand: with Ar[m, 2j] = Re As, Ar[m, 2j+1] = -Im As,
+//     z_im[m, t] = sum_j Im As Re z_j + Re As Im z_j = sum_k Ar[m, k] Zr[t, k],  Zr[t, 2j] = Im z_j, Zr[t, 2j+1] = -Re z_j,
+// so each (Re, Im) pair of a Zt fragment is swapped and its new second element negated in registers: Ai is never
+// loaded, halving the template traffic of the two real GEMMs.
+__device__ __forceinline__ uint32_t rot_bf16x2(const uint32_t w)
+{
+    return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x80000000u;     // (re, im) -> (im, -re)
+}
+__device__ __forceinline__ uint4 rot_bf16(const uint4 v)
+{
+    return make_uint4(rot_bf16x2(v.x), rot_bf16x2(v.y), rot_bf16x2(v.z), rot_bf16x2(v.w));
+}
+__device__ __forceinline__ uint32_t rot_fp8x4(const uint32_t w)
+{
+    return (((w >> 8) & 0x00ff00ffu) | ((w << 8) & 0xff00ff00u)) ^ 0x80008000u;   // bytes (re, im) -> (im, -re)
+}
+__device__ __forceinline__ uint4 rot_fp8(const uint4 v)
+{
+    return make_uint4(rot_fp8x4(v.x), rot_fp8x4(v.y), rot_fp8x4(v.z), rot_fp8x4(v.w));
+}
+
 struct Job {
     const GemmItem* items;   // null: one segment at row 0, items derived from tpb, ntile, Tn
     int nitems, nmb, tpb, ntile, Tn, M, per;
@@ -285,7 +306,7 @@ template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8
 template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4; };
 template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16; };
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool ROT_>
 __global__ void __launch_bounds__(NT, 2)
 gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
                    const long lda, const int K, const Job J, float* __restrict__ V, int* __restrict__ I,
@@ -296,9 +317,10 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
     constexpr int CPR = KC / EPV;                  // 16-byte chunks per row of a k chunk
     constexpr int CPT = (BT * CPR) / NT;           // per thread, per operand tile of 128 rows
     static_assert((BT * CPR) % NT == 0, "tile shape");
+    constexpr bool ROT = ROT_ && sizeof(T) <= 2;   // imaginary part from rotated Zt fragments (no Ai)
     __shared__ __attribute__((aligned(16))) T zs[BT * PITCH];
     __shared__ __attribute__((aligned(16))) T as_r[BMT * PITCH];
-    __shared__ __attribute__((aligned(16))) T as_i[BMT * PITCH];
+    __shared__ __attribute__((aligned(16))) T as_i[ROT ? 16 : BMT * PITCH];
 
     int mb;
     GemmItem it;
@@ -322,7 +344,8 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
                         : make_uint4(0, 0, 0, 0);
             const bool ain = kin && (m0 + row) < M;
             rr[j] = ain ? *(const uint4*)(Ar + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
-            ri[j] = ain ? *(const uint4*)(Ai + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
+            if constexpr (!ROT)
+                ri[j] = ain ? *(const uint4*)(Ai + (size_t)(m0 + row) * lda + cc) : make_uint4(0, 0, 0, 0);
         }
     };
     auto sstore = [&]() {
@@ -331,7 +354,7 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
             const int c = tid + j * NT, row = c / CPR, cc = (c % CPR) * EPV;
             *(uint4*)(&zs[row * PITCH + cc]) = rz[j];
             *(uint4*)(&as_r[row * PITCH + cc]) = rr[j];
-            *(uint4*)(&as_i[row * PITCH + cc]) = ri[j];
+            if constexpr (!ROT) *(uint4*)(&as_i[row * PITCH + cc]) = ri[j];
         }
     };
 
@@ -358,27 +381,42 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
 #pragma unroll
             for (int ks = 0; ks < KC / 64; ks++) {
                 const T* pa = &as_r[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
-                const T* pi = &as_i[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
                 const i32x8 a_r = join8(*(const uint4*)pa, *(const uint4*)(pa + 16));
-                const i32x8 a_i = join8(*(const uint4*)pi, *(const uint4*)(pi + 16));
+                i32x8 a_i = a_r;
+                if constexpr (!ROT) {
+                    const T* pi = &as_i[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
+                    a_i = join8(*(const uint4*)pi, *(const uint4*)(pi + 16));
+                }
 #pragma unroll
                 for (int ts = 0; ts < 4; ts++) {
                     const T* pz = &zs[(ts * 32 + col) * PITCH + ks * 64 + half * 32];
-                    const i32x8 z = join8(*(const uint4*)pz, *(const uint4*)(pz + 16));
+                    const uint4 z0 = *(const uint4*)pz, z1 = *(const uint4*)(pz + 16);
+                    const i32x8 z = join8(z0, z1);
                     cr[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(z, a_r, cr[ts], 0, 0, 0, 127, 0, 127);
-                    ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(z, a_i, ci[ts], 0, 0, 0, 127, 0, 127);
+                    if constexpr (ROT)
+                        ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(join8(rot_fp8(z0), rot_fp8(z1)), a_r,
+                                                                                ci[ts], 0, 0, 0, 127, 0, 127);
+                    else
+                        ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(z, a_i, ci[ts], 0, 0, 0, 127, 0, 127);
                 }
             }
         } else if constexpr (sizeof(T) == 2) {
 #pragma unroll
             for (int ks = 0; ks < KC / 16; ks++) {
                 const bf16x8 a_r = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_r[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
-                const bf16x8 a_i = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_i[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
+                bf16x8 a_i = a_r;
+                if constexpr (!ROT)
+                    a_i = __builtin_bit_cast(bf16x8, *(const uint4*)(&as_i[(wave * 32 + col) * PITCH + ks * 16 + half * 8]));
 #pragma unroll
                 for (int ts = 0; ts < 4; ts++) {
-                    const bf16x8 z = __builtin_bit_cast(bf16x8, *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]));
+                    const uint4 zv = *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]);
+                    const bf16x8 z = __builtin_bit_cast(bf16x8, zv);
                     cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_r, cr[ts], 0, 0, 0);
-                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
+                    if constexpr (ROT)
+                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
+                                                                         ci[ts], 0, 0, 0);
+                    else
+                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
                 }
             }
         } else {
@@ -391,6 +429,126 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
                     const float z = zs[(ts * 32 + col) * PITCH + 2 * ks + half];
                     cr[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, a_r, cr[ts], 0, 0, 0);
                     ci[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, a_i, ci[ts], 0, 0, 0);
+                }
+            }
+        }
+        if (kcix == nkc - 1) {
+            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+        }
+    }
+}
+
+// bf16 / fp8 K loop with only the Zt chunk staged through LDS (all 4 waves read it) and each wave's Ar fragments
+// loaded straight into registers in the MFMA operand layout (no other wave reads them), one step ahead (DB) or at
+// the start of the step; the imaginary part comes from rotated Zt fragments (complex-stacked operands only).
+template <typename T, int MODE, bool DB>
+__global__ void __launch_bounds__(NT, 2)
+gemm_peaks_kloop_ra_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
+                      const long lda, const int K, const Job J, float* __restrict__ V, int* __restrict__ I,
+                      const long ldv)
+{
+    static_assert(sizeof(T) <= 2, "bf16 or fp8 operands");
+    using TR = KloopTraits<T>;
+    constexpr int KC = TR::KC, EPV = TR::EPV, PITCH = KC + TR::PAD;
+    constexpr int CPR = KC / EPV;
+    constexpr int CPT = (BT * CPR) / NT;
+    constexpr int NA = sizeof(T) == 2 ? KC / 16 : 2 * (KC / 64);   // 16-byte fragments per operand per chunk
+    static_assert((BT * CPR) % NT == 0, "tile shape");
+    __shared__ __attribute__((aligned(16))) T zs[BT * PITCH];
+
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m0 = mb * BMT;
+    const int m = m0 + wave * 32 + col;
+    const int M = J.M;
+    const int nkc = (K + KC - 1) / KC;
+    const int nsteps = (it.t_end - it.t_first) * nkc;
+    const T* __restrict__ arow = Ar + (size_t)(m < M ? m : 0) * lda;
+
+    uint4 rz[CPT];
+    auto gload = [&](const int step) {
+        const int tt = it.t_first + step / nkc, kc = (step % nkc) * KC;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = kc + (c % CPR) * EPV;
+            rz[j] = cc < K ? *(const uint4*)(Zt + ((size_t)it.zrow0 + (size_t)tt * BT + row) * ldz + cc)
+                           : make_uint4(0, 0, 0, 0);
+        }
+    };
+    // fragment q of the chunk: bf16 k-step q (8 elements at q 16 + half 8); fp8 k-step q / 2, 16-byte part q % 2
+    // (32 bytes at (q / 2) 64 + half 32); K is a multiple of the fragment width, so a fragment is all in or all out
+    auto aload = [&](const int step, uint4 (&dr)[NA]) {
+        const int kc = (step % nkc) * KC;
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int k = sizeof(T) == 2 ? kc + q * 16 + half * 8 : kc + (q >> 1) * 64 + half * 32 + (q & 1) * 16;
+            dr[q] = m < M && k < K ? *(const uint4*)(arow + k) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = (c % CPR) * EPV;
+            *(uint4*)(&zs[row * PITCH + cc]) = rz[j];
+        }
+    };
+
+    f32x16 cr[4], ci[4];
+    uint4 fr[NA], nr[NA];
+    float best = -1.f;
+    int bi = it.t_first * BT;
+    gload(0);
+    if (DB) aload(0, nr);
+    for (int step = 0; step < nsteps; step++) {
+        const int kcix = step % nkc, tt = it.t_first + step / nkc;
+        if (kcix == 0) {
+#pragma unroll
+            for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+            }
+        }
+        if (!DB) aload(step, fr);
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (DB) {
+#pragma unroll
+            for (int q = 0; q < NA; q++) fr[q] = nr[q];
+        }
+        if (step + 1 < nsteps) {
+            gload(step + 1);
+            if (DB) aload(step + 1, nr);
+        }
+        if constexpr (sizeof(T) == 1) {
+#pragma unroll
+            for (int ks = 0; ks < KC / 64; ks++) {
+                const i32x8 a_r = join8(fr[2 * ks], fr[2 * ks + 1]);
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const T* pz = &zs[(ts * 32 + col) * PITCH + ks * 64 + half * 32];
+                    const uint4 z0 = *(const uint4*)pz, z1 = *(const uint4*)(pz + 16);
+                    cr[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(join8(z0, z1), a_r, cr[ts], 0, 0, 0, 127,
+                                                                            0, 127);
+                    ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(join8(rot_fp8(z0), rot_fp8(z1)), a_r,
+                                                                            ci[ts], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KC / 16; ks++) {
+                const bf16x8 a_r = __builtin_bit_cast(bf16x8, fr[ks]);
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const uint4 zv = *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]);
+                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, zv), a_r, cr[ts], 0, 0,
+                                                                     0);
+                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
+                                                                     ci[ts], 0, 0, 0);
                 }
             }
         }
@@ -435,36 +593,66 @@ resident_fn<float> f32_kernel(int K, int mode)
     }
 }
 
+template <typename T, bool ROT>
+void launch_kloop(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, const T* Ai,
+                  long lda, int K, const Job& J, float* V, int* I, long ldv)
+{
+    hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<T, 1, ROT> : gemm_peaks_kloop_k<T, 0, ROT>), grid, block, 0, s, Zt,
+                       ldz, Ar, Ai, lda, K, J, V, I, ldv);
+}
+
+template <typename T, bool DB>
+void launch_ra(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, const T* Ai,
+               long lda, int K, const Job& J, float* V, int* I, long ldv)
+{
+    hipLaunchKernelGGL((mode ? gemm_peaks_kloop_ra_k<T, 1, DB> : gemm_peaks_kloop_ra_k<T, 0, DB>), grid, block, 0, s,
+                       Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+}
+
+// bf16 / fp8 K loops: variant 2 (or operands without the complex stacking) LDS-staged, 3 register fragments at the
+// step start, else register fragments one chunk ahead (rot) / LDS-staged
+template <typename T>
+void launch_kloop_any(int variant, bool rot, int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz,
+                      const T* Ar, const T* Ai, long lda, int K, const Job& J, float* V, int* I, long ldv)
+{
+    if (!rot) launch_kloop<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    else if (variant == 2) launch_kloop<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    else if (variant == 3) launch_ra<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    else if (variant == 1) launch_ra<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    else launch_kloop<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+}
+
 int launch(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda, int K, int mode,
            const Job& J, float* V, int* I, long ldv, hipStream_t s)
 {
+    const bool rot = (mode & 4) != 0;
+    mode &= 3;
     const long nblocks = (long)J.nitems * J.nmb;
     GWGS_CHECK(nblocks < (1L << 30), GWGS_ESIZE);
     if (nblocks == 0) return GWGS_OK;
     const dim3 grid((unsigned)((nblocks + 7) / 8 * 8)), block(NT);
     const bool aligned = ((uintptr_t)Zt & 15) == 0 && ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0;
-    const bool kloop = variant == 1 || dtype == 2 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
+    const bool kloop = variant >= 1 || dtype == 2 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
     if (dtype == 1) {
         GWGS_CHECK(aligned && ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0, GWGS_EINVAL);
         if (kloop)
-            hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<uint16_t, 1> : gemm_peaks_kloop_k<uint16_t, 0>), grid, block,
-                               0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar, (const uint16_t*)Ai, lda, K, J, V,
-                               I, ldv);
+            launch_kloop_any<uint16_t>(variant, rot, mode, grid, block, s, (const uint16_t*)Zt, ldz,
+                                       (const uint16_t*)Ar, (const uint16_t*)Ai, lda, K, J, V, I, ldv);
         else
             hipLaunchKernelGGL(bf16_kernel(K, mode), grid, block, 0, s, (const uint16_t*)Zt, ldz, (const uint16_t*)Ar,
                                (const uint16_t*)Ai, lda, J, V, I, ldv);
     } else if (dtype == 0) {
         GWGS_CHECK(aligned && ldz % 4 == 0 && lda % 4 == 0 && K % 4 == 0, GWGS_EINVAL);
         if (kloop)
-            hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<float, 1> : gemm_peaks_kloop_k<float, 0>), grid, block, 0, s,
-                               (const float*)Zt, ldz, (const float*)Ar, (const float*)Ai, lda, K, J, V, I, ldv);
+            launch_kloop<float, false>(mode, grid, block, s, (const float*)Zt, ldz, (const float*)Ar, (const float*)Ai,
+                                       lda, K, J, V, I, ldv);
         else
             hipLaunchKernelGGL(f32_kernel(K, mode), grid, block, 0, s, (const float*)Zt, ldz, (const float*)Ar,
                                (const float*)Ai, lda, J, V, I, ldv);
     } else if (dtype == 2) {
         GWGS_CHECK(aligned && ldz % 16 == 0 && lda % 16 == 0 && K % 16 == 0, GWGS_EINVAL);
-        hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<uint8_t, 1> : gemm_peaks_kloop_k<uint8_t, 0>), grid, block, 0, s,
-                           (const uint8_t*)Zt, ldz, (const uint8_t*)Ar, (const uint8_t*)Ai, lda, K, J, V, I, ldv);
+        launch_kloop_any<uint8_t>(variant, rot, mode, grid, block, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
+                                  (const uint8_t*)Ai, lda, K, J, V, I, ldv);
     } else {
         return GWGS_EUNSUP;
     }
@@ -481,18 +669,18 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
                              const float* rowscale, void* stream)
 {
     GWGS_CHECK(Zt && Ar && Ai && V && I, GWGS_EINVAL);
-    GWGS_CHECK(M > 0 && Tn > 0 && K > 0 && (mode == 0 || mode == 1), GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && Tn > 0 && K > 0 && ((mode & 3) == 0 || (mode & 3) == 1) && (mode & ~7) == 0, GWGS_EINVAL);
     GWGS_CHECK(ldz >= K && lda >= K && ldv > 0, GWGS_EINVAL);
     const int ntile = (Tn + BT - 1) / BT;
-    if (mode == 1) {
+    if ((mode & 3) == 1) {
         GWGS_CHECK(per > 0 && Tn % (per * BT) == 0, GWGS_EINVAL);
         GWGS_CHECK(ldv >= Tn / (per * BT), GWGS_EINVAL);
     } else {
         per = 1;
         GWGS_CHECK(ldv >= ntile, GWGS_EINVAL);
     }
-    int tpb = tiles_per_block > 0 ? tiles_per_block : (mode == 1 ? per : 16);
-    if (mode == 1 && tpb % per) tpb = (tpb / per + 1) * per;
+    int tpb = tiles_per_block > 0 ? tiles_per_block : ((mode & 3) == 1 ? per : 16);
+    if ((mode & 3) == 1 && tpb % per) tpb = (tpb / per + 1) * per;
     Job J;
     J.items = nullptr; J.nmb = (M + BMT - 1) / BMT; J.tpb = tpb; J.ntile = ntile; J.Tn = Tn; J.M = M; J.per = per;
     J.nitems = (ntile + tpb - 1) / tpb; J.rowscale = rowscale;
@@ -506,11 +694,12 @@ GWGS_API int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long 
                                    int* I, long ldv, const float* rowscale, void* stream)
 {
     GWGS_CHECK(Zt && Ar && Ai && V && I && items && nitems >= 0, GWGS_EINVAL);
-    GWGS_CHECK(M > 0 && K > 0 && (mode == 0 || mode == 1) && ldz >= K && lda >= K && ldv > 0, GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && K > 0 && ((mode & 3) == 0 || (mode & 3) == 1) && (mode & ~7) == 0 && ldz >= K && lda >= K &&
+               ldv > 0, GWGS_EINVAL);
     GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
     Job J;
     J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
-    J.Tn = 0; J.M = M; J.per = mode == 1 ? per : 1; J.rowscale = rowscale;
+    J.Tn = 0; J.M = M; J.per = (mode & 3) == 1 ? per : 1; J.rowscale = rowscale;
     GWGS_CHECK(J.per > 0, GWGS_EINVAL);
     return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, mode, J, V, I, ldv, (hipStream_t)stream);
 }

@@ -121,11 +121,17 @@ class ExactStage:
         rows, samples = req.rows, req.samples
         npk = len(rows)
         if npk == 0:
-            return PendingValues(self, seg, rows, samples, 1, None, None, None)
+            return PendingValues(self, seg, rows, samples[:, None], None, None, None)
         n = self.tile.H.shape[0]
         assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg.tlen
-        ur, inv = np.unique(rows, return_inverse=True)
         t = self.tile
+        # peaks sorted by row (stable): distinct rows, each peak's edge row, and row-sorted groups for the sums
+        ob = np.argsort(rows, kind='stable')
+        rs = rows[ob]
+        first = np.r_[True, rs[1:] != rs[:-1]]
+        ur = rs[first]
+        inv = np.empty(npk, np.int64)
+        inv[ob] = np.cumsum(first) - 1
         E_dev = edges_rows_dev(ur, self._nbe(ur), t.kmin, t.kmax[ur], t.H, seg.IP)
         zero = np.flatnonzero(t.nbins[ur] == 0)
         if len(zero):
@@ -135,21 +141,21 @@ class ExactStage:
             hi = np.full(npk, seg.tlen, np.int64) if hi is None else np.asarray(hi, np.int64)
             off = np.arange(-refine, refine + 1)
             cand = np.clip(samples[:, None] + off[None, :], lo[:, None], hi[:, None] - 1)   # repeats at the bounds,
-            noff = len(off)                                                                # removed by argmax order
-            rr, ei, times = np.repeat(rows, noff), np.repeat(inv, noff), cand.ravel()
-        else:
-            cand, noff, rr, ei, times = samples[:, None], 1, rows, inv, samples
-        o, out, zo = chisq_kernel.launch_sums(t.H, rr, times, E_dev, ei, want_z=True, d=seg.D, n_time=seg.tlen,
-                                              row_span=np.maximum(t.kmax - t.kmin, 0))
-        return PendingValues(self, seg, rows, cand, noff, o, _dev.Fetch(out, zo, E_dev), inv)
+        else:                                                                              # removed by argmax order
+            cand = samples[:, None]
+        noff = cand.shape[1]
+        out, zo = chisq_kernel.launch_sorted(t.H, np.repeat(rs, noff), cand[ob].ravel(), E_dev,
+                                             np.repeat(inv[ob], noff), want_z=True, d=seg.D, n_time=seg.tlen,
+                                             row_span=np.maximum(t.kmax - t.kmin, 0))
+        return PendingValues(self, seg, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
 
 
 class PendingValues:
     """Exact values of one evaluate_async call; result() waits for the device work and builds the PeakValues (the
     refine choice, fpeak from the edges, and the first-call self-check)."""
 
-    def __init__(self, stage, seg, rows, cand, noff, order, fetch, inv):
-        self.stage, self.seg, self.rows, self.cand, self.noff = stage, seg, rows, cand, noff
+    def __init__(self, stage, seg, rows, cand, order, fetch, inv):
+        self.stage, self.seg, self.rows, self.cand = stage, seg, rows, cand
         self.order, self.fetch, self.inv = order, fetch, inv
 
     def result(self) -> PeakValues:
@@ -159,15 +165,15 @@ class PendingValues:
             return PeakValues(np.zeros(0, np.complex64), np.zeros(0, np.float32), np.zeros(0), np.zeros(0, np.int64))
         st = self.stage
         ho, hz, Eu = self.fetch.get()
-        nall = npk * self.noff
-        pw_all = np.empty(nall, np.float32); pw_all[self.order] = ho
-        z_all = np.empty(nall, np.complex64); z_all[self.order] = hz
-        if self.noff > 1:
-            mag = np.abs(z_all.astype(np.complex128)).reshape(npk, self.noff)
-            pick = np.arange(npk) * self.noff + np.argmax(mag, axis=1)   # first index: the earliest sample on ties
-            samples, pw, z = self.cand.ravel()[pick], pw_all[pick], z_all[pick]
+        noff = self.cand.shape[1]
+        pw_all = np.empty((npk, noff), np.float32); pw_all[self.order] = ho.reshape(npk, noff)
+        z_all = np.empty((npk, noff), np.complex64); z_all[self.order] = hz.reshape(npk, noff)
+        if noff > 1:
+            j = np.argmax(np.abs(z_all.astype(np.complex128)), axis=1)  # first index: the earliest sample on ties
+            r = np.arange(npk)
+            samples, pw, z = self.cand[r, j], pw_all[r, j], z_all[r, j]
         else:
-            samples, pw, z = self.cand.ravel(), pw_all, z_all
+            samples, pw, z = self.cand[:, 0], pw_all[:, 0], z_all[:, 0]
         E = Eu.astype(np.int64)[self.inv]
         out = PeakValues(z, pw, _fpeak(E, st.tile.nbins[rows], self.seg.df), samples)
         if st.self_check and st.check is None:

@@ -58,6 +58,7 @@ class Options:
     self_check: bool = True                # the exact stage's first-call check
     timing: bool = False                   # synchronize around stages to fill the timing dict
     segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
+    depth: int = 2                         # batches enqueued ahead of the one being decoded
     refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
                                            # samples inside its window (0: the reference's rebuilt-argmax sample)
 
@@ -246,6 +247,11 @@ class Filter:
             self._q = torch.zeros((nb, rp, self.tlen), dtype=torch.complex64, device='cuda')
         return self._q[:nb, :rp]
 
+    def _ones(self, n):
+        if getattr(self, '_ones_buf', None) is None or self._ones_buf.numel() < n:
+            self._ones_buf = torch.ones(max(n, 1024), dtype=torch.float32, device='cuda')
+        return self._ones_buf[:n]
+
     def _workbuf(self, nbytes):
         """Workspace of the fused basis transform (the column-pass output), grown on demand and reused."""
         if self._work is None or self._work.numel() < nbytes:
@@ -303,13 +309,18 @@ class Filter:
         return Zt, acs
 
     def run(self, plan: BankPlan):
-        """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset.
+        """Filter every tile of the plan against every segment of every dataset. Returns one Triggers per dataset."""
+        return self.run_banks([plan])[0]
 
-        Software pipeline on one stream: batch i + 1's transforms, GEMM and candidate selection are enqueued before
-        batch i's candidates are read back, and batch i's exact stage is enqueued before batch i - 1's results are
-        collected, so the GPU always has queued work while the host decodes and finalises. Copies to the host run on
-        a side stream after the events they depend on. With opts.timing every stage is synchronised instead."""
-        outs = [Triggers() for _ in self.data]
+    def run_banks(self, plans):
+        """Every tile of every plan (bank) against every segment; returns, per plan, one Triggers per dataset.
+
+        Software pipeline on one stream: the transforms, GEMM and candidate selection of the next `depth` batches
+        are enqueued before a batch's candidates are read back, and a batch's exact stage is enqueued before the
+        previous batch's results are collected, so the GPU has queued work while the host decodes and finalises,
+        across tiles and banks. Copies to the host run on a side stream after the events they depend on. With
+        opts.timing every stage is synchronised instead."""
+        outs = [[Triggers() for _ in self.data] for _ in plans]
         segs = []
         for di, m in enumerate(self.data):
             for s, (a0, a1) in enumerate(m.analyze):
@@ -317,31 +328,31 @@ class Filter:
                 if nw > 0:
                     segs.append((di, s, self.seg0[di] + s, a0, nw))
         B = max(1, self.opts.segment_batch)
-        prev, fin = None, deque()
-        for tb in plan.tiles:
-            t0 = self._sync()
-            e0 = _dev.METER.start()
-            sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
-            _dev.METER.stop('sigmasq', e0, 8 * len(tb.ids) * (tb.kend - self.kmin)
-                            + 4 * self.IPall.shape[0] * (tb.kend - self.kmin))
-            tile_host = _dev.Fetch(sig, norm)
-            t1 = self._sync(); self.T['norm_s'] += t1 - t0
-            for b0 in range(0, len(segs), B):
-                cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host)
-                if self.opts.timing:
-                    self.decode_batch(cur); self.collect_batch(cur, outs)
-                    continue
-                if prev is not None:
-                    self.decode_batch(prev)
-                    fin.append(prev)
-                while len(fin) > 1:
-                    self.collect_batch(fin.popleft(), outs)
-                prev = cur
-        if prev is not None:
-            self.decode_batch(prev)
-            fin.append(prev)
+        launched, fin = deque(), deque()
+        for pi, plan in enumerate(plans):
+            for tb in plan.tiles:
+                t0 = self._sync()
+                e0 = _dev.METER.start()
+                sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
+                _dev.METER.stop('sigmasq', e0, 8 * len(tb.ids) * (tb.kend - self.kmin)
+                                + 4 * self.IPall.shape[0] * (tb.kend - self.kmin))
+                tile_host = _dev.Fetch(sig, norm)
+                t1 = self._sync(); self.T['norm_s'] += t1 - t0
+                for b0 in range(0, len(segs), B):
+                    cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host)
+                    cur['outs'] = outs[pi]
+                    if self.opts.timing:
+                        self.decode_batch(cur); self.collect_batch(cur)
+                        continue
+                    launched.append(cur)
+                    while len(launched) > self.opts.depth:
+                        fin.append(self.decode_batch(launched.popleft()))
+                    while len(fin) > 1:
+                        self.collect_batch(fin.popleft())
+        while launched:
+            fin.append(self.decode_batch(launched.popleft()))
         while fin:
-            self.collect_batch(fin.popleft(), outs)
+            self.collect_batch(fin.popleft())
         return outs
 
     def launch_batch(self, tb: TileBasis, segs, norm, tile_host):
@@ -360,8 +371,8 @@ class Filter:
         Zt, acs = self.basis_operand(tb, segs, Tns, row0)
         t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
         Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
-        ones = torch.ones(n, dtype=torch.float32, device='cuda')
-        gsl = [gs for (_, _, gs, _, _) in segs]
+        ones = self._ones(n)
+        gsl = _dev.to_dev(np.array([gs for (_, _, gs, _, _) in segs], np.int64))
         rows2 = (norm[:, gsl] ** 2).T.to(torch.float32)               # (nb, n): norm^2 per (segment, template)
         if self.code == 2:
             pws = torch.empty(n, dtype=torch.float32, device='cuda')
@@ -381,7 +392,8 @@ class Filter:
         vw = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')
         iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
         e0 = _dev.METER.start()
-        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, items, 1, per, o.gemm_cfg, vw, iw, rows2)
+        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, items, 1 | fp_gemm.STACKED, per, o.gemm_cfg, vw,
+                                 iw, rows2)
         # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
         _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(Tns)))
         cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
@@ -398,7 +410,7 @@ class Filter:
         st['pend'] = []
         self.T['n_candidates'] += len(g)
         if len(g) == 0:
-            return
+            return st
         tb, segs, nwsum, col0, row0, K = st['tb'], st['segs'], st['nwsum'], st['col0'], st['row0'], st['K']
         n = len(tb.ids)
         m_ = (g // nwsum).astype(np.int64)
@@ -406,9 +418,8 @@ class Filter:
         si = np.searchsorted(col0, c_, side='right') - 1                # batch-local segment of each candidate
         il = il.astype(np.int64)
         approx = np.sqrt(val)
-        rows_d = _dev.to_dev(m_.astype(np.int32))
-        tcol = _dev.to_dev((row0[si] + il).astype(np.int32))
-        sidx = _dev.to_dev((si * n + m_).astype(np.int32))
+        rows_d, tcol, sidx = _dev.to_dev_many(m_.astype(np.int32), (row0[si] + il).astype(np.int32),
+                                              (si * n + m_).astype(np.int32))
         zreb_d = torch.empty(len(g), dtype=torch.complex64, device='cuda')
         _lib.call('gwgs_rebuilt_z', self.code, st['Zt'].data_ptr(), K, st['Ar'].data_ptr(), st['Ai'].data_ptr(), K,
                   K, rows_d.data_ptr(), tcol.data_ptr(), sidx.data_ptr(), len(g), st['rows2'].data_ptr(),
@@ -436,12 +447,14 @@ class Filter:
             st['pend'].append((di, s, gs, sel, tb_, samp, ax, ex, pv))
         # the batch's device buffers stay referenced by st until collect_batch
         self.T['rebuilt_z_s'] += time.perf_counter() - t0
+        return st
 
-    def collect_batch(self, st, outs):
+    def collect_batch(self, st):
         """Stage B2: wait for the batch's exact values and finalise its triggers (fp_filter.py:768-853)."""
         t0 = time.perf_counter()
         if not st['pend']:
             return
+        outs = st['outs']
         zreb, = st['zreb'].get()
         sig_h, norm_h = st['tile_host'].get()
         tb = st['tb']

@@ -58,8 +58,10 @@ def _prep_operands(Are, Aim, Z, Tn, zt):
 
 
 def _cfg(cfg):
-    """cfg: None, an int (time tiles per block) or a tuple (tiles_per_block[, variant]); variant 1 forces the
-    K-loop kernel, 0 picks by K."""
+    """cfg: None, an int (time tiles per block) or a tuple (tiles_per_block[, variant]). variant 0 picks by K
+    (register-resident templates up to K = 256 bf16 / 128 fp32, else the K loop); 1 forces the K loop (bf16 / fp8:
+    template fragments loaded into registers one chunk ahead); 2 the K loop with both operands staged through LDS;
+    3 the register-fragment K loop without the one-chunk-ahead buffer."""
     if cfg is None:
         return 0, 0
     if isinstance(cfg, (tuple, list)):
@@ -76,6 +78,10 @@ def gemm_peaks_raw(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, Tn, mode, per, cfg, V
               Ai.data_ptr(), lda, M, K, Tn, mode, per, tpb, V.data_ptr(), I.data_ptr(), V.stride(0),
               _dev.ptr(rowscale), _dev.stream())
 
+
+# mode flag: the operands are complex-stacked (Ai[m, 2j] = -Ar[m, 2j+1], Ai[m, 2j+1] = Ar[m, 2j], as stack_a
+# writes them), so the bf16 / fp8 K loops form the imaginary part from rotated Zt fragments and never read Ai
+STACKED = 4
 
 ITEM = np.dtype([('zrow0', '<i8'), ('t_first', '<i4'), ('t_end', '<i4'), ('col0', '<i4'), ('seg', '<i4'),
                  ('Tn', '<i4'), ('pad', '<i4')])
@@ -96,7 +102,7 @@ def items_table(segments, tpb):
         for t in range(0, ntile, tpb):
             rows.append((zrow0, t, min(t + tpb, ntile), col0, seg, Tn, 0))
     tab = np.array(rows, dtype=ITEM)
-    return torch.from_numpy(tab.view(np.uint8)).cuda(), len(rows)
+    return _dev.to_dev(tab.view(np.uint8)), len(rows)
 
 
 def gemm_peaks_multi(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, mode, per, cfg, V, I, rowscale=None):
@@ -135,7 +141,7 @@ def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False, rowscale=None
 
 
 TPB_CANDIDATES = (16, 32, 64, 128)
-VARIANTS = (0, 1)
+VARIANTS = (0, 1, 2, 3)
 
 
 def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=2048, reps=3):

@@ -26,10 +26,14 @@ extern "C" {
  * z_re[m, t] = sum_k Ar[m, k] Zt[t, k], z_im[m, t] = sum_k Ai[m, k] Zt[t, k], pw = z_re^2 + z_im^2.
  * dtype: 0 = float32 operands, 1 = bfloat16 (bits in uint16), 2 = fp8 e4m3 bytes; fp32 accumulation.
  * rowscale (M floats or NULL) multiplies template m's reported |z|^2.
- * variant: 0 = choose by K (template operands held in registers up to K = 256 bf16 / 128 fp32), 1 = K-loop kernel.
+ * variant: 0 = choose by K (template operands held in registers up to K = 256 bf16 / 128 fp32, else the K loop).
  * Zt: (>= ceil(Tn/128)*128 rows, ldz) time-major; Ar, Ai: (M, lda).
  * mode 0: V, I (M, ldv >= ceil(Tn/128)): per 128 samples the max pw (columns >= Tn masked to -1) and its sample.
  * mode 1: V, I (M, ldv >= Tn/(128 per)): per window of `per` blocks the max and its sample; Tn % (128 per) == 0.
+ * mode | 4: the operands are complex-stacked (Ai[m, 2j] = -Ar[m, 2j+1], Ai[m, 2j+1] = Ar[m, 2j], as gwgs_stack_a
+ * writes them): the bf16 / fp8 K loops then form z_im from Ar and pair-rotated Zt fragments and do not read Ai.
+ * variant (K loop): 1 = template fragments in registers one chunk ahead, 2 = both operands staged through LDS,
+ * 3 = template fragments in registers at each chunk; the register forms need mode | 4.
  * First index wins ties at both levels. tiles_per_block: 128-sample tiles per workgroup (0 = default). */
 int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                     int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,

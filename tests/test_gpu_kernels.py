@@ -23,7 +23,8 @@ def _operands(rng, M, K, Tn, dtype):
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
 @pytest.mark.parametrize('M,K,Tn,variant', [(45, 64, 5 * 128 + 37, 0), (128, 40, 1024, 0), (300, 256, 2048, 0),
-                                             (77, 520, 1000, 0), (130, 64, 1024, 1), (1, 16, 128, 1)])
+                                             (77, 520, 1000, 0), (130, 64, 1024, 1), (1, 16, 128, 1),
+                                             (77, 520, 1000, 2), (200, 840, 1500, 3), (33, 72, 300, 3)])
 def test_fused_peaks_vs_oracle(gpu, dtype, M, K, Tn, variant):
     """fp_gemm.fused_peaks (mode 0) against the float64 GEMM + block max of the same (rounded) operands. Tolerance:
     relative 1e-4 on the block maxima (fp32 accumulation); the chosen sample must reach the float64 max to 1e-4."""
@@ -42,6 +43,45 @@ def test_fused_peaks_vs_oracle(gpu, dtype, M, K, Tn, variant):
     assert np.all(got >= Vr * (1 - 1e-4))
     assert np.all((I // 128) == np.arange(I.shape[1])[None, :])
     assert (I == Ir).mean() > 0.99
+
+
+@pytest.mark.parametrize('code,variant', [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 3)])
+def test_stacked_operands_rotated_path(gpu, code, variant):
+    """Complex-stacked operands (Ai derived from Ar, as stack_a writes them) through the rotated-Zt K loops
+    (mode | STACKED, Ai never read: a garbage Ai is passed) against the float64 GEMM of the same values, and equal
+    window maxima and samples to the general path that reads Ai, up to fp32 summation order (relative 1e-5)."""
+    import fp_gemm
+    rng = np.random.default_rng(17 + variant + 10 * code)
+    M, r, cw, nw = 150, 197, 2048, 3
+    K = 2 * (-(-r // 16) * 16)
+    Tn = cw * nw
+    A = (rng.standard_normal((M, r)) + 1j * rng.standard_normal((M, r))) * (20 if code == 2 else 1)
+    Zc = (rng.standard_normal((Tn, r)) + 1j * rng.standard_normal((Tn, r))) * (10 if code == 2 else 1)
+    Ar = np.zeros((M, K), np.float32); Ai = np.zeros((M, K), np.float32); Zt = np.zeros((Tn, K), np.float32)
+    Ar[:, 0:2 * r:2] = A.real; Ar[:, 1:2 * r:2] = -A.imag
+    Ai[:, 0:2 * r:2] = A.imag; Ai[:, 1:2 * r:2] = A.real
+    Zt[:, 0:2 * r:2] = Zc.real; Zt[:, 1:2 * r:2] = Zc.imag
+    cast = (lambda x: _fp8(np.clip(x, -448, 448))) if code == 2 else \
+        (lambda x: torch.from_numpy(x).to(torch.bfloat16))
+    qa, qi, qz = cast(Ar), cast(Ai), cast(Zt)
+    da, di, dz = (q.to(torch.float32).numpy().astype(np.float64) for q in (qa, qi, qz))
+    view = (lambda t: t.view(torch.uint8)) if code == 2 else (lambda t: t)
+    qa, qi, qz = (view(q).cuda() for q in (qa, qi, qz))
+    per = cw // 128
+    items = fp_gemm.items_table([(0, Tn // 128, 0, 0, Tn)], 2 * per)
+    out = {}
+    for mode, ai in ((1 | fp_gemm.STACKED, torch.full_like(qi, 0x7b)), (1, qi)):
+        V = torch.empty((M, nw), device='cuda'); I = torch.empty((M, nw), dtype=torch.int32, device='cuda')
+        fp_gemm.gemm_peaks_multi(code, qz, K, qa, ai, K, M, K, items, mode, per, (2 * per, variant if mode > 1 else 2),
+                                 V, I)
+        out[mode] = (V.cpu().numpy(), I.cpu().numpy())
+    V0, I0 = core.gemm_absmax(da, di, dz.T, Tn)
+    Vr, Ir = core.window_max(V0, I0, nw, per)
+    V, I = out[1 | fp_gemm.STACKED]
+    np.testing.assert_allclose(V, Vr, rtol=1e-4)
+    np.testing.assert_allclose(V, out[1][0], rtol=1e-5)
+    pw = (da @ dz.T) ** 2 + (di @ dz.T) ** 2
+    assert np.all(np.take_along_axis(pw, I.astype(np.int64), 1) >= Vr * (1 - 1e-4))
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
@@ -208,8 +248,9 @@ def _fp8(x):
     return torch.from_numpy(np.asarray(x, np.float32)).to(torch.float8_e4m3fn)
 
 
-@pytest.mark.parametrize('M,K,Tn,mode', [(77, 128, 1024, 0), (130, 784, 2048, 1), (5, 48, 300, 0)])
-def test_fp8_gemm_vs_oracle(gpu, M, K, Tn, mode):
+@pytest.mark.parametrize('M,K,Tn,mode,variant', [(77, 128, 1024, 0, 1), (130, 784, 2048, 1, 1), (5, 48, 300, 0, 1),
+                                                 (130, 784, 2048, 1, 2), (77, 208, 1024, 0, 3)])
+def test_fp8_gemm_vs_oracle(gpu, M, K, Tn, mode, variant):
     """fp8 e4m3 operands (bytes rounded on the host) against the float64 GEMM of the same dequantized values, with a
     per-template row scale applied to |z|^2. Products of e4m3 values are exact in fp32: relative 1e-4."""
     import fp_gemm
@@ -220,10 +261,11 @@ def test_fp8_gemm_vs_oracle(gpu, M, K, Tn, mode):
     da, di, dz = (q.to(torch.float32).numpy().astype(np.float64) for q in (qa, qi, qz))
     rs = (rng.random(M) + 0.5).astype(np.float32)
     if mode == 0:
-        V, I = fp_gemm.fused_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, (16, 1), rowscale=torch.from_numpy(rs).cuda())
+        V, I = fp_gemm.fused_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, (16, variant),
+                                   rowscale=torch.from_numpy(rs).cuda())
         Vr, Ir = core.gemm_absmax(da, di, dz, Tn)
     else:
-        V, I = fp_gemm.fused_window_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, 2048, (16, 1),
+        V, I = fp_gemm.fused_window_peaks(qa.cuda(), qi.cuda(), qz.cuda(), M, Tn, 2048, (16, variant),
                                           rowscale=torch.from_numpy(rs).cuda())
         V0, I0 = core.gemm_absmax(da, di, dz, Tn)
         Vr, Ir = core.window_max(V0, I0, Tn // 2048, 16)
@@ -267,7 +309,8 @@ def test_fp8_operand_conversion(gpu):
     assert np.abs(got - exp).max() <= 1e-6 * 448
 
 
-@pytest.mark.parametrize('dtype,variant', [(torch.bfloat16, 0), (torch.bfloat16, 1), (torch.float32, 0)])
+@pytest.mark.parametrize('dtype,variant', [(torch.bfloat16, 0), (torch.bfloat16, 1), (torch.float32, 0),
+                                           (torch.bfloat16, 2), (torch.bfloat16, 3)])
 def test_multi_segment_launch_equals_single(gpu, dtype, variant):
     """One launch over three segments of different lengths (item table, per-segment row scales) equals the
     single-segment launches, bit for bit."""
