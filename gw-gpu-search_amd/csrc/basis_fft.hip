@@ -5,7 +5,7 @@
 // n = n1 n2 (powers of two, 256..2048 each). With k = k1 + n1 k2 and t = t2 + n2 t1:
 //   z_j[t] = sum_k1 w_n^(k1 t2) w_n1^(k1 t1) sum_k2 x_j[k1 + n1 k2] w_n2^(k2 t2),   w_m = e^(+2 pi i / m),
 //   x_j[k] = Vc[j, k] D[k] on [kmin, kend) and 0 elsewhere.
-// cols_k: per basis row j and 16 (or 8) columns k1, the length-n2 transform over k2 (form of x fused into the load,
+// cols_k: per basis row j and 8 to 32 columns k1, the length-n2 transform over k2 (form of x fused into the load,
 // zeros above kend never read), times w_n^(k1 t2), to Y[j, t2, k1] (complex64, r x n).
 // rows_k: per t2 and 8 to 32 basis rows, the length-n1 transform over k1 of Y[j, t2, :], written as
 // Zt[t - a0, 2j] = Re z_j[t] cs_j, Zt[t - a0, 2j + 1] = Im z_j[t] cs_j for t in [a0, a0 + Tn) (fp32, bf16 or fp8), and
@@ -106,11 +106,33 @@ __device__ __forceinline__ void advance(double& c, double& s, const double dc, c
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
+// Storage of the intermediate Y: complex64, or two fp16 (round to nearest even) of the value times a per-row scale
+// (1 / rms of the row's Y, so the half range is never approached and its 11-bit precision applies relative to the
+// row's own level; used for the bf16 / fp8 operand forms, whose own rounding is coarser).
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+struct YF32 {
+    using E = float2;
+    static __device__ __forceinline__ E put(const float2 v, const float) { return v; }
+    static __device__ __forceinline__ float2 get(const E e) { return e; }
+};
+struct YF16 {
+    using E = h16x2;
+    static __device__ __forceinline__ E put(const float2 v, const float s)
+    {
+        E h;
+        h.x = (_Float16)(v.x * s);
+        h.y = (_Float16)(v.y * s);
+        return h;
+    }
+    static __device__ __forceinline__ float2 get(const E e) { return make_float2((float)e.x, (float)e.y); }
+};
+
 // Column transforms: block (k1 block, j); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1
-template <int P, int Q, int C>
+template <int P, int Q, int C, typename YS>
 __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, const long ldv,
                                                 const float2* __restrict__ D, const long kmin, const long kend,
-                                                const int n1, const int lgn, float2* __restrict__ Y)
+                                                const int n1, const int lgn, const float* __restrict__ ysc,
+                                                typename YS::E* __restrict__ Y)
 {
     constexpr int L = P * Q, ROW = pitch(P * C);
     __shared__ float ex[Q * ROW];
@@ -162,7 +184,8 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
                 if (part) u[i][pp].y = x; else u[i][pp].x = x;
             }
     }
-    float2* __restrict__ yb = Y + (size_t)j * n;
+    typename YS::E* __restrict__ yb = Y + (size_t)j * n;
+    const float ys = ysc ? ysc[j] : 1.f;
 #pragma unroll
     for (int i = 0; i < Q / P; i++) {
         const int m2 = mb + P * i;
@@ -175,7 +198,7 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
 #pragma unroll
         for (int m1 = 0; m1 < P; m1++) {
             if (m1) advance(wc, ws, dc, ds);
-            (yb + (size_t)Q * m1 * n1)[yo] = cmulf(u[i][m1], wc, ws);
+            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(cmulf(u[i][m1], wc, ws), ys);
         }
     }
 }
@@ -218,10 +241,11 @@ struct Out<uint8_t> {
 constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
 
 // Row transforms: block (t2, row block); threads JB x P, thread (p, jr) = (tid % P, tid / P) in step 1
-template <int P, int Q, int JB, typename T>
-__global__ void __launch_bounds__(JB * P) rows_k(const float2* __restrict__ Y, const int r, const int rp,
+template <int P, int Q, int JB, typename T, typename YS>
+__global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restrict__ Y, const int r, const int rp,
                                                  const int lgn1, const int lgn, const long a0, const long Tn,
-                                                 const float* __restrict__ cs, T* __restrict__ Zt, const long ldz)
+                                                 const float* __restrict__ cs, const float* __restrict__ ysc,
+                                                 T* __restrict__ Zt, const long ldz)
 {
     constexpr int L = P * Q, ROW = pitch(P * (JB + 1));
     __shared__ float ex[Q * ROW];
@@ -235,10 +259,10 @@ __global__ void __launch_bounds__(JB * P) rows_k(const float2* __restrict__ Y, c
     {
         // Y[j0 + jr, t2, p + P q]: uniform base plus a 32-bit lane offset (jr n + p < 2^26)
         const bool live = j0 + jr < r;
-        const float2* __restrict__ yb = Y + (size_t)j0 * n + t2 * n1;
+        const typename YS::E* __restrict__ yb = Y + (size_t)j0 * n + t2 * n1;
         const unsigned yo = (unsigned)(jr * n + p);
 #pragma unroll
-        for (int q = 0; q < Q; q++) v[q] = live ? (yb + P * q)[yo] : make_float2(0.f, 0.f);
+        for (int q = 0; q < Q; q++) v[q] = live ? YS::get((yb + P * q)[yo]) : make_float2(0.f, 0.f);
     }
     dft_reg<Q>(v);
     {
@@ -269,7 +293,7 @@ __global__ void __launch_bounds__(JB * P) rows_k(const float2* __restrict__ Y, c
     }
     const int j = j0 + jr2;
     if (j >= rp) return;
-    const float sc = (cs && j < r) ? cs[j] : 1.f;
+    const float sc = j < r ? (cs ? cs[j] : 1.f) / (ysc ? ysc[j] : 1.f) : 1.f;
     // Zt as a buffer of Tn rows: row t - a0 = tl + n2 Q m1; rows outside [0, Tn) go to offset Tn ldz (dropped)
     constexpr unsigned SZ = 2 * sizeof(T);
     const unsigned nbytes = (unsigned)(Tn * ldz * sizeof(T));
@@ -317,38 +341,65 @@ bool split(const int lgl, int& P)
     return true;
 }
 
-template <int P, int Q>
-void launch_cols(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r, float2* Y,
-                 hipStream_t s)
+template <int P, int Q, typename YS>
+void launch_cols(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
+                 const float* ysc, void* Y, hipStream_t s)
 {
-    constexpr int C = Q <= 32 ? 16 : 8;
-    hipLaunchKernelGGL((cols_k<P, Q, C>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin, kend, n1,
-                       lgn, Y);
+    // columns per block: 16 complex64 or 32 fp16 pairs give 128-byte runs per Y row (8 for the 2048-point lines)
+    constexpr int C = Q > 32 ? 8 : sizeof(typename YS::E) == 8 ? 16 : 32;
+    hipLaunchKernelGGL((cols_k<P, Q, C, YS>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin, kend,
+                       n1, lgn, ysc, (typename YS::E*)Y);
 }
 
-template <int P, int Q, typename T>
-void launch_rows(const float2* Y, int r, int rp, int lgn1, int lgn, long a0, long Tn, const float* cs, T* Zt, long ldz,
-                 hipStream_t s)
+template <typename YS>
+int cols_dispatch(const int lgn2, const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn,
+                  int r, const float* ysc, void* Y, hipStream_t s)
+{
+    switch (lgn2) {
+    case 8: launch_cols<16, 16, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
+    case 9: launch_cols<16, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
+    case 10: launch_cols<32, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
+    case 11: launch_cols<32, 64, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
+    default: return GWGS_EUNSUP;
+    }
+    return GWGS_OK;
+}
+
+template <int P, int Q, typename T, typename YS>
+void launch_rows(const void* Y, int r, int rp, int lgn1, int lgn, long a0, long Tn, const float* cs, const float* ysc,
+                 T* Zt, long ldz, hipStream_t s)
 {
     // rows per block: 16 float32 pairs or 32 bf16 / fp8 pairs give 128- / 128- / 64-byte runs per output row
     constexpr int JB = Q > 32 ? 8 : sizeof(T) == 4 ? 16 : 32;
     const long n2 = 1L << (lgn - lgn1);
-    hipLaunchKernelGGL((rows_k<P, Q, JB, T>), dim3((unsigned)n2, (rp + JB - 1) / JB), dim3(JB * P), 0, s, Y, r, rp,
-                       lgn1, lgn, a0, Tn, cs, Zt, ldz);
+    hipLaunchKernelGGL((rows_k<P, Q, JB, T, YS>), dim3((unsigned)n2, (rp + JB - 1) / JB), dim3(JB * P), 0, s,
+                       (const typename YS::E*)Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz);
 }
 
-template <typename T>
-int rows_dispatch(const int lgn1, const float2* Y, int r, int rp, int lgn, long a0, long Tn, const float* cs, T* Zt,
-                  long ldz, hipStream_t s)
+template <typename T, typename YS>
+int rows_dispatch(const int lgn1, const void* Y, int r, int rp, int lgn, long a0, long Tn, const float* cs,
+                  const float* ysc, T* Zt, long ldz, hipStream_t s)
 {
     switch (lgn1) {
-    case 8: launch_rows<16, 16, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
-    case 9: launch_rows<16, 32, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
-    case 10: launch_rows<32, 32, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
-    case 11: launch_rows<32, 64, T>(Y, r, rp, lgn1, lgn, a0, Tn, cs, Zt, ldz, s); break;
+    case 8: launch_rows<16, 16, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
+    case 9: launch_rows<16, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
+    case 10: launch_rows<32, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
+    case 11: launch_rows<32, 64, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
     default: return GWGS_EUNSUP;
     }
     return GWGS_OK;
+}
+
+template <typename YS>
+int series(int dtype, int lg, const float2* vc, long ldv, int r, int rp, const float2* d, long kmin, long kend,
+           long a0, long Tn, const float* cs, const float* ysc, void* Y, void* Zt, long ldz, hipStream_t s)
+{
+    const int lgn1 = lg / 2, lgn2 = lg - lg / 2;
+    int e = cols_dispatch<YS>(lgn2, vc, ldv, d, kmin, kend, 1 << lgn1, lg, r, ysc, Y, s);
+    if (e) return e;
+    if (dtype == 0) return rows_dispatch<float, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (float*)Zt, ldz, s);
+    if (dtype == 1) return rows_dispatch<uint16_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint16_t*)Zt, ldz, s);
+    return rows_dispatch<uint8_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint8_t*)Zt, ldz, s);
 }
 
 int log2_exact(const long n)
@@ -372,10 +423,11 @@ GWGS_API long gwgs_basis_series_workspace(int r, long n)
 
 // dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (clamped to +-448). Vc (>= r rows, row pitch ldv >= kend) and D
 // (>= kend) complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt (Tn rows, ldz >= 2 rp elements, even);
-// colscale: per basis row, or null.
+// colscale: per basis row, or null. ytype 0: the intermediate in complex64; 1: in fp16 pairs scaled by yscale[j]
+// (r floats, e.g. sqrt(n1 / sum_k |x_jk|^2); null = 1), divided out again in the row pass.
 GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend,
-                               long n, long a0, long Tn, const float* colscale, void* work, void* Zt, long ldz,
-                               void* stream)
+                               long n, long a0, long Tn, const float* colscale, int ytype, const float* yscale,
+                               void* work, void* Zt, long ldz, void* stream)
 {
     GWGS_CHECK(Vc && D && work && Zt && r > 0 && rp >= r && kmin >= 0 && kmin <= kend && kend <= ldv && kend <= n &&
                a0 >= 0 && Tn >= 0 && a0 + Tn <= n && ldz >= 2 * (long)rp && ldz % 2 == 0, GWGS_EINVAL);
@@ -384,24 +436,12 @@ GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int r
     // the row kernel addresses Zt as one buffer with 32-bit byte offsets up to (Tn + 1) rows
     const double esz = dtype == 0 ? 4.0 : dtype == 1 ? 2.0 : 1.0;
     if ((double)(Tn + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;
-    if (dtype < 0 || dtype > 2) return GWGS_EUNSUP;
+    if (dtype < 0 || dtype > 2 || ytype < 0 || ytype > 1) return GWGS_EUNSUP;
     hipStream_t s = (hipStream_t)stream;
-    const int lgn1 = lg / 2, lgn2 = lg - lg / 2;
-    const int n1 = 1 << lgn1;
-    const float2* vc = (const float2*)Vc;
-    const float2* d = (const float2*)D;
-    float2* Y = (float2*)work;
-    switch (lgn2) {
-    case 8: launch_cols<16, 16>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
-    case 9: launch_cols<16, 32>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
-    case 10: launch_cols<32, 32>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
-    case 11: launch_cols<32, 64>(vc, ldv, d, kmin, kend, n1, lg, r, Y, s); break;
-    default: return GWGS_EUNSUP;
-    }
-    int e;
-    if (dtype == 0) e = rows_dispatch<float>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (float*)Zt, ldz, s);
-    else if (dtype == 1) e = rows_dispatch<uint16_t>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (uint16_t*)Zt, ldz, s);
-    else e = rows_dispatch<uint8_t>(lgn1, Y, r, rp, lg, a0, Tn, colscale, (uint8_t*)Zt, ldz, s);
+    const int e = ytype == 0 ? series<YF32>(dtype, lg, (const float2*)Vc, ldv, r, rp, (const float2*)D, kmin, kend, a0,
+                                            Tn, colscale, yscale, work, Zt, ldz, s)
+                             : series<YF16>(dtype, lg, (const float2*)Vc, ldv, r, rp, (const float2*)D, kmin, kend, a0,
+                                            Tn, colscale, yscale, work, Zt, ldz, s);
     if (e) return e;
     return gwgs_status();
 }

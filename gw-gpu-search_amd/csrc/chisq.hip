@@ -110,42 +110,76 @@ __device__ __forceinline__ double block_exscan_f64(const double x, double* sh, d
     return base + inc - x;
 }
 
-// One block per requested row: float64 sums of |h|^2 IP over contiguous thread chunks of [0, k1), a block scan, then
-// each thread walks its own chunk for the edges whose target j/nb * total falls in it (first k with cumsum > target,
-// multi_filter.py:264-282; chisq_edges.py:211-245). Columns j >= nb and the last column hold k1; edges are clipped
-// to [kmin, k1].
+// One block per requested row (multi_filter.py:264-282; chisq_edges.py:211-245): edge j = first k with the float64
+// cumulative sum of |h|^2 IP over [0, k1) above j / nb * total, clipped to [kmin, k1]; columns j >= nb and the last
+// hold k1. Pass 1: chunk sums, each chunk read by one wave in coalesced 64-bin strides; a block scan of the chunk
+// sums. Pass 2: each target's chunk by binary search, then one wave walks it in 64-bin strides with a wave prefix
+// sum and takes the first lane above the target (the next chunk if rounding left it just short).
+constexpr int EMAXCH = 4096;
+
+__device__ __forceinline__ double pw_ip(const float2* __restrict__ hr, const float* __restrict__ ips, const long k)
+{
+    const float2 v = hr[k];
+    return ((double)v.x * v.x + (double)v.y * v.y) * (double)ips[k];
+}
+
 __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h, const long ldh,
                                                     const int* __restrict__ rows, const float* __restrict__ ips,
                                                     const int* __restrict__ k1s, const int* __restrict__ nbs,
                                                     const int nedge, const int kmin, int* __restrict__ out)
 {
-    const int g = blockIdx.x, t = threadIdx.x;
-    const float2* hr = h + (size_t)rows[g] * ldh;
-    const int k1 = k1s[g], nb = nbs[g];
-    const int chunk = (k1 + NT - 1) / NT, a = min(t * chunk, k1), b = min(a + chunk, k1);
-    double s = 0.0;
-    for (int k = a; k < b; k++) {
-        const float2 v = hr[k];
-        s += ((double)v.x * v.x + (double)v.y * v.y) * (double)ips[k];
-    }
+    __shared__ double ex[EMAXCH + 1];
     __shared__ double sh[NT / 64];
-    double tot;
-    const double lo = block_exscan_f64(s, sh, tot);
-    const double hi = lo + s;
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float2* __restrict__ hr = h + (size_t)rows[g] * ldh;
+    const long k1 = k1s[g];
+    const int nb = nbs[g];
     int* eo = out + (size_t)g * nedge;
-    for (int i = t; i < nedge; i += NT) eo[i] = k1;
-    __syncthreads();
-    if (b <= a) return;
-    int j = 0;
-    while (j < nb && ((double)j / (double)nb) * tot < lo) j++;
-    if (j >= nb || ((double)j / (double)nb) * tot >= hi) return;
-    double cs = lo;
-    for (int k = a; k < b && j < nb; k++) {
-        const float2 v = hr[k];
-        cs += ((double)v.x * v.x + (double)v.y * v.y) * (double)ips[k];
-        while (j < nb && cs > ((double)j / (double)nb) * tot) { eo[j] = min(max(k, kmin), k1); j++; }
+    for (int i = tid; i < nedge; i += NT) eo[i] = (int)k1;
+    // chunk length: a multiple of 64, at least 2048, and at most EMAXCH chunks per row
+    const long ech = max(2048L, ((k1 + EMAXCH - 1) / EMAXCH + 63) / 64 * 64);
+    const int nch = (int)((k1 + ech - 1) / ech);
+    for (int c = wave; c < nch; c += NT / 64) {
+        const long c0 = (long)c * ech, c1 = min(c0 + ech, k1);
+        double s = 0.0;
+#pragma unroll 8
+        for (long k = c0 + lane; k < c1; k += 64) s += pw_ip(hr, ips, k);
+        s = wave_sum_f64(s);
+        if (lane == 0) ex[c] = s;
     }
-    while (j < nb && ((double)j / (double)nb) * tot < hi) { eo[j] = min(max(b, kmin), k1); j++; }
+    __syncthreads();
+    // exclusive scan of the chunk sums in place (each thread a contiguous run, then a block scan of the runs)
+    const int per = (nch + NT - 1) / NT, a = min(tid * per, nch), b = min(a + per, nch);
+    double loc = 0.0;
+    for (int c = a; c < b; c++) loc += ex[c];
+    double tot;
+    double run = block_exscan_f64(loc, sh, tot);
+    __syncthreads();
+    for (int c = a; c < b; c++) { const double x = ex[c]; ex[c] = run; run += x; }
+    if (tid == 0) ex[nch] = tot;
+    __syncthreads();
+    if (nch == 0) return;
+    for (int j = wave; j < nb; j += NT / 64) {
+        const double target = ((double)j / (double)nb) * tot;
+        // last chunk whose start lies at or below the target
+        int lo = 0, hi = nch - 1;
+        while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (ex[mid] <= target) lo = mid; else hi = mid - 1; }
+        long k = k1;
+        double acc = ex[lo];
+        for (long base = (long)lo * ech; base < k1; base += 64) {
+            const long kk = base + lane;
+            double v = kk < k1 ? pw_ip(hr, ips, kk) : 0.0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double y = __shfl_up(v, o, 64);
+                if (lane >= o) v += y;
+            }
+            const unsigned long long m = __ballot(acc + v > target && kk < k1);
+            if (m) { k = base + __ffsll((long long)m) - 1; break; }
+            acc += __shfl(v, 63, 64);
+        }
+        if (lane == 0) eo[j] = (int)min(max(k, (long)kmin), k1);
+    }
 }
 
 } // namespace

@@ -282,14 +282,16 @@ class Filter:
                 zcs, acs = _fp8_scales(pw)
             work = self._workbuf(ws)
             esz = Zt.element_size()
+            # intermediate in complex64 (the fp16 option halves its bytes but, with 4-byte lanes, measured slower)
+            ytype = 0
             for i, (di, s, gs, a0, nw) in enumerate(segs):
                 e0 = _dev.METER.start()
                 _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
                           self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
-                          work.data_ptr(), Zt[row0[i]].data_ptr(), K, _dev.stream())
+                          ytype, None, work.data_ptr(), Zt[row0[i]].data_ptr(), K, _dev.stream())
                 # HBM bytes: Vc rows and D over [kmin, kend), Y written and read, Zt written
                 _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin) + 8 * (tb.kend - self.kmin)
-                                + 16 * tb.r * self.tlen + Tns[i] * K * esz)
+                                + (8 if ytype else 16) * tb.r * self.tlen + Tns[i] * K * esz)
             return Zt, acs
         q = self._qbuf(nb, rp)
         for i, (di, s, gs, a0, nw) in enumerate(segs):

@@ -117,8 +117,11 @@ int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double*
  * NULL. Vc: >= r rows of pitch ldv complex64; D complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt: Tn
  * rows of ldz >= 2 rp elements (ldz even). Returns GWGS_EUNSUP for other n, GWGS_ESIZE when Zt exceeds 4 GiB. */
 long gwgs_basis_series_workspace(int r, long n);
+/* ytype 0 keeps the intermediate in complex64; 1 in fp16 pairs of the value times yscale[j] (r floats; NULL = 1),
+ * divided out in the second pass (half the intermediate's traffic). */
 int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend, long n,
-                      long a0, long Tn, const float* colscale, void* work, void* Zt, long ldz, void* stream);
+                      long a0, long Tn, const float* colscale, int ytype, const float* yscale, void* work, void* Zt,
+                      long ldz, void* stream);
 /* out[j] = sum over kmin <= k < kend of |Vc[j, k] D[k]|^2 (double), j < r: the transform input's power by rows, from
  * which pipeline/fp_filter.py:609-610's rms scale follows by Parseval without forming q */
 int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, long kend, double* out, void* stream);

@@ -41,10 +41,11 @@ def test_bad_arguments_rejected(lib):
     for n in (1 << 15, 3 << 16, 1 << 23, 0):
         assert lib.gwgs_basis_series_workspace(10, n) == 0
     p = ctypes.c_void_p(256)
-    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 3 << 16, 0, 10, None, p, p, 32, None) == -3
-    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 101, 1 << 16, 0, 10, None, p, p, 32, None) == -1
-    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, p, p, 30, None) == -1
-    assert lib.gwgs_basis_series(9, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, p, p, 32, None) == -3
+    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 3 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3
+    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 101, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -1
+    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 30, None) == -1
+    assert lib.gwgs_basis_series(9, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3
+    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 2, None, p, p, 32, None) == -3
     assert lib.gwgs_vd_power(None, 100, 10, p, 0, 50, p, None) == -1
     with pytest.raises(AssertionError):
         _lib.check(-1, 'x')

@@ -387,7 +387,7 @@ def test_basis_series_vs_oracle(gpu, n, r, kmin, kend, a0, Tn):
     for code, dt in ((0, torch.float32), (1, torch.bfloat16), (2, torch.uint8)):
         Zt = torch.full((Tn, K), 7, dtype=dt, device='cuda')
         _lib.call('gwgs_basis_series', code, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kend, n, a0, Tn,
-                  csd.data_ptr() if code == 2 else None, work.data_ptr(), Zt.data_ptr(), K, _dev.stream())
+                  csd.data_ptr() if code == 2 else None, 0, None, work.data_ptr(), Zt.data_ptr(), K, _dev.stream())
         outs[code] = Zt.cpu()
     z32 = outs[0].numpy()
     got = z32[:, 0:2 * r:2].astype(np.float64) + 1j * z32[:, 1:2 * r:2]
@@ -398,6 +398,15 @@ def test_basis_series_vs_oracle(gpu, n, r, kmin, kend, a0, Tn):
     assert torch.equal(outs[1], outs[0].to(torch.bfloat16))
     scaled = np.clip(z32 * np.repeat(cs, 2)[None, :], -448, 448)
     assert np.array_equal(outs[2].numpy(), _fp8(scaled).view(torch.uint8).numpy())
+    # fp16 intermediate (with a per-row scale): float32 output within 5e-4 rms / 5e-3 max of the series' rms
+    ysc = torch.from_numpy((rng.random(r) * 4 + 0.25).astype(np.float32)).cuda()
+    Zh = torch.full((Tn, K), 7, dtype=torch.float32, device='cuda')
+    _lib.call('gwgs_basis_series', 0, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kend, n, a0, Tn, None, 1,
+              ysc.data_ptr(), work.data_ptr(), Zh.data_ptr(), K, _dev.stream())
+    zh = Zh.cpu().numpy()
+    eh = np.abs(zh[:, 0:2 * r:2].astype(np.float64) + 1j * zh[:, 1:2 * r:2] - ref)
+    assert np.sqrt(np.mean(eh ** 2)) < 5e-4 * rms and eh.max() < 5e-3 * rms, (eh.max() / rms)
+    assert np.all(zh[:, 2 * r:] == 0)
 
 
 def test_vd_power_vs_float64(gpu):
