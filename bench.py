@@ -129,6 +129,97 @@ def cpu_baseline(streams, banks, cond, ntemp=128):
                        f'templates/SVD excluded ({T["templates_s"] + T["svd_s"]:.1f} s)')
 
 
+class ProductionWorkload:
+    """2 detectors x 512 s at 2048 Hz (lines, glitches, 20 injections), both banks; one 512 s segment each."""
+
+    def __init__(self, opts):
+        import torch
+        import fp_filter, workloads
+        self.streams, self.banks, self.cond = workloads.production()
+        self.x_dev = [torch.from_numpy(st.x).cuda() for st in self.streams]
+        ds0 = workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
+        IPall, sw, isw = fp_filter.svd_metric(ds0)
+        self.opts = opts
+        self.plans = [fp_filter.BankPlan(b.m1, b.m2, b.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
+                                         name=b.name) for b in self.banks]
+
+    def step(self, prec, check=False):
+        import fp_filter, workloads
+        ds = workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
+        o = self.opts[prec]
+        if check:
+            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
+        f = fp_filter.Filter(ds, o)
+        outs = [[t.to_dict(gates=d.gates) for t, d in zip(tr, ds)] for tr in f.run_banks(self.plans)]
+        return ds, [f], outs
+
+    def ranks(self):
+        return {p.name: p.ranks for p in self.plans}
+
+    def describe(self, tile):
+        return ('2 detectors x 512 s at 2048 Hz (lines, glitches, 20 injections), 4,612 circular + 1,268 eccentric '
+                'SPA templates, tiles of %d, SVD worst lost power 1e-4, 16 chi-squared bins, SNR threshold 5.5, 1 s '
+                'windows; BF16 GEMM, FP32 exact stage' % tile)
+
+    def compare(self, a, b):
+        return [compare_triggers(a[i][d], b[i][d]) for i in range(len(a)) for d in range(len(a[i]))]
+
+    def direct_check(self, ds, outs):
+        return direct_snr_check(ds, self.plans, outs)
+
+
+class BatchedWorkload:
+    """Uneven chunks (64-1024 s in 16 s steps, 2 detectors, noise with lines), both banks, grouped by segment grid
+    (batched.BatchedSearch). Datasets and trigger lists are flattened over grids in grid order."""
+
+    def __init__(self, n_chunks, opts):
+        import torch
+        import batched, workloads
+        chunks, self.banks = workloads.batched(n_chunks)
+        x_dev = [torch.from_numpy(st.x).cuda() for st, _ in chunks]
+        self.n_chunks = n_chunks
+        self.search = batched.BatchedSearch(chunks, self.banks, x_dev, opts)
+        self.plans = None
+
+    def step(self, prec, check=False):
+        ds, outs, filters = self.search.step(prec, self_check=check)
+        g = self.search.grids
+        self._last_ds = ds
+        flat_ds = [d for k in g for d in ds[k]]
+        flat = [[t for k in g for t in outs[k][b]] for b in range(len(self.banks))]
+        return flat_ds, filters, flat
+
+    def ranks(self):
+        return self.search.ranks()
+
+    def describe(self, tile):
+        return ('batched: %d chunks per detector, 64-1024 s in 16 s steps, 2 detectors at 2048 Hz (noise with lines), '
+                'grouped on 512 s and 128 s segment grids; 4,612 circular + 1,268 eccentric SPA templates, tiles of '
+                '%d, SVD worst lost power 1e-4, 16 chi-squared bins, SNR threshold 5.5, 1 s windows; BF16 GEMM, FP32 '
+                'exact stage' % (self.n_chunks, tile))
+
+    def compare(self, a, b):
+        # per bank, the lists of every dataset merged (keys are unique: each dataset has its own end times)
+        out = []
+        for i in range(len(a)):
+            out.append(compare_triggers(_merge(a[i]), _merge(b[i])))
+        return out
+
+    def direct_check(self, ds, outs):
+        # the float64 direct filter on the datasets of the first grid (the flat lists start with them; that grid's
+        # plans hold the dense templates on its frequency bins)
+        g = self.search.grids[0]
+        nd = len(self._last_ds[g])
+        part = [outs[b][:nd] for b in range(len(outs))]
+        return direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=12)
+
+
+def _merge(dicts):
+    import numpy as np
+    keys = dicts[0].keys()
+    return {k: np.concatenate([d[k] for d in dicts]) for k in keys}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -141,6 +232,9 @@ def main():
     ap.add_argument('--gemm-cfg', default=None, help='tiles_per_block[,variant]')
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
     ap.add_argument('--timing', action='store_true', help='one extra step with stage timing (synchronised) to stderr')
+    ap.add_argument('--workload', choices=('production', 'batched'), default='production',
+                    help='production: 2 detectors x 512 s with lines, glitches and injections; batched: uneven chunks')
+    ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); the '
@@ -154,32 +248,19 @@ def main():
     import _dev, fp_filter, workloads
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     t_setup = time.perf_counter()
-    streams, banks, cond = workloads.production()
-    x_dev = [torch.from_numpy(st.x).cuda() for st in streams]
-    log('synthetic data ready', round(time.perf_counter() - t_setup, 1))
-    ds0 = workloads.condition_streams(streams, cond, [x.clone() for x in x_dev])
-    IPall, sw, isw = fp_filter.svd_metric(ds0)
     cfg = tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
                                  refine=a.refine) for p in ('bf16', 'fp32')}
-    plans = []
-    for bank in banks:
-        plans.append(fp_filter.BankPlan(bank.m1, bank.m2, bank.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
-                                        name=bank.name))
-        log(f'plan {bank.name}: {len(bank)} templates, ranks {plans[-1].ranks}', plans[-1].T)
+    if a.workload == 'batched':
+        wl = BatchedWorkload(a.chunks, opts)
+    else:
+        wl = ProductionWorkload(opts)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t_setup
+    log('setup', round(setup_s, 1), 's; ranks', wl.ranks())
+    banks = wl.banks
     cw = 2048
-
-    def step(prec, check=False):
-        xs = [x.clone() for x in x_dev]
-        ds = workloads.condition_streams(streams, cond, xs)
-        o = opts[prec]
-        if check:
-            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
-        f = fp_filter.Filter(ds, o)
-        outs = [[t.to_dict(gates=d.gates) for t, d in zip(tr, ds)] for tr in f.run_banks(plans)]
-        return ds, [f], outs
+    step = wl.step
 
     log('warmup')
     for _ in range(a.warmup):
@@ -195,6 +276,7 @@ def main():
             opts[p].timing = False
     an_s = workloads.analysed_seconds(ds, cw)
     ntemp = sum(len(b) for b in banks)
+    plans = wl.plans
     meter = _dev.METER
     meter.reset(); meter.on = True
     torch.cuda.synchronize()
@@ -229,22 +311,19 @@ def main():
                              share_of_step=t_ / a.steps / dt)
     diffs = None
     if outs_fp is not None:
-        diffs = [compare_triggers(outs_bf[b][d], outs_fp[b][d]) for b in range(len(plans)) for d in range(len(ds))]
+        diffs = wl.compare(outs_bf, outs_fp)
     err = err32 = None
     nchk = 0
     if not a.no_check:
-        err, nchk = direct_snr_check(ds, plans, outs_bf)
+        err, nchk = wl.direct_check(ds, outs_bf)
         if outs_fp is not None:
-            err32, _ = direct_snr_check(ds, plans, outs_fp)
+            err32, _ = wl.direct_check(ds, outs_fp)
     res = dict(
         metric=METRIC, value=value, unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
         ms_per_step=dt * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
         data='synthetic',
-        config=dict(workload='2 detectors x 512 s at 2048 Hz (lines, glitches, 20 injections), 4,612 circular + '
-                             '1,268 eccentric SPA templates, tiles of %d, SVD worst lost power 1e-4, 16 chi-squared '
-                             'bins, SNR threshold 5.5, 1 s windows; BF16 GEMM, FP32 exact stage' % a.tile,
-                    templates=ntemp, analysed_s=an_s, tile=a.tile,
-                    ranks={p.name: p.ranks for p in plans}, setup_s=round(setup_s, 1)),
+        config=dict(workload=wl.describe(a.tile), templates=ntemp, analysed_s=an_s, tile=a.tile,
+                    ranks=wl.ranks(), setup_s=round(setup_s, 1)),
         roofline=dict(bound='mfma', achieved=fl / t_k / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
                       frac=fl / t_k / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
                       kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=n_l,
@@ -259,7 +338,8 @@ def main():
     )
     if not a.no_cpu:
         log('cpu baseline')
-        res['cpu_baseline'] = cpu_baseline(streams, banks, cond)
+        streams, banks_p, cond = workloads.production()
+        res['cpu_baseline'] = cpu_baseline(streams, banks_p, cond)
     else:
         res['cpu_baseline'] = None
     print(json.dumps(res), flush=True)
