@@ -140,6 +140,7 @@ class ProductionWorkload:
         ds0 = workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
         IPall, sw, isw = fp_filter.svd_metric(ds0)
         self.opts = opts
+        self.filters = {}
         self.plans = [fp_filter.BankPlan(b.m1, b.m2, b.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
                                          name=b.name) for b in self.banks]
 
@@ -147,9 +148,18 @@ class ProductionWorkload:
         import fp_filter, workloads
         ds = workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
         o = self.opts[prec]
-        if check:
-            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
-        f = fp_filter.Filter(ds, o)
+        if o.graphs:
+            # one Filter per form for the whole run: its captured graphs read its own D / IP, refreshed by rebind
+            f = self.filters.get(prec)
+            if f is None:
+                f = self.filters[prec] = fp_filter.Filter(ds, o)
+            else:
+                f.rebind(ds)
+            ds = f.data
+        else:
+            if check:
+                o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
+            f = fp_filter.Filter(ds, o)
         outs = [[t.to_dict(gates=d.gates) for t, d in zip(tr, ds)] for tr in f.run_banks(self.plans)]
         return ds, [f], outs
 
@@ -235,6 +245,8 @@ def main():
     ap.add_argument('--workload', choices=('production', 'batched'), default='production',
                     help='production: 2 detectors x 512 s with lines, glitches and injections; batched: uneven chunks')
     ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
+    ap.add_argument('--graphs', action='store_true', help='capture stage A of every batch in HIP graphs (replayed '
+                    'from the second step on; production workload)')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); the '
@@ -250,7 +262,7 @@ def main():
     t_setup = time.perf_counter()
     cfg = tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
-                                 refine=a.refine) for p in ('bf16', 'fp32')}
+                                 refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32')}
     if a.workload == 'batched':
         wl = BatchedWorkload(a.chunks, opts)
     else:
@@ -287,9 +299,21 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
     ms = meter.stats()
-    n_l, t_k, fl = ms['gemm_bf16']
     value = ntemp * an_s / dt
     meter.on = False
+    def metered_plain(prec):
+        # kernels inside replayed graphs carry no events: one more step of the same work without graphs, metered
+        opts[prec].graphs = False
+        meter.reset(); meter.on = True
+        step(prec)
+        st = meter.stats()
+        meter.on = False
+        opts[prec].graphs = a.graphs
+        return st
+
+    if a.graphs:
+        ms = metered_plain('bf16')
+    n_l, t_k, fl = ms['gemm_bf16']
     dt32 = outs_fp = None
     if a.fp32_steps > 0:
         log('fp32 form')
@@ -301,14 +325,15 @@ def main():
             _, f32f, outs_fp = step('fp32')
         torch.cuda.synchronize()
         dt32 = (time.perf_counter() - t1) / a.fp32_steps
-        n32, t32, fl32 = meter.stats()['gemm_fp32']
         meter.on = False
+        n32, t32, fl32 = (metered_plain('fp32') if a.graphs else meter.stats())['gemm_fp32']
     hbm = {}
+    msteps = 1 if a.graphs else a.steps
     for name, (n_, t_, by) in ms.items():
         if not name.startswith('gemm'):
             hbm[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 1e9 / 8000.0,
                              launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
-                             share_of_step=t_ / a.steps / dt)
+                             share_of_step=t_ / msteps / dt)
     diffs = None
     if outs_fp is not None:
         diffs = wl.compare(outs_bf, outs_fp)
@@ -327,7 +352,7 @@ def main():
         roofline=dict(bound='mfma', achieved=fl / t_k / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
                       frac=fl / t_k / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
                       kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=n_l,
-                      avg_launch_ms=t_k / max(n_l, 1) * 1e3, gemm_share_of_step=t_k / a.steps / dt),
+                      avg_launch_ms=t_k / max(n_l, 1) * 1e3, gemm_share_of_step=t_k / msteps / dt),
         fp32=None if dt32 is None else dict(
             value=ntemp * an_s / dt32, ms_per_step=dt32 * 1e3, gemm_tflops=fl32 / t32 / 1e12,
             gemm_frac=fl32 / t32 / 1e12 / PEAK['fp32'], gemm_share_of_step=t32 / dt32),

@@ -136,7 +136,7 @@ class Meter:
         self.rec = {}
 
     def start(self):
-        if not self.on:
+        if not self.on or torch.cuda.is_current_stream_capturing():
             return None
         e = torch.cuda.Event(enable_timing=True)
         e.record()

@@ -59,6 +59,9 @@ class Options:
     timing: bool = False                   # synchronize around stages to fill the timing dict
     segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
     depth: int = 2                         # batches enqueued ahead of the one being decoded
+    graphs: bool = False                   # capture each tile's norms and each batch's stage A in a HIP graph on
+                                           # the first run and replay them on later runs (same segmentation; new
+                                           # data through rebind)
     refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
                                            # samples inside its window (0: the reference's rebuilt-argmax sample)
 
@@ -196,7 +199,10 @@ class Triggers:
 def _fp8_scales(pw):
     """Per-basis fp8 scales from (segments, rp) powers sum_k |q_jk|^2: Z is multiplied by 1 / rms and A by rms, with
     rms = sqrt of the mean over segments (0 rows stay 0)."""
-    rms = torch.sqrt(pw.mean(0)).to(torch.float32)
+    acc = pw[0].clone()
+    for i in range(1, pw.shape[0]):          # elementwise (no reduction kernel: the stage may be graph-captured)
+        acc += pw[i]
+    rms = torch.sqrt(acc / pw.shape[0]).to(torch.float32)
     zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
     return zcs, rms.contiguous()
 
@@ -229,6 +235,8 @@ class Filter:
             self.seg0.append(off); off += len(m.analyze)
         self._q = None
         self._work = None
+        self._bufs = None                  # graph mode: three rotating buffer sets
+        self._graphs = {}                  # graph mode: captured tile norms and stage-A batches
         self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
         self.rel = []
@@ -259,7 +267,7 @@ class Filter:
             self._work = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
         return self._work
 
-    def basis_operand(self, tb: TileBasis, segs, Tns, row0):
+    def basis_operand(self, tb: TileBasis, segs, Tns, row0, Zt=None):
         """Zt (sum Tn, 2 rp) in the GEMM's operand type: the basis series of every segment of the batch over its
         analysed span, stacked by rows (fp_filter.py:597-613). Power-of-two segment lengths 2^16..2^22 take the
         fused four-step transform (csrc/basis_fft.hip): no transform input is materialised and the output is written
@@ -272,7 +280,8 @@ class Filter:
         lib = _lib.load()
         ws = lib.gwgs_basis_series_workspace(tb.r, self.tlen)
         zcs = acs = None
-        Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
+        if Zt is None:
+            Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
         if ws > 0:
             if self.code == 2:
                 pw = torch.zeros((nb, rp), dtype=torch.float64, device='cuda')
@@ -331,17 +340,19 @@ class Filter:
                     segs.append((di, s, self.seg0[di] + s, a0, nw))
         B = max(1, self.opts.segment_batch)
         launched, fin = deque(), deque()
+        if self.opts.graphs and self._bufs is None:
+            self._alloc_rotation(plans, segs, B)
+        ordinal = 0
         for pi, plan in enumerate(plans):
-            for tb in plan.tiles:
+            for ti, tb in enumerate(plan.tiles):
                 t0 = self._sync()
-                e0 = _dev.METER.start()
-                sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
-                _dev.METER.stop('sigmasq', e0, 8 * len(tb.ids) * (tb.kend - self.kmin)
-                                + 4 * self.IPall.shape[0] * (tb.kend - self.kmin))
+                sig, norm = self.tile_norms(tb, (id(plan), ti))
                 tile_host = _dev.Fetch(sig, norm)
                 t1 = self._sync(); self.T['norm_s'] += t1 - t0
                 for b0 in range(0, len(segs), B):
-                    cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host)
+                    cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
+                                            ordinal=ordinal)
+                    ordinal += 1
                     cur['outs'] = outs[pi]
                     if self.opts.timing:
                         self.decode_batch(cur); self.collect_batch(cur)
@@ -357,25 +368,94 @@ class Filter:
             self.collect_batch(fin.popleft())
         return outs
 
-    def launch_batch(self, tb: TileBasis, segs, norm, tile_host):
-        """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
-        (basis_operand), one GEMM launch over all their windows (an item table of (segment, run of tiles)) with each
-        segment's template norm^2 applied to |z|^2 as a per-(segment, template) row scale, and the candidate list
-        (fp_filter.py:741-757) compacted on the device. Returns the batch state."""
-        o = self.opts
-        n, rp = len(tb.ids), tb.rp
-        K = 2 * rp
-        nb = len(segs)
-        dt = OPERAND_DTYPES[self.code]
-        t0 = self._sync()
+    def tile_norms(self, tb: TileBasis, key):
+        """sigma^2 and the norm of every template of the tile in every segment (fp_filter.py:710-714); with graphs
+        captured once per tile and replayed."""
+        work = 8 * len(tb.ids) * (tb.kend - self.kmin) + 4 * self.IPall.shape[0] * (tb.kend - self.kmin)
+        if not self.opts.graphs:
+            e0 = _dev.METER.start()
+            sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
+            _dev.METER.stop('sigmasq', e0, work)
+            return sig, norm
+        rec = self._graphs.get(('tile',) + key)
+        if rec is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self._capture_stream()):
+                sig, norm = tmpl.sigmasq(tb.H, self.IPall, self.df, self.kmin, tb.kend)
+            rec = self._graphs[('tile',) + key] = dict(graph=g, sig=sig, norm=norm)
+        rec['graph'].replay()
+        return rec['sig'], rec['norm']
+
+    def _capture_stream(self):
+        if getattr(self, '_cap', None) is None:
+            self._cap = torch.cuda.Stream()
+        return self._cap
+
+    def _batch_geometry(self, tb: TileBasis, segs):
+        """Host-side layout of one batch: rows of each segment in Zt, output columns, the GEMM item table and the
+        segments' indices into the norm table (device copies made here, outside any capture)."""
         Tns = [nw * self.cw for (_, _, _, _, nw) in segs]
         row0 = np.concatenate([[0], np.cumsum(Tns)]).astype(np.int64)
-        Zt, acs = self.basis_operand(tb, segs, Tns, row0)
-        t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
-        Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
-        ones = self._ones(n)
+        nws = [nw for (_, _, _, _, nw) in segs]
+        col0 = np.concatenate([[0], np.cumsum(nws)]).astype(np.int64)
+        per = self.cw // BN
+        items = fp_gemm.items_table([(int(row0[i]), Tns[i] // BN, int(col0[i]), i, Tns[i]) for i in range(len(segs))],
+                                    fp_gemm.tiles_per_item(self.opts.gemm_cfg, per))
         gsl = _dev.to_dev(np.array([gs for (_, _, gs, _, _) in segs], np.int64))
-        rows2 = (norm[:, gsl] ** 2).T.to(torch.float32)               # (nb, n): norm^2 per (segment, template)
+        return dict(Tns=Tns, row0=row0, nwsum=sum(nws), col0=col0, items=items, gsl=gsl, K=2 * tb.rp, per=per)
+
+    def _alloc_rotation(self, plans, segs, B):
+        """Three buffer sets, each sized for the largest batch of the run, for the captured stage-A graphs: batch i
+        of a run writes set i mod 3 (every reader of a batch's buffers is enqueued before batch i + 3 is)."""
+        zt = a = w = nmax = 0
+        for plan in plans:
+            for tb in plan.tiles:
+                n, K = len(tb.ids), 2 * tb.rp
+                for b0 in range(0, len(segs), B):
+                    sg = segs[b0:b0 + B]
+                    T = sum(nw * self.cw for (_, _, _, _, nw) in sg)
+                    nw = sum(x[4] for x in sg)
+                    zt, a, w, nmax = max(zt, T * K), max(a, n * K), max(w, n * nw), max(nmax, n)
+        nwork = int(_lib.load().gwgs_select_workspace(w)) + 1
+        dt = OPERAND_DTYPES[self.code]
+        self._bufs = [dict(zt=torch.empty(zt, dtype=dt, device='cuda'), ar=torch.empty(a, dtype=dt, device='cuda'),
+                           ai=torch.empty(a, dtype=dt, device='cuda'),
+                           vw=torch.empty(w, dtype=torch.float32, device='cuda'),
+                           iw=torch.empty(w, dtype=torch.int32, device='cuda'),
+                           sel=(torch.empty(w, dtype=torch.int32, device='cuda'),
+                                torch.empty(w, dtype=torch.int32, device='cuda'),
+                                torch.empty(w, dtype=torch.float32, device='cuda'),
+                                torch.empty(nwork, dtype=torch.int32, device='cuda'))) for _ in range(3)]
+        ws = max((_lib.load().gwgs_basis_series_workspace(tb.r, self.tlen) for plan in plans for tb in plan.tiles),
+                 default=0)
+        if ws > 0:
+            self._workbuf(ws)
+        self._ones(nmax)
+
+    def _batch_device(self, tb: TileBasis, segs, norm, geo, bufs=None):
+        """All device work of stage A (no host reads, no host-to-device copies): basis series, template operands,
+        row scales, the GEMM with the window epilogue and the candidate compaction. bufs: a rotation set to write
+        into (graph capture) or None (fresh allocations)."""
+        o = self.opts
+        n, rp, K = len(tb.ids), tb.rp, geo['K']
+        dt = OPERAND_DTYPES[self.code]
+        T, nwsum = int(geo['row0'][-1]), geo['nwsum']
+        if bufs is None:
+            Zt = Ar = Ai = vw = iw = sel = None
+        else:
+            Zt = bufs['zt'][:T * K].view(T, K)
+            Ar, Ai = bufs['ar'][:n * K].view(n, K), bufs['ai'][:n * K].view(n, K)
+            vw, iw = bufs['vw'][:n * nwsum].view(n, nwsum), bufs['iw'][:n * nwsum].view(n, nwsum)
+            sel = bufs['sel']
+        t0 = self._sync()
+        Zt, acs = self.basis_operand(tb, segs, geo['Tns'], geo['row0'], Zt)
+        t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
+        if Ar is None:
+            Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
+            vw = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')
+            iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
+        ones = self._ones(n)
+        rows2 = (norm[:, geo['gsl']] ** 2).T.to(torch.float32)         # (nb, n): norm^2 per (segment, template)
         if self.code == 2:
             pws = torch.empty(n, dtype=torch.float32, device='cuda')
             _lib.call('gwgs_stack_a_fp8', tb.A.data_ptr(), n, tb.r, ones.data_ptr(), acs.data_ptr(), K,
@@ -385,24 +465,54 @@ class Filter:
             _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, ones.data_ptr(), None, n, K,
                       Ar.data_ptr(), Ai.data_ptr(), _dev.stream())
         rows2 = rows2.contiguous()
-        per = self.cw // BN
-        nws = [nw for (_, _, _, _, nw) in segs]
-        nwsum = sum(nws)
-        col0 = np.concatenate([[0], np.cumsum(nws)]).astype(np.int64)
-        items = fp_gemm.items_table([(int(row0[i]), Tns[i] // BN, int(col0[i]), i, Tns[i]) for i in range(nb)],
-                                    fp_gemm.tiles_per_item(o.gemm_cfg, per))
-        vw = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')
-        iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
         e0 = _dev.METER.start()
-        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, items, 1 | fp_gemm.STACKED, per, o.gemm_cfg, vw,
-                                 iw, rows2)
+        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, geo['items'], 1 | fp_gemm.STACKED, geo['per'],
+                                 o.gemm_cfg, vw, iw, rows2)
         # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
-        _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(Tns)))
+        _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(geo['Tns'])))
         cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
-        sel = peak_kernel.select_peaks(vw, iw, float(cut))
+        idx, samp, val, cnt = peak_kernel.select_launch(vw, iw, float(cut), sel)
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
-        return dict(tb=tb, segs=segs, Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, nwsum=nwsum, col0=col0, row0=row0, K=K,
-                    sel=sel, tile_host=tile_host, t=t2)
+        return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt))
+
+    def launch_batch(self, tb: TileBasis, segs, norm, tile_host, key=None, ordinal=0):
+        """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
+        (basis_operand), one GEMM launch over all their windows (an item table of (segment, run of tiles)) with each
+        segment's template norm^2 applied to |z|^2 as a per-(segment, template) row scale, and the candidate list
+        (fp_filter.py:741-757) compacted on the device. With opts.graphs the stage is captured in a HIP graph on
+        the first run (into rotation set ordinal mod 3) and replayed afterwards. Returns the batch state."""
+        if self.opts.graphs:
+            rec = self._graphs.get(key)
+            if rec is None:
+                geo = self._batch_geometry(tb, segs)
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize()
+                with torch.cuda.graph(g, stream=self._capture_stream()):
+                    dev = self._batch_device(tb, segs, norm, geo, self._bufs[ordinal % 3])
+                rec = self._graphs[key] = dict(graph=g, geo=geo, dev=dev)
+            rec['graph'].replay()
+            geo, dev = rec['geo'], rec['dev']
+        else:
+            geo = self._batch_geometry(tb, segs)
+            dev = self._batch_device(tb, segs, norm, geo)
+        idx, samp, val, cnt = dev['sel']
+        sel = peak_kernel.Selection(idx, samp, val, cnt, _dev.mark())
+        return dict(tb=tb, segs=segs, Zt=dev['Zt'], Ar=dev['Ar'], Ai=dev['Ai'], rows2=dev['rows2'],
+                    nwsum=geo['nwsum'], col0=geo['col0'], row0=geo['row0'], K=geo['K'], sel=sel,
+                    tile_host=tile_host)
+
+    def rebind(self, datasets):
+        """New data with the same segmentation: D and IP are copied into this Filter's tensors (their addresses are
+        what captured graphs read); gates and start times are taken from the new datasets."""
+        assert len(datasets) == len(self.data)
+        for m, d in zip(self.data, datasets):
+            assert m.D.shape == d.D.shape and m.tlen == d.tlen and list(m.analyze) == list(d.analyze)
+            m.D.copy_(d.D)
+            m.IP.copy_(d.IP)
+            m.gates, m.gps_start, m.seg_start = list(d.gates), d.gps_start, list(d.seg_start)
+        torch.cat([m.IP for m in self.data], 0, out=self.IPall)
+        self.T = {k: 0 if isinstance(v, int) else 0.0 for k, v in self.T.items()}
+        self.rel = []
 
     def decode_batch(self, st):
         """Stage B1: read the batch's candidate list back (waiting only for its own stage A), split it by segment,

@@ -72,17 +72,27 @@ class Selection:
         return g.astype(np.int64), s, v
 
 
+def select_launch(vw, iw, cut, out=None):
+    """Enqueue the one-pass compaction of vw > cut (fp_filter.py:747-757): returns device (idx, samp, val, count)
+    with capacity vw.numel(); out: preallocated (idx, samp, val, work) to write into (graph capture)."""
+    total = vw.numel()
+    nwork = _lib.load().gwgs_select_workspace(total)
+    cap = max(total, 1)
+    if out is None:
+        work = torch.empty(int(nwork) + 1, dtype=torch.int32, device=vw.device)
+        idx = torch.empty(cap, dtype=torch.int32, device=vw.device)
+        samp = torch.empty(cap, dtype=torch.int32, device=vw.device)
+        val = torch.empty(cap, dtype=torch.float32, device=vw.device)
+    else:
+        idx, samp, val, work = (x[:n] for x, n in zip(out, (cap, cap, cap, int(nwork) + 1)))
+    cnt = work[int(nwork):int(nwork) + 1]
+    _lib.call('gwgs_select_peaks', vw.data_ptr(), iw.data_ptr(), total, float(cut), idx.data_ptr(), samp.data_ptr(),
+              val.data_ptr(), cap, work.data_ptr(), cnt.data_ptr(), _dev.stream())
+    return idx, samp, val, cnt
+
+
 def select_peaks(vw, iw, cut):
     """fp_filter.py:747-757 on the device: entries of vw > cut (strict, row-major order) with their window samples
     iw and values, compacted in one pass (capacity: every entry). Returns a Selection without waiting."""
-    total = vw.numel()
-    nwork = _lib.load().gwgs_select_workspace(total)
-    work = torch.empty(int(nwork) + 1, dtype=torch.int32, device=vw.device)
-    cnt = work[-1:]
-    cap = max(total, 1)
-    idx = torch.empty(cap, dtype=torch.int32, device=vw.device)
-    samp = torch.empty(cap, dtype=torch.int32, device=vw.device)
-    val = torch.empty(cap, dtype=torch.float32, device=vw.device)
-    _lib.call('gwgs_select_peaks', vw.data_ptr(), iw.data_ptr(), total, float(cut), idx.data_ptr(), samp.data_ptr(),
-              val.data_ptr(), cap, work.data_ptr(), cnt.data_ptr(), _dev.stream())
+    idx, samp, val, cnt = select_launch(vw, iw, cut)
     return Selection(idx, samp, val, cnt, _dev.mark())
