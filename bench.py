@@ -260,7 +260,15 @@ def main():
     import _dev, fp_filter, workloads
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     t_setup = time.perf_counter()
-    cfg = tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None
+    import fp_gemm
+    tuned = None
+    if a.gemm_cfg:
+        cfg = tuple(int(v) for v in a.gemm_cfg.split(','))
+    else:
+        # the autotuner on the filter's shape: a 512-template tile, K = 2 x 416, two 448 s segments of windows
+        cfg, tuned = fp_gemm.pick_cfg(M=512, K=832, Tn=2 * 448 * 2048, dtype=torch.bfloat16)
+        log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
+                                          for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
                                  refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32')}
     if a.workload == 'batched':
@@ -348,7 +356,8 @@ def main():
         ms_per_step=dt * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
         data='synthetic',
         config=dict(workload=wl.describe(a.tile), templates=ntemp, analysed_s=an_s, tile=a.tile,
-                    ranks=wl.ranks(), setup_s=round(setup_s, 1)),
+                    ranks=wl.ranks(), setup_s=round(setup_s, 1), gemm_cfg=list(cfg) if cfg else None,
+                    autotune_ms={k: round(v * 1e3, 3) for k, v in (tuned or {}).items() if isinstance(v, float)}),
         roofline=dict(bound='mfma', achieved=fl / t_k / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
                       frac=fl / t_k / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
                       kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=n_l,

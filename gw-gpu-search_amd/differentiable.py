@@ -1,0 +1,138 @@
+"""Differentiable peak SNR of compressed templates and refinement of loud triggers off the discrete bank.
+
+The rebuilt SNR of fp_filter.py:759-764 depends on a template only through its coefficient vector a in the tile's
+SVD basis (z = sum_j a_j Zc_j). Here it is evaluated at a continuous time t + delta, exactly, from the frequency domain
+(csrc/refine.hip phasor_data_k, then the basis response R = E Vc^T as a plain GEMM), normalised by the template's own
+sigma (sigma^2 = 4 df a^T G conj(a), G the tile's basis metric under the segment's 1/psd), and differentiated by a
+hand-written kernel (snr_grad_k): rho^2 and its gradients with respect to a (2r real coordinates), delta and the
+phase. PeakSNR2 wraps it as a torch.autograd.Function; refine() runs projected gradient ascent on the convex hull of
+the nearest bank templates' coefficient vectors (and on delta), the search the north star names for the loudest
+triggers.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import _dev
+import _lib
+
+
+class ResponseContext:
+    """One (tile, segment): the tile's basis Vc over [kmin, kend) (transposed for the GEMM), the segment's data D and
+    1/psd IP, and the metric G[j, j'] = sum_k Vc[j, k] conj(Vc[j', k]) IP[k] (formed in complex128)."""
+
+    def __init__(self, tb, D, IP, df, tlen, kmin):
+        self.k0, k1 = int(kmin), int(tb.kend)
+        self.kb = k1 - self.k0
+        self.r, self.n, self.df = tb.r, int(tlen), float(df)
+        V = tb.Vc[:tb.r, self.k0:k1]
+        self.Vt = V.T.contiguous()                                              # (kb, r) complex64
+        V64 = V.to(torch.complex128)
+        self.G = ((V64 * IP[self.k0:k1].to(torch.float64)) @ V64.conj().T).to(torch.complex64).contiguous()
+        self.D = D
+
+    def response(self, t, delta):
+        """Basis responses R, dR/ddelta (npk, r) complex64 at times t + delta (t int64, delta float64 tensors)."""
+        npk = t.shape[0]
+        E = torch.empty((npk, self.kb), dtype=torch.complex64, device='cuda')
+        Ed = torch.empty_like(E)
+        td, dd = t.to('cuda', torch.int64).contiguous(), delta.to('cuda', torch.float64).contiguous()
+        _lib.call('gwgs_phasor_data', self.D.data_ptr(), self.k0, self.kb, td.data_ptr(), dd.data_ptr(), self.n, npk,
+                  E.data_ptr(), Ed.data_ptr(), self.kb, _dev.stream())
+        return E @ self.Vt, Ed @ self.Vt
+
+
+def snr2(ctx: ResponseContext, a, t, delta, phi=None, want_grad=True):
+    """rho^2, z, d rho^2 / d delta, the phase derivative and (want_grad) d rho^2 / d a for coefficient vectors
+    a (npk, r) complex64 at times t + delta. Device tensors."""
+    npk, r = a.shape
+    assert r == ctx.r
+    R, Rd = ctx.response(t, delta)
+    a = a.contiguous()
+    rho2 = torch.empty(npk, dtype=torch.float32, device='cuda')
+    z = torch.empty(npk, dtype=torch.complex64, device='cuda')
+    dt = torch.empty_like(rho2)
+    dphi = torch.empty_like(rho2)
+    grad = torch.empty((npk, r), dtype=torch.complex64, device='cuda') if want_grad else None
+    ph = None if phi is None else phi.to('cuda', torch.float32).contiguous()
+    _lib.call('gwgs_snr_grad', R.data_ptr(), Rd.data_ptr(), r, a.data_ptr(), r, ctx.G.data_ptr(), r, r, npk,
+              ctx.df, _dev.ptr(ph), rho2.data_ptr(), z.data_ptr(), dt.data_ptr(), dphi.data_ptr(), _dev.ptr(grad), r,
+              _dev.stream())
+    return dict(rho2=rho2, z=z, drho2_ddelta=dt, drho2_dphi=dphi, grad=grad)
+
+
+class PeakSNR2(torch.autograd.Function):
+    """rho^2(a, delta): a as real (npk, r, 2) float32 (Re, Im), delta (npk,) float64; ctx and t are constants. The
+    kernel returns the gradients with the value; backward scales them by the incoming gradient."""
+
+    @staticmethod
+    def forward(fctx, a_ri, delta, ctx, t):
+        out = snr2(ctx, torch.view_as_complex(a_ri.detach().contiguous()), t, delta.detach())
+        fctx.save_for_backward(torch.view_as_real(out['grad']), out['drho2_ddelta'])
+        return out['rho2']
+
+    @staticmethod
+    def backward(fctx, g):
+        ga, gt = fctx.saved_tensors
+        return ga * g[:, None, None], (gt * g).to(torch.float64), None, None
+
+
+def _project_simplex(w):
+    """Euclidean projection of each row of w onto the probability simplex (sort-based)."""
+    u, _ = torch.sort(w, dim=1, descending=True)
+    css = u.cumsum(1) - 1
+    ind = torch.arange(1, w.shape[1] + 1, device=w.device, dtype=w.dtype)
+    cond = u - css / ind > 0
+    rho = cond.to(torch.int64).cumsum(1).argmax(1, keepdim=True)               # last index where cond holds
+    theta = css.gather(1, rho) / (rho + 1).to(w.dtype)
+    return (w - theta).clamp(min=0)
+
+
+def refine(ctx: ResponseContext, A, rows, t, neighbours=8, steps=20, step0=0.5, dmax=1.5):
+    """Projected gradient ascent of rho^2 from bank templates `rows` (of the tile's coefficient matrix A (n, r)
+    complex64) at integer samples t: a = sum_i w_i A[nb_i] over the `neighbours` templates nearest in normalised
+    coefficient space, each phase-aligned to the starting template (w on the simplex: the bank's local coefficient
+    hull), delta in [-dmax, dmax]. Each step is
+    normalised by rho^2 and accepted per trigger only if rho^2 grows (the step then grows by 1.5, else halves).
+    Returns dict(rho2_start, rho2, w, nb, delta, a)."""
+    rows = torch.as_tensor(rows, device='cuda', dtype=torch.int64)
+    t = torch.as_tensor(t, device='cuda', dtype=torch.int64)
+    npk = rows.shape[0]
+    An = A / A.abs().pow(2).sum(1, keepdim=True).sqrt().clamp(min=1e-30)
+    ov = An[rows] @ An.conj().T                                                 # (npk, n) overlaps in the basis
+    L = min(neighbours, A.shape[0])
+    nb = ov.abs().topk(L, dim=1).indices                                       # the row itself first
+    # each neighbour's overall phase aligned to the starting template, so that convex combinations add coherently
+    ph = torch.gather(ov, 1, nb).conj()
+    ph = ph / ph.abs().clamp(min=1e-30)
+    Ari = torch.view_as_real((A[nb] * ph.conj()[:, :, None]).contiguous())     # (npk, L, r, 2)
+    w = torch.zeros((npk, L), dtype=torch.float32, device='cuda')
+    w[torch.arange(npk, device='cuda'), (nb == rows[:, None]).to(torch.int64).argmax(1)] = 1.0
+    delta = torch.zeros(npk, dtype=torch.float64, device='cuda')
+    eta = torch.full((npk,), step0, dtype=torch.float64, device='cuda')
+
+    def value_grad(w, delta):
+        wv = w.clone().requires_grad_(True)
+        dv = delta.clone().requires_grad_(True)
+        a_ri = (wv[:, :, None, None] * Ari).sum(1)
+        rho2 = PeakSNR2.apply(a_ri, dv, ctx, t)
+        rho2.sum().backward()
+        return rho2.detach(), wv.grad.detach(), dv.grad.detach()
+
+    rho2, gw, gd = value_grad(w, delta)
+    start = rho2.clone()
+    for _ in range(steps):
+        scale = (eta / rho2.to(torch.float64).clamp(min=1e-30))
+        w_new = _project_simplex(w + (scale[:, None] * gw.to(torch.float64)).to(torch.float32))
+        d_new = (delta + scale * gd.to(torch.float64)).clamp(-dmax, dmax)
+        r_new, gw_new, gd_new = value_grad(w_new, d_new)
+        up = r_new > rho2
+        w = torch.where(up[:, None], w_new, w)
+        delta = torch.where(up, d_new, delta)
+        gw = torch.where(up[:, None], gw_new, gw)
+        gd = torch.where(up, gd_new, gd)
+        rho2 = torch.where(up, r_new, rho2)
+        eta = torch.where(up, eta * 1.5, eta * 0.5)
+    a = torch.view_as_complex((w[:, :, None, None] * Ari).sum(1).contiguous())
+    return dict(rho2_start=start, rho2=rho2, w=w, nb=nb, delta=delta, a=a)

@@ -144,35 +144,46 @@ TPB_CANDIDATES = (16, 32, 64, 128)
 VARIANTS = (0, 1, 2, 3)
 
 
-def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=2048, reps=3):
-    """Time every candidate launch shape once warm and keep the fastest on a representative shape (the reference's
-    sweep, fp_gemm.py:47-64). Candidates: time tiles per block (window mode needs a multiple of cw / BN). Returns
-    (cfg, {str(cfg): seconds or error}). Raises RuntimeError if none runs."""
+def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=2048, reps=3, stacked=True):
+    """The project's autotuner (the reference's sweep, fp_gemm.py:47-64): time every candidate launch shape once warm
+    and keep the fastest on a representative shape. Candidates: time tiles per work item (window mode needs a
+    multiple of cw / BN) x kernel variant. stacked: complex-stacked operands (Ai derived from Ar, mode | STACKED),
+    the filter's own case. Returns (cfg, {str(cfg): seconds or error}); raises RuntimeError if none runs."""
     g = torch.Generator(device='cpu').manual_seed(0)
-    Ar = torch.randn((M, K), generator=g).to(dev, dtype)
-    Ai = torch.randn((M, K), generator=g).to(dev, dtype)
+    K = _kpad(K) if K % 2 == 0 else _kpad(K + 1)
+    A = torch.randn((M, K), generator=g)
+    Ai = torch.empty_like(A)
+    Ai[:, 0::2], Ai[:, 1::2] = -A[:, 1::2], A[:, 0::2]
+    code = DTYPES[dtype]
     Tn = Tn // cw * cw
-    Zs = torch.randn((Tn, K), generator=g).to(dev, dtype)
+    Zs = torch.randn((Tn, K), generator=g)
+    if code == 2:
+        Ar_d, Ai_d, Z_d = (x.clamp(-448, 448).to(dev).to(torch.float8_e4m3fn).view(torch.uint8) for x in (A, Ai, Zs))
+    else:
+        Ar_d, Ai_d, Z_d = (x.to(dev, dtype) for x in (A, Ai, Zs))
     per = cw // BN
+    V = torch.empty((M, Tn // cw), device=dev); I = torch.empty((M, Tn // cw), dtype=torch.int32, device=dev)
+    mode = 1 | (STACKED if stacked else 0)
     best, res = None, {}
     for tpb in TPB_CANDIDATES:
-      for variant in VARIANTS:
-        c = (max(per, tpb // per * per), variant)
-        if str(c) in res:
-            continue
-        try:
-            fused_window_peaks(Ar, Ai, Zs, M, Tn, cw, c, zt=True)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                fused_window_peaks(Ar, Ai, Zs, M, Tn, cw, c, zt=True)
-            torch.cuda.synchronize()
-            res[str(c)] = (time.perf_counter() - t0) / reps
-        except Exception as ex:   # a shape that fails is recorded, not fatal
-            res[str(c)] = repr(ex)[:300]
-            continue
-        if best is None or res[str(c)] < res[str(best)]:
-            best = c
+        for variant in VARIANTS:
+            c = (max(per, tpb // per * per), variant)
+            if str(c) in res:
+                continue
+            try:
+                items = items_table([(0, Tn // BN, 0, 0, Tn)], c[0])
+                gemm_peaks_multi(code, Z_d, K, Ar_d, Ai_d, K, M, K, items, mode, per, c, V, I)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    gemm_peaks_multi(code, Z_d, K, Ar_d, Ai_d, K, M, K, items, mode, per, c, V, I)
+                torch.cuda.synchronize()
+                res[str(c)] = (time.perf_counter() - t0) / reps
+            except Exception as ex:   # a shape that fails is recorded, not fatal
+                res[str(c)] = repr(ex)[:300]
+                continue
+            if best is None or res[str(c)] < res[str(best)]:
+                best = c
     if best is None:
         raise RuntimeError('no GEMM config ran: %r' % res)
     return best, res

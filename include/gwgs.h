@@ -167,6 +167,19 @@ int gwgs_gate_mag(const double* wt, long pad_start, long n, long cl, double df, 
 /* gate_taper (gpu_prep.py:212-231): x[offset + i] *= inverted Tukey window(i), i in [i1, i2) */
 int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, long pad, long win, void* stream);
 
+/* ---- differentiable refinement (no reference function: the backward kernel of the peak SNR; its forward value
+ * is pipeline/fp_filter.py:759-764's rebuilt SNR, evaluated off the sample grid and off the bank) -------------------
+ * E[b, i] = D[k0 + i] e^{2 pi i (k0 + i)(t[b] + delta[b]) / n}, Ed = (2 pi i (k0 + i) / n) E (npk, lde) complex64:
+ * the data phased to each peak's time; R = E Vc^T is the basis response (a plain GEMM). */
+int gwgs_phasor_data(const void* D, long k0, long kb, const long long* t, const double* delta, long long n, int npk,
+                     void* E, void* Ed, long lde, void* stream);
+/* Per peak (one wave): rho^2 = |z|^2 (4 df)^2 / sigma^2 with z = a . R, sigma^2 = 4 df a^T G conj(a); z; the delta
+ * derivative of rho^2; the phi derivative of Re(e^{-i phi} z)^2 (4 df)^2 / sigma^2 (phi NULL: phi = arg z); and
+ * (grad != NULL) d rho^2 / d a as complex gradients d/dRe + i d/dIm. Double accumulation. */
+int gwgs_snr_grad(const void* R, const void* Rd, long ldr, const void* a, long lda, const void* G, long ldg, int r,
+                  int npk, double df, const float* phi, float* rho2, void* z, float* drho2_dt, float* drho2_dphi,
+                  void* grad, long ldgr, void* stream);
+
 /* ---- time slides (pipeline/coinc_rank.py:100-138 find_pairs) -------------------------------------------------
  * L triggers sorted by key lkey = tid * big + t (with copies at t - P and t + P), lt their unwrapped times, li their
  * indices; per H trigger (key hkey, time hx) the pairs with |tH - tL - k step| <= window, |k| <= K, within

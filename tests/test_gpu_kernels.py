@@ -421,3 +421,32 @@ def test_vd_power_vs_float64(gpu):
     _lib.call('gwgs_vd_power', Vd.data_ptr(), flen, r, Dd.data_ptr(), kmin, kend, out.data_ptr(), _dev.stream())
     exp = (np.abs(Vc[:, kmin:kend].astype(np.complex128) * D[kmin:kend]) ** 2).sum(1)
     np.testing.assert_allclose(out.cpu().numpy(), exp, rtol=1e-12)
+
+
+def test_autotuner_picks_a_running_config(gpu):
+    """fp_gemm.pick_cfg (the reference's sweep, fp_gemm.py:47-64) on a small stacked shape: every candidate runs,
+    the chosen one is among the fastest, and every variant gives the same window maxima and samples to fp32
+    summation order (variants differ only in how operands reach the MFMAs)."""
+    import fp_gemm
+    cfg, res = fp_gemm.pick_cfg(M=96, K=160, Tn=8 * 2048, reps=1)
+    times = {k: v for k, v in res.items() if isinstance(v, float)}
+    assert len(times) == len(res), res
+    assert res[str(cfg)] == min(times.values())
+    rng = np.random.default_rng(3)
+    M, r, Tn, per = 96, 80, 4 * 2048, 16
+    K = 2 * r
+    A = rng.standard_normal((M, r)) + 1j * rng.standard_normal((M, r))
+    Ar = np.zeros((M, K), np.float32); Ai = np.zeros((M, K), np.float32)
+    Ar[:, 0::2], Ar[:, 1::2] = A.real, -A.imag
+    Ai[:, 0::2], Ai[:, 1::2] = A.imag, A.real
+    Z = rng.standard_normal((Tn, K)).astype(np.float32)
+    qa, qi, qz = (torch.from_numpy(x).cuda().to(torch.bfloat16) for x in (Ar, Ai, Z))
+    outs = []
+    for variant in fp_gemm.VARIANTS:
+        V = torch.empty((M, Tn // 2048), device='cuda'); I = torch.empty_like(V, dtype=torch.int32)
+        items = fp_gemm.items_table([(0, Tn // 128, 0, 0, Tn)], 32)
+        fp_gemm.gemm_peaks_multi(1, qz, K, qa, qi, K, M, K, items, 1 | fp_gemm.STACKED, per, (32, variant), V, I)
+        outs.append((V.cpu().numpy(), I.cpu().numpy()))
+    for V, I in outs[1:]:
+        np.testing.assert_allclose(V, outs[0][0], rtol=1e-5)
+        assert (I == outs[0][1]).mean() > 0.99
