@@ -129,6 +129,65 @@ def cpu_baseline(streams, banks, cond, ntemp=128):
                        f'templates/SVD excluded ({T["templates_s"] + T["svd_s"]:.1f} s)')
 
 
+def refinement_report(plans, streams, ds, outs, top, neighbours=8, steps=20):
+    """The differentiable refinement on a step's triggers: the `top` loudest exact triggers refined off the bank
+    (differentiable.refine_triggers, `steps` projected-gradient steps on the hull of `neighbours` templates), timed;
+    the gain rho / rho_start (the compressed template's SNR before and after) over all of them, and per injection the
+    loudest exact trigger within 0.1 s of its merger time (the injections' masses are drawn off the bank)."""
+    import numpy as np
+    import torch
+    import differentiable
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = differentiable.refine_triggers(plans, ds, outs, top=top, neighbours=neighbours, steps=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    gain = r['rho'] / np.maximum(r['rho_start'], 1e-30)
+    inj = []
+    for d, st in enumerate(streams):
+        for j in st.injections:
+            near = np.flatnonzero((r['dataset'] == d) & (np.abs(r['end_time'] - (st.gps_start + j.t0)) < 0.1))
+            if len(near):
+                k = near[np.argmax(r['snr'][near])]
+                inj.append(dict(det=st.ifo, t0=round(j.t0, 3), m1=round(j.m1, 2), m2=round(j.m2, 2),
+                                snr_optimal=round(j.snr, 2), exact_snr=round(float(r['snr'][k]), 4),
+                                nearest=round(float(r['rho_start'][k]), 4), refined=round(float(r['rho'][k]), 4),
+                                gain=float(gain[k]), delta_samples=round(float(r['delta'][k]), 3)))
+    ig = np.array([x['gain'] for x in inj])
+    return dict(n_refined=int(len(gain)), steps=steps, neighbours=neighbours, seconds=dt,
+                triggers_per_s=len(gain) / max(dt, 1e-9),
+                gain_median=float(np.median(gain)) if len(gain) else None,
+                gain_max=float(gain.max()) if len(gain) else None,
+                frac_improved=float((gain > 1 + 1e-6).mean()) if len(gain) else None,
+                injections_found=len(inj), injection_gain_median=float(np.median(ig)) if len(ig) else None,
+                injection_gain_max=float(ig.max()) if len(ig) else None, injections=inj)
+
+
+def slides_report(ds, outs, names, K=500, step=0.1, window=0.012):
+    """Time slides over the two detectors' trigger lists of one step, per bank (slides.find_pairs, the reference's
+    coinc_rank.find_pairs with its defaults: 0.1 s step, 12 ms window): L1 shifted by k step for |k| <= K over the
+    analysed span P, same template, ranked by the quadrature sum of newsnr. Times the pair search."""
+    import numpy as np
+    import torch
+    import slides
+    gps0 = min(d.gps_start for d in ds)
+    P = max(d.gps_start + (d.seg_start[-1] + d.tlen) / d.sr for d in ds) - gps0
+    res = []
+    for b, name in enumerate(names):
+        H, L = outs[b][0], outs[b][1]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hh, ll, kk, stat = slides.find_pairs(dict(x=H['end_time'] - gps0, tid=H['template_id']),
+                                             dict(x=L['end_time'] - gps0, tid=L['template_id']), P, step, K, window,
+                                             stat_fn=lambda h, l, dt: np.sqrt(H['newsnr'][h] ** 2 +
+                                                                              L['newsnr'][l] ** 2))
+        dt = time.perf_counter() - t0
+        res.append(dict(bank=name, slides=2 * K, span_s=P, triggers=(len(H['snr']), len(L['snr'])), pairs=len(hh),
+                        zero_lag_pairs=int((kk == 0).sum()), loudest_zero_lag=float(stat[kk == 0].max())
+                        if (kk == 0).any() else None, seconds=dt))
+    return res
+
+
 class ProductionWorkload:
     """2 detectors x 512 s at 2048 Hz (lines, glitches, 20 injections), both banks; one 512 s segment each."""
 
@@ -247,6 +306,10 @@ def main():
     ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
     ap.add_argument('--graphs', action='store_true', help='capture stage A of every batch in HIP graphs (replayed '
                     'from the second step on; production workload)')
+    ap.add_argument('--refine-top', type=int, default=10000, help='refine this many of the loudest exact triggers '
+                    'off the bank after the timed steps (production workload; 0: skip)')
+    ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
+                    'trigger lists after the timed steps (production workload; 0: skip)')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); the '
@@ -370,6 +433,12 @@ def main():
         snr_rel_err_vs_fp64=dict(bf16=err, fp32=err32, n_checked=nchk),
         trigger_diff_bf16_vs_fp32=diffs,
     )
+    if a.workload == 'production' and a.refine_top > 0:
+        log('refinement of the loudest triggers')
+        res['refinement'] = refinement_report(wl.plans, wl.streams, ds, outs_bf, a.refine_top)
+    if a.workload == 'production' and a.slides > 0:
+        log('time slides')
+        res['time_slides'] = slides_report(ds, outs_bf, [b.name for b in banks], K=a.slides)
     if not a.no_cpu:
         log('cpu baseline')
         streams, banks_p, cond = workloads.production()

@@ -75,6 +75,27 @@ __device__ __forceinline__ uint4 rot_fp8(const uint4 v)
     return make_uint4(rot_fp8x4(v.x), rot_fp8x4(v.y), rot_fp8x4(v.z), rot_fp8x4(v.w));
 }
 
+// The same identity on the template side: Ai[m, 2j] = -Ar[m, 2j+1], Ai[m, 2j+1] = Ar[m, 2j], i.e. each (lo, hi) pair
+// of an Ar fragment becomes (-hi, lo). A wave rotates its own template fragment once per k step and uses it for all
+// of its time sub-tiles, a quarter of the rotations of the Zt-side form.
+__device__ __forceinline__ uint32_t arot_bf16x2(const uint32_t w)
+{
+    return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x00008000u;     // (lo, hi) -> (-hi, lo)
+}
+__device__ __forceinline__ uint4 arot_bf16(const uint4 v)
+{
+    return make_uint4(arot_bf16x2(v.x), arot_bf16x2(v.y), arot_bf16x2(v.z), arot_bf16x2(v.w));
+}
+__device__ __forceinline__ uint32_t arot_fp8x4(const uint32_t w)
+{
+    // v_perm_b32 byte select (1, 0, 3, 2): swap the bytes of each 16-bit pair, then negate the new low byte
+    return __builtin_amdgcn_perm(w, w, 0x02030001u) ^ 0x00800080u;
+}
+__device__ __forceinline__ uint4 arot_fp8(const uint4 v)
+{
+    return make_uint4(arot_fp8x4(v.x), arot_fp8x4(v.y), arot_fp8x4(v.z), arot_fp8x4(v.w));
+}
+
 struct Job {
     const GemmItem* items;   // null: one segment at row 0, items derived from tpb, ntile, Tn
     int nitems, nmb, tpb, ntile, Tn, M, per;
@@ -559,6 +580,257 @@ gemm_peaks_kloop_ra_k(const T* __restrict__ Zt, const long ldz, const T* __restr
     }
 }
 
+// Complex-stacked K loop with two LDS stages: the chunk after next is fetched into registers while the current
+// chunk's MFMAs run and is written to the idle stage after them, so a chunk costs one barrier and its global loads
+// have a whole chunk of MFMAs to land. Only Zt and Ar are staged (73,728 bytes of LDS per workgroup, two workgroups
+// per CU); each wave forms its imaginary-part template fragment from its Ar fragment in registers (arot_*).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(NT, 2)
+gemm_peaks_ring_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const long lda, const int K,
+                  const Job J, float* __restrict__ V, int* __restrict__ I, const long ldv)
+{
+    static_assert(sizeof(T) <= 2, "bf16 or fp8 operands");
+    using TR = KloopTraits<T>;
+    constexpr int KC = TR::KC, EPV = TR::EPV, PITCH = KC + TR::PAD;
+    constexpr int CPR = KC / EPV;
+    constexpr int CPT = (BT * CPR) / NT;
+    constexpr int STAGE = BT * PITCH;              // elements per operand per stage (BT == BMT rows each)
+    static_assert((BT * CPR) % NT == 0 && BT == BMT, "tile shape");
+    __shared__ __attribute__((aligned(16))) T zs[2 * STAGE];
+    __shared__ __attribute__((aligned(16))) T as[2 * STAGE];
+
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m0 = mb * BMT;
+    const int m = m0 + wave * 32 + col;
+    const int M = J.M;
+    const int nkc = (K + KC - 1) / KC;
+    const int nsteps = (it.t_end - it.t_first) * nkc;
+    if (nsteps <= 0) return;                       // uniform over the workgroup
+
+    uint4 rz[CPT], rr[CPT];
+    auto gload = [&](const int step) {
+        const int tt = it.t_first + step / nkc, kc = (step % nkc) * KC;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = kc + (c % CPR) * EPV;
+            const bool kin = cc < K;
+            rz[j] = kin ? *(const uint4*)(Zt + ((size_t)it.zrow0 + (size_t)tt * BT + row) * ldz + cc)
+                        : make_uint4(0, 0, 0, 0);
+            rr[j] = kin && (m0 + row) < M ? *(const uint4*)(Ar + (size_t)(m0 + row) * lda + cc)
+                                          : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto sstore = [&](const int stage) {
+        T* z = zs + stage * STAGE;
+        T* a = as + stage * STAGE;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * NT, row = c / CPR, cc = (c % CPR) * EPV;
+            *(uint4*)(&z[row * PITCH + cc]) = rz[j];
+            *(uint4*)(&a[row * PITCH + cc]) = rr[j];
+        }
+    };
+
+    f32x16 cr[4], ci[4];
+    float best = -1.f;
+    int bi = it.t_first * BT;
+    gload(0);
+    sstore(0);
+    if (nsteps > 1) gload(1);
+    __syncthreads();
+    for (int step = 0; step < nsteps; step++) {
+        const int kcix = step % nkc, tt = it.t_first + step / nkc;
+        const T* z = zs + (step & 1) * STAGE;
+        const T* a = as + (step & 1) * STAGE;
+        if (kcix == 0) {
+#pragma unroll
+            for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+            }
+        }
+        if constexpr (sizeof(T) == 1) {
+#pragma unroll 1
+            for (int ks = 0; ks < KC / 64; ks++) {
+                const T* pa = &a[(wave * 32 + col) * PITCH + ks * 64 + half * 32];
+                const uint4 a0 = *(const uint4*)pa, a1 = *(const uint4*)(pa + 16);
+                const i32x8 a_r = join8(a0, a1), a_i = join8(arot_fp8(a0), arot_fp8(a1));
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const T* pz = &z[(ts * 32 + col) * PITCH + ks * 64 + half * 32];
+                    const i32x8 zf = join8(*(const uint4*)pz, *(const uint4*)(pz + 16));
+                    cr[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(zf, a_r, cr[ts], 0, 0, 0, 127, 0, 127);
+                    ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(zf, a_i, ci[ts], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KC / 16; ks++) {
+                const uint4 av = *(const uint4*)(&a[(wave * 32 + col) * PITCH + ks * 16 + half * 8]);
+                const bf16x8 a_r = __builtin_bit_cast(bf16x8, av), a_i = __builtin_bit_cast(bf16x8, arot_bf16(av));
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const bf16x8 zf = __builtin_bit_cast(bf16x8, *(const uint4*)(&z[(ts * 32 + col) * PITCH + ks * 16 + half * 8]));
+                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_r, cr[ts], 0, 0, 0);
+                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_i, ci[ts], 0, 0, 0);
+                }
+            }
+        }
+        if (step + 1 < nsteps) {
+            sstore((step + 1) & 1);
+            if (step + 2 < nsteps) gload(step + 2);
+        }
+        if (kcix == nkc - 1) {
+            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+        }
+        __syncthreads();
+    }
+}
+
+// Complex-stacked K loop fed by LDS DMA (global_load_lds_dwordx4): the hardware writes each 16-byte piece straight
+// from L2 into LDS, so no staging registers are held and S stages of 64-byte rows (KC = 32 bf16 / 64 fp8 per row)
+// are in flight at once: while stage s is multiplied, stages s+1 .. s+S-1 load. Each wave issues 4 DMA instructions
+// per stage (2 for its 32 Zt rows, 2 for its 32 Ar rows; one instruction = 64 lanes x 16 bytes = 16 rows) and waits
+// for them with an explicit vmcnt before the stage barrier. The 16-byte slots of row r are stored XOR-swizzled by
+// ((r >> 1) ^ (r >> 2)) & 3, which keeps the MFMA fragment reads (32 consecutive rows, one slot) free of LDS bank
+// conflicts; the DMA side chooses which global piece each lane loads, so the swizzle costs nothing there.
+// The DMA instructions are inline assembly: the compiler does not count them in its vmcnt bookkeeping, so the loop
+// holds no compiler-issued vector memory load (the row scale is read before it) and every wait on the stages is
+// written out here. The kernel ends only after its last DMA has landed (an exiting workgroup's LDS may be reassigned).
+constexpr int DMA_STAGES = 3;
+
+__device__ __forceinline__ int dma_swizzle(const int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+__device__ __forceinline__ void dma_load16(const uint32_t lds_addr, const uint32_t voff, const void* sbase)
+{
+    // M0 holds the wave's LDS destination (lane i writes M0 + 16 i); the global address is sbase + voff
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" :: "s"(lds_addr), "v"(voff),
+                 "s"(sbase) : "memory");
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(NT, 2)
+gemm_peaks_dma_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const long lda, const int K,
+                 const Job J, float* __restrict__ V, int* __restrict__ I, const long ldv)
+{
+    static_assert(sizeof(T) <= 2, "bf16 or fp8 operands");
+    constexpr int S = DMA_STAGES;
+    constexpr int ROWB = 64;                       // bytes per row per stage
+    constexpr int KC = ROWB / (int)sizeof(T);      // k per stage
+    constexpr int OPB = BT * ROWB, STB = 2 * OPB;  // operand and stage bytes (Zt rows, then Ar rows)
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[S * STB];
+
+    int mb;
+    GemmItem it;
+    if (!block_item(J, mb, it)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int m0 = mb * BMT;
+    const int m = m0 + wave * 32 + col;
+    const int M = J.M;
+    const int nkc = K / KC;
+    const int ntl = it.t_end - it.t_first;
+    if (ntl <= 0 || nkc <= 0) return;              // uniform over the workgroup, before any DMA
+    const float sc = (J.rowscale && m < M) ? J.rowscale[(size_t)it.seg * M + m] : 1.f;
+
+    // DMA lane constants: instruction i of this wave fills rows (2 wave + i) 16 .. + 15, lane -> (row, slot)
+    uint32_t zoff[2], aoff[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int R = (wave * 2 + i) * 16 + (lane >> 2);
+        const int c = (lane & 3) ^ dma_swizzle(R);
+        zoff[i] = (uint32_t)((size_t)R * ldz * sizeof(T) + c * 16);
+        aoff[i] = (uint32_t)((size_t)(min(m0 + R, M - 1) - m0) * lda * sizeof(T) + c * 16);   // rows >= M repeat M-1
+    }
+    const uint32_t lds0 = (uint32_t)(uintptr_t)lds + __builtin_amdgcn_readfirstlane((uint32_t)(wave * 2 * 1024));
+    // issue cursor: the next chunk to load (its tile's first Zt row, its k chunk) and its LDS stage
+    const uint8_t* zrow = (const uint8_t*)(Zt + ((size_t)it.zrow0 + (size_t)it.t_first * BT) * ldz);
+    const uint8_t* const arow = (const uint8_t*)(Ar + (size_t)m0 * lda);
+    const size_t tile_bytes = (size_t)BT * ldz * sizeof(T);
+    int ikc = 0, ist = 0;
+    auto issue = [&]() {
+        const uint8_t* zb = zrow + ikc * ROWB;
+        const uint8_t* ab = arow + ikc * ROWB;
+        const uint32_t st = lds0 + (uint32_t)(ist * STB);
+        dma_load16(st, zoff[0], zb);
+        dma_load16(st + 1024, zoff[1], zb);
+        dma_load16(st + OPB, aoff[0], ab);
+        dma_load16(st + OPB + 1024, aoff[1], ab);
+        if (++ikc == nkc) { ikc = 0; zrow += tile_bytes; }
+        if (++ist == S) ist = 0;
+    };
+
+    const int fs = dma_swizzle(col);               // rows ts 32 + col and wave 32 + col share col's swizzle
+    f32x16 cr[4], ci[4];
+    float best = -1.f;
+    int bi = it.t_first * BT;
+    const int nsteps = ntl * nkc;
+#pragma unroll
+    for (int p = 0; p < S - 1; p++)
+        if (p < nsteps) issue();
+    int left = nsteps;                             // chunks not yet multiplied
+    int cst = 0;                                   // the LDS stage multiplied next
+    for (int tl = 0; tl < ntl; tl++) {
+        const int tt = it.t_first + tl;
+#pragma unroll
+        for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) { cr[ts][r] = 0.f; ci[ts][r] = 0.f; }
+        }
+        for (int kc = 0; kc < nkc; kc++) {
+            // this stage's DMA done (at most the S - 2 later stages, 4 instructions each, still in flight), then
+            // every wave's part of it, and every wave done reading the stage the next issue overwrites
+            if (left >= S - 1) __builtin_amdgcn_s_waitcnt(0x0F70 | (4 * (S - 2)));   // vmcnt(4 (S - 2))
+            else __builtin_amdgcn_s_waitcnt(0x0F70);                                   // vmcnt(0)
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            if (left >= S) issue();
+            const uint8_t* z = lds + cst * STB;
+            const uint8_t* a = z + OPB;
+            if constexpr (sizeof(T) == 1) {
+                // one 64-wide k step: lanes 0-31 / 32-63 hold bytes 0-31 / 32-63 (slots 2 half, 2 half + 1)
+                const int s0 = ((2 * half) ^ fs) * 16, s1 = ((2 * half + 1) ^ fs) * 16;
+                const uint8_t* pa = a + (wave * 32 + col) * ROWB;
+                const uint4 a0 = *(const uint4*)(pa + s0), a1 = *(const uint4*)(pa + s1);
+                const i32x8 a_r = join8(a0, a1), a_i = join8(arot_fp8(a0), arot_fp8(a1));
+#pragma unroll
+                for (int ts = 0; ts < 4; ts++) {
+                    const uint8_t* pz = z + (ts * 32 + col) * ROWB;
+                    const i32x8 zf = join8(*(const uint4*)(pz + s0), *(const uint4*)(pz + s1));
+                    cr[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(zf, a_r, cr[ts], 0, 0, 0, 127, 0, 127);
+                    ci[ts] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(zf, a_i, ci[ts], 0, 0, 0, 127, 0, 127);
+                }
+            } else {
+#pragma unroll
+                for (int ks = 0; ks < KC / 16; ks++) {
+                    const int so = ((2 * ks + half) ^ fs) * 16;
+                    const uint4 av = *(const uint4*)(a + (wave * 32 + col) * ROWB + so);
+                    const bf16x8 a_r = __builtin_bit_cast(bf16x8, av), a_i = __builtin_bit_cast(bf16x8, arot_bf16(av));
+#pragma unroll
+                    for (int ts = 0; ts < 4; ts++) {
+                        const bf16x8 zf = __builtin_bit_cast(bf16x8, *(const uint4*)(z + (ts * 32 + col) * ROWB + so));
+                        cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_r, cr[ts], 0, 0, 0);
+                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_i, ci[ts], 0, 0, 0);
+                    }
+                }
+            }
+            if (++cst == S) cst = 0;
+            left--;
+        }
+        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+        if (MODE == 0 || (tt + 1) % J.per == 0) {
+            emit(best, bi, half, m, M, sc, V, I, ldv, it.col0 + (MODE == 0 ? tt : (tt + 1) / J.per - 1));
+            best = -1.f;
+            bi = (tt + 1) * BT;
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): no DMA in flight when the workgroup ends
+}
+
 template <typename T>
 using resident_fn = void (*)(const T*, long, const T*, const T*, long, const Job, float*, int*, long);
 
@@ -609,13 +881,35 @@ void launch_ra(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long
                        Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
 }
 
-// bf16 / fp8 K loops: variant 2 (or operands without the complex stacking) LDS-staged, 3 register fragments at the
-// step start, else register fragments one chunk ahead (rot) / LDS-staged
+template <typename T>
+void launch_ring(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, long lda, int K,
+                 const Job& J, float* V, int* I, long ldv)
+{
+    hipLaunchKernelGGL((mode ? gemm_peaks_ring_k<T, 1> : gemm_peaks_ring_k<T, 0>), grid, block, 0, s, Zt, ldz, Ar,
+                       lda, K, J, V, I, ldv);
+}
+
+template <typename T>
+void launch_dma(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, long lda, int K,
+                const Job& J, float* V, int* I, long ldv)
+{
+    hipLaunchKernelGGL((mode ? gemm_peaks_dma_k<T, 1> : gemm_peaks_dma_k<T, 0>), grid, block, 0, s, Zt, ldz, Ar, lda,
+                       K, J, V, I, ldv);
+}
+
+// bf16 / fp8 K loops for complex-stacked operands: variant 0 or 5 the LDS-DMA loop (when K is a multiple of its
+// 64-byte chunk, else the ring), 4 the two-stage ring, 2 LDS-staged with rotated Zt, 3 register fragments at the step
+// start, 1 register fragments one chunk ahead; without the stacking, LDS-staged with both template operands
 template <typename T>
 void launch_kloop_any(int variant, bool rot, int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz,
                       const T* Ar, const T* Ai, long lda, int K, const Job& J, float* V, int* I, long ldv)
 {
+    const bool dma_ok = K % (64 / (int)sizeof(T)) == 0;
     if (!rot) launch_kloop<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    else if ((variant == 0 || variant == 5) && dma_ok) launch_dma<T>(mode, grid, block, s, Zt, ldz, Ar, lda, K, J, V,
+                                                                     I, ldv);
+    else if (variant == 4 || variant == 5 || variant == 0) launch_ring<T>(mode, grid, block, s, Zt, ldz, Ar, lda, K, J,
+                                                                          V, I, ldv);
     else if (variant == 2) launch_kloop<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
     else if (variant == 3) launch_ra<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
     else if (variant == 1) launch_ra<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);

@@ -136,3 +136,52 @@ def refine(ctx: ResponseContext, A, rows, t, neighbours=8, steps=20, step0=0.5, 
         eta = torch.where(up, eta * 1.5, eta * 0.5)
     a = torch.view_as_complex((w[:, :, None, None] * Ari).sum(1).contiguous())
     return dict(rho2_start=start, rho2=rho2, w=w, nb=nb, delta=delta, a=a)
+
+
+def refine_triggers(plans, datasets, outs, top=10000, neighbours=8, steps=20, chunk=256):
+    """The loudest exact triggers of a filtering pass refined off the discrete bank: the `top` loudest exact triggers
+    over banks and datasets (outs[b][d]: the trigger dict of plans[b] on datasets[d], fields as Triggers.to_dict),
+    each refined by refine() from its own template on its tile's coefficient hull at its sample, grouped by
+    (bank, dataset, segment, tile) and taken `chunk` triggers per call. Returns numpy arrays over the refined triggers:
+    bank, dataset, index (into outs[b][d]), template_id, snr (the exact SNR), end_time, rho_start (the compressed
+    template's SNR at the trigger's sample), rho (refined), delta (samples)."""
+    cand = []
+    for b in range(len(plans)):
+        for d in range(len(datasets)):
+            tr = outs[b][d]
+            cand += [(float(tr['snr'][i]), b, d, int(i)) for i in np.flatnonzero(tr['exact'])]
+    cand.sort(key=lambda c: -c[0])
+    cand = cand[:top]
+    seg0 = np.cumsum([0] + [len(ds.analyze) for ds in datasets])
+    tile_of = [{int(t): (ti, r) for ti, tb in enumerate(p.tiles) for r, t in enumerate(tb.ids)} for p in plans]
+    groups = {}
+    for _, b, d, i in cand:
+        tr = outs[b][d]
+        ti, r = tile_of[b][int(tr['template_id'][i])]
+        s = int(tr['segment'][i]) - int(seg0[d])
+        groups.setdefault((b, d, s, ti), []).append((i, r, int(tr['sample'][i])))
+    cols = {k: [] for k in ('bank', 'dataset', 'index', 'rho_start', 'rho', 'delta')}
+    for (b, d, s, ti), lst in groups.items():
+        tb, ds = plans[b].tiles[ti], datasets[d]
+        ctx = ResponseContext(tb, ds.D[s], ds.IP[s], ds.df, ds.tlen, plans[b].kmin)
+        for c in range(0, len(lst), chunk):
+            part = lst[c:c + chunk]
+            res = refine(ctx, tb.A, [p[1] for p in part], [p[2] for p in part], neighbours=neighbours, steps=steps)
+            cols['bank'] += [b] * len(part)
+            cols['dataset'] += [d] * len(part)
+            cols['index'] += [p[0] for p in part]
+            cols['rho_start'].append(res['rho2_start'].sqrt().cpu().numpy())
+            cols['rho'].append(res['rho2'].sqrt().cpu().numpy())
+            cols['delta'].append(res['delta'].cpu().numpy())
+    out = {k: (np.concatenate(v) if v and isinstance(v[0], np.ndarray) else np.asarray(v, np.int64))
+           for k, v in cols.items()}
+    for k in ('rho_start', 'rho', 'delta'):
+        if len(out[k]) == 0:
+            out[k] = np.zeros(0)
+    out['template_id'] = np.asarray([outs[b][d]['template_id'][i] for b, d, i in
+                                     zip(out['bank'], out['dataset'], out['index'])], np.int64)
+    out['snr'] = np.asarray([outs[b][d]['snr'][i] for b, d, i in zip(out['bank'], out['dataset'], out['index'])],
+                            np.float32)
+    out['end_time'] = np.asarray([outs[b][d]['end_time'][i] for b, d, i in
+                                  zip(out['bank'], out['dataset'], out['index'])], np.float64)
+    return out

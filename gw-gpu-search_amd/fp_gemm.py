@@ -61,7 +61,9 @@ def _cfg(cfg):
     """cfg: None, an int (time tiles per block) or a tuple (tiles_per_block[, variant]). variant 0 picks by K
     (register-resident templates up to K = 256 bf16 / 128 fp32, else the K loop); 1 forces the K loop (bf16 / fp8:
     template fragments loaded into registers one chunk ahead); 2 the K loop with both operands staged through LDS;
-    3 the register-fragment K loop without the one-chunk-ahead buffer."""
+    3 the register-fragment K loop without the one-chunk-ahead buffer; 4 (complex-stacked operands) the two-stage
+    LDS ring with the imaginary-part template fragments rotated in registers; 5 (complex-stacked operands, also what
+    0 selects for them in the K loop) the three-stage ring filled by LDS DMA."""
     if cfg is None:
         return 0, 0
     if isinstance(cfg, (tuple, list)):
@@ -141,7 +143,7 @@ def fused_window_peaks(Are, Aim, Z, M, Tn, cw, cfg=None, zt=False, rowscale=None
 
 
 TPB_CANDIDATES = (16, 32, 64, 128)
-VARIANTS = (0, 1, 2, 3)
+VARIANTS = (0, 1, 2, 3, 4, 5)
 
 
 def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=2048, reps=3, stacked=True):

@@ -31,9 +31,11 @@ extern "C" {
  * mode 0: V, I (M, ldv >= ceil(Tn/128)): per 128 samples the max pw (columns >= Tn masked to -1) and its sample.
  * mode 1: V, I (M, ldv >= Tn/(128 per)): per window of `per` blocks the max and its sample; Tn % (128 per) == 0.
  * mode | 4: the operands are complex-stacked (Ai[m, 2j] = -Ar[m, 2j+1], Ai[m, 2j+1] = Ar[m, 2j], as gwgs_stack_a
- * writes them): the bf16 / fp8 K loops then form z_im from Ar and pair-rotated Zt fragments and do not read Ai.
- * variant (K loop): 1 = template fragments in registers one chunk ahead, 2 = both operands staged through LDS,
- * 3 = template fragments in registers at each chunk; the register forms need mode | 4.
+ * writes them): the bf16 / fp8 K loops then form z_im from Ar and pair-rotated fragments and do not read Ai.
+ * variant (K loop, mode | 4): 0 or 5 = operands loaded by LDS DMA through a 3-stage ring (K a multiple of 32 bf16 /
+ * 64 fp8, else as 4), 4 = two LDS stages filled from registers, 1 = template fragments in registers one chunk ahead,
+ * 2 = both operands staged through LDS, 3 = template fragments in registers at each chunk. Without mode | 4 every
+ * K-loop variant stages Ar and Ai through LDS.
  * First index wins ties at both levels. tiles_per_block: 128-sample tiles per workgroup (0 = default). */
 int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                     int M, int K, int Tn, int mode, int per, int tiles_per_block, float* V, int* I, long ldv,

@@ -45,14 +45,16 @@ def test_fused_peaks_vs_oracle(gpu, dtype, M, K, Tn, variant):
     assert (I == Ir).mean() > 0.99
 
 
-@pytest.mark.parametrize('code,variant', [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 3)])
+@pytest.mark.parametrize('code,variant', [(1, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5), (2, 0), (2, 1), (2, 3),
+                                          (2, 4), (2, 5)])
 def test_stacked_operands_rotated_path(gpu, code, variant):
-    """Complex-stacked operands (Ai derived from Ar, as stack_a writes them) through the rotated-Zt K loops
+    """Complex-stacked operands (Ai derived from Ar, as stack_a writes them) through the rotated K loops
     (mode | STACKED, Ai never read: a garbage Ai is passed) against the float64 GEMM of the same values, and equal
-    window maxima and samples to the general path that reads Ai, up to fp32 summation order (relative 1e-5)."""
+    window maxima and samples to the general path that reads Ai, up to fp32 summation order (relative 1e-5; bit for
+    bit for the template-side rotation of variants 0, 4 and 5)."""
     import fp_gemm
     rng = np.random.default_rng(17 + variant + 10 * code)
-    M, r, cw, nw = 150, 197, 2048, 3
+    M, r, cw, nw = 150, (185 if code == 2 and variant in (0, 5) else 197), 2048, 3   # fp8 DMA: K % 64 == 0
     K = 2 * (-(-r // 16) * 16)
     Tn = cw * nw
     A = (rng.standard_normal((M, r)) + 1j * rng.standard_normal((M, r))) * (20 if code == 2 else 1)
@@ -80,6 +82,10 @@ def test_stacked_operands_rotated_path(gpu, code, variant):
     V, I = out[1 | fp_gemm.STACKED]
     np.testing.assert_allclose(V, Vr, rtol=1e-4)
     np.testing.assert_allclose(V, out[1][0], rtol=1e-5)
+    if variant in (0, 4, 5):
+        # the template-side rotation forms exactly the Ai the general path reads: the same products in the same order
+        np.testing.assert_array_equal(V, out[1][0])
+        np.testing.assert_array_equal(I, out[1][1])
     pw = (da @ dz.T) ** 2 + (di @ dz.T) ** 2
     assert np.all(np.take_along_axis(pw, I.astype(np.int64), 1) >= Vr * (1 - 1e-4))
 
