@@ -42,6 +42,7 @@ def host(t):
 
 
 _SIDE = None
+_READY = None
 
 
 def side_stream():
@@ -51,6 +52,16 @@ def side_stream():
     if _SIDE is None:
         _SIDE = torch.cuda.Stream()
     return _SIDE
+
+
+def ready_stream():
+    """The stream for copies whose inputs are already computed (default priority). The side stream is in order, so
+    a copy queued there also waits for every earlier copy's dependency (work enqueued later on the compute stream);
+    a copy that needs no wait goes here instead."""
+    global _READY
+    if _READY is None:
+        _READY = torch.cuda.Stream()
+    return _READY
 
 
 def to_dev(a, dtype=None):
@@ -95,12 +106,17 @@ def mark():
 
 class Fetch:
     """Device -> host copies of tensors on the side stream, ordered after `after` (an event; default: everything
-    enqueued so far on the current stream). get() waits for these copies only and returns numpy arrays."""
+    enqueued so far on the current stream); when `after` has already completed, on the ready stream, so that the
+    copies do not queue behind the side stream's waits. get() waits for these copies only and returns numpy
+    arrays."""
 
     def __init__(self, *tensors, after=None):
-        ev = after if after is not None else mark()
-        s = side_stream()
-        s.wait_event(ev)
+        if after is not None and after.query():
+            s = ready_stream()
+        else:
+            ev = after if after is not None else mark()
+            s = side_stream()
+            s.wait_event(ev)
         self.host = []
         with torch.cuda.stream(s):
             for t in tensors:
