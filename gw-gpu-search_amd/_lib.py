@@ -37,7 +37,8 @@ SIGNATURES = {
     'gwgs_select_above': [_p, _l, _f, _p, _i, _p, _p, _p],
     'gwgs_select_peaks': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
-    'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
+    'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
+    'gwgs_chisq_points_workspace': [_i, _i],
     'gwgs_chisq_peaks_per_group': [],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_form_q': [_p, _i, _l, _p, _l, _p, _p],
@@ -67,7 +68,8 @@ SIGNATURES = {
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],
     'gwgs_slide_write': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p, _p, _p, _p],
 }
-_RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l, 'gwgs_sigmasq_workspace': _l}
+_RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l, 'gwgs_sigmasq_workspace': _l,
+            'gwgs_chisq_points_workspace': _l}
 
 _lib = None
 _lock = threading.Lock()

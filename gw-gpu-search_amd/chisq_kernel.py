@@ -52,10 +52,12 @@ def _launch(q, times_sorted, grow, gstart, gcount, gedges_dev, want_z, d, N, spa
                                                                                          for x in (grow, gstart, gcount)))
     ge = gedges_dev if isinstance(gedges_dev, torch.Tensor) else _dev.to_dev(gedges_dev)
     ge = ge.contiguous()
+    nwork = int(_lib.load().gwgs_chisq_points_workspace(len(gstart), ge.shape[1]))
+    work = torch.empty(max(nwork, 8), dtype=torch.uint8, device=q.device)
     e0 = _dev.METER.start()
     _lib.call('gwgs_chisq_points', q.data_ptr(), q.shape[1], _dev.ptr(dd), int(N), g_row.data_ptr(),
               g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(), ge.shape[1], len(gstart),
-              out.data_ptr(), _dev.ptr(zo), _dev.stream())
+              out.data_ptr(), _dev.ptr(zo), work.data_ptr(), _dev.stream())
     # HBM bytes: per group the row (and the data) over its bins, 8 B each
     _dev.METER.stop('chisq', e0, span * (16 if d is not None else 8))
     return out, zo

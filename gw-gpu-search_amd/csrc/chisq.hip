@@ -6,89 +6,116 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int P = 8;       // peaks per block (one template row's peaks share its read of q / h)
-constexpr int R = 32;      // consecutive bins per thread run: the phasor is reseeded exactly at each run start
+constexpr int R = 32;      // phasor steps per lane run: the phasor is reseeded exactly at each run start
 
-// exp(2 pi i ((k t) mod N) / N), exact integer phase, double-precision sincos
+// exp(2 pi i ((k t) mod N) / N) from the exact integer phase m = (k t) mod N; mask = N - 1 when N is a power of two
+// up to 2^24 (then the modulo is a mask and 2 m / N is exact in float), else 0 (double division); float sincospi.
 __device__ __forceinline__ float2 phasor(const unsigned long long k, const unsigned long long t,
-                                         const unsigned long long N)
+                                         const unsigned long long N, const unsigned long long mask)
 {
-    const unsigned long long m = (k * t) % N;
-    double s, c;
-    sincospi(2.0 * (double)m / (double)N, &s, &c);
-    return make_float2((float)c, (float)s);
+    const unsigned long long kt = k * t;
+    float x;
+    if (mask) {
+        x = (float)(kt & mask) * (2.f / (float)N);          // both factors exact: N a power of two
+    } else {
+        x = (float)(2.0 * (double)(kt % N) / (double)N);
+    }
+    float s, c;
+    sincospif(x, &s, &c);
+    return make_float2(c, s);
 }
 
-// One block per group of <= P peaks of one row. For each bin j = [e_j, e_j+1) the threads take runs of R consecutive
-// bins (run r of the bin -> thread r mod NT), reseed each peak's phasor at the run start and advance it by a complex
-// rotation per bin. Per bin: wave butterfly sums, then the 4 waves in fixed order, so the result is deterministic.
-// z = sum_j Z_j and pw = sum_j |Z_j|^2 accumulate in double. FUSE: the summand is conj(h_k) d_k formed here.
+// One wave per (group of <= P peaks of one row, chi-squared bin j), four per workgroup: lanes take the bin's bins
+// [e_j, e_j+1) interleaved (lane l: ka + l, ka + l + 64, ... for runs of 64 R bins: coalesced loads), each peak's
+// phasor reseeded from the exact integer phase at the lane's first bin of a run and advanced by exp(2 pi i 64 t / N)
+// per step; a wave butterfly sum gives Z_j of every peak, written to zp[(g nbin + j) P + p]; chisq_finish_k sums
+// over j. Wave i = g nbin + j: a group's bins run on neighbouring waves (they share the row in L2). FUSE: the summand
+// is conj(h_k) d_k formed here.
 template <bool FUSE>
-__global__ void __launch_bounds__(NT) chisq_pts_k(const float2* __restrict__ q, const long ldq,
-                                                  const float2* __restrict__ dv, const unsigned long long N,
-                                                  const int* __restrict__ grow, const int* __restrict__ gstart,
-                                                  const int* __restrict__ gcount, const long long* __restrict__ ptime,
-                                                  const int* __restrict__ edges, const int nedge,
-                                                  float* __restrict__ out, float2* __restrict__ zout)
+__global__ void __launch_bounds__(NT) chisq_bins_k(const float2* __restrict__ q, const long ldq,
+                                                   const float2* __restrict__ dv, const unsigned long long N,
+                                                   const int* __restrict__ grow, const int* __restrict__ gstart,
+                                                   const int* __restrict__ gcount, const long long* __restrict__ ptime,
+                                                   const int* __restrict__ edges, const int nedge, const int ngroups,
+                                                   float2* __restrict__ zp)
 {
-    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbin = nedge - 1;
+    const int lane = threadIdx.x & 63;
+    const long wi = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (wi >= (long)ngroups * nbin) return;   // whole waves
+    const int g = (int)(wi / nbin), j = (int)(wi % nbin);
     const int row = grow[g], p0 = gstart[g], np_ = gcount[g];
-    const float2* qr = q + (size_t)row * ldq;
     const int* e = edges + (size_t)g * nedge;
+    const int k0 = e[j], k1 = e[j + 1];
+    float2* zo = zp + (size_t)wi * P;
+    if (k0 >= k1) {                           // empty bin: the finish pass skips it
+        if (lane < P) zo[lane] = make_float2(0.f, 0.f);
+        return;
+    }
+    const unsigned long long mask = (N & (N - 1)) == 0 && N <= (1ull << 24) ? N - 1 : 0;
+    const float2* qr = q + (size_t)row * ldq;
     unsigned long long t[P];
     float2 rot[P];
 #pragma unroll
     for (int p = 0; p < P; p++) {
         t[p] = p < np_ ? (unsigned long long)ptime[p0 + p] % N : 0ull;
-        rot[p] = phasor(1ull, t[p], N);
+        rot[p] = phasor(64ull, t[p], N, mask);
     }
-    __shared__ float2 red[NT / 64][P];
-    double tot = 0.0, zx = 0.0, zy = 0.0;    // meaningful in thread p < np_
-    for (int j = 0; j < nedge - 1; j++) {
-        const int k0 = e[j], k1 = e[j + 1];
-        if (k0 >= k1) continue;              // uniform over the block
-        float2 acc[P];
+    float2 acc[P];
 #pragma unroll
-        for (int p = 0; p < P; p++) acc[p] = make_float2(0.f, 0.f);
-        const int nrun = (k1 - k0 + R - 1) / R;
-        for (int r = tid; r < nrun; r += NT) {
-            const int ka = k0 + r * R, kb = min(ka + R, k1);
-            float2 w[P];
+    for (int p = 0; p < P; p++) acc[p] = make_float2(0.f, 0.f);
+    constexpr int RW = 64 * R;                // bins per run
+    for (int ra = k0; ra < k1; ra += RW) {
+        const int ka = ra + lane, kb = min(ra + RW, k1);
+        if (ka >= kb) continue;
+        float2 w[P];
 #pragma unroll
-            for (int p = 0; p < P; p++) w[p] = p < np_ ? phasor((unsigned long long)ka, t[p], N) : make_float2(0.f, 0.f);
-            for (int k = ka; k < kb; k++) {
-                float2 v = qr[k];
-                if (FUSE) {
-                    const float2 dk = dv[k];
-                    v = make_float2(v.x * dk.x + v.y * dk.y, v.x * dk.y - v.y * dk.x);
-                }
+        for (int p = 0; p < P; p++) w[p] = p < np_ ? phasor((unsigned long long)ka, t[p], N, mask) : make_float2(0.f, 0.f);
+#pragma unroll 4
+        for (int k = ka; k < kb; k += 64) {
+            float2 v = qr[k];
+            if (FUSE) {
+                const float2 dk = dv[k];
+                v = make_float2(v.x * dk.x + v.y * dk.y, v.x * dk.y - v.y * dk.x);
+            }
 #pragma unroll
-                for (int p = 0; p < P; p++) {
-                    acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
-                    acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
-                    const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
-                    w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
-                    w[p].x = wx;
-                }
+            for (int p = 0; p < P; p++) {
+                acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
+                acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
+                const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
+                w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
+                w[p].x = wx;
             }
         }
+    }
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-            const float ax = wave_sum_f32(acc[p].x), ay = wave_sum_f32(acc[p].y);
-            if (lane == 0) red[wave][p] = make_float2(ax, ay);
-        }
-        __syncthreads();
-        if (tid < np_) {
-            float ax = 0.f, ay = 0.f;
-            for (int w = 0; w < NT / 64; w++) { ax += red[w][tid].x; ay += red[w][tid].y; }
-            tot += (double)ax * ax + (double)ay * ay;
-            zx += ax; zy += ay;
-        }
-        __syncthreads();
+    for (int p = 0; p < P; p++) {
+        const float ax = wave_sum_f32(acc[p].x), ay = wave_sum_f32(acc[p].y);
+        if (lane == p) zo[p] = make_float2(ax, ay);
     }
-    if (tid < np_) {
-        out[p0 + tid] = (float)tot;
-        if (zout) zout[p0 + tid] = make_float2((float)zx, (float)zy);
+}
+
+// One thread per (group, peak): z = sum_j Z_j and pw = sum_j |Z_j|^2 over the non-empty bins in increasing j, in
+// double (the fixed order makes the result deterministic).
+__global__ void __launch_bounds__(NT) chisq_finish_k(const float2* __restrict__ zp, const int* __restrict__ gstart,
+                                                     const int* __restrict__ gcount, const int* __restrict__ edges,
+                                                     const int nedge, const int ngroups, float* __restrict__ out,
+                                                     float2* __restrict__ zout)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int g = i / P, p = i % P;
+    if (g >= ngroups || p >= gcount[g]) return;
+    const int nbin = nedge - 1;
+    const int* e = edges + (size_t)g * nedge;
+    double tot = 0.0, zx = 0.0, zy = 0.0;
+    for (int j = 0; j < nbin; j++) {
+        if (e[j] >= e[j + 1]) continue;
+        const float2 z = zp[((size_t)g * nbin + j) * P + p];
+        tot += (double)z.x * z.x + (double)z.y * z.y;
+        zx += z.x; zy += z.y;
     }
+    out[gstart[g] + p] = (float)tot;
+    if (zout) zout[gstart[g] + p] = make_float2((float)zx, (float)zy);
 }
 
 // Exclusive scan of one double per thread over a 256-thread block, in a fixed order (deterministic).
@@ -186,21 +213,37 @@ __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h
 
 // Groups: for g in [0, ngroups): row grow[g] of q (or h in fused mode), peaks gstart[g] .. gstart[g] + gcount[g]
 // (gcount <= 8) of ptime, bin edges edges[g, 0:nedge]. N: the time length of the phase factor (any positive value;
-// the phase k t mod N is exact in 64-bit integers). d: data (fused mode) or null. zout may be null.
+// the phase k t mod N is exact in 64-bit integers). d: data (fused mode) or null. zout may be null. work: device
+// buffer of gwgs_chisq_points_workspace(ngroups, nedge) bytes (the per-bin sums).
+GWGS_API long gwgs_chisq_points_workspace(int ngroups, int nedge)
+{
+    return ngroups > 0 && nedge >= 2 ? (long)ngroups * (nedge - 1) * P * (long)sizeof(float2) : 0;
+}
+
 GWGS_API int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const int* grow, const int* gstart,
                                const int* gcount, const long long* ptime, const int* edges, int nedge, int ngroups,
-                               float* out, void* zout, void* stream)
+                               float* out, void* zout, void* work, void* stream)
 {
     GWGS_CHECK(q && grow && gstart && gcount && ptime && edges && out && N > 0 && nedge >= 1 && ngroups >= 0,
                GWGS_EINVAL);
     if (ngroups == 0) return GWGS_OK;
     hipStream_t s = (hipStream_t)stream;
+    if (nedge == 1) {                        // no bins: every sum is zero
+        hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                           (const float2*)nullptr, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
+        return gwgs_status();
+    }
+    GWGS_CHECK(work != nullptr, GWGS_EINVAL);
+    GWGS_CHECK((long)ngroups * (nedge - 1) < (1L << 33), GWGS_ESIZE);
+    const dim3 grid((unsigned)(((long)ngroups * (nedge - 1) + NT / 64 - 1) / (NT / 64)));
     if (d)
-        hipLaunchKernelGGL(chisq_pts_k<true>, dim3(ngroups), dim3(NT), 0, s, (const float2*)q, ldq, (const float2*)d,
-                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, out, (float2*)zout);
+        hipLaunchKernelGGL(chisq_bins_k<true>, grid, dim3(NT), 0, s, (const float2*)q, ldq, (const float2*)d,
+                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work);
     else
-        hipLaunchKernelGGL(chisq_pts_k<false>, dim3(ngroups), dim3(NT), 0, s, (const float2*)q, ldq, nullptr,
-                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, out, (float2*)zout);
+        hipLaunchKernelGGL(chisq_bins_k<false>, grid, dim3(NT), 0, s, (const float2*)q, ldq, nullptr,
+                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work);
+    hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const float2*)work, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
     return gwgs_status();
 }
 

@@ -81,10 +81,13 @@ int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar, const vo
  * Group g: row grow[g] of q (or of the templates h when d != NULL: summand conj(h_k) d_k), peaks
  * gstart[g] .. gstart[g] + gcount[g] - 1 (gcount <= gwgs_chisq_peaks_per_group()) at samples ptime[],
  * bin edges edges[g, 0:nedge]. out[p] = sum_j |Z_j|^2, zout[p] = sum_j Z_j (may be NULL), with
- * Z_j = sum_{e_j <= k < e_j+1} v_k exp(2 pi i ((k t) mod N) / N). Any N > 0 (64-bit integer phase). */
+ * Z_j = sum_{e_j <= k < e_j+1} v_k exp(2 pi i ((k t) mod N) / N). Any N > 0 (64-bit integer phase). One
+ * workgroup per (group, bin); the per-bin sums go to `work`, then a second pass adds them in bin order. */
 int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const int* grow, const int* gstart,
                       const int* gcount, const long long* ptime, const int* edges, int nedge, int ngroups, float* out,
-                      void* zout, void* stream);
+                      void* zout, void* work, void* stream);
+/* bytes of the device workspace gwgs_chisq_points needs (the per-bin sums of every group; 0 when there is none) */
+long gwgs_chisq_points_workspace(int ngroups, int nedge);
 int gwgs_chisq_peaks_per_group(void);
 /* pipeline/chisq_edges.py:90-143 edges_fused (and edges_blocked, multi_filter.py:264-282 edges_per_template):
  * per request i, row rows[i] of h: edge j = first k with float64 cumsum(|h|^2 ips)[k] > j / nbs[i] * total over
