@@ -104,7 +104,7 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(streams, banks, cond, ntemp=128):
+def cpu_baseline(streams, banks, cond, ntemp=512):
     """The oracle's plain method (numpy / scipy.fft / OpenBLAS float32) on a bounded sample: H1 only, the first
     `ntemp` circular templates in chirp-time order as one tile; conditioning plus the per-segment work (basis
     transforms, GEMM and window maxima, exact stage) is timed, the tile's templates and SVD are not (as on the GPU)."""
