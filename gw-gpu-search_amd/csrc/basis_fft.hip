@@ -224,8 +224,7 @@ struct Out<uint16_t> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
     {
-        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)f32_to_bf16(re) | ((uint32_t)f32_to_bf16(im) << 16), rs,
-                                              (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_bf16x2(re, im), rs, (int)off, 0, 0);
     }
 };
 template <>
@@ -233,8 +232,7 @@ struct Out<uint8_t> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
     {
-        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)((uint32_t)f32_to_fp8(re) | ((uint32_t)f32_to_fp8(im) << 8)),
-                                              rs, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(f32x2_to_fp8x2(re, im), rs, (int)off, 0, 0);
     }
 };
 

@@ -49,13 +49,21 @@ __device__ __forceinline__ float wave_sum_f32(float x)
     return x;
 }
 
-// round-to-nearest-even float -> bfloat16 bits (NaN kept quiet)
+// two floats -> a packed bfloat16 pair (a in the low half), round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+__device__ __forceinline__ uint32_t f32x2_to_bf16x2(const float a, const float b)
+{
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    f32x2_t v;
+    v.x = a;
+    v.y = b;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+// round-to-nearest-even float -> bfloat16 bits
 __device__ __forceinline__ uint16_t f32_to_bf16(const float x)
 {
-    const uint32_t u = __float_as_uint(x);
-    const uint32_t rn = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-    const bool nan = (u & 0x7fffffffu) > 0x7f800000u;
-    return (uint16_t)(nan ? ((u >> 16) | 0x40u) : rn);     // a select, not a branch
+    return (uint16_t)(f32x2_to_bf16x2(x, 0.f) & 0xffffu);
 }
 
 // float -> fp8 e4m3 (OCP, gfx950 v_cvt_pk_fp8_f32: round to nearest even), saturated to +-448 first
@@ -63,6 +71,13 @@ __device__ __forceinline__ uint8_t f32_to_fp8(float x)
 {
     x = fminf(fmaxf(x, -448.f), 448.f);
     return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false) & 0xff);
+}
+
+// two floats -> two fp8 e4m3 bytes (a in the low byte), each as f32_to_fp8, in one conversion
+__device__ __forceinline__ uint16_t f32x2_to_fp8x2(const float a, const float b)
+{
+    return (uint16_t)(__builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(a, -448.f), 448.f), fminf(fmaxf(b, -448.f), 448.f),
+                                                      0, false) & 0xffff);
 }
 
 __device__ __forceinline__ float fp8_to_f32(const uint8_t b)
