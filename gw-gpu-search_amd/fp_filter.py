@@ -181,7 +181,7 @@ class Triggers:
         keep = np.ones(len(t), bool)
         for g_t, g_w, g_taper in gates:
             keep &= np.abs(t - g_t) > g_w + g_taper + 0.5
-        sel = np.nonzero(keep)[0][np.lexsort((t[keep], tid[keep]))]
+        sel = np.nonzero(keep)[0][_order_by_template_time(tid[keep], t[keep])]
         out = dict(template_id=tid[sel], snr=snr[sel], end_time=t[sel],
                    coa_phase=self.cat('ph', np.float32)[sel], peak_sigmasq=self.cat('sg', np.float64)[sel],
                    exact=self.cat('ex', bool)[sel], sample=self.cat('samp', np.int64)[sel],
@@ -194,6 +194,24 @@ class Triggers:
             ns[m] = ns[m] / ((1 + rc[m].astype(np.float64) ** 3) / 2.0) ** (1.0 / 6)
             out.update(chisq=chi, chisq_dof=dof, newsnr=ns)
         return out
+
+
+def _order_by_template_time(tid, t):
+    """The permutation np.lexsort((t, tid)) returns (stable), found as one float64 argsort of the key
+    tid M + (t - min t), M = 2 span + 2, about twice as fast.  Rounding keeps the key monotone in (tid, t), so
+    the only way it can differ from the lexsort is two distinct (tid, t) rounding to one key; the result is
+    checked in one pass (keys sorted, equal pairs in index order) and the lexsort is taken when it fails."""
+    if len(t) < 2:
+        return np.arange(len(t), dtype=np.int64)
+    t0, span = float(t.min()), float(t.max() - t.min())
+    m = 2.0 * span + 2.0
+    if tid.min() >= 0 and float(tid.max()) * m + m < 2.0 ** 40:
+        o = np.argsort(tid * m + (t - t0), kind='stable')
+        a, b = tid[o], t[o]
+        da, db = np.diff(a), np.diff(b)
+        if np.all((da > 0) | ((da == 0) & ((db > 0) | ((db == 0) & (np.diff(o) > 0))))):
+            return o
+    return np.lexsort((t, tid))
 
 
 def _fp8_scales(pw):

@@ -72,6 +72,22 @@ def test_trigger_output_rules():
     assert abs(out['newsnr'][0] - 8 / ((1 + 27) / 2) ** (1 / 6)) < 1e-12 and out['newsnr'][1] == 6.0
 
 
+def test_trigger_order_equals_lexsort():
+    """The single-key sort of the trigger list is the stable lexsort by (template_id, end_time), including ties,
+    GPS-sized times and two end times one ulp apart (where the key cannot separate them and the lexsort runs)."""
+    import fp_filter
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        n = int(rng.integers(0, 2000))
+        tid = rng.integers(0, int(rng.integers(1, 30000)), n)
+        t = (1.2e9 if trial % 2 else 0.0) + np.round(rng.uniform(0, 2048, n) * 4096) / 4096
+        if trial % 5 == 0 and n:
+            t[: n // 2] = t[0]
+        if trial % 7 == 0 and n > 3:
+            t[1], tid[1] = np.nextafter(t[0], np.inf), tid[0]
+        assert np.array_equal(fp_filter._order_by_template_time(tid, t), np.lexsort((t, tid))), trial
+
+
 def test_gemm_cfg_parsing():
     import fp_gemm
     assert fp_gemm._cfg(None) == (0, 0) and fp_gemm._cfg(32) == (32, 0) and fp_gemm._cfg((16, 1)) == (16, 1)
