@@ -6,7 +6,9 @@
 //   z_j[t] = sum_k1 w_n^(k1 t2) w_n1^(k1 t1) sum_k2 x_j[k1 + n1 k2] w_n2^(k2 t2),   w_m = e^(+2 pi i / m),
 //   x_j[k] = Vc[j, k] D[k] on [kmin, kend) and 0 elsewhere.
 // cols_k: per basis row j and 8 to 32 columns k1, the length-n2 transform over k2 (form of x fused into the load,
-// zeros above kend never read), times w_n^(k1 t2), to Y[j, t2, k1] (complex64, r x n).
+// zeros above kend never read), times w_n^(k1 t2), to Y[j, t2, k1] (complex64, r x n). The band [kmin, kend) is
+// usually a small part of n, so x[k1 + n1 k2] is zero for all k2 >= kend / n1: the kernel is instantiated for the
+// number of leading nonzero inputs per thread (Q / 8, Q / 4, Q / 2, Q) and its first DFT skips the known zeros.
 // rows_k: per t2 and 8 to 32 basis rows, the length-n1 transform over k1 of Y[j, t2, :], written as
 // Zt[t - a0, 2j] = Re z_j[t] cs_j, Zt[t - a0, 2j + 1] = Im z_j[t] cs_j for t in [a0, a0 + Tn) (fp32, bf16 or fp8), and
 // zeros for rows r <= j < rp.
@@ -14,6 +16,9 @@
 // the twiddle w_L^(p m2), an exchange through LDS (real then imaginary parts, so two blocks fit a CU), and
 // Q / P P-point DFTs per thread. Twiddles of the in-register DFTs are compile-time constants; the others come from
 // float64 sincospi seeds advanced by float64 products (rounded once to float32).
+// Inputs are read through buffer resources whose range check returns zeros outside the band (cols_k) or past the
+// last basis row (rows_k): every load of a thread is issued unconditionally, back to back, where a per-load branch
+// made the compiler wait for each load (or pair of loads) before issuing the next.
 // HBM traffic per (tile, segment): 8 r (kend - kmin) bytes read of Vc, 8 r n written and read of Y, 2 Tn rp
 // elements of the output type written: two passes over Y instead of the batched FFT's several plus its input and
 // transpose passes.
@@ -52,22 +57,70 @@ __device__ constexpr Twiddles<E> kTw{};
 constexpr int ilog2(int e) { int l = 0; while ((1 << l) < e) l++; return l; }
 constexpr int brev(int i, int lg) { int r = 0; for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b); return r; }
 
-// v[m] <- sum_n v[n] e^(+2 pi i n m / E), radix-2 decimation in time, every index a compile-time constant
-template <int E>
+// Zero<E, NZ>::m[s] bit i: before the radix-2 stage of span 2^s, w[i] is a known zero (every input feeding it, in
+// bit-reversed order, has index NZ or above)
+template <int E, int NZ>
+struct Zero {
+    unsigned long long m[ilog2(E) + 1];
+    constexpr Zero() : m()
+    {
+        constexpr int LG = ilog2(E);
+        for (int st = 0; st <= LG; st++) {
+            const int h = 1 << st;
+            unsigned long long bits = 0;
+            for (int i = 0; i < E; i++) {
+                bool z = true;
+                for (int x = i & ~(h - 1); x < (i & ~(h - 1)) + h; x++)
+                    if (brev(x, LG) < NZ) z = false;
+                if (z) bits |= 1ull << i;
+            }
+            m[st] = bits;
+        }
+    }
+};
+
+// v[m] <- sum_n v[n] e^(+2 pi i n m / E), radix-2 decimation in time, every index a compile-time constant. Inputs
+// v[n] with n >= NZ are zero and not read: butterflies with a known-zero operand become copies. A butterfly with a
+// general twiddle (c, s) is a +- g (b + i u b) with (g, u) = (c, s / c) when |c| >= |s| and (s, -c / s) times i
+// otherwise: six fused multiply-adds instead of four products and four sums.
+template <int E, int NZ = E>
 __device__ __forceinline__ void dft_reg(float2 (&v)[E])
 {
     constexpr int LG = ilog2(E);
+    constexpr Zero<E, NZ> zm{};
     float2 w[E];
 #pragma unroll
-    for (int i = 0; i < E; i++) w[brev(i, LG)] = v[i];
+    for (int i = 0; i < NZ; i++) w[brev(i, LG)] = v[i];
 #pragma unroll
-    for (int h = 1; h < E; h <<= 1)
+    for (int st = 0; st < LG; st++)
 #pragma unroll
-        for (int k = 0; k < h; k++) {
+        for (int k = 0; k < (1 << st); k++) {
+            const int h = 1 << st;
             const int ti = k * (E / (2 * h));
 #pragma unroll
             for (int i = k; i < E; i += 2 * h) {
+                const bool za = (zm.m[st] >> i) & 1, zb = (zm.m[st] >> (i + h)) & 1;
+                if (za && zb) continue;
                 const float2 a = w[i], b0 = w[i + h];
+                if (zb) { w[i + h] = a; continue; }
+                const bool general = ti != 0 && 4 * ti != E;
+                if (general && !za) {
+                    const float c = kTw<E>.c[ti], s = kTw<E>.s[ti];
+                    float2 bb;
+                    float g;
+                    if (__builtin_fabsf(c) >= __builtin_fabsf(s)) {
+                        const float u = (float)((double)s / (double)c);
+                        bb = make_float2(__builtin_fmaf(-u, b0.y, b0.x), __builtin_fmaf(u, b0.x, b0.y));
+                        g = c;
+                    } else {
+                        const float u = (float)((double)c / (double)s);
+                        bb = make_float2(__builtin_fmaf(u, b0.x, -b0.y), __builtin_fmaf(u, b0.y, b0.x));
+                        g = s;
+                    }
+                    w[i] = make_float2(__builtin_fmaf(g, bb.x, a.x), __builtin_fmaf(g, bb.y, a.y));
+                    w[i + h] = make_float2(__builtin_fmaf(-g, bb.x, a.x), __builtin_fmaf(-g, bb.y, a.y));
+                    continue;
+                }
                 float2 b;
                 if (ti == 0) b = b0;
                 else if (4 * ti == E) b = make_float2(-b0.y, b0.x);
@@ -75,8 +128,13 @@ __device__ __forceinline__ void dft_reg(float2 (&v)[E])
                     const float c = kTw<E>.c[ti], s = kTw<E>.s[ti];
                     b = make_float2(b0.x * c - b0.y * s, b0.x * s + b0.y * c);
                 }
-                w[i] = make_float2(a.x + b.x, a.y + b.y);
-                w[i + h] = make_float2(a.x - b.x, a.y - b.y);
+                if (za) {
+                    w[i] = b;
+                    w[i + h] = make_float2(-b.x, -b.y);
+                } else {
+                    w[i] = make_float2(a.x + b.x, a.y + b.y);
+                    w[i + h] = make_float2(a.x - b.x, a.y - b.y);
+                }
             }
         }
 #pragma unroll
@@ -106,6 +164,23 @@ __device__ __forceinline__ void advance(double& c, double& s, const double dc, c
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
+
+// a buffer of nbytes bytes at p, capped at 2^31 - 1 (offsets used here stay below 2^30)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, const long nbytes)
+{
+    const int nrec = nbytes <= 0 ? 0 : nbytes > 0x7fffffffL ? 0x7fffffff : (int)nbytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, nrec, kRsrcWord3);
+}
+
+// 8 bytes at byte offset off of a buffer (zeros when off is at or past its size)
+__device__ __forceinline__ float2 load_f2(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+    return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+
 // Storage of the intermediate Y: complex64, or two fp16 (round to nearest even) of the value times a per-row scale
 // (1 / rms of the row's Y, so the half range is never approached and its 11-bit precision applies relative to the
 // row's own level; used for the bf16 / fp8 operand forms, whose own rounding is coarser).
@@ -114,6 +189,10 @@ struct YF32 {
     using E = float2;
     static __device__ __forceinline__ E put(const float2 v, const float) { return v; }
     static __device__ __forceinline__ float2 get(const E e) { return e; }
+    static __device__ __forceinline__ float2 load(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
+    {
+        return load_f2(rs, off);
+    }
 };
 struct YF16 {
     using E = h16x2;
@@ -125,10 +204,15 @@ struct YF16 {
         return h;
     }
     static __device__ __forceinline__ float2 get(const E e) { return make_float2((float)e.x, (float)e.y); }
+    static __device__ __forceinline__ float2 load(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
+    {
+        return get(__builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0)));
+    }
 };
 
-// Column transforms: block (k1 block, j); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1
-template <int P, int Q, int C, typename YS>
+// Column transforms: block (k1 block, j); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1; inputs
+// k1 + n1 (p + P q) with q >= NZ are at or above kend (zero, not read)
+template <int P, int Q, int C, int NZ, typename YS>
 __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, const long ldv,
                                                 const float2* __restrict__ D, const long kmin, const long kend,
                                                 const int n1, const int lgn, const float* __restrict__ ysc,
@@ -146,18 +230,24 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
     const unsigned off = (unsigned)(k1 + n1 * p);
 
     float2 v[Q];
+    {
+        // x over [kmin, kend) through buffers of that range (8 (kend - kmin) < 2^25 bytes): loads outside it return
+        // zeros; k < kmin maps to an offset past the end
+        const long span = kend - kmin;
+        const __amdgpu_buffer_rsrc_t ra = buffer_of(vr + kmin, span * 8), rd = buffer_of(D + kmin, span * 8);
+        float2 a[NZ], d[NZ];
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const long kq = (long)n1 * P * q;
-        const long k = kq + off;
-        float2 x = make_float2(0.f, 0.f);
-        if (k >= kmin && k < kend) {
-            const float2 a = (vr + kq)[off], d = (D + kq)[off];
-            x = make_float2(a.x * d.x - a.y * d.y, a.x * d.y + a.y * d.x);
+        for (int q = 0; q < NZ; q++) {
+            const long kk = (long)n1 * P * q + off - kmin;
+            const unsigned o = kk < 0 ? 0x80000000u : (unsigned)kk * 8u;
+            a[q] = load_f2(ra, o);
+            d[q] = load_f2(rd, o);
         }
-        v[q] = x;
+#pragma unroll
+        for (int q = 0; q < NZ; q++)
+            v[q] = make_float2(a[q].x * d[q].x - a[q].y * d[q].y, a[q].x * d[q].y + a[q].y * d[q].x);
     }
-    dft_reg<Q>(v);
+    dft_reg<Q, NZ>(v);
     {
         double dc, ds, wc = 1.0, ws = 0.0;
         unit(p, L, dc, ds);
@@ -236,9 +326,8 @@ struct Out<uint8_t> {
     }
 };
 
-constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
-
-// Row transforms: block (t2, row block); threads JB x P, thread (p, jr) = (tid % P, tid / P) in step 1
+// Row transforms: block (row block, t2), so that the blocks running together complete whole output rows; threads
+// JB x P, thread (p, jr) = (tid % P, tid / P) in step 1
 template <int P, int Q, int JB, typename T, typename YS>
 __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restrict__ Y, const int r, const int rp,
                                                  const int lgn1, const int lgn, const long a0, const long Tn,
@@ -250,17 +339,18 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
     const int tid = threadIdx.x;
     const int p = tid % P, jr = tid / P;
     const long n = 1L << lgn, n1 = 1L << lgn1, n2 = n >> lgn1;
-    const long t2 = blockIdx.x;
-    const int j0 = blockIdx.y * JB;
+    const long t2 = blockIdx.y;
+    const int j0 = blockIdx.x * JB;
 
     float2 v[Q];
     {
-        // Y[j0 + jr, t2, p + P q]: uniform base plus a 32-bit lane offset (jr n + p < 2^26)
-        const bool live = j0 + jr < r;
-        const typename YS::E* __restrict__ yb = Y + (size_t)j0 * n + t2 * n1;
-        const unsigned yo = (unsigned)(jr * n + p);
+        // Y[j0 + jr, t2, p + P q] from a buffer that ends with Y, so rows j >= r read zeros; byte offsets
+        // (jr n + p + P q) E < 2^30
+        constexpr unsigned ES = sizeof(typename YS::E);
+        const __amdgpu_buffer_rsrc_t rs = buffer_of(Y + (size_t)j0 * n + t2 * n1, ((long)(r - j0) * n - t2 * n1) * ES);
+        const unsigned yo = (unsigned)(jr * n + p) * ES;
 #pragma unroll
-        for (int q = 0; q < Q; q++) v[q] = live ? YS::get((yb + P * q)[yo]) : make_float2(0.f, 0.f);
+        for (int q = 0; q < Q; q++) v[q] = YS::load(rs, yo + P * q * ES);
     }
     dft_reg<Q>(v);
     {
@@ -339,14 +429,26 @@ bool split(const int lgl, int& P)
     return true;
 }
 
+template <int P, int Q, int NZ, typename YS>
+void launch_cols_nz(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
+                    const float* ysc, void* Y, hipStream_t s)
+{
+    // columns per block: 16 complex64 or 32 fp16 pairs give 128-byte runs per Y row (8 for the 2048-point lines)
+    constexpr int C = Q > 32 ? 8 : sizeof(typename YS::E) == 8 ? 16 : 32;
+    hipLaunchKernelGGL((cols_k<P, Q, C, NZ, YS>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin,
+                       kend, n1, lgn, ysc, (typename YS::E*)Y);
+}
+
+// the thread of column k1 and lane p holds k = k1 + n1 (p + P q): all k >= kend for q >= NZ when n1 P NZ >= kend
 template <int P, int Q, typename YS>
 void launch_cols(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
                  const float* ysc, void* Y, hipStream_t s)
 {
-    // columns per block: 16 complex64 or 32 fp16 pairs give 128-byte runs per Y row (8 for the 2048-point lines)
-    constexpr int C = Q > 32 ? 8 : sizeof(typename YS::E) == 8 ? 16 : 32;
-    hipLaunchKernelGGL((cols_k<P, Q, C, YS>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin, kend,
-                       n1, lgn, ysc, (typename YS::E*)Y);
+    const long span = (long)n1 * P;
+    if (kend <= span * (Q / 8)) launch_cols_nz<P, Q, Q / 8, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
+    else if (kend <= span * (Q / 4)) launch_cols_nz<P, Q, Q / 4, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
+    else if (kend <= span * (Q / 2)) launch_cols_nz<P, Q, Q / 2, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
+    else launch_cols_nz<P, Q, Q, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
 }
 
 template <typename YS>
@@ -370,7 +472,7 @@ void launch_rows(const void* Y, int r, int rp, int lgn1, int lgn, long a0, long 
     // rows per block: 16 float32 pairs or 32 bf16 / fp8 pairs give 128- / 128- / 64-byte runs per output row
     constexpr int JB = Q > 32 ? 8 : sizeof(T) == 4 ? 16 : 32;
     const long n2 = 1L << (lgn - lgn1);
-    hipLaunchKernelGGL((rows_k<P, Q, JB, T, YS>), dim3((unsigned)n2, (rp + JB - 1) / JB), dim3(JB * P), 0, s,
+    hipLaunchKernelGGL((rows_k<P, Q, JB, T, YS>), dim3((rp + JB - 1) / JB, (unsigned)n2), dim3(JB * P), 0, s,
                        (const typename YS::E*)Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz);
 }
 

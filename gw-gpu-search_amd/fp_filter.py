@@ -225,6 +225,15 @@ def _fp8_scales(pw):
     return zcs, rms.contiguous()
 
 
+def _y16_scales(pw, n1):
+    """Per-(segment, basis row) scales of the transform's fp16 intermediate from the powers sum_k |q_jk|^2: the
+    intermediate (the length-n2 column transforms of q) holds n2 sum_k |q_jk|^2 over n values, so its rms is
+    sqrt(power / n1) and the scale is 1 / rms (1 for rows of zero power)."""
+    ms = pw / n1
+    sc = torch.where(ms > 0, torch.rsqrt(ms.clamp(min=1e-300)), torch.ones_like(ms)).clamp(1e-30, 1e30)
+    return sc.to(torch.float32).contiguous()
+
+
 def svd_metric(datasets):
     """fp_filter.py:360-364: IPall (all segments, flen), sw = sqrt(mean IPall) in float64, isw = 1/sw (0 where 0)."""
     IPall = torch.cat([m.IP for m in datasets], 0)
@@ -301,21 +310,27 @@ class Filter:
         if Zt is None:
             Zt = torch.empty((int(row0[-1]), K), dtype=dt, device='cuda')
         if ws > 0:
-            if self.code == 2:
+            # the transform's intermediate: complex64 for the fp32 form; two fp16 scaled to unit rms per basis row
+            # for the bf16 / fp8 forms (half the bytes; its 11-bit rounding is well below theirs)
+            ytype = 0 if self.code == 0 else 1
+            ysc = None
+            if ytype or self.code == 2:
                 pw = torch.zeros((nb, rp), dtype=torch.float64, device='cuda')
                 for i, (di, s, gs, a0, nw) in enumerate(segs):
                     _lib.call('gwgs_vd_power', tb.Vc.data_ptr(), self.flen, tb.r, self.data[di].D[s].data_ptr(),
                               self.kmin, tb.kend, pw[i].data_ptr(), _dev.stream())
-                zcs, acs = _fp8_scales(pw)
+                if self.code == 2:
+                    zcs, acs = _fp8_scales(pw)
+                if ytype:
+                    ysc = _y16_scales(pw, 1 << ((int(self.tlen).bit_length() - 1) // 2))
             work = self._workbuf(ws)
             esz = Zt.element_size()
-            # intermediate in complex64 (the fp16 option halves its bytes but, with 4-byte lanes, measured slower)
-            ytype = 0
             for i, (di, s, gs, a0, nw) in enumerate(segs):
                 e0 = _dev.METER.start()
                 _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
                           self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
-                          ytype, None, work.data_ptr(), Zt[row0[i]].data_ptr(), K, _dev.stream())
+                          ytype, _dev.ptr(ysc[i]) if ytype else None, work.data_ptr(), Zt[row0[i]].data_ptr(), K,
+                          _dev.stream())
                 # HBM bytes: Vc rows and D over [kmin, kend), Y written and read, Zt written
                 _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin) + 8 * (tb.kend - self.kmin)
                                 + (8 if ytype else 16) * tb.r * self.tlen + Tns[i] * K * esz)

@@ -365,6 +365,7 @@ def test_time_slides_vs_brute_force(gpu):
     (1 << 17, 16, 0, (1 << 16) + 1, 0, 1 << 17),      # 256 x 512, whole series
     (1 << 19, 9, 3840, 120000, 4096, 65536),          # 512 x 1024
     (1 << 20, 21, 7680, 230000, 98304, 917504),       # 1024 x 1024, a 512 s segment at 2048 Hz
+    (1 << 20, 35, 10240, 128240, 98304, 917504),      # the same below 250 Hz: 4 of 32 column inputs per thread
     (1 << 21, 3, 100, 500000, 5, 1000),               # 1024 x 2048
     (1 << 22, 2, 0, 1 << 21, (1 << 22) - 4096, 4096),  # 2048 x 2048, the last rows
 ])
@@ -413,6 +414,20 @@ def test_basis_series_vs_oracle(gpu, n, r, kmin, kend, a0, Tn):
     eh = np.abs(zh[:, 0:2 * r:2].astype(np.float64) + 1j * zh[:, 1:2 * r:2] - ref)
     assert np.sqrt(np.mean(eh ** 2)) < 5e-4 * rms and eh.max() < 5e-3 * rms, (eh.max() / rms)
     assert np.all(zh[:, 2 * r:] == 0)
+
+
+def test_basis_series_empty_band(gpu):
+    """kmin == kend: every input is outside the band, so the series (and the padding rows) are exactly zero."""
+    import _dev, _lib
+    n, r, rp, flen = 1 << 17, 5, 16, (1 << 16) + 1
+    Vd = torch.ones((rp, flen), dtype=torch.complex64, device='cuda')
+    Dd = torch.ones(flen, dtype=torch.complex64, device='cuda')
+    work = torch.empty(_lib.load().gwgs_basis_series_workspace(r, n), dtype=torch.uint8, device='cuda')
+    for kmin in (0, 3000):
+        Zt = torch.full((4096, 2 * rp), 7, dtype=torch.float32, device='cuda')
+        _lib.call('gwgs_basis_series', 0, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kmin, n, 100, 4096, None,
+                  0, None, work.data_ptr(), Zt.data_ptr(), 2 * rp, _dev.stream())
+        assert torch.count_nonzero(Zt).item() == 0
 
 
 def test_vd_power_vs_float64(gpu):
