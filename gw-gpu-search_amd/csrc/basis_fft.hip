@@ -164,23 +164,6 @@ __device__ __forceinline__ void advance(double& c, double& s, const double dc, c
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
-constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
-
-// a buffer of nbytes bytes at p, capped at 2^31 - 1 (offsets used here stay below 2^30)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, const long nbytes)
-{
-    const int nrec = nbytes <= 0 ? 0 : nbytes > 0x7fffffffL ? 0x7fffffff : (int)nbytes;
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, nrec, kRsrcWord3);
-}
-
-// 8 bytes at byte offset off of a buffer (zeros when off is at or past its size)
-__device__ __forceinline__ float2 load_f2(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
-{
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
-    return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
-}
-
 // Storage of the intermediate Y: complex64, or two fp16 (round to nearest even) of the value times a per-row scale
 // (1 / rms of the row's Y, so the half range is never approached and its 11-bit precision applies relative to the
 // row's own level; used for the bf16 / fp8 operand forms, whose own rounding is coarser).

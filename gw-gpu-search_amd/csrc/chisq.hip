@@ -41,7 +41,9 @@ __global__ void __launch_bounds__(NT) chisq_bins_k(const float2* __restrict__ q,
 {
     const int nbin = nedge - 1;
     const int lane = threadIdx.x & 63;
-    const long wi = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    // wave index, uniform over the wave by construction (stated, so that the row, its edges and the buffer
+    // descriptors below live in scalar registers)
+    const long wi = (long)blockIdx.x * (NT / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wi >= (long)ngroups * nbin) return;   // whole waves
     const int g = (int)(wi / nbin), j = (int)(wi % nbin);
     const int row = grow[g], p0 = gstart[g], np_ = gcount[g];
@@ -64,28 +66,49 @@ __global__ void __launch_bounds__(NT) chisq_bins_k(const float2* __restrict__ q,
     float2 acc[P];
 #pragma unroll
     for (int p = 0; p < P; p++) acc[p] = make_float2(0.f, 0.f);
+    // q (and d) through buffers that end at k1: a lane past the bin's end reads zeros and adds exact zeros, so every
+    // lane of a run takes the same number of steps, rounded up to a multiple of 4 (full runs have R = 32), and the
+    // loads carry no branch. Four registers rotate by unrolling, so each step's loads are issued three steps ahead
+    // (a register copy would wait for the load). Byte offsets 8 k < 2^31.
+    const __amdgpu_buffer_rsrc_t rq = buffer_of(qr, (long)k1 * 8), rd = buffer_of(FUSE ? dv : qr, (long)k1 * 8);
     constexpr int RW = 64 * R;                // bins per run
+    static_assert(R % 4 == 0, "runs are whole groups of 4 steps");
+    auto step = [&](const float2 qv, const float2 dk, float2 (&w)[P]) {
+        float2 v = qv;
+        if (FUSE) v = make_float2(qv.x * dk.x + qv.y * dk.y, qv.x * dk.y - qv.y * dk.x);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
+            acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
+            const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
+            w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
+            w[p].x = wx;
+        }
+    };
     for (int ra = k0; ra < k1; ra += RW) {
-        const int ka = ra + lane, kb = min(ra + RW, k1);
-        if (ka >= kb) continue;
+        const int ka = ra + lane;
+        const int ns = ((min(ra + RW, k1) - ra + 255) >> 8) << 2;   // steps of this run, the same for every lane
         float2 w[P];
 #pragma unroll
         for (int p = 0; p < P; p++) w[p] = p < np_ ? phasor((unsigned long long)ka, t[p], N, mask) : make_float2(0.f, 0.f);
-#pragma unroll 4
-        for (int k = ka; k < kb; k += 64) {
-            float2 v = qr[k];
-            if (FUSE) {
-                const float2 dk = dv[k];
-                v = make_float2(v.x * dk.x + v.y * dk.y, v.x * dk.y - v.y * dk.x);
-            }
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
-                acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
-                const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
-                w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
-                w[p].x = wx;
-            }
+        const unsigned o0 = (unsigned)ka * 8u;
+        float2 qa = load_f2(rq, o0), qb = load_f2(rq, o0 + 512u), qc = load_f2(rq, o0 + 1024u), qd = qa;
+        float2 da = qa, db = qb, dc = qc, dd = qa;
+        if (FUSE) { da = load_f2(rd, o0); db = load_f2(rd, o0 + 512u); dc = load_f2(rd, o0 + 1024u); }
+        for (int s = 0; s < ns; s += 4) {
+            const unsigned o = o0 + 512u * (unsigned)s;
+            qd = load_f2(rq, o + 1536u);
+            if (FUSE) dd = load_f2(rd, o + 1536u);
+            step(qa, da, w);
+            qa = load_f2(rq, o + 2048u);
+            if (FUSE) da = load_f2(rd, o + 2048u);
+            step(qb, db, w);
+            qb = load_f2(rq, o + 2560u);
+            if (FUSE) db = load_f2(rd, o + 2560u);
+            step(qc, dc, w);
+            qc = load_f2(rq, o + 3072u);
+            if (FUSE) dc = load_f2(rd, o + 3072u);
+            step(qd, dd, w);
         }
     }
 #pragma unroll

@@ -84,3 +84,28 @@ __device__ __forceinline__ float fp8_to_f32(const uint8_t b)
 {
     return __builtin_amdgcn_cvt_f32_fp8((int)b, 0);
 }
+
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer descriptor word 3 (32-bit data format, no swizzle)
+
+// A raw buffer of nbytes bytes at p (capped at 2^31 - 1; callers' offsets stay below that). Loads at or past its
+// size return zeros and stores there are dropped by the hardware, so a range check needs no branch: a load under a
+// per-lane branch makes the compiler wait for it before the next one is issued.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, const long nbytes)
+{
+    const int nrec = nbytes <= 0 ? 0 : nbytes > 0x7fffffffL ? 0x7fffffff : (int)nbytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, nrec, kRsrcWord3);
+}
+
+// 8 bytes at byte offset off of a buffer (zeros when off is at or past its size)
+__device__ __forceinline__ float2 load_f2(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+    return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+
+// 4 bytes at byte offset off of a buffer (zero when off is at or past its size)
+__device__ __forceinline__ float load_f1(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+}

@@ -54,32 +54,40 @@ __global__ void spa_templates_k(const double* __restrict__ m1, const double* __r
 // |h_ik|^2 IP_sk in double. Rows vary fastest in the grid so the blocks in flight share the IP chunk in L2.
 constexpr int CHK = 8192;
 
+// NS >= nseg segments are summed; row i and each segment's IP over [k0, k1) are read as buffers, so bins past k1 and
+// segments s >= nseg read zeros (adding exact zeros): the CHK / NT steps are the same for every lane and carry no
+// branch, and their loads issue back to back.
+template <int NS>
 __global__ void __launch_bounds__(NT) sigmasq_part_k(const float2* __restrict__ H, const long ldh, const long kmin,
                                                      const long kend, const float* __restrict__ IP, const long ldip,
                                                      const int nseg, double* __restrict__ part)
 {
     const int i = blockIdx.x, c = blockIdx.y, n = gridDim.x;
     const long k0 = kmin + (long)c * CHK, k1 = min(kend, k0 + CHK);
-    double acc[MAXSEG];
+    double acc[NS];
 #pragma unroll
-    for (int s = 0; s < MAXSEG; s++) acc[s] = 0.0;
-    const float2* __restrict__ h = H + (size_t)i * ldh;
-#pragma unroll 4
-    for (long k = k0 + threadIdx.x; k < k1; k += NT) {
-        const float2 v = h[k];
+    for (int s = 0; s < NS; s++) acc[s] = 0.0;
+    const __amdgpu_buffer_rsrc_t rh = buffer_of(H + (size_t)i * ldh + k0, (k1 - k0) * 8);
+    __amdgpu_buffer_rsrc_t ri[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) ri[s] = buffer_of(IP + (size_t)s * ldip + k0, s < nseg ? (k1 - k0) * 4 : 0);
+#pragma unroll 2
+    for (int j = 0; j < CHK / NT; j++) {
+        const unsigned o = (unsigned)(threadIdx.x + NT * j);
+        const float2 v = load_f2(rh, o * 8u);
+        float ip[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) ip[s] = load_f1(ri[s], o * 4u);
         const double p = (double)v.x * v.x + (double)v.y * v.y;
 #pragma unroll
-        for (int s = 0; s < MAXSEG; s++)
-            if (s < nseg) acc[s] += p * (double)IP[(size_t)s * ldip + k];
+        for (int s = 0; s < NS; s++) acc[s] += p * (double)ip[s];
     }
-    __shared__ double sh[NT / 64][MAXSEG];
+    __shared__ double sh[NT / 64][NS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int s = 0; s < MAXSEG; s++) {
-        if (s < nseg) {
-            const double x = wave_sum_f64(acc[s]);
-            if (lane == 0) sh[wave][s] = x;
-        }
+    for (int s = 0; s < NS; s++) {
+        const double x = wave_sum_f64(acc[s]);
+        if (lane == 0) sh[wave][s] = x;
     }
     __syncthreads();
     if (threadIdx.x < nseg) {
@@ -135,8 +143,13 @@ GWGS_API int gwgs_sigmasq(const void* H, long ldh, long kmin, long kend, int n, 
     if (nch > 0) {
         GWGS_CHECK(work, GWGS_EINVAL);
         GWGS_CHECK(nch < 65536 && n < (1 << 30), GWGS_ESIZE);
-        hipLaunchKernelGGL(sigmasq_part_k, dim3(n, nch), dim3(NT), 0, st, (const float2*)H, ldh, kmin, kend, IP, ldip,
-                           nseg, work);
+        const dim3 grid(n, nch);
+        const float2* h = (const float2*)H;
+        if (nseg == 1) hipLaunchKernelGGL(sigmasq_part_k<1>, grid, dim3(NT), 0, st, h, ldh, kmin, kend, IP, ldip, nseg, work);
+        else if (nseg == 2) hipLaunchKernelGGL(sigmasq_part_k<2>, grid, dim3(NT), 0, st, h, ldh, kmin, kend, IP, ldip, nseg, work);
+        else if (nseg <= 4) hipLaunchKernelGGL(sigmasq_part_k<4>, grid, dim3(NT), 0, st, h, ldh, kmin, kend, IP, ldip, nseg, work);
+        else if (nseg <= 8) hipLaunchKernelGGL(sigmasq_part_k<8>, grid, dim3(NT), 0, st, h, ldh, kmin, kend, IP, ldip, nseg, work);
+        else hipLaunchKernelGGL(sigmasq_part_k<MAXSEG>, grid, dim3(NT), 0, st, h, ldh, kmin, kend, IP, ldip, nseg, work);
     }
     const long tot = (long)n * nseg;
     hipLaunchKernelGGL(sigmasq_sum_k, dim3((unsigned)((tot + NT - 1) / NT)), dim3(NT), 0, st, work, n, nch, nseg, df,
