@@ -225,15 +225,6 @@ def _fp8_scales(pw):
     return zcs, rms.contiguous()
 
 
-def _y16_scales(pw, n1):
-    """Per-(segment, basis row) scales of the transform's fp16 intermediate from the powers sum_k |q_jk|^2: the
-    intermediate (the length-n2 column transforms of q) holds n2 sum_k |q_jk|^2 over n values, so its rms is
-    sqrt(power / n1) and the scale is 1 / rms (1 for rows of zero power)."""
-    ms = pw / n1
-    sc = torch.where(ms > 0, torch.rsqrt(ms.clamp(min=1e-300)), torch.ones_like(ms)).clamp(1e-30, 1e30)
-    return sc.to(torch.float32).contiguous()
-
-
 def svd_metric(datasets):
     """fp_filter.py:360-364: IPall (all segments, flen), sw = sqrt(mean IPall) in float64, isw = 1/sw (0 where 0)."""
     IPall = torch.cat([m.IP for m in datasets], 0)
@@ -268,11 +259,35 @@ class Filter:
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
         self.rel = []
         self.code = PRECISIONS[opts.precision]
+        self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
+        self._y16_update()
 
     def _sync(self):
         if self.opts.timing:
             torch.cuda.synchronize()
         return time.perf_counter()
+
+    def _y16_update(self):
+        """Scale of the basis transform's fp16 intermediate per segment, one value for every basis row, in rows of
+        self._ysc (segments of all datasets in order; opts.tile + 16 >= rp columns): the basis rows are unit vectors
+        vh_j over the band (q_jk = conj(vh_jk) D_k / sw_k), so their power sum_k |q_jk|^2 is a weighted mean of
+        |D_k / sw_k|^2 and the intermediate's rms is about sqrt(c / n1) with c the segment's mean of |D_k / sw_k|^2
+        over [kmin, flen) (sw = sqrt(mean IP), as the SVD metric). The scale is 1 / (64 rms): fp16 then holds values
+        from 4e-3 to 4e6 times the rms as normal numbers, far beyond what one row's spread or a loud segment needs,
+        and its rounding stays relative (2^-11). Computed when data is bound, outside any captured graph."""
+        if self.code == 0:
+            return
+        isw2 = torch.where(self.IPall.double().mean(0) > 0, 1.0 / self.IPall.double().mean(0).clamp(min=1e-300),
+                           torch.zeros((), dtype=torch.float64, device=self.IPall.device))[self.kmin:]
+        n1 = 1 << ((int(self.tlen).bit_length() - 1) // 2)
+        vals = []
+        for m in self.data:
+            c = ((m.D[:, self.kmin:].abs().double() ** 2) * isw2).mean(1)
+            vals.append(torch.where(c > 0, torch.sqrt(n1 / c.clamp(min=1e-300)) / 64.0, torch.ones_like(c)))
+        v = torch.cat(vals).clamp(1e-30, 1e30).to(torch.float32)
+        if self._ysc is None:
+            self._ysc = torch.empty((len(v), self.opts.tile + 16), dtype=torch.float32, device='cuda')
+        self._ysc.copy_(v[:, None].expand_as(self._ysc))
 
     def _qbuf(self, nb, rp):
         """(nb, rp, tlen) complex64 transform input whose columns >= flen stay zero (fp_filter.py:595): allocated
@@ -313,24 +328,21 @@ class Filter:
             # the transform's intermediate: complex64 for the fp32 form; two fp16 scaled to unit rms per basis row
             # for the bf16 / fp8 forms (half the bytes; its 11-bit rounding is well below theirs)
             ytype = 0 if self.code == 0 else 1
-            ysc = None
-            if ytype or self.code == 2:
+            if self.code == 2:
                 pw = torch.zeros((nb, rp), dtype=torch.float64, device='cuda')
                 for i, (di, s, gs, a0, nw) in enumerate(segs):
                     _lib.call('gwgs_vd_power', tb.Vc.data_ptr(), self.flen, tb.r, self.data[di].D[s].data_ptr(),
                               self.kmin, tb.kend, pw[i].data_ptr(), _dev.stream())
-                if self.code == 2:
-                    zcs, acs = _fp8_scales(pw)
-                if ytype:
-                    ysc = _y16_scales(pw, 1 << ((int(self.tlen).bit_length() - 1) // 2))
+                zcs, acs = _fp8_scales(pw)
+            assert not ytype or rp <= self._ysc.shape[1], 'tile larger than this Filter\'s opts.tile'
             work = self._workbuf(ws)
             esz = Zt.element_size()
             for i, (di, s, gs, a0, nw) in enumerate(segs):
                 e0 = _dev.METER.start()
                 _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
                           self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
-                          ytype, _dev.ptr(ysc[i]) if ytype else None, work.data_ptr(), Zt[row0[i]].data_ptr(), K,
-                          _dev.stream())
+                          ytype, self._ysc[self.seg0[di] + s].data_ptr() if ytype else None, work.data_ptr(),
+                          Zt[row0[i]].data_ptr(), K, _dev.stream())
                 # HBM bytes: Vc rows and D over [kmin, kend), Y written and read, Zt written
                 _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin) + 8 * (tb.kend - self.kmin)
                                 + (8 if ytype else 16) * tb.r * self.tlen + Tns[i] * K * esz)
@@ -544,6 +556,7 @@ class Filter:
             m.IP.copy_(d.IP)
             m.gates, m.gps_start, m.seg_start = list(d.gates), d.gps_start, list(d.seg_start)
         torch.cat([m.IP for m in self.data], 0, out=self.IPall)
+        self._y16_update()
         self.T = {k: 0 if isinstance(v, int) else 0.0 for k, v in self.T.items()}
         self.rel = []
 
