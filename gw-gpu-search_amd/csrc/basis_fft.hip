@@ -161,6 +161,54 @@ __device__ __forceinline__ void advance(double& c, double& s, const double dc, c
     c = t;
 }
 
+// A twiddle e^(2 pi i e / m) advanced by a fixed step: float64 seeds and products, rounded once to float32 where
+// applied (Rot<false>); or float32 throughout (Rot<true>, for the forms whose intermediate is fp16: sincospif seeds
+// of exact arguments and float products drift by a few 1e-7 over the 64 steps of a line, far below fp16's 2^-11).
+template <bool LP>
+struct Rot;
+template <>
+struct Rot<false> {
+    double c = 1.0, s = 0.0, dc, ds;
+    __device__ __forceinline__ void seed(const long e, const long m) { unit(e, m, c, s); }
+    __device__ __forceinline__ void step(const long e, const long m) { unit(e, m, dc, ds); }
+    __device__ __forceinline__ void next() { advance(c, s, dc, ds); }
+    __device__ __forceinline__ float2 apply(const float2 a) const { return cmulf(a, c, s); }
+};
+template <>
+struct Rot<true> {
+    float c = 1.f, s = 0.f, dc, ds;
+    static __device__ __forceinline__ void unitf(const long e, const long m, float& co, float& si)
+    {
+        // (e mod m) < m <= 2^22: 2 (e mod m) / m is exact in float
+        sincospif(2.f * (float)(e & (m - 1)) / (float)m, &si, &co);
+    }
+    __device__ __forceinline__ void seed(const long e, const long m) { unitf(e, m, c, s); }
+    __device__ __forceinline__ void step(const long e, const long m) { unitf(e, m, dc, ds); }
+    __device__ __forceinline__ void next()
+    {
+        const float t = c * dc - s * ds;
+        s = c * ds + s * dc;
+        c = t;
+    }
+    __device__ __forceinline__ float2 apply(const float2 a) const
+    {
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+};
+
+// v[m2] *= w_L^(p m2) for 0 < m2 < Q
+template <int Q, bool LP>
+__device__ __forceinline__ void twiddle_line(float2 (&v)[Q], const int p, const int L)
+{
+    Rot<LP> w;
+    w.step(p, L);
+#pragma unroll
+    for (int m2 = 1; m2 < Q; m2++) {
+        w.next();
+        v[m2] = w.apply(v[m2]);
+    }
+}
+
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
@@ -202,6 +250,7 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
                                                 typename YS::E* __restrict__ Y)
 {
     constexpr int L = P * Q, ROW = pitch(P * C);
+    constexpr bool LP = sizeof(typename YS::E) == 4;     // fp16 intermediate: float32 twiddles
     __shared__ float ex[Q * ROW];
     const int tid = threadIdx.x;
     const int c = tid % C, p = tid / C;
@@ -231,15 +280,7 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
             v[q] = make_float2(a[q].x * d[q].x - a[q].y * d[q].y, a[q].x * d[q].y + a[q].y * d[q].x);
     }
     dft_reg<Q, NZ>(v);
-    {
-        double dc, ds, wc = 1.0, ws = 0.0;
-        unit(p, L, dc, ds);
-#pragma unroll
-        for (int m2 = 1; m2 < Q; m2++) {
-            advance(wc, ws, dc, ds);
-            v[m2] = cmulf(v[m2], wc, ws);
-        }
-    }
+    twiddle_line<Q, LP>(v, p, L);
     // exchange: U(c, p, m2) at m2 ROW + p C + c; step 2 takes tasks (c, m2 = tid / C + P i)
     const int mb = tid / C;
     float2 u[Q / P][P];
@@ -264,14 +305,14 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
         const int m2 = mb + P * i;
         dft_reg<P>(u[i]);
         // w_n^(k1 t2), t2 = m2 + Q m1: Y[j, t2, k1] at the uniform base Q m1 n1 plus the lane offset m2 n1 + k1
-        double wc, ws, dc, ds;
-        unit((long)k1 * m2, n, wc, ws);
-        unit((long)k1 * Q, n, dc, ds);
+        Rot<LP> w;
+        w.seed((long)k1 * m2, n);
+        w.step((long)k1 * Q, n);
         const unsigned yo = (unsigned)(m2 * n1 + k1);
 #pragma unroll
         for (int m1 = 0; m1 < P; m1++) {
-            if (m1) advance(wc, ws, dc, ds);
-            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(cmulf(u[i][m1], wc, ws), ys);
+            if (m1) w.next();
+            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(w.apply(u[i][m1]), ys);
         }
     }
 }
@@ -318,6 +359,7 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
                                                  T* __restrict__ Zt, const long ldz)
 {
     constexpr int L = P * Q, ROW = pitch(P * (JB + 1));
+    constexpr bool LP = sizeof(typename YS::E) == 4;     // fp16 intermediate: float32 twiddles
     __shared__ float ex[Q * ROW];
     const int tid = threadIdx.x;
     const int p = tid % P, jr = tid / P;
@@ -336,15 +378,7 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
         for (int q = 0; q < Q; q++) v[q] = YS::load(rs, yo + P * q * ES);
     }
     dft_reg<Q>(v);
-    {
-        double dc, ds, wc = 1.0, ws = 0.0;
-        unit(p, L, dc, ds);
-#pragma unroll
-        for (int m2 = 1; m2 < Q; m2++) {
-            advance(wc, ws, dc, ds);
-            v[m2] = cmulf(v[m2], wc, ws);
-        }
-    }
+    twiddle_line<Q, LP>(v, p, L);
     // exchange: U(jr, p, m2) at m2 ROW + p (JB + 1) + jr; step 2 takes tasks (jr2 = tid % JB, m2 = tid / JB + P i)
     const int jr2 = tid % JB, mb = tid / JB;
     float2 u[Q / P][P];
