@@ -163,15 +163,10 @@ __device__ __forceinline__ double block_exscan_f64(const double x, double* sh, d
 // One block per requested row (multi_filter.py:264-282; chisq_edges.py:211-245): edge j = first k with the float64
 // cumulative sum of |h|^2 IP over [0, k1) above j / nb * total, clipped to [kmin, k1]; columns j >= nb and the last
 // hold k1. Pass 1: chunk sums, each chunk read by one wave in coalesced 64-bin strides; a block scan of the chunk
-// sums. Pass 2: each target's chunk by binary search, then one wave walks it in 64-bin strides with a wave prefix
-// sum and takes the first lane above the target (the next chunk if rounding left it just short).
+// sums. Pass 2: each target's chunk by binary search, then one wave walks it in 64-bin strides (loaded eight strides
+// at a time) with a wave prefix sum and takes the first lane above the target (the next chunk if rounding left it
+// just short).
 constexpr int EMAXCH = 4096;
-
-__device__ __forceinline__ double pw_ip(const float2* __restrict__ hr, const float* __restrict__ ips, const long k)
-{
-    const float2 v = hr[k];
-    return ((double)v.x * v.x + (double)v.y * v.y) * (double)ips[k];
-}
 
 __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h, const long ldh,
                                                     const int* __restrict__ rows, const float* __restrict__ ips,
@@ -180,20 +175,28 @@ __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h
 {
     __shared__ double ex[EMAXCH + 1];
     __shared__ double sh[NT / 64];
-    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const float2* __restrict__ hr = h + (size_t)rows[g] * ldh;
     const long k1 = k1s[g];
     const int nb = nbs[g];
     int* eo = out + (size_t)g * nedge;
     for (int i = tid; i < nedge; i += NT) eo[i] = (int)k1;
+    // h and 1/psd of the row as buffers ending at k1 (byte offsets below 2^31): bins past k1 read zeros and add
+    // exact zeros, so the loops below have the same trip count on every lane and carry no branch around their loads
+    const __amdgpu_buffer_rsrc_t rh = buffer_of(hr, k1 * 8), ri = buffer_of(ips, k1 * 4);
+    auto pw_at = [&](const long k) {
+        const float2 v = load_f2(rh, (unsigned)k * 8u);
+        return ((double)v.x * v.x + (double)v.y * v.y) * (double)load_f1(ri, (unsigned)k * 4u);
+    };
     // chunk length: a multiple of 64, at least 2048, and at most EMAXCH chunks per row
     const long ech = max(2048L, ((k1 + EMAXCH - 1) / EMAXCH + 63) / 64 * 64);
     const int nch = (int)((k1 + ech - 1) / ech);
     for (int c = wave; c < nch; c += NT / 64) {
-        const long c0 = (long)c * ech, c1 = min(c0 + ech, k1);
+        const long c0 = (long)c * ech + lane;
         double s = 0.0;
 #pragma unroll 8
-        for (long k = c0 + lane; k < c1; k += 64) s += pw_ip(hr, ips, k);
+        for (int i = 0; i < (int)(ech >> 6); i++) s += pw_at(c0 + 64 * i);
         s = wave_sum_f64(s);
         if (lane == 0) ex[c] = s;
     }
@@ -216,17 +219,31 @@ __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h
         while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (ex[mid] <= target) lo = mid; else hi = mid - 1; }
         long k = k1;
         double acc = ex[lo];
-        for (long base = (long)lo * ech; base < k1; base += 64) {
-            const long kk = base + lane;
-            double v = kk < k1 ? pw_ip(hr, ips, kk) : 0.0;
+        // the walk in groups of WG strides of 64 bins: a group's loads are issued together, then scanned in order
+        constexpr int WG = 8;
+        bool found = false;
+        for (long base = (long)lo * ech; base < k1 && !found; base += 64 * WG) {
+            float2 hv[WG];
+            float iv[WG];
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const double y = __shfl_up(v, o, 64);
-                if (lane >= o) v += y;
+            for (int s = 0; s < WG; s++) {
+                const unsigned o = (unsigned)(base + 64 * s + lane);
+                hv[s] = load_f2(rh, o * 8u);
+                iv[s] = load_f1(ri, o * 4u);
             }
-            const unsigned long long m = __ballot(acc + v > target && kk < k1);
-            if (m) { k = base + __ffsll((long long)m) - 1; break; }
-            acc += __shfl(v, 63, 64);
+#pragma unroll
+            for (int s = 0; s < WG; s++) {
+                const long kk = base + 64 * s + lane;
+                double v = ((double)hv[s].x * hv[s].x + (double)hv[s].y * hv[s].y) * (double)iv[s];
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const double y = __shfl_up(v, o, 64);
+                    if (lane >= o) v += y;
+                }
+                const unsigned long long m = __ballot(acc + v > target && kk < k1);
+                if (m) { k = base + 64 * s + __ffsll((long long)m) - 1; found = true; break; }
+                acc += __shfl(v, 63, 64);
+            }
         }
         if (lane == 0) eo[j] = (int)min(max(k, (long)kmin), k1);
     }
