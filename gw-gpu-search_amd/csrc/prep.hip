@@ -20,47 +20,57 @@ __global__ void welch_segments_k(const float* __restrict__ x, const long ldx, co
     segs[((size_t)r * nseg + s) * L + i] = (double)x[(size_t)r * ldx + start + (size_t)s * stride + i] * w;
 }
 
-// One thread per (row, bin): p_s = |X_s|^2 dt^2 (halved at DC and Nyquist), median over the nseg segments
-// (numpy's: the middle order statistic, or the mean of the two middle ones), / bias, * fac; or the mean.
-// WB bins per block, their nseg <= WMAX values staged in LDS.
-template <int WB, int WMAX>
-__global__ void __launch_bounds__(WB) welch_reduce_k(const double2* __restrict__ X, const int nseg, const int nf,
-                                                    const double dt2, const int mean, const double bias,
-                                                    const double fac, double* __restrict__ out)
+// Per (row, bin): p_s = |X_s|^2 dt^2 (halved at DC and Nyquist), median over the nseg segments (numpy's: the middle
+// order statistic, or the mean of the two middle ones), / bias, * fac; or the mean (summed in segment order).
+// A block of 256 threads takes WB = 8192 / NP bins: their nseg <= NP values are staged in LDS (64 KB, bins fastest, so
+// a row of segments is read coalesced), padded to NP with +inf, and each bin's column is sorted by a bitonic network
+// (NP / 2 compare-exchanges per stage over all WB columns at once, one barrier per stage). The order statistics are
+// the same values whatever the algorithm, so the result is the one the selection by counting gave.
+template <int NP>
+__global__ void __launch_bounds__(256) welch_reduce_k(const double2* __restrict__ X, const int nseg, const int nf,
+                                                     const double dt2, const int mean, const double bias,
+                                                     const double fac, double* __restrict__ out)
 {
-    __shared__ double v[WMAX][WB];
-    const int r = blockIdx.y, b = blockIdx.x * WB + threadIdx.x, tl = threadIdx.x;
-    const bool ok = b < nf;
-    for (int s = 0; s < nseg; s++) {
-        double p = 0.0;
-        if (ok) {
-            const double2 z = X[((size_t)r * nseg + s) * nf + b];
-            p = (z.x * z.x + z.y * z.y) * dt2;
-            if (b == 0 || b == nf - 1) p /= 2;
-        }
-        v[s][tl] = p;
+    constexpr int WB = 8192 / NP;
+    __shared__ double v[NP * WB];               // v[s WB + bin]
+    const int r = blockIdx.y, b0 = blockIdx.x * WB, tid = threadIdx.x;
+    const double2* __restrict__ xr = X + (size_t)r * nseg * nf;
+    for (int e = tid; e < NP * WB; e += 256) {
+        const int s = e / WB, b = b0 + e % WB;
+        const bool ok = s < nseg && b < nf;
+        const double2 z = xr[ok ? (size_t)s * nf + b : 0];     // unconditional load of a valid element
+        double p = (z.x * z.x + z.y * z.y) * dt2;
+        if (b == 0 || b == nf - 1) p /= 2;
+        v[e] = ok ? p : s < nseg ? 0.0 : __builtin_inf();
     }
-    if (!ok) return;
-    double res;
+    __syncthreads();
     if (mean) {
-        double acc = 0.0;
-        for (int s = 0; s < nseg; s++) acc += v[s][tl];
-        res = acc / nseg;
-    } else {
-        // k-th order statistic by counting: the value with #less <= k < #less + #equal
-        auto kth = [&](const int k) {
-            for (int s = 0; s < nseg; s++) {
-                const double x = v[s][tl];
-                int lt = 0, eq = 0;
-                for (int u = 0; u < nseg; u++) { const double y = v[u][tl]; lt += y < x; eq += y == x; }
-                if (lt <= k && k < lt + eq) return x;
-            }
-            return 0.0;
-        };
-        res = (nseg & 1) ? kth(nseg / 2) : 0.5 * (kth(nseg / 2 - 1) + kth(nseg / 2));
-        res /= bias;
+        if (tid < WB && b0 + tid < nf) {
+            double acc = 0.0;
+            for (int s = 0; s < nseg; s++) acc += v[s * WB + tid];
+            out[(size_t)r * nf + b0 + tid] = acc / nseg * fac;
+        }
+        return;
     }
-    out[(size_t)r * nf + b] = res * fac;
+    for (int k = 2; k <= NP; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int e = tid; e < (NP / 2) * WB; e += 256) {
+                const int bl = e % WB, pi = e / WB;
+                const int i = 2 * j * (pi / j) + pi % j, l = i + j;    // pair (i, i + j), bit j of i clear
+                const double a = v[i * WB + bl], c = v[l * WB + bl];
+                if ((a > c) == ((i & k) == 0)) {
+                    v[i * WB + bl] = c;
+                    v[l * WB + bl] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (tid < WB && b0 + tid < nf) {
+        const int m = nseg / 2;
+        double res = (nseg & 1) ? v[m * WB + tid] : 0.5 * (v[(m - 1) * WB + tid] + v[m * WB + tid]);
+        res /= bias;
+        out[(size_t)r * nf + b0 + tid] = res * fac;
+    }
 }
 
 // np.interp of P (rows, nold) sampled at k df_old onto i df_new, i < nnew (right end held)
@@ -210,12 +220,21 @@ GWGS_API int gwgs_welch_reduce(const void* X, int rows, int nseg, int nf, double
 {
     GWGS_CHECK(X && out && rows > 0 && nseg > 0 && nf > 0, GWGS_EINVAL);
     GWGS_CHECK(nseg <= 512, GWGS_ESIZE);
-    if (nseg <= 128)
-        hipLaunchKernelGGL((welch_reduce_k<64, 128>), dim3((nf + 63) / 64, rows), dim3(64), 0, (hipStream_t)stream,
-                           (const double2*)X, nseg, nf, dt * dt, mean, bias, fac, out);
+    const double2* x = (const double2*)X;
+    hipStream_t st = (hipStream_t)stream;
+    const double dt2 = dt * dt;
+    if (nseg <= 64)
+        hipLaunchKernelGGL(welch_reduce_k<64>, dim3((nf + 127) / 128, rows), dim3(256), 0, st, x, nseg, nf, dt2, mean,
+                           bias, fac, out);
+    else if (nseg <= 128)
+        hipLaunchKernelGGL(welch_reduce_k<128>, dim3((nf + 63) / 64, rows), dim3(256), 0, st, x, nseg, nf, dt2, mean,
+                           bias, fac, out);
+    else if (nseg <= 256)
+        hipLaunchKernelGGL(welch_reduce_k<256>, dim3((nf + 31) / 32, rows), dim3(256), 0, st, x, nseg, nf, dt2, mean,
+                           bias, fac, out);
     else
-        hipLaunchKernelGGL((welch_reduce_k<16, 512>), dim3((nf + 15) / 16, rows), dim3(16), 0, (hipStream_t)stream,
-                           (const double2*)X, nseg, nf, dt * dt, mean, bias, fac, out);
+        hipLaunchKernelGGL(welch_reduce_k<512>, dim3((nf + 15) / 16, rows), dim3(256), 0, st, x, nseg, nf, dt2, mean,
+                           bias, fac, out);
     return gwgs_status();
 }
 
