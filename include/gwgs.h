@@ -122,8 +122,10 @@ int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double*
  * NULL. Vc: >= r rows of pitch ldv complex64; D complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt: Tn
  * rows of ldz >= 2 rp elements (ldz even). Returns GWGS_EUNSUP for other n, GWGS_ESIZE when Zt exceeds 4 GiB. */
 long gwgs_basis_series_workspace(int r, long n);
-/* ytype 0 keeps the intermediate in complex64; 1 in fp16 pairs of the value times yscale[j] (r floats; NULL = 1),
- * divided out in the second pass (half the intermediate's traffic). */
+/* ytype 0 keeps the intermediate in complex64 (float64 twiddles); 1 in fp16 pairs of the value times yscale[j]
+ * (r floats; NULL = 1), divided out in the second pass (half the intermediate's traffic; float32 twiddles). The
+ * filter passes ytype 1 for its bf16 / fp8 operands with one yscale value per segment, 1 / (64 rms), and ytype 0
+ * for fp32. */
 int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend, long n,
                       long a0, long Tn, const float* colscale, int ytype, const float* yscale, void* work, void* Zt,
                       long ldz, void* stream);
