@@ -197,16 +197,17 @@ class Triggers:
 
 
 def _order_by_template_time(tid, t):
-    """The permutation np.lexsort((t, tid)) returns (stable), found as one float64 argsort of the key
-    tid M + (t - min t), M = 2 span + 2, about twice as fast.  Rounding keeps the key monotone in (tid, t), so
-    the only way it can differ from the lexsort is two distinct (tid, t) rounding to one key; the result is
-    checked in one pass (keys sorted, equal pairs in index order) and the lexsort is taken when it fails."""
+    """The permutation np.lexsort((t, tid)) returns (stable), found as one float64 argsort (introsort) of the key
+    tid M + (t - min t), M = 2 span + 2, about ten times as fast.  Rounding keeps the key monotone in (tid, t), and a
+    trigger list holds each (tid, t) once, so the sorted order is the lexsort's unless two distinct (tid, t) round to
+    one key or a pair repeats; the result is checked in one pass (keys sorted, equal pairs in index order) and the
+    lexsort is taken when it fails."""
     if len(t) < 2:
         return np.arange(len(t), dtype=np.int64)
     t0, span = float(t.min()), float(t.max() - t.min())
     m = 2.0 * span + 2.0
     if tid.min() >= 0 and float(tid.max()) * m + m < 2.0 ** 40:
-        o = np.argsort(tid * m + (t - t0), kind='stable')
+        o = np.argsort(tid * m + (t - t0))
         a, b = tid[o], t[o]
         da, db = np.diff(a), np.diff(b)
         if np.all((da > 0) | ((da == 0) & ((db > 0) | ((db == 0) & (np.diff(o) > 0))))):
