@@ -414,6 +414,15 @@ def test_basis_series_vs_oracle(gpu, n, r, kmin, kend, a0, Tn):
     eh = np.abs(zh[:, 0:2 * r:2].astype(np.float64) + 1j * zh[:, 1:2 * r:2] - ref)
     assert np.sqrt(np.mean(eh ** 2)) < 5e-4 * rms and eh.max() < 5e-3 * rms, (eh.max() / rms)
     assert np.all(zh[:, 2 * r:] == 0)
+    # the filter's bf16 form: fp16 intermediate and bf16 output, within the bf16 rounding (2^-9 of each value) plus
+    # the intermediate's; padding columns exactly zero
+    Zb = torch.full((Tn, K), 7, dtype=torch.bfloat16, device='cuda')
+    _lib.call('gwgs_basis_series', 1, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kend, n, a0, Tn, None, 1,
+              ysc.data_ptr(), work.data_ptr(), Zb.data_ptr(), K, _dev.stream())
+    zb = Zb.float().cpu().numpy()
+    eb = np.abs(zb[:, 0:2 * r:2].astype(np.float64) + 1j * zb[:, 1:2 * r:2] - ref)
+    assert np.sqrt(np.mean(eb ** 2)) < 3e-3 * rms and eb.max() < 3e-2 * rms, (eb.max() / rms)
+    assert np.all(zb[:, 2 * r:] == 0)
 
 
 def test_basis_series_empty_band(gpu):
