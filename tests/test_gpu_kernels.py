@@ -439,6 +439,32 @@ def test_basis_series_empty_band(gpu):
         assert torch.count_nonzero(Zt).item() == 0
 
 
+def test_basis_series_padding_rows(gpu):
+    """rp well above r (a whole row block past the last basis row, read through an empty buffer): the padding
+    columns are zero and the live ones equal those of a call with the smallest padding, bit for bit, for the
+    complex64 and the fp16 intermediate."""
+    import _dev, _lib
+    rng = np.random.default_rng(11)
+    n, r, flen, kmin, kend = 1 << 17, 3, (1 << 16) + 1, 40, 30000
+    Vc = np.zeros((64, flen), np.complex64)
+    Vc[:r, kmin:kend] = rng.standard_normal((r, kend - kmin)) + 1j * rng.standard_normal((r, kend - kmin))
+    D = (rng.standard_normal(flen) + 1j * rng.standard_normal(flen)).astype(np.complex64)
+    Vd, Dd = torch.from_numpy(Vc).cuda(), torch.from_numpy(D).cuda()
+    work = torch.empty(_lib.load().gwgs_basis_series_workspace(r, n), dtype=torch.uint8, device='cuda')
+    ysc = torch.full((64,), 0.5, device='cuda')
+    Tn, a0 = 8192, 1000
+    for ytype in (0, 1):
+        outs = []
+        for rp in (16, 48):
+            Zt = torch.full((Tn, 2 * rp), 7, dtype=torch.bfloat16, device='cuda')
+            _lib.call('gwgs_basis_series', 1, Vd.data_ptr(), flen, r, rp, Dd.data_ptr(), kmin, kend, n, a0, Tn, None,
+                      ytype, ysc.data_ptr() if ytype else None, work.data_ptr(), Zt.data_ptr(), 2 * rp, _dev.stream())
+            outs.append(Zt.cpu())
+        assert torch.count_nonzero(outs[1][:, 2 * r:]).item() == 0
+        assert torch.equal(outs[0][:, :2 * r], outs[1][:, :2 * r])
+        assert torch.count_nonzero(outs[0][:, :2 * r]).item() > 0
+
+
 def test_vd_power_vs_float64(gpu):
     """gwgs_vd_power: sum over [kmin, kend) of |Vc D|^2 per row, against float64 (relative 1e-12)."""
     import _dev, _lib
