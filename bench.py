@@ -27,7 +27,7 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 METRIC = 'template·s filtered per GPU-s at 5,880 templates, BF16 vs FP32 at SNR err≤1e-3'
-PEAK = {'bf16': 2500.0, 'fp32': 157.3}      # dense MFMA TFLOP/s, MI355X
+PEAK = {'bf16': 2500.0, 'fp32': 157.3, 'fp8': 5000.0}      # dense MFMA TFLOP/s, MI355X
 
 
 def log(*a):
@@ -295,6 +295,9 @@ def main():
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--fp32-steps', type=int, default=1)
+    ap.add_argument('--fp8-steps', type=int, default=1, help='timed steps of the FP8 form (production workload)')
+    ap.add_argument('--fp8-margin', type=float, default=0.04, help='candidate margin of the FP8 form')
+    ap.add_argument('--fp8-refine', type=int, default=3, help='exact-stage re-centring half width of the FP8 form')
     ap.add_argument('--tile', type=int, default=512)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-check', action='store_true', help='skip the float64 direct-filter SNR check')
@@ -333,7 +336,8 @@ def main():
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
-                                 refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32')}
+                                 refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32', 'fp8')}
+    opts['fp8'].margin, opts['fp8'].refine = a.fp8_margin, a.fp8_refine
     if a.workload == 'batched':
         wl = BatchedWorkload(a.chunks, opts)
     else:
@@ -398,6 +402,20 @@ def main():
         dt32 = (time.perf_counter() - t1) / a.fp32_steps
         meter.on = False
         n32, t32, fl32 = (metered_plain('fp32') if a.graphs else meter.stats())['gemm_fp32']
+    dt8 = outs_f8 = None
+    if a.fp8_steps > 0 and a.workload == 'production':
+        # the FP8 e4m3 form of the same step (per-basis and per-template scales), reported beside the headline
+        log('fp8 form')
+        step('fp8')
+        meter.reset(); meter.on = True
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(a.fp8_steps):
+            _, _, outs_f8 = step('fp8')
+        torch.cuda.synchronize()
+        dt8 = (time.perf_counter() - t2) / a.fp8_steps
+        meter.on = False
+        n8, t8, fl8 = (metered_plain('fp8') if a.graphs else meter.stats())['gemm_fp8']
     hbm = {}
     msteps = 1 if a.graphs else a.steps
     for name, (n_, t_, by) in ms.items():
@@ -408,12 +426,14 @@ def main():
     diffs = None
     if outs_fp is not None:
         diffs = wl.compare(outs_bf, outs_fp)
-    err = err32 = None
+    err = err32 = err8 = None
     nchk = 0
     if not a.no_check:
         err, nchk = wl.direct_check(ds, outs_bf)
         if outs_fp is not None:
             err32, _ = wl.direct_check(ds, outs_fp)
+        if outs_f8 is not None:
+            err8, _ = wl.direct_check(ds, outs_f8)
     res = dict(
         metric=METRIC, value=value, unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
         ms_per_step=dt * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
@@ -432,6 +452,11 @@ def main():
         speedup_bf16_vs_fp32=None if dt32 is None else dt32 / dt,
         snr_rel_err_vs_fp64=dict(bf16=err, fp32=err32, n_checked=nchk),
         trigger_diff_bf16_vs_fp32=diffs,
+        fp8=None if dt8 is None else dict(
+            value=ntemp * an_s / dt8, ms_per_step=dt8 * 1e3, gemm_tflops=fl8 / t8 / 1e12,
+            gemm_frac=fl8 / t8 / 1e12 / PEAK['fp8'], gemm_avg_launch_ms=t8 / max(n8, 1) * 1e3,
+            speedup_vs_fp32=None if dt32 is None else dt32 / dt8, snr_rel_err_vs_fp64=err8,
+            trigger_diff_fp8_vs_fp32=None if outs_fp is None else wl.compare(outs_f8, outs_fp)),
     )
     if a.workload == 'production' and a.refine_top > 0:
         log('refinement of the loudest triggers')
