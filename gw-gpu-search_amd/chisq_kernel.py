@@ -1,8 +1,9 @@
-"""GPU power chi-squared at peaks (the reference's chisq_kernel.py:1-121): one block per (template row, up to P
-peaks); for each equal-power bin j the block sums q[row, k] e^{+2 pi i k t / N} over the bin for each of its peaks
-and accumulates |Z_j|^2 and Z_j. csrc/chisq.hip chisq_pts_k: runs of 32 bins per thread with the phasor reseeded
-exactly from the 64-bit integer phase k t mod N and advanced by a rotation in between, wave64 butterfly sums, and
-float64 accumulation over bins. N need not be a power of two (the reference's 32-bit wrap needed one)."""
+"""GPU power chi-squared at peaks (the reference's chisq_kernel.py:1-121): the peaks of one template row are taken
+in groups of up to P samples; for each group and each equal-power bin j one wave sums q[row, k] e^{+2 pi i k t / N}
+over the bin for each sample (csrc/chisq.hip chisq_bins_k: lanes on interleaved bins, the phasor reseeded exactly
+from the 64-bit integer phase k t mod N at each run of 32 steps and advanced by a rotation in between, a wave
+butterfly sum), and chisq_finish_k accumulates |Z_j|^2 and Z_j over the bins in float64, in bin order. N need not
+be a power of two (the reference's 32-bit wrap needed one). P = 7 is one peak's 7 re-centring samples."""
 from __future__ import annotations
 
 import numpy as np
@@ -11,7 +12,7 @@ import torch
 import _dev
 import _lib
 
-P = 8
+P = 7                 # samples per group; the same constant as csrc/chisq.hip's P
 THREADS = 256
 
 

@@ -5,7 +5,7 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int P = 8;       // peaks per block (one template row's peaks share its read of q / h)
+constexpr int P = 7;       // peak samples per group (one template row's; a peak and its +-3 re-centring samples)
 constexpr int R = 32;      // phasor steps per lane run: the phasor is reseeded exactly at each run start
 
 // exp(2 pi i ((k t) mod N) / N) from the exact integer phase m = (k t) mod N; mask = N - 1 when N is a power of two
@@ -252,9 +252,10 @@ __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h
 } // namespace
 
 // Groups: for g in [0, ngroups): row grow[g] of q (or h in fused mode), peaks gstart[g] .. gstart[g] + gcount[g]
-// (gcount <= 8) of ptime, bin edges edges[g, 0:nedge]. N: the time length of the phase factor (any positive value;
-// the phase k t mod N is exact in 64-bit integers). d: data (fused mode) or null. zout may be null. work: device
-// buffer of gwgs_chisq_points_workspace(ngroups, nedge) bytes (the per-bin sums).
+// (gcount <= P, which gwgs_chisq_peaks_per_group returns) of ptime, bin edges edges[g, 0:nedge]. N: the time length
+// of the phase factor (any positive value; the phase k t mod N is exact in 64-bit integers). d: data (fused mode) or
+// null. zout may be null. work: device buffer of gwgs_chisq_points_workspace(ngroups, nedge) bytes (the per-bin
+// sums).
 GWGS_API long gwgs_chisq_points_workspace(int ngroups, int nedge)
 {
     return ngroups > 0 && nedge >= 2 ? (long)ngroups * (nedge - 1) * P * (long)sizeof(float2) : 0;

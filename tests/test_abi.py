@@ -34,7 +34,10 @@ def test_bad_arguments_rejected(lib):
     assert lib.gwgs_gemm_peaks(1, 0, None, 64, None, None, 64, 10, 64, 1024, 0, 1, 0, None, None, 8, None, None) == -1
     assert lib.gwgs_chisq_points(None, 10, None, 16, None, None, None, None, None, 2, 1, None, None, None, None) == -1
     assert lib.gwgs_chisq_points(p, 10, None, 16, p, p, p, p, p, 17, 3, p, None, None, None) == -1   # no workspace
-    assert lib.gwgs_chisq_points_workspace(3, 17) == 3 * 16 * 8 * 8
+    import chisq_kernel
+    pg = lib.gwgs_chisq_peaks_per_group()
+    assert pg == chisq_kernel.P                       # the host groups peaks as the kernel sizes its registers
+    assert lib.gwgs_chisq_points_workspace(3, 17) == 3 * 16 * pg * 8
     assert lib.gwgs_chisq_points_workspace(0, 17) == 0 and lib.gwgs_chisq_points_workspace(3, 1) == 0
     assert lib.gwgs_edges_fused(None, 10, None, None, None, None, 2, 0, 1, None, None) == -1
     assert lib.gwgs_welch_reduce(ctypes.c_void_p(16), 1, 600, 8, 1.0, 0, 1.0, 1.0, ctypes.c_void_p(16), None) == -2

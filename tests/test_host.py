@@ -49,13 +49,16 @@ def test_argv_helpers():
 
 
 def test_peak_groups_like_reference():
-    """chisq_kernel.py:96-107: sort by (row, time), groups of <= 8 peaks of one row."""
+    """chisq_kernel.py:96-107: sort by (row, time), groups of <= P peaks of one row (the reference's blocks take 8;
+    each peak's sums do not depend on the grouping, and P = 7 fits one peak's re-centring samples)."""
     import chisq_kernel
+    P = chisq_kernel.P
     rows = np.array([3, 1, 1, 3, 1, 2] + [5] * 19)
     times = np.arange(len(rows))[::-1]
     o, grow, gstart, gcount = chisq_kernel.plan_groups(rows, times)
     assert np.all(np.diff(rows[o]) >= 0)
-    assert grow.tolist() == [1, 2, 3, 5, 5, 5] and gcount.tolist() == [3, 1, 2, 8, 8, 3]
+    n5 = [P] * (19 // P) + ([19 % P] if 19 % P else [])
+    assert grow.tolist() == [1, 2, 3] + [5] * len(n5) and gcount.tolist() == [3, 1, 2] + n5
     for s, c in zip(gstart, gcount):
         assert len(set(rows[o][s:s + c])) == 1
 
