@@ -67,9 +67,11 @@ SIGNATURES = {
     'gwgs_snr_grad': [_p, _p, _l, _p, _l, _p, _l, _i, _i, _d, _p, _p, _p, _p, _p, _p, _l, _p],
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],
     'gwgs_slide_write': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p, _p, _p, _p],
+    'gwgs_pre_keep_workspace': [_l, _l],
+    'gwgs_pre_keep': [_p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _d, _i, _f, _f, _f, _f, _f, _i, _i, _p, _l, _p],
 }
 _RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l, 'gwgs_sigmasq_workspace': _l,
-            'gwgs_chisq_points_workspace': _l}
+            'gwgs_chisq_points_workspace': _l, 'gwgs_pre_keep_workspace': _l}
 
 _lib = None
 _lock = threading.Lock()
@@ -80,8 +82,9 @@ def sources():
 
 
 # per-file flags: the peak kernels compute |z|^2 without fused multiply-add so that they are bit-exact against
-# numpy's float32 arithmetic on the same input
-FILE_FLAGS = {'peaks.hip': ['-ffp-contract=off']}
+# numpy's float32 arithmetic on the same input, and the pre-consolidation computes its end times and windows as the
+# host does in float64
+FILE_FLAGS = {'peaks.hip': ['-ffp-contract=off'], 'consolidate.hip': ['-ffp-contract=off']}
 
 
 def _compile(src):

@@ -14,7 +14,10 @@ Per tile and data segment (Filter.run: launch_batch, decode_batch, collect_batch
   - windows whose rebuilt SNR is above thr (1 - margin) are candidates (fp_filter.py:747); the rebuilt complex SNR
     at each gives the rebuilt phase (:759-764); candidates below exact_snr (1 - margin) keep their rebuilt values
     (:768-781), the rest go to the exact stage: edges and chi-squared sums over the exact template in fp32 with
-    float64 finalization (:783-853).
+    float64 finalization (:783-853);
+  - with opts.consolidate, every dataset's peaks are consolidated after each tile (keep-loudest per interval,
+    fp_filter.py:629-663), and with opts.pre_consolidate the candidates that consolidation would drop are removed
+    on the device before the candidate list is read back (fp_filter.py:665-695).
 So snr, coa_phase and chisq of exact peaks are exact; the compressed filter chooses which windows to look at and the
 sample within the window.
 
@@ -32,6 +35,7 @@ import torch
 
 import _dev
 import _lib
+import consolidate
 import fp_gemm
 import peak_kernel
 import templates as tmpl
@@ -64,6 +68,9 @@ class Options:
                                            # data through rebind)
     refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
                                            # samples inside its window (0: the reference's rebuilt-argmax sample)
+    consolidate: object = None             # consolidate.KeepLoudest: consolidate every dataset's peaks after each
+                                           # tile (None: keep every peak)
+    pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
 
 
 @dataclass
@@ -176,7 +183,7 @@ class Triggers:
 
     def to_dict(self, gates=(), chisq=True):
         """The peak-file fields (fp_filter.py:910-937): gate veto |t - g_t| > w + taper + 0.5, sorted by
-        (template_id, end_time); newsnr as coinc_rank.py:39-44."""
+        (template_id, end_time); newsnr as PyCBC's (fp_filter.py:930)."""
         tid, t, snr = self.cat('tid', np.int64), self.cat('t', np.float64), self.cat('snr', np.float32)
         keep = np.ones(len(t), bool)
         for g_t, g_w, g_taper in gates:
@@ -188,12 +195,14 @@ class Triggers:
                    segment=self.cat('seg', np.int64)[sel])
         if chisq:
             chi, dof = self.cat('chi', np.float32)[sel], self.cat('dof', np.int32)[sel]
-            rc = chi / np.maximum(dof, 1)
-            ns = np.array(out['snr'], np.float64)
-            m = rc > 1
-            ns[m] = ns[m] / ((1 + rc[m].astype(np.float64) ** 3) / 2.0) ** (1.0 / 6)
-            out.update(chisq=chi, chisq_dof=dof, newsnr=ns)
+            out.update(chisq=chi, chisq_dof=dof, newsnr=consolidate.newsnr(out['snr'], chi / np.maximum(dof, 1)))
         return out
+
+
+def candidate_cut(thr, margin):
+    """The candidate cut on |z|^2 of fp_filter.py:747, vw > (thr (1 - margin))^2: the square in float64, then
+    float32, as torch compares a float32 tensor with a Python float."""
+    return np.float32((thr * (1 - margin)) ** 2)
 
 
 def _order_by_template_time(tid, t):
@@ -257,9 +266,15 @@ class Filter:
         self._bufs = None                  # graph mode: three rotating buffer sets
         self._graphs = {}                  # graph mode: captured tile norms and stage-A batches
         self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
-                      n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0)
+                      n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0, consolidate_s=0.0,
+                      n_consolidated_out=0)
         self.rel = []
         self.code = PRECISIONS[opts.precision]
+        assert not opts.pre_consolidate or (opts.consolidate is not None and not opts.graphs), \
+            'pre_consolidate needs consolidate, and the candidate tables change between runs (no graphs)'
+        self._prekeep = (consolidate.PreKeep(opts.consolidate, opts.snr_threshold, opts.exact_snr, opts.margin,
+                                             self.sr, self.cw) if opts.pre_consolidate else None)
+        self._floors = {}                  # (plan index, dataset index): the floor consolidation left (device)
         self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
         self._y16_update()
 
@@ -397,22 +412,38 @@ class Filter:
                 t1 = self._sync(); self.T['norm_s'] += t1 - t0
                 for b0 in range(0, len(segs), B):
                     cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
-                                            ordinal=ordinal)
+                                            ordinal=ordinal, pi=pi)
                     ordinal += 1
-                    cur['outs'] = outs[pi]
+                    cur['outs'], cur['pi'], cur['last'] = outs[pi], pi, b0 + B >= len(segs)
                     if self.opts.timing:
-                        self.decode_batch(cur); self.collect_batch(cur)
+                        self.decode_batch(cur); self.finish_batch(cur)
                         continue
                     launched.append(cur)
                     while len(launched) > self.opts.depth:
                         fin.append(self.decode_batch(launched.popleft()))
                     while len(fin) > 1:
-                        self.collect_batch(fin.popleft())
+                        self.finish_batch(fin.popleft())
         while launched:
             fin.append(self.decode_batch(launched.popleft()))
         while fin:
-            self.collect_batch(fin.popleft())
+            self.finish_batch(fin.popleft())
         return outs
+
+    def finish_batch(self, st):
+        """collect_batch, and after a tile's last batch the consolidation of the plan's datasets (fp_filter.py:886-888):
+        their floors feed the pre-consolidation of batches launched from then on."""
+        self.collect_batch(st)
+        o = self.opts
+        if o.consolidate is None or not st.get('last'):
+            return
+        t0 = time.perf_counter()
+        for di, m in enumerate(self.data):
+            floor, dropped = consolidate.consolidate(st['outs'][di], m.gates, m.gps_start, m.sr, o.consolidate,
+                                                     want_floor=o.pre_consolidate)
+            self.T['n_consolidated_out'] += dropped
+            if floor is not None:
+                self._floors[(st['pi'], di)] = _dev.to_dev(floor)
+        self.T['consolidate_s'] += time.perf_counter() - t0
 
     def tile_norms(self, tb: TileBasis, key):
         """sigma^2 and the norm of every template of the tile in every segment (fp_filter.py:710-714); with graphs
@@ -478,10 +509,11 @@ class Filter:
             self._workbuf(ws)
         self._ones(nmax)
 
-    def _batch_device(self, tb: TileBasis, segs, norm, geo, bufs=None):
-        """All device work of stage A (no host reads, no host-to-device copies): basis series, template operands,
-        row scales, the GEMM with the window epilogue and the candidate compaction. bufs: a rotation set to write
-        into (graph capture) or None (fresh allocations)."""
+    def _batch_device(self, tb: TileBasis, segs, norm, geo, bufs=None, pk=None):
+        """All device work of stage A (no host reads, no host-to-device copies but the pre-consolidation's tables):
+        basis series, template operands, row scales, the GEMM with the window epilogue, the pre-consolidation (pk:
+        its per-segment tables, or None) and the candidate compaction. bufs: a rotation set to write into (graph
+        capture) or None (fresh allocations)."""
         o = self.opts
         n, rp, K = len(tb.ids), tb.rp, geo['K']
         dt = OPERAND_DTYPES[self.code]
@@ -516,12 +548,14 @@ class Filter:
                                  o.gemm_cfg, vw, iw, rows2)
         # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
         _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(geo['Tns'])))
-        cut = np.float32(o.snr_threshold * (1 - o.margin)) ** 2
+        cut = candidate_cut(o.snr_threshold, o.margin)
+        if pk is not None:
+            self._prekeep.launch(vw, iw, cut, pk)
         idx, samp, val, cnt = peak_kernel.select_launch(vw, iw, float(cut), sel)
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
         return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt))
 
-    def launch_batch(self, tb: TileBasis, segs, norm, tile_host, key=None, ordinal=0):
+    def launch_batch(self, tb: TileBasis, segs, norm, tile_host, key=None, ordinal=0, pi=0):
         """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
         (basis_operand), one GEMM launch over all their windows (an item table of (segment, run of tiles)) with each
         segment's template norm^2 applied to |z|^2 as a per-(segment, template) row scale, and the candidate list
@@ -540,7 +574,12 @@ class Filter:
             geo, dev = rec['geo'], rec['dev']
         else:
             geo = self._batch_geometry(tb, segs)
-            dev = self._batch_device(tb, segs, norm, geo)
+            pk = None
+            if self._prekeep is not None:
+                pk = [(self.data[di].seg_start[s] + a0, self.data[di].gps_start, int(geo['col0'][i]), nw,
+                       self.data[di].gates, self._floors.get((pi, di)))
+                      for i, (di, s, gs, a0, nw) in enumerate(segs)]
+            dev = self._batch_device(tb, segs, norm, geo, pk=pk)
         idx, samp, val, cnt = dev['sel']
         sel = peak_kernel.Selection(idx, samp, val, cnt, _dev.mark())
         return dict(tb=tb, segs=segs, Zt=dev['Zt'], Ar=dev['Ar'], Ai=dev['Ai'], rows2=dev['rows2'],

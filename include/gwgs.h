@@ -199,6 +199,25 @@ int gwgs_slide_write(const double* hkey, const double* hx, int nh, const double*
                      const int* li, long nl, double reach, double step, int K, double window,
                      const long long* offsets, int* oh, int* ol, int* ok, double* odt, void* stream);
 
+/* ---- pre-consolidation of the candidates (pipeline/fp_filter.py:665-695 pre_keep, the keep-loudest consolidation
+ * of fp_filter.py:629-663 applied to one tile and segment's candidates on the device) ----------------------------
+ * vw (n, nwsum) float32, iw (n, nwsum) int32: one batch's window maxima and window samples as the GEMM's epilogue
+ * writes them. Every candidate vw[g] > cut that the consolidation drops is overwritten with -1 in place, so the
+ * compaction that follows (gwgs_select_peaks) skips it. segi: nseg rows of 10 int64 (host-built, device copy):
+ * sbase (segment start + first analysed sample, in samples of its dataset), wt0 and nwin (its keep-loudest windows
+ * [wt0, wt0 + nwin), wt = trunc(rint((t - gps) sr) / wdiv)), col0 and ncol (its columns of vw), wbase (its first
+ * window in the batch's nwin windows), gate0 and ngate (its dataset's gates in `gates`, pairs (time, w + taper +
+ * 0.5) in float64), fl0 and nfl (its dataset's floor in `floors`, nfl = 0: none). segd: nseg float64 dataset start
+ * times. Kept: apx = sqrt(vw) >= exact_cut; rebuilt candidates with apx > thr, (apx >= nthr or exempt), outside
+ * every gate and, with flags & 2 (an interval), apx > floor[wt] and among the num loudest of their segment's window
+ * (apx descending, then row-major order); exempt (flags & 1, apx >= exempt_snr) ones are not ranked. work:
+ * gwgs_pre_keep_workspace(n * nwsum, nwin) bytes, 8-byte aligned (only with flags & 2). */
+long gwgs_pre_keep_workspace(long total, long nwin);
+int gwgs_pre_keep(float* vw, const int* iw, int n, int nwsum, int nseg, const long* segi, const double* segd,
+                  const double* gates, const double* floors, double sr, double wdiv, int num, float cut,
+                  float exact_cut, float thr, float nthr, float exempt_snr, int flags, int nwin, void* work,
+                  long work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

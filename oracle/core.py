@@ -364,9 +364,10 @@ def window_max(V, I, nw, per):
 
 
 def candidates(vw, iw, thr, margin, a0):
-    """fp_filter.py:747-753: windows with vw > (thr (1 - margin))^2 (strict), row-major order. Returns tb (row),
+    """fp_filter.py:747-753: windows with vw > (thr (1 - margin))^2 (strict), row-major order; for float32 vw the
+    float64 square is rounded to float32 (torch's compare of a float32 tensor with a Python float). Returns tb (row),
     wb (window), samp = a0 + iw, approx = sqrt(vw) (float32)."""
-    cut = np.float32(thr * (1 - margin)) ** 2 if vw.dtype == np.float32 else (thr * (1 - margin)) ** 2
+    cut = np.float32((thr * (1 - margin)) ** 2) if vw.dtype == np.float32 else (thr * (1 - margin)) ** 2
     tb, wb = np.nonzero(vw > cut)
     return tb, wb, (a0 + iw[tb, wb]).astype(np.int64), np.sqrt(vw[tb, wb])
 
