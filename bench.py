@@ -283,6 +283,46 @@ class BatchedWorkload:
         return direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=12)
 
 
+def consolidation_report(step, nsteps, work, outs_headline):
+    """The production cuts (SNR threshold 4, exact band from 5.5, newsnr >= 4, the 30 loudest per 1.072 s window,
+    exempt from SNR 5, consolidated after every tile: fp_filter.py:629-663) on the same step, with the consolidation
+    on the host only and with the device pre-consolidation of the candidates (fp_filter.py:665-695). Reports step
+    times, candidate counts, the pre-consolidation kernels' rate and whether the two trigger lists are identical."""
+    import numpy as np
+    import torch
+    import _dev
+    res = {}
+    outs = {}
+    for name in ('cons', 'pre'):
+        step(name)
+        _dev.METER.reset(); _dev.METER.on = True
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            _, fs, outs[name] = step(name)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / nsteps
+        _dev.METER.on = False
+        st = _dev.METER.stats()
+        r = dict(ms_per_step=dt * 1e3, value=work / dt,
+                 n_candidates=int(sum(f.T['n_candidates'] for f in fs)),
+                 n_exact_candidates=int(sum(f.T['n_exact_candidates'] for f in fs)),
+                 n_triggers=int(sum(len(o['template_id']) for ob in outs[name] for o in ob)),
+                 consolidate_host_s=float(sum(f.T['consolidate_s'] for f in fs)))
+        if 'pre_keep' in st:
+            n_, t_, by = st['pre_keep']
+            r['pre_keep_kernels'] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 8e12,
+                                         launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
+                                         share_of_step=t_ / nsteps / dt)
+        res[name] = r
+    same = all(all(np.array_equal(x[k], y[k]) for k in x) for a_, b_ in zip(outs['cons'], outs['pre'])
+               for x, y in zip(a_, b_))
+    res['identical_triggers'] = bool(same)
+    res['speedup_pre_vs_host_only'] = res['cons']['ms_per_step'] / res['pre']['ms_per_step']
+    res['n_triggers_headline_cuts'] = int(sum(len(o['template_id']) for ob in outs_headline for o in ob))
+    return res
+
+
 def _merge(dicts):
     import numpy as np
     keys = dicts[0].keys()
@@ -311,6 +351,10 @@ def main():
                     'from the second step on; production workload)')
     ap.add_argument('--refine-top', type=int, default=10000, help='refine this many of the loudest exact triggers '
                     'off the bank after the timed steps (production workload; 0: skip)')
+    ap.add_argument('--consolidate-steps', type=int, default=1,
+                    help='timed steps of the production cuts (SNR threshold 4, exact band from 5.5, keep the 30 '
+                    'loudest per 1.072 s, exempt from SNR 5) with host consolidation only and with the device '
+                    'pre-consolidation (production workload; 0: skip)')
     ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
                     'trigger lists after the timed steps (production workload; 0: skip)')
     a = ap.parse_args()
@@ -338,6 +382,12 @@ def main():
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
                                  refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32', 'fp8')}
     opts['fp8'].margin, opts['fp8'].refine = a.fp8_margin, a.fp8_refine
+    import consolidate
+    kl = consolidate.KeepLoudest(newsnr_threshold=4.0, interval=1.072, num=30, exempt_snr=5.0)
+    for name, pre in (('cons', False), ('pre', True)):
+        opts[name] = fp_filter.Options(tile=a.tile, precision='bf16', gemm_cfg=cfg, self_check=False,
+                                       refine=a.refine, snr_threshold=4.0, exact_snr=5.5, consolidate=kl,
+                                       pre_consolidate=pre)
     if a.workload == 'batched':
         wl = BatchedWorkload(a.chunks, opts)
     else:
@@ -458,6 +508,9 @@ def main():
             speedup_vs_fp32=None if dt32 is None else dt32 / dt8, snr_rel_err_vs_fp64=err8,
             trigger_diff_fp8_vs_fp32=None if outs_fp is None else wl.compare(outs_f8, outs_fp)),
     )
+    if a.workload == 'production' and a.consolidate_steps > 0:
+        log('production cuts with consolidation')
+        res['consolidation'] = consolidation_report(step, a.consolidate_steps, ntemp * an_s, outs_bf)
     if a.workload == 'production' and a.refine_top > 0:
         log('refinement of the loudest triggers')
         res['refinement'] = refinement_report(wl.plans, wl.streams, ds, outs_bf, a.refine_top)

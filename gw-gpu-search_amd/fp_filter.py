@@ -550,7 +550,10 @@ class Filter:
         _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(geo['Tns'])))
         cut = candidate_cut(o.snr_threshold, o.margin)
         if pk is not None:
+            e0 = _dev.METER.start()
             self._prekeep.launch(vw, iw, cut, pk)
+            # algorithmic bytes: one read of the window maxima (the candidates' samples and drops are a few percent)
+            _dev.METER.stop('pre_keep', e0, 4.0 * vw.numel())
         idx, samp, val, cnt = peak_kernel.select_launch(vw, iw, float(cut), sel)
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
         return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt))
