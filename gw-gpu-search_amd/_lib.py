@@ -68,6 +68,7 @@ SIGNATURES = {
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],
     'gwgs_slide_write': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p, _p, _p, _p],
     'gwgs_pre_keep_workspace': [_l, _l],
+    'gwgs_sg_tiles': [_p, _p, _l, _l, _d, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_pre_keep': [_p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _d, _i, _f, _f, _f, _f, _f, _i, _i, _p, _l, _p],
 }
 _RESTYPE = {'gwgs_select_workspace': _l, 'gwgs_basis_series_workspace': _l, 'gwgs_sigmasq_workspace': _l,

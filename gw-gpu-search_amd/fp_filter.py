@@ -38,6 +38,7 @@ import _lib
 import consolidate
 import fp_gemm
 import peak_kernel
+import sgchisq
 import templates as tmpl
 from exact_stage import ExactStage, PeakRequest, SegmentData, TemplateTile
 
@@ -71,6 +72,8 @@ class Options:
     consolidate: object = None             # consolidate.KeepLoudest: consolidate every dataset's peaks after each
                                            # tile (None: keep every peak)
     pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
+    sgchisq_snr_threshold: float = None    # sine-Gaussian chi-squared of exact peaks with newsnr >= this, for the
+                                           # templates BankPlan(sg_locations=...) gives tiles (None: off)
 
 
 @dataclass
@@ -137,12 +140,14 @@ class BankPlan:
     SVD bases. sw: the SVD metric sqrt(mean IP) (fp_filter.py:363); it sets how well the basis compresses, never the
     SNR definition (conj(h sw) (D / sw) = conj(h) D)."""
 
-    def __init__(self, m1, m2, e0, flen, df, opts: Options, sw, isw, name='bank'):
+    def __init__(self, m1, m2, e0, flen, df, opts: Options, sw, isw, name='bank', sg_locations=None):
         self.name, self.opts = name, opts
         self.flen, self.df = flen, df
         self.kmin = int(opts.low_frequency_cutoff / df)
         self.m1, self.m2 = np.asarray(m1, float), np.asarray(m2, float)
         self.e0 = np.zeros(len(self.m1)) if e0 is None else np.asarray(e0, float)
+        # --sgchisq-locations: per template its sine-Gaussian tile list [(q, offset Hz)] or None
+        self.sg_locs = sgchisq.parse_locations(sg_locations, self.m1, self.m2) if sg_locations else None
         fl = opts.low_frequency_cutoff
         tau0 = 5.0 / (256.0 * np.pi * fl * (self.m1 * self.m2 / (self.m1 + self.m2) ** 2)) * \
             (np.pi * (self.m1 + self.m2) * 4.925490947641267e-06 * fl) ** (-5.0 / 3.0)
@@ -168,13 +173,18 @@ class BankPlan:
 
 
 class Triggers:
-    """Peaks of one dataset and bank, accumulated over tiles and segments (fp_filter.py:323, 772-853)."""
-    KEYS = ('tid', 't', 'snr', 'ph', 'chi', 'dof', 'sg', 'ex', 'samp', 'seg')
+    """Peaks of one dataset and bank, accumulated over tiles and segments (fp_filter.py:323, 772-853); sgc / sgd:
+    the sine-Gaussian chi-squared (1 where not computed) and where it was computed."""
+    KEYS = ('tid', 't', 'snr', 'ph', 'chi', 'dof', 'sg', 'ex', 'samp', 'seg', 'sgc', 'sgd')
 
-    def __init__(self):
+    def __init__(self, sg=False):
         self.parts = {k: [] for k in self.KEYS}
+        self.sg = sg                       # the peak file carries sg_chisq and sg_chisq_computed
 
     def add(self, **kw):
+        n = len(kw['snr'])
+        kw.setdefault('sgc', np.ones(n, np.float32))
+        kw.setdefault('sgd', np.zeros(n, bool))
         for k in self.KEYS:
             self.parts[k].append(np.asarray(kw[k]))
 
@@ -196,6 +206,8 @@ class Triggers:
         if chisq:
             chi, dof = self.cat('chi', np.float32)[sel], self.cat('dof', np.int32)[sel]
             out.update(chisq=chi, chisq_dof=dof, newsnr=consolidate.newsnr(out['snr'], chi / np.maximum(dof, 1)))
+        if self.sg:
+            out.update(sg_chisq=self.cat('sgc', np.float32)[sel], sg_chisq_computed=self.cat('sgd', bool)[sel])
         return out
 
 
@@ -267,7 +279,7 @@ class Filter:
         self._graphs = {}                  # graph mode: captured tile norms and stage-A batches
         self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0, consolidate_s=0.0,
-                      n_consolidated_out=0)
+                      n_consolidated_out=0, sg_chisq_s=0.0, n_sg_chisq_points=0)
         self.rel = []
         self.code = PRECISIONS[opts.precision]
         assert not opts.pre_consolidate or (opts.consolidate is not None and not opts.graphs), \
@@ -392,7 +404,8 @@ class Filter:
         previous batch's results are collected, so the GPU has queued work while the host decodes and finalises,
         across tiles and banks. Copies to the host run on a side stream after the events they depend on. With
         opts.timing every stage is synchronised instead."""
-        outs = [[Triggers() for _ in self.data] for _ in plans]
+        sg_on = self.opts.sgchisq_snr_threshold is not None
+        outs = [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
         segs = []
         for di, m in enumerate(self.data):
             for s, (a0, a1) in enumerate(m.analyze):
@@ -414,7 +427,7 @@ class Filter:
                     cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
                                             ordinal=ordinal, pi=pi)
                     ordinal += 1
-                    cur['outs'], cur['pi'], cur['last'] = outs[pi], pi, b0 + B >= len(segs)
+                    cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(segs), plan.sg_locs
                     if self.opts.timing:
                         self.decode_batch(cur); self.finish_batch(cur)
                         continue
@@ -650,6 +663,32 @@ class Filter:
         self.T['rebuilt_z_s'] += time.perf_counter() - t0
         return st
 
+    def _sg_values(self, locs, m, s, tid, samples, snr, chi, dof, fpeak):
+        """fp_filter.py:854-883: the sine-Gaussian chi-squared of exact peaks with newsnr >= the threshold whose
+        template has a tile list, one launch per distinct tile list (sgchisq.sg_chisq_points on the segment's D and
+        IP, the template's kmin as the first tile's lower bound). Returns (values float32, 1 where not computed;
+        computed mask)."""
+        n = len(snr)
+        sgc, sgd = np.ones(n, np.float32), np.zeros(n, bool)
+        thr = self.opts.sgchisq_snr_threshold
+        if thr is None or locs is None or n == 0:
+            return sgc, sgd
+        t0 = time.perf_counter()
+        ns = consolidate.newsnr(snr, chi / np.maximum(dof, 1))
+        groups = {}
+        for i in np.nonzero(ns >= thr)[0]:
+            if locs[tid[i]] is not None:
+                groups.setdefault(tuple(locs[tid[i]]), []).append(i)
+        for vals, gi in groups.items():
+            gi = np.array(gi)
+            v_, ok_ = sgchisq.sg_chisq_points(m.D[s], m.IP[s], samples[gi], fpeak[gi], list(vals),
+                                              np.full(len(gi), self.kmin), self.df, self.tlen)
+            sgc[gi[ok_]] = v_[ok_]
+            sgd[gi[ok_]] = True
+        self.T['n_sg_chisq_points'] += int(sgd.sum())
+        self.T['sg_chisq_s'] += time.perf_counter() - t0
+        return sgc, sgd
+
     def collect_batch(self, st):
         """Stage B2: wait for the batch's exact values and finalise its triggers (fp_filter.py:768-853)."""
         t0 = time.perf_counter()
@@ -671,7 +710,7 @@ class Filter:
                 out.add(tid=tb.ids[tb_[rb]], t=tstamp(samp[rb]), snr=ax[rb],
                         ph=np.angle(zreb[sel][rb]).astype(np.float32), chi=np.zeros(nrb, np.float32),
                         dof=np.zeros(nrb, np.int32), sg=sig_h[tb_[rb], gs], ex=np.zeros(nrb, bool), samp=samp[rb],
-                        seg=np.full(nrb, gs))
+                        seg=np.full(nrb, gs), sgc=np.ones(nrb, np.float32), sgd=np.zeros(nrb, bool))
                 self.T['n_rebuilt_peaks'] += nrb
             tbx = tb_[ex]
             self.T['n_exact_candidates'] += len(tbx)
@@ -687,8 +726,10 @@ class Filter:
             nbin = tb.stage.tile.nbins[tbx]
             chi = ((nbin * v.pw.astype(np.float64) - np.abs(zx) ** 2) * nr ** 2).astype(np.float32)
             dof = (2 * nbin - 2).astype(np.int32)
+            sgc, sgd = self._sg_values(st.get('sg_locs'), m, s, tb.ids[tbx[k]], sx[k], snr[k], chi[k], dof[k],
+                                       v.fpeak[k])
             out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
                     ph=np.angle(v.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
-                    ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs))
+                    ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs), sgc=sgc, sgd=sgd)
             self.T['n_chisq_points'] += int(k.sum())
         self.T['refine_chisq_s'] += time.perf_counter() - t0

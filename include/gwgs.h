@@ -218,6 +218,17 @@ int gwgs_pre_keep(float* vw, const int* iw, int n, int nwsum, int nseg, const lo
                   float exact_cut, float thr, float nthr, float exempt_snr, int flags, int nwin, void* work,
                   long work_bytes, void* stream);
 
+/* ---- sine-Gaussian chi-squared (pipeline/sg_kernel.py sg_tiles / sg_chisq_points; pipeline/fp_filter.py:463-486
+ * sg_chisq_one) ---------------------------------------------------------------------------------------------------
+ * For every (peak p, tile j), b = p * ntile + j, over bins k in [kk[b], k1[b]) with k >= kg[b] (k1 <= flen):
+ * zsum[b] = sum g_k D_k e^{2 pi i (k t[p] mod N) / N} (double2) and gsum[b] = sum g_k^2 IP_k, with
+ * g_k = (sqrt(pi) / 2 tau) exp(-(pi tau (k df - fcen))^2) (1 + exp(-q2 k df / fcen)) (per-b fcen, tau, q2 in
+ * float64). D complex64 and IP float32 of one segment (flen bins, time length N). The caller finishes
+ * z_j = 4 df zsum / sqrt(4 df gsum) and the value sum_j |z_j|^2 / (2 ntile). */
+int gwgs_sg_tiles(const void* D, const float* IP, long flen, long N, double df, const long* kk, const long* k1,
+                  const long* kg, const double* fcen, const double* tau, const double* q2, const long* t, int npk,
+                  int ntile, void* zsum, double* gsum, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
