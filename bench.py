@@ -200,8 +200,9 @@ class ProductionWorkload:
         IPall, sw, isw = fp_filter.svd_metric(ds0)
         self.opts = opts
         self.filters = {}
+        # the production run's sine-Gaussian tiles (used only by forms with a sgchisq threshold)
         self.plans = [fp_filter.BankPlan(b.m1, b.m2, b.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
-                                         name=b.name) for b in self.banks]
+                                         name=b.name, sg_locations=SG_LOCATIONS) for b in self.banks]
 
     def step(self, prec, check=False):
         import fp_filter, workloads
@@ -283,11 +284,16 @@ class BatchedWorkload:
         return direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=12)
 
 
+SG_LOCATIONS = ['mtotal>30:20-15,20-30,20-45,20-60,20-75,20-90,20-105,20-120']
+
+
 def consolidation_report(step, nsteps, work, outs_headline):
     """The production cuts (SNR threshold 4, exact band from 5.5, newsnr >= 4, the 30 loudest per 1.072 s window,
-    exempt from SNR 5, consolidated after every tile: fp_filter.py:629-663) on the same step, with the consolidation
-    on the host only and with the device pre-consolidation of the candidates (fp_filter.py:665-695). Reports step
-    times, candidate counts, the pre-consolidation kernels' rate and whether the two trigger lists are identical."""
+    exempt from SNR 5, consolidated after every tile: fp_filter.py:629-663; the sine-Gaussian chi-squared of exact
+    peaks with newsnr >= 6 of templates with mtotal > 30, 8 tiles: ops/run_prod_o4a.sh:31) on the same step, with the
+    consolidation on the host only and with the device pre-consolidation of the candidates (fp_filter.py:665-695).
+    Reports step times, candidate counts, the pre-consolidation and sine-Gaussian kernels' rates and whether the two
+    trigger lists are identical."""
     import numpy as np
     import torch
     import _dev
@@ -308,12 +314,19 @@ def consolidation_report(step, nsteps, work, outs_headline):
                  n_candidates=int(sum(f.T['n_candidates'] for f in fs)),
                  n_exact_candidates=int(sum(f.T['n_exact_candidates'] for f in fs)),
                  n_triggers=int(sum(len(o['template_id']) for ob in outs[name] for o in ob)),
-                 consolidate_host_s=float(sum(f.T['consolidate_s'] for f in fs)))
+                 consolidate_host_s=float(sum(f.T['consolidate_s'] for f in fs)),
+                 n_sg_chisq_points=int(sum(f.T['n_sg_chisq_points'] for f in fs)),
+                 sg_chisq_host_s=float(sum(f.T['sg_chisq_s'] for f in fs)))
         if 'pre_keep' in st:
             n_, t_, by = st['pre_keep']
             r['pre_keep_kernels'] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 8e12,
                                          launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
                                          share_of_step=t_ / nsteps / dt)
+        if 'sg_tiles' in st:
+            n_, t_, by = st['sg_tiles']
+            r['sg_tiles_kernel'] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 8e12,
+                                        launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
+                                        share_of_step=t_ / nsteps / dt)
         res[name] = r
     same = all(all(np.array_equal(x[k], y[k]) for k in x) for a_, b_ in zip(outs['cons'], outs['pre'])
                for x, y in zip(a_, b_))
@@ -353,8 +366,8 @@ def main():
                     'off the bank after the timed steps (production workload; 0: skip)')
     ap.add_argument('--consolidate-steps', type=int, default=1,
                     help='timed steps of the production cuts (SNR threshold 4, exact band from 5.5, keep the 30 '
-                    'loudest per 1.072 s, exempt from SNR 5) with host consolidation only and with the device '
-                    'pre-consolidation (production workload; 0: skip)')
+                    'loudest per 1.072 s, exempt from SNR 5, sine-Gaussian chi-squared from newsnr 6) with host '
+                    'consolidation only and with the device pre-consolidation (production workload; 0: skip)')
     ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
                     'trigger lists after the timed steps (production workload; 0: skip)')
     a = ap.parse_args()
@@ -387,7 +400,7 @@ def main():
     for name, pre in (('cons', False), ('pre', True)):
         opts[name] = fp_filter.Options(tile=a.tile, precision='bf16', gemm_cfg=cfg, self_check=False,
                                        refine=a.refine, snr_threshold=4.0, exact_snr=5.5, consolidate=kl,
-                                       pre_consolidate=pre)
+                                       pre_consolidate=pre, sgchisq_snr_threshold=6.0)
     if a.workload == 'batched':
         wl = BatchedWorkload(a.chunks, opts)
     else:

@@ -56,16 +56,18 @@ def _window(t, gps_start, sr, interval):
 
 
 def consolidate(tr, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
-    """fp_filter.py:629-663 on one dataset's Triggers (in place): the gate veto as at write, newsnr >= the threshold
+    """fp_filter.py:629-663 on one dataset's Triggers (in place; on its leading parts whose fields are final, the
+    later parts kept after them unchanged): the gate veto as at write, newsnr >= the threshold
     (or exempt), and with an interval the `num` loudest by newsnr per window among the non-exempt peaks, ties by
     position. Returns the floor for the pre-consolidation (want_floor and an interval): per window up to the last
     ranked one, the num-th loudest newsnr of every window holding at least num ranked peaks, -inf elsewhere; None
     when there is no floor (or no peak: the caller keeps its previous floor). Also returns the number dropped."""
     o = tr.parts
-    ks = [k for k, v in o.items() if v]
+    n = tr.ready_parts() if hasattr(tr, 'ready_parts') else len(o['snr'])   # leading parts with final fields
+    ks = [k for k, v in o.items() if v[:n]]
     if not ks:
         return None, 0
-    c = {k: np.concatenate(o[k]) for k in ks}
+    c = {k: np.concatenate(o[k][:n]) for k in ks}
     snr = c['snr'].astype(np.float32, copy=False)
     ns = newsnr(snr, c['chi'].astype(np.float32, copy=False) / np.maximum(c['dof'].astype(np.int32, copy=False), 1))
     exempt = snr >= np.float32(kl.exempt_snr) if kl.exempt_snr is not None else np.zeros(len(ns), bool)
@@ -91,7 +93,7 @@ def consolidate(tr, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
             floor[wt_s[first[full]]] = ns[idx[od[first[full] + kl.num - 1]]]
     dropped = int((~keep).sum())
     for k in ks:
-        o[k] = [c[k][keep]]
+        o[k] = [c[k][keep]] + o[k][n:]
     return floor, dropped
 
 

@@ -180,6 +180,14 @@ class Triggers:
     def __init__(self, sg=False):
         self.parts = {k: [] for k in self.KEYS}
         self.sg = sg                       # the peak file carries sg_chisq and sg_chisq_computed
+        self.unready = []                  # sgc arrays of parts whose sine-Gaussian values are still on the device
+
+    def ready_parts(self):
+        """Number of leading parts whose fields are final (consolidation works on these)."""
+        if not self.unready:
+            return len(self.parts['snr'])
+        ids = {id(a) for a in self.unready}
+        return next(i for i, a in enumerate(self.parts['sgc']) if id(a) in ids)
 
     def add(self, **kw):
         n = len(kw['snr'])
@@ -189,6 +197,7 @@ class Triggers:
             self.parts[k].append(np.asarray(kw[k]))
 
     def cat(self, k, dt):
+        assert not self.unready, 'sine-Gaussian values still pending'
         return np.concatenate(self.parts[k]).astype(dt) if self.parts[k] else np.zeros(0, dt)
 
     def to_dict(self, gates=(), chisq=True):
@@ -287,6 +296,7 @@ class Filter:
         self._prekeep = (consolidate.PreKeep(opts.consolidate, opts.snr_threshold, opts.exact_snr, opts.margin,
                                              self.sr, self.cw) if opts.pre_consolidate else None)
         self._floors = {}                  # (plan index, dataset index): the floor consolidation left (device)
+        self._sg_pending = []              # (Triggers, sgc, sgd, [(indices, sgchisq.PendingSG)])
         self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
         self._y16_update()
 
@@ -406,6 +416,8 @@ class Filter:
         opts.timing every stage is synchronised instead."""
         sg_on = self.opts.sgchisq_snr_threshold is not None
         outs = [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
+        # parts left out of a tile's consolidation (values pending) are consolidated at the end
+        self._sg_consolidate_pending = sg_on and self.opts.consolidate is not None
         segs = []
         for di, m in enumerate(self.data):
             for s, (a0, a1) in enumerate(m.analyze):
@@ -440,23 +452,50 @@ class Filter:
             fin.append(self.decode_batch(launched.popleft()))
         while fin:
             self.finish_batch(fin.popleft())
+        self._resolve_sg()
+        if self._sg_consolidate_pending:
+            for pi in range(len(plans)):
+                self._consolidate(pi, outs[pi])
         return outs
 
     def finish_batch(self, st):
-        """collect_batch, and after a tile's last batch the consolidation of the plan's datasets (fp_filter.py:886-888):
-        their floors feed the pre-consolidation of batches launched from then on."""
+        """The sine-Gaussian values of earlier batches (their kernels ran right after this batch's exact stage, which
+        collect_batch waits for anyway), collect_batch, and after a tile's last batch the consolidation of the plan's
+        datasets (fp_filter.py:886-888) over their parts with final fields: the floors feed the pre-consolidation of
+        batches launched from then on. Parts still waiting for sine-Gaussian values are consolidated later; applying
+        the consolidation to any prefix of the peak list leaves the final list unchanged."""
+        self._resolve_sg()
         self.collect_batch(st)
+        if st.get('last'):
+            self._consolidate(st['pi'], st['outs'])
+
+    def _consolidate(self, pi, outs):
         o = self.opts
-        if o.consolidate is None or not st.get('last'):
+        if o.consolidate is None:
             return
         t0 = time.perf_counter()
         for di, m in enumerate(self.data):
-            floor, dropped = consolidate.consolidate(st['outs'][di], m.gates, m.gps_start, m.sr, o.consolidate,
+            floor, dropped = consolidate.consolidate(outs[di], m.gates, m.gps_start, m.sr, o.consolidate,
                                                      want_floor=o.pre_consolidate)
             self.T['n_consolidated_out'] += dropped
             if floor is not None:
-                self._floors[(st['pi'], di)] = _dev.to_dev(floor)
+                self._floors[(pi, di)] = _dev.to_dev(floor)
         self.T['consolidate_s'] += time.perf_counter() - t0
+
+    def _resolve_sg(self):
+        """Wait for every enqueued sine-Gaussian launch and write its values into its parts."""
+        if not self._sg_pending:
+            return
+        t0 = time.perf_counter()
+        for tr, sgc, sgd, pend in self._sg_pending:
+            for gi, pv in pend:
+                v, ok = pv.result()
+                sgc[gi[ok]] = v[ok]
+                sgd[gi[ok]] = True
+                self.T['n_sg_chisq_points'] += int(ok.sum())
+            tr.unready = [a for a in tr.unready if a is not sgc]
+        self._sg_pending = []
+        self.T['sg_chisq_s'] += time.perf_counter() - t0
 
     def tile_norms(self, tb: TileBasis, key):
         """sigma^2 and the norm of every template of the tile in every segment (fp_filter.py:710-714); with graphs
@@ -665,29 +704,27 @@ class Filter:
 
     def _sg_values(self, locs, m, s, tid, samples, snr, chi, dof, fpeak):
         """fp_filter.py:854-883: the sine-Gaussian chi-squared of exact peaks with newsnr >= the threshold whose
-        template has a tile list, one launch per distinct tile list (sgchisq.sg_chisq_points on the segment's D and
-        IP, the template's kmin as the first tile's lower bound). Returns (values float32, 1 where not computed;
-        computed mask)."""
+        template has a tile list, one launch per distinct tile list (sgchisq.sg_chisq_launch on the segment's D and
+        IP, the template's kmin as the first tile's lower bound), enqueued only. Returns (values float32, 1 where not
+        computed; computed mask; [(indices, PendingSG)]): _resolve_sg fills the first two in place."""
         n = len(snr)
         sgc, sgd = np.ones(n, np.float32), np.zeros(n, bool)
         thr = self.opts.sgchisq_snr_threshold
         if thr is None or locs is None or n == 0:
-            return sgc, sgd
+            return sgc, sgd, []
         t0 = time.perf_counter()
         ns = consolidate.newsnr(snr, chi / np.maximum(dof, 1))
         groups = {}
         for i in np.nonzero(ns >= thr)[0]:
             if locs[tid[i]] is not None:
                 groups.setdefault(tuple(locs[tid[i]]), []).append(i)
+        pend = []
         for vals, gi in groups.items():
             gi = np.array(gi)
-            v_, ok_ = sgchisq.sg_chisq_points(m.D[s], m.IP[s], samples[gi], fpeak[gi], list(vals),
-                                              np.full(len(gi), self.kmin), self.df, self.tlen)
-            sgc[gi[ok_]] = v_[ok_]
-            sgd[gi[ok_]] = True
-        self.T['n_sg_chisq_points'] += int(sgd.sum())
+            pend.append((gi, sgchisq.sg_chisq_launch(m.D[s], m.IP[s], samples[gi], fpeak[gi], list(vals),
+                                                     np.full(len(gi), self.kmin), self.df, self.tlen)))
         self.T['sg_chisq_s'] += time.perf_counter() - t0
-        return sgc, sgd
+        return sgc, sgd, pend
 
     def collect_batch(self, st):
         """Stage B2: wait for the batch's exact values and finalise its triggers (fp_filter.py:768-853)."""
@@ -726,10 +763,13 @@ class Filter:
             nbin = tb.stage.tile.nbins[tbx]
             chi = ((nbin * v.pw.astype(np.float64) - np.abs(zx) ** 2) * nr ** 2).astype(np.float32)
             dof = (2 * nbin - 2).astype(np.int32)
-            sgc, sgd = self._sg_values(st.get('sg_locs'), m, s, tb.ids[tbx[k]], sx[k], snr[k], chi[k], dof[k],
-                                       v.fpeak[k])
+            sgc, sgd, pend = self._sg_values(st.get('sg_locs'), m, s, tb.ids[tbx[k]], sx[k], snr[k], chi[k],
+                                             dof[k], v.fpeak[k])
             out.add(tid=tb.ids[tbx[k]], t=tstamp(sx[k]), snr=snr[k].astype(np.float32),
                     ph=np.angle(v.z[k]).astype(np.float32), chi=chi[k], dof=dof[k], sg=sig_h[tbx[k], gs],
                     ex=np.ones(int(k.sum()), bool), samp=sx[k], seg=np.full(int(k.sum()), gs), sgc=sgc, sgd=sgd)
+            if pend:
+                out.unready.append(sgc)
+                self._sg_pending.append((out, sgc, sgd, pend))
             self.T['n_chisq_points'] += int(k.sum())
         self.T['refine_chisq_s'] += time.perf_counter() - t0

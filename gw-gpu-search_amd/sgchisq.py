@@ -33,23 +33,36 @@ def tile_ranges(fpeak, vals, k_start, df):
     return kk_o, k1_o, kg_o, fc_o, tau_o, q2_o
 
 
-def sg_chisq_points(D, IP, t, fpeak, vals, k_start, df, tlen):
-    """sg_chisq at samples t (int) of one segment for peaks sharing the tile list vals [(q, offset Hz)]; fpeak and
-    k_start per peak. D complex64 (flen,), IP float32 (flen,) on the GPU. Returns (values float64, computed mask): a
-    peak whose tiles reach above 0.9 x Nyquist is not computed (value 1), as the reference's sg_chisq_points."""
+class PendingSG:
+    """sg_chisq values being computed on the device; result() waits for them and finishes in float64."""
+
+    def __init__(self, n, ok, idx, nt, df, fetch):
+        self.n, self.ok, self.idx, self.nt, self.df, self.fetch = n, ok, idx, nt, df, fetch
+
+    def result(self):
+        out = np.ones(self.n, np.float64)
+        if self.fetch is not None:
+            zh, gh = self.fetch.get()
+            zz = (zh[0::2] + 1j * zh[1::2]).reshape(-1, self.nt) * (4.0 * self.df)
+            gsig = np.sqrt(4 * self.df * gh).reshape(-1, self.nt)
+            out[self.idx] = (np.abs(zz / gsig) ** 2).sum(1) / (2 * self.nt)
+        return out, self.ok
+
+
+def sg_chisq_launch(D, IP, t, fpeak, vals, k_start, df, tlen):
+    """Enqueue sg_chisq_points' kernel and the copy of its sums; returns a PendingSG (no host wait)."""
     flen = int(D.shape[-1])
     assert D.dtype == torch.complex64 and IP.dtype == torch.float32 and D.is_contiguous() and IP.is_contiguous()
     assert IP.shape[-1] == flen and flen == tlen // 2 + 1
     fpeak = np.asarray(fpeak, np.float64)
     t = np.asarray(t, np.int64)
     ok = np.all([fpeak + off + 50 <= flen * df * 0.9 for _, off in vals], axis=0)
-    out = np.ones(len(t), np.float64)
     idx = np.flatnonzero(ok)
+    nt = len(vals)
     if len(idx) == 0:
-        return out, ok
+        return PendingSG(len(t), ok, idx, nt, df, None)
     kk, k1, kg, fc, tau, q2 = tile_ranges(fpeak[idx], vals, np.asarray(k_start)[idx], df)
     assert k1.max() <= flen and kk.min() >= 0
-    nt = len(vals)
     tabs = _dev.to_dev_many(kk.ravel(), k1.ravel(), kg.ravel(), fc.ravel(), tau.ravel(), q2.ravel(), t[idx])
     z = torch.empty(2 * len(idx) * nt, dtype=torch.float64, device=D.device)
     gs = torch.empty(len(idx) * nt, dtype=torch.float64, device=D.device)
@@ -58,11 +71,14 @@ def sg_chisq_points(D, IP, t, fpeak, vals, k_start, df, tlen):
               *[x.data_ptr() for x in tabs], len(idx), nt, z.data_ptr(), gs.data_ptr(), _dev.stream())
     # algorithmic bytes: D and IP (12 B) per bin of every (peak, tile)
     _dev.METER.stop('sg_tiles', e0, 12.0 * float((k1 - kk).sum()))
-    zh, gh = _dev.Fetch(z, gs).get()
-    zz = (zh[0::2] + 1j * zh[1::2]).reshape(-1, nt) * (4.0 * df)
-    gsig = np.sqrt(4 * df * gh).reshape(-1, nt)
-    out[idx] = (np.abs(zz / gsig) ** 2).sum(1) / (2 * nt)
-    return out, ok
+    return PendingSG(len(t), ok, idx, nt, df, _dev.Fetch(z, gs, after=_dev.mark()))
+
+
+def sg_chisq_points(D, IP, t, fpeak, vals, k_start, df, tlen):
+    """sg_chisq at samples t (int) of one segment for peaks sharing the tile list vals [(q, offset Hz)]; fpeak and
+    k_start per peak. D complex64 (flen,), IP float32 (flen,) on the GPU. Returns (values float64, computed mask): a
+    peak whose tiles reach above 0.9 x Nyquist is not computed (value 1), as the reference's sg_chisq_points."""
+    return sg_chisq_launch(D, IP, t, fpeak, vals, k_start, df, tlen).result()
 
 
 _COND = re.compile(r'^\s*(mass1|mass2|mtotal|mchirp|eta|q)\s*(<=|>=|<|>|==)\s*([-+0-9.eE]+)\s*$')

@@ -160,6 +160,33 @@ def test_host_consolidate_matches_oracle():
             np.testing.assert_array_equal(floor, fl)
 
 
+def test_prefix_consolidation_equals_whole():
+    """Consolidating only the leading parts whose fields are final (later parts pending), then everything, keeps
+    the same peaks as consolidating everything once."""
+    import consolidate
+    import fp_filter
+    rng = np.random.default_rng(12)
+    kl = consolidate.KeepLoudest(4.0, 1.072, 2, 5.0)
+    for trial in range(40):
+        a, b = fp_filter.Triggers(sg=True), fp_filter.Triggers(sg=True)
+        for part in range(int(rng.integers(2, 6))):
+            c = _random_peaks(rng, int(rng.integers(0, 300)))
+            n = len(c['snr'])
+            c.update(tid=rng.integers(0, 500, n), ph=np.zeros(n, np.float32), sg=np.ones(n), ex=c['dof'] > 0,
+                     samp=np.zeros(n, np.int64), seg=np.zeros(n, np.int64))
+            a.add(**c)
+            b.add(**c)
+            if part == 1:
+                a.unready.append(a.parts['sgc'][-1])           # this part's sine-Gaussian values pending
+                consolidate.consolidate(a, [], GPS, SR, kl)
+                assert a.parts['sgc'][-1] is a.unready[0]     # left as it was, after the consolidated prefix
+        a.unready = []
+        consolidate.consolidate(a, [], GPS, SR, kl)
+        consolidate.consolidate(b, [], GPS, SR, kl)
+        assert sorted(zip(a.cat('tid', np.int64).tolist(), a.cat('t', np.float64).tolist())) == \
+            sorted(zip(b.cat('tid', np.int64).tolist(), b.cat('t', np.float64).tolist()))
+
+
 def test_candidate_cut_rounds_like_torch():
     """fp_filter.py:747 compares the float32 window maxima with the Python float (thr (1 - margin))^2: torch rounds
     the float64 square to float32. At thr 5.5, margin 0.04 that is 27.8784 (float32), not the square of the

@@ -140,15 +140,24 @@ def test_filter_consolidation(loud_data, precision):
     kl = consolidate.KeepLoudest(4.0, 1.072, 3, 5.0)
     IPall, sw, isw = fp_filter.svd_metric([ds])
     outs = {}
-    for name, kw in (('none', {}), ('cons', dict(consolidate=kl)), ('pre', dict(consolidate=kl, pre_consolidate=True))):
+    locs = ['mtotal>12:20-15,20-30,20-45']
+    for name, kw in (('none', {}), ('cons', dict(consolidate=kl)), ('pre', dict(consolidate=kl, pre_consolidate=True)),
+                     ('pre_sg', dict(consolidate=kl, pre_consolidate=True, sgchisq_snr_threshold=4.5))):
         opts = fp_filter.Options(tile=16, precision=precision, snr_threshold=4.0, exact_snr=5.5, **kw)
-        plan = fp_filter.BankPlan(bank.m1, bank.m2, None, ds.flen, ds.df, opts, sw, isw)
+        plan = fp_filter.BankPlan(bank.m1, bank.m2, None, ds.flen, ds.df, opts, sw, isw,
+                                  sg_locations=locs if name == 'pre_sg' else None)
         f = fp_filter.Filter([ds], opts)
         outs[name] = (f.run(plan)[0], f.T)
     a, b = outs['cons'][0].to_dict(gates=ds.gates), outs['pre'][0].to_dict(gates=ds.gates)
     assert len(a['template_id']) > 0
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    # with the sine-Gaussian values arriving one batch late, tiles consolidate only the parts already complete:
+    # the same peaks, now with sg_chisq for some
+    c_ = outs['pre_sg'][0].to_dict(gates=ds.gates)
+    for k in a:
+        np.testing.assert_array_equal(a[k], c_[k], err_msg=k)
+    assert c_['sg_chisq_computed'].sum() > 0 and outs['pre_sg'][1]['n_sg_chisq_points'] >= c_['sg_chisq_computed'].sum()
     assert outs['pre'][1]['n_candidates'] < outs['cons'][1]['n_candidates']
     # the unconsolidated peaks in the order they were found (ties in a window go to the earlier peak)
     tr = outs['none'][0]
