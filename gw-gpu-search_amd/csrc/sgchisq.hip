@@ -6,15 +6,18 @@
 // in float64 (the reference's index arithmetic, bit for bit).
 //
 // One 256-thread workgroup per (peak, tile): thread i takes bins kk + i, kk + i + 256, ... (coalesced loads of D
-// and IP, which stay in L2 across the peaks of one segment). Each bin's phasor is exact: the 64-bit integer phase
-// k t mod N, then sincospif(2 ph / N) in float (2 ph / N is exact in float for power-of-two N <= 2^24), so no
-// recurrence error accumulates; g in float with expf; sums in float64 per thread, reduced in a fixed order (wave
-// butterflies, then the four waves in order): the result does not depend on scheduling.
+// and IP, which stay in L2 across the peaks of one segment). Each thread's phasor is seeded exactly every R of its
+// bins (the 64-bit integer phase k t mod N, then sincospif(2 ph / N) in float, exact in its argument for
+// power-of-two N <= 2^24) and advanced in between by the fixed rotation e^{2 pi i (256 t mod N) / N}, so its
+// error stays within about R float roundings; g in float with expf; float partial sums over those R bins are added
+// in float64, then reduced in a fixed order (wave butterflies, then the four waves in order): the result does not
+// depend on scheduling.
 #include "common.h"
 
 namespace {
 
 constexpr int NT = 256;
+constexpr int R = 16;            // bins of one thread between exact phasor seeds
 
 __global__ void __launch_bounds__(NT) sg_tiles_k(const float2* __restrict__ D, const float* __restrict__ IP,
                                                  const long N, const int pow2, const double df,
@@ -31,20 +34,33 @@ __global__ void __launch_bounds__(NT) sg_tiles_k(const float2* __restrict__ D, c
     const float pit = 3.14159265358979f * ta;
     const long tp = t[p];
     const float inv = 2.0f / (float)N;
+    const long dstep = pow2 ? (((long)NT * tp) & (N - 1)) : (((long)NT * tp) % N);
+    float ws, wc;                                           // the phasor's rotation from one of its bins to the next
+    sincospif((float)dstep * inv, &ws, &wc);
     double sre = 0.0, sim = 0.0, s2 = 0.0;
-    for (long k = max(a, g0) + threadIdx.x; k < e; k += NT) {
-        const long prod = k * tp;
+    for (long k0 = max(a, g0) + threadIdx.x; k0 < e; k0 += (long)NT * R) {
+        const long prod = k0 * tp;
         const long ph = pow2 ? (prod & (N - 1)) : (prod % N);
         float sn, cs;
         sincospif((float)ph * inv, &sn, &cs);
-        const float f = (float)((double)k * df);
-        const float x = pit * (f - fc);
-        const float g = amp * expf(-x * x) * (1.0f + expf(-qq * f));
-        const float2 d = D[k];
-        const float wr = d.x * cs - d.y * sn, wi = d.x * sn + d.y * cs;     // D_k e^{i phase}
-        sre += (double)(g * wr);
-        sim += (double)(g * wi);
-        s2 += (double)(g * g * IP[k]);
+        const long k1e = min(e, k0 + (long)NT * R);
+        float fr = 0.f, fi = 0.f, f2 = 0.f;
+        for (long k = k0; k < k1e; k += NT) {
+            const float f = (float)((double)k * df);
+            const float x = pit * (f - fc);
+            const float g = amp * expf(-x * x) * (1.0f + expf(-qq * f));
+            const float2 d = D[k];
+            const float wr = d.x * cs - d.y * sn, wi = d.x * sn + d.y * cs;     // D_k e^{i phase}
+            fr += g * wr;
+            fi += g * wi;
+            f2 += g * g * IP[k];
+            const float nc = cs * wc - sn * ws;
+            sn = cs * ws + sn * wc;
+            cs = nc;
+        }
+        sre += (double)fr;
+        sim += (double)fi;
+        s2 += (double)f2;
     }
     sre = wave_sum_f64(sre);
     sim = wave_sum_f64(sim);
