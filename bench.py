@@ -287,6 +287,43 @@ class BatchedWorkload:
 SG_LOCATIONS = ['mtotal>30:20-15,20-30,20-45,20-60,20-75,20-90,20-105,20-120']
 
 
+def chunked_report(wl, cfg, tile, hist_s=8.0, chunk_batch=(64,)):
+    """Overlap-save chunks (fp_filter.py:324-357 --chunk / --chunk2): the circular bank's templates shorter than 6 s
+    from 15 Hz against the production data as whole 512 s segments, as 16 s chunks (8 s history, 2 s future) and as
+    the same chunks band-limited to 512 Hz (sampled at 1024 Hz). One warmup and one timed run each; reports step time,
+    template*s per GPU-s over each form's own analysed span and the loudest trigger of each dataset."""
+    import numpy as np
+    import torch
+    import fp_filter, window_filter, workloads
+    b = wl.banks[0]
+    sel = np.nonzero(b.tau0(15.0) < 6.0)[0]
+    ds = workloads.condition_streams(wl.streams, wl.cond, [x.clone() for x in wl.x_dev])
+    res = dict(templates=int(len(sel)))
+    forms = [('whole', None, 4)] + [(f'chunks_b{b_}', '16,%g,2' % hist_s, b_) for b_ in chunk_batch] + \
+        [('chunks_512hz', '16,%g,2,512' % hist_s, chunk_batch[-1])]
+    for name, spec, nb in forms:
+        d = ds if spec is None else window_filter.to_chunks(ds, spec)[0]
+        opts = fp_filter.Options(tile=tile, precision='bf16', gemm_cfg=cfg, self_check=False, segment_batch=nb)
+        IPall, sw, isw = fp_filter.svd_metric(d)
+        plan = fp_filter.BankPlan(b.m1[sel], b.m2[sel], None, d[0].flen, d[0].df, opts, sw, isw)
+        fp_filter.Filter(d, opts).run(plan)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = [t.to_dict(gates=x.gates) for t, x in zip(fp_filter.Filter(d, opts).run(plan), d)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        an = workloads.analysed_seconds(d, int(d[0].sr))
+        loud = []
+        for o in outs:
+            i = int(np.argmax(o['snr'])) if len(o['snr']) else None
+            loud.append(None if i is None else dict(snr=float(o['snr'][i]), end_time=float(o['end_time'][i]),
+                                                     template=int(sel[o['template_id'][i]])))
+        res[name] = dict(spec=spec, segment_batch=nb, ms=dt * 1e3, analysed_s=an, value=len(sel) * an / dt,
+                         segments=sum(len(x.analyze) for x in d), tlen=d[0].tlen, ranks=plan.ranks,
+                         n_triggers=int(sum(len(o['snr']) for o in outs)), loudest=loud)
+    return res
+
+
 def consolidation_report(step, nsteps, work, outs_headline):
     """The production cuts (SNR threshold 4, exact band from 5.5, newsnr >= 4, the 30 loudest per 1.072 s window,
     exempt from SNR 5, consolidated after every tile: fp_filter.py:629-663; the sine-Gaussian chi-squared of exact
@@ -368,6 +405,8 @@ def main():
                     help='timed steps of the production cuts (SNR threshold 4, exact band from 5.5, keep the 30 '
                     'loudest per 1.072 s, exempt from SNR 5, sine-Gaussian chi-squared from newsnr 6) with host '
                     'consolidation only and with the device pre-consolidation (production workload; 0: skip)')
+    ap.add_argument('--chunked', type=int, default=1, help='run the overlap-save chunked forms beside whole segments '
+                    '(production workload; 0: skip)')
     ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
                     'trigger lists after the timed steps (production workload; 0: skip)')
     a = ap.parse_args()
@@ -524,6 +563,9 @@ def main():
     if a.workload == 'production' and a.consolidate_steps > 0:
         log('production cuts with consolidation')
         res['consolidation'] = consolidation_report(step, a.consolidate_steps, ntemp * an_s, outs_bf)
+    if a.workload == 'production' and a.chunked > 0:
+        log('overlap-save chunks')
+        res['chunked'] = chunked_report(wl, cfg, a.tile)
     if a.workload == 'production' and a.refine_top > 0:
         log('refinement of the loudest triggers')
         res['refinement'] = refinement_report(wl.plans, wl.streams, ds, outs_bf, a.refine_top)
