@@ -55,19 +55,13 @@ def _window(t, gps_start, sr, interval):
     return (np.round((np.asarray(t, np.float64) - gps_start) * sr) / (interval * sr)).astype(np.int64)
 
 
-def consolidate(tr, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
-    """fp_filter.py:629-663 on one dataset's Triggers (in place; on its leading parts whose fields are final, the
-    later parts kept after them unchanged): the gate veto as at write, newsnr >= the threshold
-    (or exempt), and with an interval the `num` loudest by newsnr per window among the non-exempt peaks, ties by
-    position. Returns the floor for the pre-consolidation (want_floor and an interval): per window up to the last
-    ranked one, the num-th loudest newsnr of every window holding at least num ranked peaks, -inf elsewhere; None
-    when there is no floor (or no peak: the caller keeps its previous floor). Also returns the number dropped."""
-    o = tr.parts
-    n = tr.ready_parts() if hasattr(tr, 'ready_parts') else len(o['snr'])   # leading parts with final fields
-    ks = [k for k, v in o.items() if v[:n]]
-    if not ks:
-        return None, 0
-    c = {k: np.concatenate(o[k][:n]) for k in ks}
+def consolidate_parts(parts, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
+    """fp_filter.py:629-663 on a list of parts (dicts of equal-length arrays: snr float32, chi float32, dof int32, t
+    float64, and any other fields): the gate veto as at write, newsnr >= the threshold (or exempt), and with an interval
+    the `num` loudest by newsnr per window among the non-exempt peaks, ties by position. Returns (the kept peaks as
+    one dict, floor, number dropped); the floor (want_floor and an interval): per window up to the last ranked one,
+    the num-th loudest newsnr of every window holding at least num ranked peaks, -inf elsewhere, else None."""
+    c = {k: np.concatenate([p_[k] for p_ in parts]) for k in parts[0]}
     snr = c['snr'].astype(np.float32, copy=False)
     ns = newsnr(snr, c['chi'].astype(np.float32, copy=False) / np.maximum(c['dof'].astype(np.int32, copy=False), 1))
     exempt = snr >= np.float32(kl.exempt_snr) if kl.exempt_snr is not None else np.zeros(len(ns), bool)
@@ -91,9 +85,32 @@ def consolidate(tr, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
             full = size >= kl.num
             floor = np.full(int(wt_s[-1]) + 1 if len(wt_s) else 1, -np.inf)
             floor[wt_s[first[full]]] = ns[idx[od[first[full] + kl.num - 1]]]
-    dropped = int((~keep).sum())
-    for k in ks:
-        o[k] = [c[k][keep]] + o[k][n:]
+    return {k: v[keep] for k, v in c.items()}, floor, int((~keep).sum())
+
+
+def snapshot(tr):
+    """The leading parts of a Triggers whose fields are final, as a list of dicts, and their count."""
+    o = tr.parts
+    n = tr.ready_parts() if hasattr(tr, 'ready_parts') else len(o['snr'])
+    return [{k: o[k][i] for k in o} for i in range(n)], n
+
+
+def replace_prefix(tr, n, kept):
+    """Put the consolidated peaks of the first n parts in their place (later parts unchanged)."""
+    for k in tr.parts:
+        tr.parts[k] = [kept[k]] + tr.parts[k][n:]
+    tr.settled = 1
+
+
+def consolidate(tr, gates, gps_start, sr, kl: KeepLoudest, want_floor=False):
+    """fp_filter.py:629-663 on one dataset's Triggers (in place; on its leading parts whose fields are final, the
+    later parts kept after them unchanged). Returns (floor or None, number dropped); (None, 0) when there is no
+    finished part (the caller keeps its previous floor)."""
+    parts, n = snapshot(tr)
+    if n == 0:
+        return None, 0
+    kept, floor, dropped = consolidate_parts(parts, gates, gps_start, sr, kl, want_floor)
+    replace_prefix(tr, n, kept)
     return floor, dropped
 
 

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -181,6 +182,7 @@ class Triggers:
         self.parts = {k: [] for k in self.KEYS}
         self.sg = sg                       # the peak file carries sg_chisq and sg_chisq_computed
         self.unready = []                  # sgc arrays of parts whose sine-Gaussian values are still on the device
+        self.settled = 0                   # 1 when the first part is a consolidation's result
 
     def ready_parts(self):
         """Number of leading parts whose fields are final (consolidation works on these)."""
@@ -297,6 +299,8 @@ class Filter:
                                              self.sr, self.cw) if opts.pre_consolidate else None)
         self._floors = {}                  # (plan index, dataset index): the floor consolidation left (device)
         self._sg_pending = []              # (Triggers, sgc, sgd, [(indices, sgchisq.PendingSG)])
+        self._cons_pool = None             # one worker thread for the consolidations (fp_filter.py:623, 888)
+        self._cons_jobs = []               # (Triggers, plan index, dataset index, parts covered, future)
         self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
         self._y16_update()
 
@@ -416,8 +420,6 @@ class Filter:
         opts.timing every stage is synchronised instead."""
         sg_on = self.opts.sgchisq_snr_threshold is not None
         outs = [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
-        # parts left out of a tile's consolidation (values pending) are consolidated at the end
-        self._sg_consolidate_pending = sg_on and self.opts.consolidate is not None
         segs = []
         for di, m in enumerate(self.data):
             for s, (a0, a1) in enumerate(m.analyze):
@@ -453,30 +455,53 @@ class Filter:
         while fin:
             self.finish_batch(fin.popleft())
         self._resolve_sg()
-        if self._sg_consolidate_pending:
-            for pi in range(len(plans)):
-                self._consolidate(pi, outs[pi])
+        if self.opts.consolidate is not None:
+            # the parts added after each dataset's last consolidation (sine-Gaussian values pending then, or found
+            # while it ran): one more pass, which leaves an already consolidated list as it is
+            self._apply_consolidated(block=True)
+            if any(len(tr.parts['snr']) > tr.settled for o_ in outs for tr in o_):
+                for pi in range(len(plans)):
+                    self._consolidate(pi, outs[pi])
+                self._apply_consolidated(block=True)
         return outs
 
     def finish_batch(self, st):
         """The sine-Gaussian values of earlier batches (their kernels ran right after this batch's exact stage, which
-        collect_batch waits for anyway), collect_batch, and after a tile's last batch the consolidation of the plan's
-        datasets (fp_filter.py:886-888) over their parts with final fields: the floors feed the pre-consolidation of
-        batches launched from then on. Parts still waiting for sine-Gaussian values are consolidated later; applying
-        the consolidation to any prefix of the peak list leaves the final list unchanged."""
+        collect_batch waits for anyway), the consolidations that have finished, collect_batch, and after a tile's last
+        batch the consolidation of the plan's datasets (fp_filter.py:886-888) over their parts with final fields, on
+        a worker thread as the reference's --pre-consolidate does: its floors feed the pre-consolidation of batches
+        launched after it is applied. Parts still waiting for sine-Gaussian values, or found while a consolidation
+        runs, are consolidated later; applying the consolidation to any prefix of the peak list leaves the final list
+        unchanged."""
         self._resolve_sg()
+        self._apply_consolidated(block=False)
         self.collect_batch(st)
         if st.get('last'):
             self._consolidate(st['pi'], st['outs'])
 
     def _consolidate(self, pi, outs):
+        """Start the consolidation of every dataset's finished parts (after the previous one of the same lists)."""
         o = self.opts
         if o.consolidate is None:
             return
-        t0 = time.perf_counter()
+        self._apply_consolidated(block=True)
+        if self._cons_pool is None:
+            self._cons_pool = ThreadPoolExecutor(1)
         for di, m in enumerate(self.data):
-            floor, dropped = consolidate.consolidate(outs[di], m.gates, m.gps_start, m.sr, o.consolidate,
-                                                     want_floor=o.pre_consolidate)
+            parts, n = consolidate.snapshot(outs[di])
+            if n == 0:
+                continue
+            fut = self._cons_pool.submit(consolidate.consolidate_parts, parts, list(m.gates), m.gps_start, m.sr,
+                                         o.consolidate, o.pre_consolidate)
+            self._cons_jobs.append((outs[di], pi, di, n, fut))
+
+    def _apply_consolidated(self, block):
+        """Put finished consolidations in place (all of them, waiting, with block) and take their floors."""
+        t0 = time.perf_counter()
+        while self._cons_jobs and (block or self._cons_jobs[0][4].done()):
+            tr, pi, di, n, fut = self._cons_jobs.pop(0)
+            kept, floor, dropped = fut.result()
+            consolidate.replace_prefix(tr, n, kept)
             self.T['n_consolidated_out'] += dropped
             if floor is not None:
                 self._floors[(pi, di)] = _dev.to_dev(floor)
