@@ -75,6 +75,7 @@ class Options:
     pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
     sgchisq_snr_threshold: float = None    # sine-Gaussian chi-squared of exact peaks with newsnr >= this, for the
                                            # templates BankPlan(sg_locations=...) gives tiles (None: off)
+    gate: object = None                    # Gate: filter a tile on a segment only near base-search triggers
 
 
 @dataclass
@@ -147,6 +148,8 @@ class BankPlan:
         self.kmin = int(opts.low_frequency_cutoff / df)
         self.m1, self.m2 = np.asarray(m1, float), np.asarray(m2, float)
         self.e0 = np.zeros(len(self.m1)) if e0 is None else np.asarray(e0, float)
+        mt = self.m1 + self.m2
+        self.mchirp = mt * (self.m1 * self.m2 / mt ** 2) ** 0.6
         # --sgchisq-locations: per template its sine-Gaussian tile list [(q, offset Hz)] or None
         self.sg_locs = sgchisq.parse_locations(sg_locations, self.m1, self.m2) if sg_locations else None
         fl = opts.low_frequency_cutoff
@@ -171,6 +174,42 @@ class BankPlan:
     @property
     def ranks(self):
         return [t.r for t in self.tiles]
+
+
+@dataclass
+class Gate:
+    """--gate / --gate-mc-ratio / --gate-window (fp_filter.py:146-149, 424-445, 724-756): base-search triggers
+    (detector name per trigger, end time in GPS seconds, chirp mass). A tile runs on a segment only if a trigger of
+    that segment's detector lies within `window` seconds of the segment's analysed range with chirp mass inside the
+    tile's range widened by `mc_ratio`, and only peaks within `window` of such a trigger are kept."""
+    ifo: np.ndarray
+    time: np.ndarray
+    mchirp: np.ndarray
+    mc_ratio: float = 1.25
+    window: float = 1.0
+
+    def per_segment(self, m: Dataset):
+        """fp_filter.py:430-438: per segment of m, the samples (relative to the segment start, sorted) and chirp
+        masses of this detector's triggers within the window of its analysed range."""
+        sel = np.asarray(self.ifo).astype(str) == m.tag.split('-')[0]
+        t, mc = np.asarray(self.time, np.float64)[sel], np.asarray(self.mchirp, np.float64)[sel]
+        o = np.argsort(t, kind='stable')
+        gs = np.round((t[o] - m.gps_start) * m.sr).astype(np.int64)
+        mc = mc[o]
+        w = int(self.window * m.sr)
+        out = []
+        for s, (a0, a1) in enumerate(m.analyze):
+            c = gs - m.seg_start[s]
+            i0, i1 = np.searchsorted(c, a0 - w), np.searchsorted(c, a1 + w, side='right')
+            out.append((c[i0:i1], mc[i0:i1]))
+        return out
+
+
+def near(samp, gs, w):
+    """fp_filter.py:441-443: samples within w of a sorted sample list gs."""
+    j = np.searchsorted(gs, samp)
+    return ((np.abs(samp - gs[np.clip(j - 1, 0, len(gs) - 1)]) <= w) |
+            (np.abs(samp - gs[np.clip(j, 0, len(gs) - 1)]) <= w))
 
 
 class Triggers:
@@ -295,6 +334,9 @@ class Filter:
         self.code = PRECISIONS[opts.precision]
         assert not opts.pre_consolidate or (opts.consolidate is not None and not opts.graphs), \
             'pre_consolidate needs consolidate, and the candidate tables change between runs (no graphs)'
+        # fp_filter.py:160: the gate drops peaks after the candidates are ranked, so the pre-consolidation's ranking
+        # would not be the consolidation's; gated batches differ from run to run (no graphs)
+        assert opts.gate is None or not (opts.pre_consolidate or opts.graphs), 'gate excludes pre_consolidate, graphs'
         self._prekeep = (consolidate.PreKeep(opts.consolidate, opts.snr_threshold, opts.exact_snr, opts.margin,
                                              self.sr, self.cw) if opts.pre_consolidate else None)
         self._floors = {}                  # (plan index, dataset index): the floor consolidation left (device)
@@ -427,21 +469,40 @@ class Filter:
                 if nw > 0:
                     segs.append((di, s, self.seg0[di] + s, a0, nw))
         B = max(1, self.opts.segment_batch)
+        gate = self.opts.gate
+        gsegs = [gate.per_segment(m) for m in self.data] if gate is not None else None
         launched, fin = deque(), deque()
         if self.opts.graphs and self._bufs is None:
             self._alloc_rotation(plans, segs, B)
         ordinal = 0
         for pi, plan in enumerate(plans):
             for ti, tb in enumerate(plan.tiles):
+                tsegs, gsel = segs, None
+                if gate is not None:
+                    # fp_filter.py:726-739: the segments where a trigger's chirp mass fits this tile
+                    mc = plan.mchirp[tb.ids]
+                    lo, hi = mc.min() / gate.mc_ratio, mc.max() * gate.mc_ratio
+                    gsel = {}
+                    for sg_ in segs:
+                        gsm, gmc = gsegs[sg_[0]][sg_[1]]
+                        g_ = gsm[(gmc >= lo) & (gmc <= hi)]
+                        if len(g_):
+                            gsel[(sg_[0], sg_[1])] = g_
+                    tsegs = [sg_ for sg_ in segs if (sg_[0], sg_[1]) in gsel]
+                    self.T['gate_runs'] = self.T.get('gate_runs', 0) + len(tsegs)
+                    self.T['gate_skips'] = self.T.get('gate_skips', 0) + len(segs) - len(tsegs)
+                    if not tsegs:
+                        continue
                 t0 = self._sync()
                 sig, norm = self.tile_norms(tb, (id(plan), ti))
                 tile_host = _dev.Fetch(sig, norm)
                 t1 = self._sync(); self.T['norm_s'] += t1 - t0
-                for b0 in range(0, len(segs), B):
-                    cur = self.launch_batch(tb, segs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
+                for b0 in range(0, len(tsegs), B):
+                    cur = self.launch_batch(tb, tsegs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
                                             ordinal=ordinal, pi=pi)
                     ordinal += 1
-                    cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(segs), plan.sg_locs
+                    cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(tsegs), plan.sg_locs
+                    cur['gate'] = gsel
                     if self.opts.timing:
                         self.decode_batch(cur); self.finish_batch(cur)
                         continue
@@ -709,6 +770,13 @@ class Filter:
             if not len(sel):
                 continue
             m = self.data[di]
+            if st.get('gate') is not None:
+                # fp_filter.py:754-757: only peaks within the window of a selected trigger
+                k_ = near(a0 + il[sel], st['gate'][(di, s)], int(o.gate.window * m.sr))
+                self.T['gate_peaks_dropped'] = self.T.get('gate_peaks_dropped', 0) + int((~k_).sum())
+                sel = sel[k_]
+                if not len(sel):
+                    continue
             tb_, samp, ax = m_[sel], a0 + il[sel], approx[sel]
             ex = ax >= o.exact_snr * (1 - o.margin)
             pv = None

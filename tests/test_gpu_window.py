@@ -68,3 +68,42 @@ def test_filter_on_chunks(heavy, spec):
     i, j = np.argmax(tr['snr']), np.argmax(res['whole']['snr'])
     assert tr['snr'][i] > 15 and abs(tr['snr'][i] / res['whole']['snr'][j] - 1) < 0.02
     assert abs(tr['end_time'][i] - res['whole']['end_time'][j]) < 0.01
+
+
+def test_gated_chunks(heavy):
+    """--gate (fp_filter.py:424-445, 724-756): the chunked search gated by the whole-segment search's loud triggers
+    (SNR > 8) keeps exactly the ungated chunked triggers whose tile has a gate trigger of matching chirp mass near the
+    chunk and which lie within the gate window of one, and skips the other (tile, chunk) pairs."""
+    import fp_filter, window_filter
+    bank, ds = heavy
+    (c,), _ = window_filter.to_chunks([ds], '16,8,2')
+    opts = fp_filter.Options(tile=8, precision='bf16', snr_threshold=5.0)
+    IPall, sw, isw = fp_filter.svd_metric([ds])
+    base = fp_filter.Filter([ds], opts).run(fp_filter.BankPlan(bank.m1, bank.m2, None, ds.flen, ds.df, opts, sw,
+                                                               isw))[0].to_dict()
+    loud = base['snr'] > 8
+    mc_all = (bank.m1 * bank.m2) ** 0.6 / (bank.m1 + bank.m2) ** 0.2
+    gate = fp_filter.Gate(np.array(['H1'] * int(loud.sum())), base['end_time'][loud], mc_all[base['template_id'][loud]],
+                          mc_ratio=1.25, window=1.0)
+    assert 0 < loud.sum()
+    IPc, swc, iswc = fp_filter.svd_metric([c])
+    plan = fp_filter.BankPlan(bank.m1, bank.m2, None, c.flen, c.df, opts, swc, iswc)
+    full = fp_filter.Filter([c], opts).run(plan)[0].to_dict()
+    fg = fp_filter.Filter([c], fp_filter.Options(tile=8, precision='bf16', snr_threshold=5.0, gate=gate))
+    got = fg.run(plan)[0].to_dict()
+    assert fg.T['gate_skips'] > 0 and fg.T['gate_runs'] > 0
+    # the rule, restated: per trigger its tile's chirp-mass range, the gate triggers of H1 in it, in samples of its chunk
+    tile_of = {int(i): tb for tb in plan.tiles for i in tb.ids}
+    gs = np.round((gate.time - c.gps_start) * c.sr).astype(np.int64)
+    w = int(1.0 * c.sr)
+    keep = np.zeros(len(full['snr']), bool)
+    for i, (tid, seg, samp) in enumerate(zip(full['template_id'], full['segment'], full['sample'])):
+        mc = mc_all[tile_of[int(tid)].ids]
+        ok = (gate.mchirp >= mc.min() / 1.25) & (gate.mchirp <= mc.max() * 1.25)
+        rel = gs[ok] - c.seg_start[seg]
+        a0, a1 = c.analyze[seg]
+        rel = rel[(rel >= a0 - w) & (rel <= a1 + w)]
+        keep[i] = len(rel) > 0 and np.abs(rel - samp).min() <= w
+    assert 0 < keep.sum() < len(keep)
+    exp = sorted(zip(full['template_id'][keep].tolist(), full['end_time'][keep].tolist(), full['snr'][keep].tolist()))
+    assert sorted(zip(got['template_id'].tolist(), got['end_time'].tolist(), got['snr'].tolist())) == exp
