@@ -290,8 +290,10 @@ SG_LOCATIONS = ['mtotal>30:20-15,20-30,20-45,20-60,20-75,20-90,20-105,20-120']
 def chunked_report(wl, cfg, tile, hist_s=8.0, chunk_batch=(64,)):
     """Overlap-save chunks (fp_filter.py:324-357 --chunk / --chunk2): the circular bank's templates shorter than 6 s
     from 15 Hz against the production data as whole 512 s segments, as 16 s chunks (8 s history, 2 s future) and as
-    the same chunks band-limited to 512 Hz (sampled at 1024 Hz). One warmup and one timed run each; reports step time,
-    template*s per GPU-s over each form's own analysed span and the loudest trigger of each dataset."""
+    the same chunks band-limited to 512 Hz (sampled at 1024 Hz), and both gated by the whole-segment search's
+    triggers above SNR 12 (--gate: a tile runs on a chunk only near such a trigger of matching chirp mass). One warmup
+    and one timed run each; reports step time, template*s per GPU-s over each form's own analysed span (the gated
+    forms: over the span they would cover ungated) and the loudest trigger of each dataset."""
     import numpy as np
     import torch
     import fp_filter, window_filter, workloads
@@ -300,10 +302,15 @@ def chunked_report(wl, cfg, tile, hist_s=8.0, chunk_batch=(64,)):
     ds = workloads.condition_streams(wl.streams, wl.cond, [x.clone() for x in wl.x_dev])
     res = dict(templates=int(len(sel)))
     forms = [('whole', None, 4)] + [(f'chunks_b{b_}', '16,%g,2' % hist_s, b_) for b_ in chunk_batch] + \
-        [('chunks_512hz', '16,%g,2,512' % hist_s, chunk_batch[-1])]
+        [('chunks_512hz', '16,%g,2,512' % hist_s, chunk_batch[-1]),
+         ('chunks_gated', '16,%g,2' % hist_s, chunk_batch[-1]), ('chunks_512hz_gated', '16,%g,2,512' % hist_s,
+                                                                 chunk_batch[-1])]
+    gate = None
+    mc_all = (b.m1 * b.m2) ** 0.6 / (b.m1 + b.m2) ** 0.2
     for name, spec, nb in forms:
         d = ds if spec is None else window_filter.to_chunks(ds, spec)[0]
-        opts = fp_filter.Options(tile=tile, precision='bf16', gemm_cfg=cfg, self_check=False, segment_batch=nb)
+        opts = fp_filter.Options(tile=tile, precision='bf16', gemm_cfg=cfg, self_check=False, segment_batch=nb,
+                                 gate=gate if name.endswith('gated') else None)
         IPall, sw, isw = fp_filter.svd_metric(d)
         plan = fp_filter.BankPlan(b.m1[sel], b.m2[sel], None, d[0].flen, d[0].df, opts, sw, isw)
         fp_filter.Filter(d, opts).run(plan)
@@ -318,6 +325,13 @@ def chunked_report(wl, cfg, tile, hist_s=8.0, chunk_batch=(64,)):
             i = int(np.argmax(o['snr'])) if len(o['snr']) else None
             loud.append(None if i is None else dict(snr=float(o['snr'][i]), end_time=float(o['end_time'][i]),
                                                      template=int(sel[o['template_id'][i]])))
+        if name == 'whole':
+            # the gate of the gated forms: this search's triggers above SNR 12 (fp_filter.py:146-149)
+            g = [(x.tag.split('-')[0], o['end_time'][k], mc_all[sel[o['template_id'][k]]])
+                 for x, o in zip(d, outs) for k in np.nonzero(o['snr'] > 12)[0]]
+            gate = fp_filter.Gate(np.array([x[0] for x in g]), np.array([x[1] for x in g]),
+                                  np.array([x[2] for x in g]), mc_ratio=1.25, window=1.0)
+            res['gate_triggers'] = len(g)
         res[name] = dict(spec=spec, segment_batch=nb, ms=dt * 1e3, analysed_s=an, value=len(sel) * an / dt,
                          segments=sum(len(x.analyze) for x in d), tlen=d[0].tlen, ranks=plan.ranks,
                          n_triggers=int(sum(len(o['snr']) for o in outs)), loudest=loud)
