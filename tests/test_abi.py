@@ -54,6 +54,18 @@ def test_bad_arguments_rejected(lib):
     assert lib.gwgs_basis_series(9, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3
     assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 2, None, p, p, 32, None) == -3
     assert lib.gwgs_vd_power(None, 100, 10, p, 0, 50, p, None) == -1
+    # pre-consolidation: empty batches return at once; bad flags, no segment table, or ranking without a workspace
+    assert lib.gwgs_pre_keep_workspace(100, 5) == 8 * 100 + 4 * 16
+    pk = lambda n, nseg, flags, work, wb: lib.gwgs_pre_keep(p, p, n, 10, nseg, p, p, p, p, 2048.0, 2195.456, 3, 1.0,
+                                                            5.28, 4.0, 4.0, 5.0, flags, 5, work, wb, None)
+    assert pk(0, 1, 2, None, 0) == 0
+    assert pk(10, 1, 4, None, 0) == -1 and pk(10, 0, 0, None, 0) == -1 and pk(10, 1, 2, None, 0) == -1
+    assert pk(10, 1, 2, p, 100) == -1                 # workspace smaller than gwgs_pre_keep_workspace
+    # sine-Gaussian tiles: no peaks returns at once; no tile, a spectrum longer than N / 2 + 1 or null tables rejected
+    sg = lambda npk, ntile, flen, ptr: lib.gwgs_sg_tiles(ptr, ptr, flen, 1 << 20, 1 / 512, ptr, ptr, ptr, ptr, ptr,
+                                                         ptr, ptr, npk, ntile, ptr, ptr, None)
+    assert sg(0, 8, 524289, p) == 0
+    assert sg(4, 0, 524289, p) == -1 and sg(4, 8, 524290, p) == -1 and sg(4, 8, 524289, None) == -1
     with pytest.raises(AssertionError):
         _lib.check(-1, 'x')
     with pytest.raises(RuntimeError):

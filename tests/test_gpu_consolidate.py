@@ -168,3 +168,37 @@ def test_filter_consolidation(loud_data, precision):
     assert keep.sum() < len(keep)
     exp = sorted(zip(tr.cat('tid', np.int64)[keep].tolist(), c['t'][keep].tolist()))
     assert sorted(zip(a['template_id'].tolist(), a['end_time'].tolist())) == exp
+
+
+@pytest.mark.parametrize('kind', ['all_exact', 'none_above_cut', 'all_tied', 'all_exempt', 'all_gated'])
+def test_pre_keep_edge_cases(gpu, kind):
+    """Edge cases against the oracle: every candidate in the exact band (all kept), no candidate (vw untouched),
+    one SNR for all (the num earliest in row-major order per window survive), every rebuilt candidate exempt, every
+    candidate inside a gate."""
+    import consolidate, fp_filter
+    rng = np.random.default_rng(21)
+    gps = 1380077576.0
+    segs = [dict(gps=gps, seg_start=0, a0=0, nw=6, gates=[], floor=None),
+            dict(gps=gps, seg_start=2097152, a0=4096, nw=5, gates=[], floor=None)]
+    n = 40
+    vw, iw = _batch(rng, n, segs, 0.7)
+    cut = fp_filter.candidate_cut(4.0, 0.04)
+    if kind == 'all_exact':
+        vw[vw > cut] = np.float32(6.0) ** 2
+    elif kind == 'none_above_cut':
+        vw = np.minimum(vw, np.float32(3.0))
+    elif kind == 'all_tied':
+        vw[vw > cut] = np.float32(4.5) ** 2
+    elif kind == 'all_exempt':
+        vw[vw > cut] = np.where(vw[vw > cut] < np.float32(5.0) ** 2, np.float32(5.1) ** 2, vw[vw > cut])
+    elif kind == 'all_gated':
+        for s in segs:
+            s['gates'] = [(gps + (s['seg_start'] + s['a0'] + 6 * 2048) / 2048.0, 8.0, 0.25)]
+    kl = consolidate.KeepLoudest(4.0, 1.072, 2, 5.0)
+    out = _run(vw, iw, segs, cut, 4.0, 5.5, 0.04, kl)
+    exp = _expected(vw, iw, segs, cut, 4.0, 5.5, 0.04, kl)
+    np.testing.assert_array_equal(out > cut, exp)
+    if kind in ('all_exact', 'none_above_cut', 'all_exempt'):
+        np.testing.assert_array_equal(out, vw)
+    if kind == 'all_gated':
+        assert not (out > cut)[np.sqrt(vw) < np.float32(5.28)].any()
