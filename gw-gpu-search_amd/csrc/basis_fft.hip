@@ -438,14 +438,6 @@ __global__ void __launch_bounds__(256) vd_power_k(const float2* __restrict__ Vc,
     if (threadIdx.x == 0) out[j] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// line length L -> threads per line P (Q = L / P values each)
-bool split(const int lgl, int& P)
-{
-    if (lgl < 8 || lgl > 11) return false;
-    P = lgl <= 9 ? 16 : 32;
-    return true;
-}
-
 template <int P, int Q, int NZ, typename YS>
 void launch_cols_nz(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
                     const float* ysc, void* Y, hipStream_t s)
@@ -473,6 +465,7 @@ int cols_dispatch(const int lgn2, const float2* Vc, long ldv, const float2* D, l
                   int r, const float* ysc, void* Y, hipStream_t s)
 {
     switch (lgn2) {
+    case 7: launch_cols<8, 16, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
     case 8: launch_cols<16, 16, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
     case 9: launch_cols<16, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
     case 10: launch_cols<32, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
@@ -498,6 +491,8 @@ int rows_dispatch(const int lgn1, const void* Y, int r, int rp, int lgn, long a0
                   const float* ysc, T* Zt, long ldz, hipStream_t s)
 {
     switch (lgn1) {
+    case 6: launch_rows<8, 8, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
+    case 7: launch_rows<8, 16, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
     case 8: launch_rows<16, 16, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
     case 9: launch_rows<16, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
     case 10: launch_rows<32, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
@@ -530,11 +525,11 @@ int log2_exact(const long n)
 } // namespace
 
 // Bytes of the intermediate Y for r basis rows and transform length n; 0 when n is not a supported length
-// (a power of two from 2^16 to 2^22).
+// (a power of two from 2^13 to 2^22).
 GWGS_API long gwgs_basis_series_workspace(int r, long n)
 {
     const int lg = log2_exact(n);
-    if (lg < 16 || lg > 22 || r <= 0) return 0;
+    if (lg < 13 || lg > 22 || r <= 0) return 0;
     return (long)r * n * 8;
 }
 
@@ -549,7 +544,7 @@ GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int r
     GWGS_CHECK(Vc && D && work && Zt && r > 0 && rp >= r && kmin >= 0 && kmin <= kend && kend <= ldv && kend <= n &&
                a0 >= 0 && Tn >= 0 && a0 + Tn <= n && ldz >= 2 * (long)rp && ldz % 2 == 0, GWGS_EINVAL);
     const int lg = log2_exact(n);
-    if (lg < 16 || lg > 22) return GWGS_EUNSUP;
+    if (lg < 13 || lg > 22) return GWGS_EUNSUP;
     // the row kernel addresses Zt as one buffer with 32-bit byte offsets up to (Tn + 1) rows
     const double esz = dtype == 0 ? 4.0 : dtype == 1 ? 2.0 : 1.0;
     if ((double)(Tn + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;

@@ -395,7 +395,7 @@ class Filter:
 
     def basis_operand(self, tb: TileBasis, segs, Tns, row0, Zt=None):
         """Zt (sum Tn, 2 rp) in the GEMM's operand type: the basis series of every segment of the batch over its
-        analysed span, stacked by rows (fp_filter.py:597-613). Power-of-two segment lengths 2^16..2^22 take the
+        analysed span, stacked by rows (fp_filter.py:597-613). Power-of-two segment lengths 2^13..2^22 take the
         fused four-step transform (csrc/basis_fft.hip): no transform input is materialised and the output is written
         time-major directly; other lengths take one batched hipFFT transform and the LDS transpose (pack_zt).
         For fp8, the per-basis scale 1 / rms (rms from the batch's mean sum_k |q_jk|^2, Parseval) is applied to Z

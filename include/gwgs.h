@@ -115,7 +115,7 @@ int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const f
 int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double* out, void* stream);
 
 /* Fused basis series (replaces pipeline/fp_filter.py:597-613 basis_series: the zero-padded q = conj(Vh) D / sw,
- * torch.fft.ifft(q) * n and the fp16 stacked Z), as a four-step FFT of two passes for power-of-two n from 2^16 to
+ * torch.fft.ifft(q) * n and the fp16 stacked Z), as a four-step FFT of two passes for power-of-two n from 2^13 to
  * 2^22: x_j[k] = Vc[j, k] D[k] on [kmin, kend) (0 elsewhere), z_j[t] = sum_k x_j[k] e^(2 pi i k t / n), written as
  * Zt[t - a0, 2j] = Re z_j[t] cs[j], Zt[t - a0, 2j + 1] = Im z_j[t] cs[j] for a0 <= t < a0 + Tn, j < r, and zeros
  * for r <= j < rp. dtype 0 float32, 1 bfloat16 (nearest even), 2 fp8 e4m3 (saturated to +-448); colscale may be

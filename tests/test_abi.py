@@ -43,12 +43,14 @@ def test_bad_arguments_rejected(lib):
     assert lib.gwgs_welch_reduce(ctypes.c_void_p(16), 1, 600, 8, 1.0, 0, 1.0, 1.0, ctypes.c_void_p(16), None) == -2
     assert lib.gwgs_gemm_max_k(7) == 0
     assert lib.gwgs_select_workspace(4097) == 3
-    # the fused basis transform: power-of-two lengths 2^16..2^22 only
+    # the fused basis transform: power-of-two lengths 2^13..2^22 only
     assert lib.gwgs_basis_series_workspace(10, 1 << 20) == 10 * (1 << 20) * 8
-    for n in (1 << 15, 3 << 16, 1 << 23, 0):
+    assert lib.gwgs_basis_series_workspace(10, 1 << 13) == 10 * (1 << 13) * 8
+    for n in (1 << 12, 3 << 16, 1 << 23, 0):
         assert lib.gwgs_basis_series_workspace(10, n) == 0
     p = ctypes.c_void_p(256)
     assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 3 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3
+    assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 12, 0, 10, None, 0, None, p, p, 32, None) == -3
     assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 101, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -1
     assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 30, None) == -1
     assert lib.gwgs_basis_series(9, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3

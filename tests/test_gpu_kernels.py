@@ -361,6 +361,9 @@ def test_time_slides_vs_brute_force(gpu):
 
 
 @pytest.mark.parametrize('n,r,kmin,kend,a0,Tn', [
+    (1 << 13, 11, 120, 2049, 512, 4096),              # 64 x 128: 16 s chunks band-limited to 256 Hz
+    (1 << 14, 23, 240, 2000, 1024, 12288),            # 128 x 128, 2 of 16 column inputs per thread
+    (1 << 15, 40, 480, (1 << 14) + 1, 2048, 12288),   # 128 x 256: 16 s chunks at 2048 Hz, the whole band
     (1 << 16, 37, 5, 20000, 300, 1 << 15),            # 256 x 256
     (1 << 17, 16, 0, (1 << 16) + 1, 0, 1 << 17),      # 256 x 512, whole series
     (1 << 19, 9, 3840, 120000, 4096, 65536),          # 512 x 1024
