@@ -541,14 +541,18 @@ class Filter:
             self._consolidate(st['pi'], st['outs'])
 
     def _consolidate(self, pi, outs):
-        """Start the consolidation of every dataset's finished parts (after the previous one of the same lists)."""
+        """Start the consolidation of every dataset's finished parts; a list whose previous consolidation is still
+        running is left for the next one (the end of the run consolidates whatever remains)."""
         o = self.opts
         if o.consolidate is None:
             return
-        self._apply_consolidated(block=True)
+        self._apply_consolidated(block=False)
         if self._cons_pool is None:
             self._cons_pool = ThreadPoolExecutor(1)
+        busy = {id(job[0]) for job in self._cons_jobs}
         for di, m in enumerate(self.data):
+            if id(outs[di]) in busy:
+                continue
             parts, n = consolidate.snapshot(outs[di])
             if n == 0:
                 continue
@@ -559,8 +563,9 @@ class Filter:
     def _apply_consolidated(self, block):
         """Put finished consolidations in place (all of them, waiting, with block) and take their floors."""
         t0 = time.perf_counter()
-        while self._cons_jobs and (block or self._cons_jobs[0][4].done()):
-            tr, pi, di, n, fut = self._cons_jobs.pop(0)
+        for job in [j for j in self._cons_jobs if block or j[4].done()]:
+            self._cons_jobs.remove(job)
+            tr, pi, di, n, fut = job
             kept, floor, dropped = fut.result()
             consolidate.replace_prefix(tr, n, kept)
             self.T['n_consolidated_out'] += dropped
