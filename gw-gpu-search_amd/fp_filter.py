@@ -524,6 +524,9 @@ class Filter:
                 for pi in range(len(plans)):
                     self._consolidate(pi, outs[pi])
                 self._apply_consolidated(block=True)
+        if self._cons_pool is not None:
+            self._cons_pool.shutdown(wait=True)
+            self._cons_pool = None
         return outs
 
     def finish_batch(self, st):
