@@ -435,8 +435,8 @@ def main():
                     'trigger lists after the timed steps (production workload; 0: skip)')
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); the '
-              f'filtering work shards by (detector, chunk) or by template tile, see DESIGN.md', file=sys.stderr)
+        print(f'bench.py runs on one GPU: there is no multi-GPU path here (asked for --gpus {a.gpus}); the '
+              f'filtering work shards by (detector, chunk) or by template tile', file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -1,4 +1,4 @@
-// Differentiable peak SNR of a compressed template (build-only: the north star's backward kernel; the forward
+// Differentiable peak SNR of a compressed template (no reference counterpart: the backward kernel; the forward
 // quantity is the reference's rebuilt SNR, fp_filter.py:759-764, evaluated off the sample grid and off the bank).
 //
 // For a template given by its coefficient vector a (r complex, z = sum_j a_j Zc_j, the rows of A in the basis of one

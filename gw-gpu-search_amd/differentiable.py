@@ -6,8 +6,8 @@ SVD basis (z = sum_j a_j Zc_j). Here it is evaluated at a continuous time t + de
 sigma (sigma^2 = 4 df a^T G conj(a), G the tile's basis metric under the segment's 1/psd), and differentiated by a
 hand-written kernel (snr_grad_k): rho^2 and its gradients with respect to a (2r real coordinates), delta and the
 phase. PeakSNR2 wraps it as a torch.autograd.Function; refine() runs projected gradient ascent on the convex hull of
-the nearest bank templates' coefficient vectors (and on delta), the search the north star names for the loudest
-triggers.
+the nearest bank templates' coefficient vectors (and on delta), which refines the loudest triggers off the
+discrete bank.
 """
 from __future__ import annotations
 

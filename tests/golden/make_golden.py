@@ -1,5 +1,5 @@
-"""Writes golden vectors from the reference's own numpy-only functions (run in the build container only, where the
-reference is mounted; the GPU box never reads it). Only functions whose integer/index conventions no closed form
+"""Writes golden vectors from the reference's own numpy-only functions (run only where the reference's sources are
+mounted; the tests read the written vectors, never the reference). Only functions whose integer/index conventions no closed form
 pins are sampled: gpu_prep.segment_slices, overlapping_psd_spans, inverse_spectrum_truncation (edge bins, Hann
 halves), welch (float32 path), median_bias, interpolate, cluster_over_window, and window_filter.chunk_starts/valid.
 
