@@ -204,16 +204,10 @@ class ProductionWorkload:
         self.plans = [fp_filter.BankPlan(b.m1, b.m2, b.e0, ds0[0].flen, ds0[0].df, opts['bf16'], sw, isw,
                                          name=b.name, sg_locations=SG_LOCATIONS) for b in self.banks]
 
-    def condition(self):
-        import workloads
-        return workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
-
-    def step(self, prec, check=False, ds=None):
-        """One pass: conditioning of both streams (unless ds, conditioned ahead by the queue) and the filtering of
-        both banks."""
-        import fp_filter
-        if ds is None:
-            ds = self.condition()
+    def step(self, prec, check=False):
+        """One pass: conditioning of both streams and the filtering of both banks."""
+        import fp_filter, workloads
+        ds = workloads.condition_streams(self.streams, self.cond, [x.clone() for x in self.x_dev])
         o = self.opts[prec]
         if o.graphs:
             # one Filter per form for the whole run: its captured graphs read its own D / IP, refreshed by rebind
@@ -426,9 +420,6 @@ def main():
                     help='timed steps of the production cuts (SNR threshold 4, exact band from 5.5, keep the 30 '
                     'loudest per 1.072 s, exempt from SNR 5, sine-Gaussian chi-squared from newsnr 6) with host '
                     'consolidation only and with the device pre-consolidation (production workload; 0: skip)')
-    ap.add_argument('--queue', type=int, default=0, help='1: condition the next step on a worker thread and stream '
-                    'while the current one filters (the reference\'s --queue; production workload); 0: one after the '
-                    'other (the default: measured faster on this workload)')
     ap.add_argument('--chunked', type=int, default=1, help='run the overlap-save chunked forms beside whole segments '
                     '(production workload; 0: skip)')
     ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
@@ -491,31 +482,11 @@ def main():
     ntemp = sum(len(b) for b in banks)
     plans = wl.plans
     meter = _dev.METER
-    queue = workloads.PrepQueue() if a.queue and a.workload == 'production' else None
-    dt_serial = None
-    if queue is not None:
-        # the same steps without the queue, for reference: conditioning and filtering one after the other
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(a.steps):
-            step('bf16')
-        torch.cuda.synchronize()
-        dt_serial = (time.perf_counter() - t0) / a.steps
     meter.reset(); meter.on = True
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    if queue is not None:
-        # fp_filter.py:300-305 (--queue): step k + 1's conditioning runs on the queue's thread and stream while step k
-        # filters; exactly a.steps conditionings and a.steps filterings fall inside the timed region
-        queue.submit(wl.condition)
     for k in range(a.steps):
-        if queue is not None:
-            dsq = queue.get()
-            if k + 1 < a.steps:
-                queue.submit(wl.condition)
-            ds, filters, outs_bf = step('bf16', ds=dsq)
-        else:
-            ds, filters, outs_bf = step('bf16')
+        ds, filters, outs_bf = step('bf16')
         log(f'step {k + 1}/{a.steps}')
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
@@ -582,11 +553,9 @@ def main():
             err8, _ = wl.direct_check(ds, outs_f8)
     res = dict(
         metric=METRIC, value=value, unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
-        ms_per_step=dt * 1e3, ms_per_step_without_queue=None if dt_serial is None else dt_serial * 1e3,
-        higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
+        ms_per_step=dt * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
         data='synthetic',
         config=dict(workload=wl.describe(a.tile), templates=ntemp, analysed_s=an_s, tile=a.tile,
-                    queue=queue is not None,
                     ranks=wl.ranks(), setup_s=round(setup_s, 1), gemm_cfg=list(cfg) if cfg else None,
                     autotune_ms={k: round(v * 1e3, 3) for k, v in (tuned or {}).items() if isinstance(v, float)}),
         roofline=dict(bound='mfma', achieved=fl / t_k / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',

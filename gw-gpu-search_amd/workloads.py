@@ -103,43 +103,5 @@ def condition_streams(streams, cond: Conditioning, x_dev=None):
     return out
 
 
-class PrepQueue:
-    """The reference's --queue (fp_filter.py:239-245, 300-305): the next datasets are conditioned on a worker thread
-    and its own stream (default priority) while the current ones are filtered. submit(fn, *args) runs fn there;
-    get() returns its datasets with the caller's stream ordered after that work and their D / IP tensors recorded as
-    used by the caller's stream, so the caching allocator keeps them until the filtering that reads them is done."""
-
-    def __init__(self):
-        import torch
-        from concurrent.futures import ThreadPoolExecutor
-        self.pool = ThreadPoolExecutor(1)
-        self.stream = torch.cuda.Stream()
-        self.fut = None
-
-    def submit(self, fn, *args):
-        import torch
-        assert self.fut is None, 'one conditioning in flight at a time'
-
-        def job():
-            with torch.cuda.stream(self.stream):
-                out = fn(*args)
-                ev = torch.cuda.Event()
-                ev.record(self.stream)
-            return out, ev
-
-        self.fut = self.pool.submit(job)
-
-    def get(self):
-        import torch
-        out, ev = self.fut.result()
-        self.fut = None
-        main = torch.cuda.current_stream()
-        main.wait_event(ev)
-        for d in out:
-            d.D.record_stream(main)
-            d.IP.record_stream(main)
-        return out
-
-
 def analysed_seconds(datasets, cw):
     return sum((a1 - a0) // cw * cw / d.sr for d in datasets for a0, a1 in d.analyze)
