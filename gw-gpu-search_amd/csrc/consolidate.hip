@@ -15,7 +15,8 @@
 //
 // Passes: count (candidates per window, atomics), exclusive scan of the window counts, scatter (every ranked
 // candidate's 64-bit key into its window's bucket; the drops of the cut, the gates and the floor are written here),
-// rank (one workgroup per window: the num-th largest key by num max-reductions; keys below it are dropped). The key
+// rank (one workgroup per window: up to 1,024 keys, each key's rank counted directly from LDS; more, the num-th
+// largest key by num max-reductions; keys below it are dropped). The key
 // is (float bits of apx) << 32 | ~g: apx > 0, so larger keys are louder, and the earlier g wins a tie. Each entry is
 // classified twice (count and scatter) from the same vw[g] and iw[g], which no pass changes before the scatter
 // writes its own entry.
@@ -154,6 +155,16 @@ __global__ void __launch_bounds__(NT) prekeep_rank_k(float* __restrict__ vw, con
     if (staged) {
         for (int i = threadIdx.x; i < n; i += NT) keys[i] = B[i];
         __syncthreads();
+    }
+    if (staged && n <= 4 * NT) {
+        // few keys: each key's rank is the number of larger keys (keys are distinct), counted from LDS
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const unsigned long long k = keys[i];
+            int rank = 0;
+            for (int j = 0; j < n; j++) rank += keys[j] > k;
+            if (rank >= num) vw[(unsigned)~(unsigned)k] = -1.0f;
+        }
+        return;
     }
     const unsigned long long* K = staged ? keys : B;
     unsigned long long lim = ~0ull;             // after pass r: the r-th largest key
