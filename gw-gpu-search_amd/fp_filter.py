@@ -325,7 +325,7 @@ class Filter:
             self.seg0.append(off); off += len(m.analyze)
         self._q = None
         self._work = None
-        self._bufs = None                  # graph mode: three rotating buffer sets
+        self._bufs = None                  # graph mode: depth + 1 rotating buffer sets
         self._graphs = {}                  # graph mode: captured tile norms and stage-A batches
         self.T = dict(norm_s=0.0, basis_fft_s=0.0, gemm_s=0.0, rebuilt_z_s=0.0, refine_chisq_s=0.0, n_candidates=0,
                       n_exact_candidates=0, n_rebuilt_peaks=0, n_chisq_points=0, consolidate_s=0.0,
@@ -345,6 +345,17 @@ class Filter:
         self._cons_jobs = []               # (Triggers, plan index, dataset index, parts covered, future)
         self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
         self._y16_update()
+
+    def _nsets(self):
+        """Rotation sets of graph mode: a batch's buffers are reused depth + 1 batches later (_alloc_rotation)."""
+        return max(1, self.opts.depth) + 1
+
+    def _reset_run_state(self):
+        """State one run hands to the next must not leak into it: the floors consolidation left (they would drop
+        the next run's candidates at or below another run's floors), pending consolidation jobs and sine-Gaussian
+        launches (both are drained by the end of every run)."""
+        assert not self._cons_jobs and not self._sg_pending, 'a previous run did not finish'
+        self._floors = {}
 
     def _sync(self):
         if self.opts.timing:
@@ -460,6 +471,7 @@ class Filter:
         previous batch's results are collected, so the GPU has queued work while the host decodes and finalises,
         across tiles and banks. Copies to the host run on a side stream after the events they depend on. With
         opts.timing every stage is synchronised instead."""
+        self._reset_run_state()
         sg_on = self.opts.sgchisq_snr_threshold is not None
         outs = [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
         segs = []
@@ -628,8 +640,11 @@ class Filter:
         return dict(Tns=Tns, row0=row0, nwsum=sum(nws), col0=col0, items=items, gsl=gsl, K=2 * tb.rp, per=per)
 
     def _alloc_rotation(self, plans, segs, B):
-        """Three buffer sets, each sized for the largest batch of the run, for the captured stage-A graphs: batch i
-        of a run writes set i mod 3 (every reader of a batch's buffers is enqueued before batch i + 3 is)."""
+        """depth + 1 buffer sets, each sized for the largest batch of the run, for the captured stage-A graphs:
+        batch i of a run writes set i mod (depth + 1). Invariant: run_banks decodes batch i (its candidate list read
+        back and awaited, its rebuilt phases enqueued on the compute stream) once batches i + 1 .. i + depth are
+        launched and before batch i + depth + 1 is, so every reader of set i mod (depth + 1) is complete or ordered
+        before the next batch that overwrites it."""
         zt = a = w = nmax = 0
         for plan in plans:
             for tb in plan.tiles:
@@ -648,7 +663,8 @@ class Filter:
                            sel=(torch.empty(w, dtype=torch.int32, device='cuda'),
                                 torch.empty(w, dtype=torch.int32, device='cuda'),
                                 torch.empty(w, dtype=torch.float32, device='cuda'),
-                                torch.empty(nwork, dtype=torch.int32, device='cuda'))) for _ in range(3)]
+                                torch.empty(nwork, dtype=torch.int32, device='cuda')))
+                      for _ in range(self._nsets())]
         ws = max((_lib.load().gwgs_basis_series_workspace(tb.r, self.tlen) for plan in plans for tb in plan.tiles),
                  default=0)
         if ws > 0:
@@ -717,7 +733,7 @@ class Filter:
                 g = torch.cuda.CUDAGraph()
                 torch.cuda.synchronize()
                 with torch.cuda.graph(g, stream=self._capture_stream()):
-                    dev = self._batch_device(tb, segs, norm, geo, self._bufs[ordinal % 3])
+                    dev = self._batch_device(tb, segs, norm, geo, self._bufs[ordinal % self._nsets()])
                 rec = self._graphs[key] = dict(graph=g, geo=geo, dev=dev)
             rec['graph'].replay()
             geo, dev = rec['geo'], rec['dev']
@@ -745,6 +761,7 @@ class Filter:
             m.IP.copy_(d.IP)
             m.gates, m.gps_start, m.seg_start = list(d.gates), d.gps_start, list(d.seg_start)
         torch.cat([m.IP for m in self.data], 0, out=self.IPall)
+        self._reset_run_state()
         self._y16_update()
         self.T = {k: 0 if isinstance(v, int) else 0.0 for k, v in self.T.items()}
         self.rel = []

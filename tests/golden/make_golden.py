@@ -1,7 +1,9 @@
 """Writes golden vectors from the reference's own numpy-only functions (run only where the reference's sources are
 mounted; the tests read the written vectors, never the reference). Only functions whose integer/index conventions no closed form
 pins are sampled: gpu_prep.segment_slices, overlapping_psd_spans, inverse_spectrum_truncation (edge bins, Hann
-halves), welch (float32 path), median_bias, interpolate, cluster_over_window, and window_filter.chunk_starts/valid.
+halves), welch (float32 path), median_bias, interpolate, cluster_over_window, window_filter.chunk_starts/valid, and
+gpu_prep.gpu_stage2 (numpy backend) end to end on a seeded strain: one PSD span, and several spans with per-segment
+PSD picks (stage2_strain() below rebuilds the inputs from their seeds; D and IP are stored on a bin subsample).
 
 Run: python tests/golden/make_golden.py  ->  tests/golden/reference_vectors.npz
 """
@@ -13,6 +15,37 @@ import numpy as np
 
 REF = '/root/reference/pipeline'
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'reference_vectors.npz')
+
+
+# gpu_stage2 cases: (name, seed, sample rate, seconds before padding, pad, pycbc_inspiral options)
+STAGE2 = [
+    ('one_span', 5, 512, 64, 4, ['--segment-length', '64', '--segment-start-pad', '8', '--segment-end-pad', '8',
+                                 '--psd-segment-length', '4', '--psd-segment-stride', '2', '--psd-num-segments', '31',
+                                 '--psd-inverse-length', '4', '--psd-estimation', 'median']),
+    ('multi_span', 6, 256, 1100, 8, ['--segment-length', '256', '--segment-start-pad', '16', '--segment-end-pad', '16',
+                                     '--psd-segment-length', '16', '--psd-segment-stride', '8', '--psd-num-segments',
+                                     '63', '--psd-inverse-length', '16', '--invpsd-trunc-method', 'hann',
+                                     '--psd-estimation', 'median']),
+]
+STAGE2_GPS = 1000000000
+STAGE2_STRIDE = 13        # D and IP are stored on every 13th bin (and in full on the 64 bins from kmin)
+
+
+def stage2_strain(seed, sr, seconds, pad):
+    """float32 strain with its padding: white noise times a slow envelope (so PSD spans differ and the per-segment
+    pick matters), plus two lines. The tests import this function to rebuild the inputs."""
+    rng = np.random.default_rng(seed)
+    n = int((seconds + 2 * pad) * sr)
+    t = np.arange(n) / sr
+    env = 1.0 + 0.6 * np.sin(2 * np.pi * t / 700.0 + 0.3)
+    x = rng.standard_normal(n) * env + 3.0 * np.sin(2 * np.pi * 60.0 * t) + 0.5 * np.sin(2 * np.pi * 33.3 * t)
+    return (x * 1e3).astype(np.float32)
+
+
+def stage2_argv(sr, seconds, pad, opts):
+    return ['--hdf-store', 'synthetic.hdf', '--channel-name', 'H1:STRAIN', '--gps-start-time', str(STAGE2_GPS),
+            '--gps-end-time', str(STAGE2_GPS + seconds), '--pad-data', str(pad), '--sample-rate', str(sr),
+            '--low-frequency-cutoff', '15'] + opts
 
 
 def main():
@@ -61,6 +94,18 @@ def main():
         a, b = w.valid(st, Lc, hist, fut, hi)
         chunks.append(dict(case=[tlen, Lc, hist, fut, lo, hi], starts=st.tolist(), a=a.tolist(), b=b.tolist()))
     out['chunk_json'] = np.array(json.dumps(chunks))
+    meta = []
+    for name, seed, sr, seconds, pad, opts in STAGE2:
+        h = stage2_strain(seed, sr, seconds, pad)
+        r = g.gpu_stage2(np, h.copy(), stage2_argv(sr, seconds, pad, opts))
+        kmin = int(15.0 / r['df'])
+        out[f's2_{name}_D'] = r['D'][:, ::STAGE2_STRIDE]
+        out[f's2_{name}_IP'] = r['IP'][:, ::STAGE2_STRIDE]
+        out[f's2_{name}_Dk'] = r['D'][:, kmin - 32:kmin + 32]
+        out[f's2_{name}_IPk'] = r['IP'][:, kmin - 32:kmin + 32]
+        meta.append(dict(name=name, flen=r['flen'], tlen=r['tlen'], df=r['df'], sr=r['sr'], gps_start=r['gps_start'],
+                         analyze=[list(map(int, a)) for a in r['analyze']], seg_start=list(map(int, r['seg_start']))))
+    out['s2_json'] = np.array(json.dumps(meta))
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, sorted(out))
 

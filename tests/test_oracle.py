@@ -263,3 +263,117 @@ def test_find_pairs_brute_force_small():
     h, l, k, dt = core.find_pairs(H, L, 100.0, 0.1, 10, 0.015)
     assert list(zip(h, l, k)) == [(0, 0, -3), (1, 1, 0)]
     assert abs(dt[0]) < 1e-12 and abs(dt[1] + 0.01) < 1e-9
+
+
+def _stage2_cases():
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    import make_golden as mg
+    meta = json.loads(str(GOLD['s2_json']))
+    for (name, seed, sr, seconds, pad, opts), m in zip(mg.STAGE2, meta):
+        o = {opts[i][2:].replace('-', '_'): opts[i + 1] for i in range(0, len(opts), 2)}
+        h = mg.stage2_strain(seed, sr, seconds, pad)
+        s = h[int(pad * sr):len(h) - int(pad * sr)]
+        args = (int(o['segment_length']), int(o['segment_start_pad']), int(o['segment_end_pad']),
+                float(o['psd_segment_length']), float(o['psd_segment_stride']), int(o['psd_num_segments']),
+                float(o['psd_inverse_length']), 15.0)
+        yield name, s, sr, mg, m, args, o.get('invpsd_trunc_method')
+
+
+def _check_stage2(name, D, IP, analyze, seg_start, meta, stride):
+    """D and IP on the golden bin subsample (the reference's float32 PSD path against a float64 one: 2e-5 per IP
+    bin, 1e-5 of max |D|), the exact zero pattern around kmin, and the segment layout exactly."""
+    gD, gIP = GOLD[f's2_{name}_D'], GOLD[f's2_{name}_IP']
+    nz = gIP > 0
+    assert np.array_equal(IP[:, ::stride] > 0, nz)
+    assert np.max(np.abs(IP[:, ::stride] - gIP)[nz] / gIP[nz]) < 2e-5
+    assert np.max(np.abs(D[:, ::stride] - gD)) < 1e-5 * np.abs(gD).max()
+    kmin = int(15.0 / meta['df'])
+    assert np.array_equal(IP[:, kmin - 32:kmin + 32] > 0, GOLD[f's2_{name}_IPk'] > 0)
+    assert not np.any(D[:, kmin - 32:kmin]) and np.all(IP[:, kmin - 32:kmin] == 0)
+    assert [list(a) for a in analyze] == meta['analyze'] and list(seg_start) == meta['seg_start']
+
+
+def test_condition_vs_reference_stage2_golden():
+    """core.condition against the reference's own gpu_stage2 (numpy backend) on seeded strains: one PSD span, and a
+    1,100 s strain with four PSD spans and five segments whose largest-overlap picks are 0, 0, 2, 3, 3 (a pick moved
+    to the next span changes IP by about 2x, far beyond the tolerance), with the Hann-tapered truncation."""
+    for name, s, sr, mg, meta, args, trunc in _stage2_cases():
+        r = core.condition(s, sr, mg.STAGE2_GPS, *args, trunc_method=trunc)
+        _check_stage2(name, r['D'], r['IP'], r['analyze'], r['seg_start'], meta, mg.STAGE2_STRIDE)
+        if name == 'multi_span':
+            seg = core.segment_slices(len(s), sr, mg.STAGE2_GPS, *args[:3])
+            spans = core.overlapping_psd_spans(len(s), int(args[3] * sr), int(args[4] * sr), args[5])
+            assert core.pick_psd(seg, spans) == [0, 0, 2, 3, 3]
+            # a planted pick (the next span) is caught by the golden IP
+            alt = [(p + 1) % len(spans) for p in core.pick_psd(seg, spans)]
+            P = r['psd'][alt]
+            ip2 = np.where(P > 0, 1.0 / np.where(P > 0, P, 1.0), 0.0)
+            ip2[:, :int(15.0 / r['df'])] = 0
+            gIP = GOLD['s2_multi_span_IP']
+            nz = gIP > 0
+            assert np.max(np.abs(ip2[:, ::mg.STAGE2_STRIDE] - gIP)[nz] / gIP[nz]) > 0.1
+
+
+def test_pick_psd_rules():
+    """Largest overlap wins, the first span wins ties, a segment touching no span gets None."""
+    segs = [((0, 100), None), ((50, 150), None), ((90, 200), None), ((1000, 1100), None)]
+    spans = [(0, 120), (40, 160), (80, 200)]
+    # overlaps: seg0 120/60/20 -> 0; seg1 70/100/70 -> 1; seg2 30/70/110 -> 2; seg3 none
+    assert core.pick_psd(segs, spans) == [0, 1, 2, None]
+    assert core.pick_psd([((10, 60), None)], [(0, 40), (30, 70)]) == [0]        # 30 vs 30: first wins
+
+
+def test_gate_taper_hand_case():
+    """pycbc gate_data at non-integer sample times near both edges: zeros start exactly at int(t_start sr) + pad,
+    the ramps are 0.5 (1 + cos(pi i / pad)) and its mirror, samples outside the window keep their value, and a gate
+    starting before the data clips at sample 0."""
+    sr, n, gps = 256.0, 4096, 1000000000.0
+    gw, pw = 0.25, 0.125
+    win, pad = int(2 * sr * (gw + pw)), int(sr * pw)                   # 192, 32
+    for gt_rel in (3.3017, 15.89):                                     # offsets 748 (near mid) and 3972 (right edge)
+        x = np.full(n, 2.0, np.float32)
+        core.gate_taper(x, sr, gps, [(gps + gt_rel, gw, pw)])
+        off = int((gt_rel - gw - pw) * sr)
+        ramp = 0.5 * (1.0 + np.cos(np.pi * np.arange(pad) / pad))
+        exp = np.full(n, 2.0)
+        w = np.concatenate([ramp, np.zeros(win - 2 * pad), ramp[::-1]])
+        i2 = min(win, n - off)
+        exp[off:off + i2] = 2.0 * w[:i2]
+        np.testing.assert_array_equal(x, exp.astype(np.float32))
+        assert x[off + pad] == 0 and x[off + pad - 1] > 0 and x[off - 1] == 2.0
+    x = np.full(n, 2.0, np.float32)
+    core.gate_taper(x, sr, gps, [(gps + 0.1, gw, pw)])                 # t_start = -0.275 s: offset -70
+    off = int((0.1 - gw - pw) * sr)
+    assert off == -70
+    w = np.concatenate([0.5 * (1 + np.cos(np.pi * np.arange(pad) / pad)), np.zeros(win - 2 * pad),
+                        0.5 * (1 + np.cos(np.pi * np.arange(pad) / pad))[::-1]])
+    exp = np.full(n, 2.0)
+    exp[0:win + off] = 2.0 * w[-off:]
+    np.testing.assert_array_equal(x, exp.astype(np.float32))
+    y = np.full(n, 2.0, np.float32)
+    core.gate_taper(y, sr, gps, [(gps + 40.0, gw, pw), (gps - 5.0, gw, pw)])       # both outside the data
+    assert np.all(y == 2.0)
+
+
+def test_white_noise_snr_normalisation():
+    """sigma^2 = 4 df sum |h|^2 IP and norm = 4 df / sigma make rho = |z| norm have <rho^2> = 2 on Gaussian noise of
+    the PSD 1/IP (D = stilde IP, E|stilde_k|^2 = S_k / (2 df)); a factor 2 slip in sigma^2, df or the one-sided sum
+    gives 1 or 4."""
+    rng = np.random.default_rng(5)
+    N, sr = 1 << 15, 2048.0
+    flen, df = N // 2 + 1, sr / N
+    f = np.arange(flen) * df
+    import synth
+    S = synth.analytic_psd(f)
+    kmin = int(15.0 / df)
+    IP = np.where(np.arange(flen) >= kmin, 1.0 / S, 0.0)
+    H, _ = pipeline.spa_bank(np.array([10.0, 30.0, 7.0]), np.array([8.0, 25.0, 6.0]), None, flen, df, 15.0)
+    sig, norm = core.sigmasq(H, IP[None, :], df)
+    acc = []
+    for _ in range(24):
+        st = np.sqrt(S / (4 * df)) * (rng.standard_normal(flen) + 1j * rng.standard_normal(flen))
+        z = core.direct_z(H, st * IP, N)
+        acc.append((np.abs(z) ** 2 * norm[:, 0][:, None] ** 2).mean(1))
+    m = np.mean(acc, 0)
+    assert np.all(np.abs(m - 2.0) < 0.06), m
