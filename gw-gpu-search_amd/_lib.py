@@ -36,6 +36,7 @@ SIGNATURES = {
     'gwgs_select_workspace': [_l],
     'gwgs_select_above': [_p, _l, _f, _p, _i, _p, _p, _p],
     'gwgs_select_peaks': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p, _p],
+    'gwgs_select_peaks_noalloc': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p],
     'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_chisq_points_workspace': [_i, _i],

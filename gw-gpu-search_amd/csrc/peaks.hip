@@ -1,6 +1,8 @@
 // Peak finding and candidate selection (the reference's peak_kernel.py:8-47 and fp_filter.py:745-764), wave64.
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int NT = 256;
@@ -216,6 +218,54 @@ GWGS_API int gwgs_select_peaks(const float* vw, const int* iw, long total, float
     hipStream_t s = (hipStream_t)stream;
     const long nc = (total + CH - 1) / CH;
     GWGS_CHECK(nc < (1L << 24), GWGS_ESIZE);
+    if (nc) hipLaunchKernelGGL(count_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work);
+    hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, s, work, (int)nc, count_dev);
+    if (nc)
+        hipLaunchKernelGGL(write_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work, out, cap, iw,
+                           out_iw, out_vw);
+    return gwgs_status();
+}
+
+// The candidate compaction without a caller-provided workspace, for callers that cannot allocate (code under
+// graph capture, serving loops). Its per-block counts live in a file-scope device array, one copy per GPU that the
+// library runs on, held until the process ends; the host takes each GPU's address of it once (hipGetSymbolAddress
+// with that GPU current) and passes that address to the kernels, never the array's name. Every block count is
+// written before it is read, so nothing is cleared between calls; calls that overlap on one GPU share the array
+// and race.
+constexpr long NOALLOC_COUNTS = 1L << 16;          // 256 KiB: total <= NOALLOC_COUNTS * CH entries (2.7e8)
+static __device__ __align__(256) int scratch[NOALLOC_COUNTS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static int* scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+static int scratch_for_current_device(int** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    GWGS_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, GWGS_ESIZE);
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        scratch_dev[dev] = (int*)ptr;
+    }
+    *out = scratch_dev[dev];
+    return GWGS_OK;
+}
+
+GWGS_API int gwgs_select_peaks_noalloc(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw,
+                                       float* out_vw, int cap, int* count_dev, void* stream)
+{
+    GWGS_CHECK((vw || total == 0) && (iw || total == 0) && ((out && out_iw && out_vw) || cap == 0) && count_dev &&
+               total >= 0 && cap >= 0, GWGS_EINVAL);
+    const long nc = (total + CH - 1) / CH;
+    GWGS_CHECK(nc <= NOALLOC_COUNTS, GWGS_ESIZE);
+    int* work = nullptr;
+    const int rc = scratch_for_current_device(&work);
+    if (rc != GWGS_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
     if (nc) hipLaunchKernelGGL(count_above_k, dim3((unsigned)nc), dim3(NT), 0, s, vw, total, cut, work);
     hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, s, work, (int)nc, count_dev);
     if (nc)

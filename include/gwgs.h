@@ -69,6 +69,13 @@ int gwgs_select_above(const float* vw, long total, float cut, int* out, int cap,
  * and out_vw[i] = vw[out[i]] for the selected entries, so the candidate list needs no host round trip. */
 int gwgs_select_peaks(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw, float* out_vw,
                       int cap, int* work, int* count_dev, void* stream);
+/* No counterpart in the reference: gwgs_select_peaks for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). The same compaction with its block counts in a static device
+ * array of 65,536 ints (256 KiB per GPU, held until the process ends) instead of a caller workspace: total must be
+ * at most 65,536 x 4,096 = 268,435,456 entries, else GWGS_ESIZE. Allocates nothing. Calls that overlap on one GPU
+ * share the array and race; calls on one stream, or ordered by events, do not. */
+int gwgs_select_peaks_noalloc(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw,
+                              float* out_vw, int cap, int* count_dev, void* stream);
 /* pipeline/fp_filter.py:759-764: rebuilt complex SNR at candidates, out[c] = (Ar[rows[c]] . Zt[tcol[c]],
  * Ai[rows[c]] . Zt[tcol[c]]) sqrt(pw_scale[sidx[c]]) in fp32 (complex64 out); sidx NULL indexes pw_scale by rows[c];
  * pw_scale NULL: no scale. dtype as gwgs_gemm_peaks. */
