@@ -1,16 +1,22 @@
 #!/usr/bin/env python
 """Throughput of the filtering path on one MI355X: template-seconds filtered per GPU-second.
 
-Workload (workloads.production): 2 detectors x 512 s at 2048 Hz of synthetic coloured noise with lines, glitches and
-20 injections; both banks, 4,612 circular + 1,268 eccentric = 5,880 templates. One step = one pass of the hot path
-over the batch with the raw strain resident in HBM: autogating, conditioning (float64 median-Welch PSD, truncation,
-whitening), template norms, per tile the basis transforms and the fused GEMM + window max, candidate selection, the
-rebuilt phases and the exact stage (edges, chi-squared sums, float64 finalization). The bank products (templates,
-float64 SVD bases) are built once before timing, as a search builds them once per run.
+Headline workload (the batched configuration, workloads.batched_device): per detector 200 chunks of 64-1024 s in
+16 s steps (segment gaps), 2 detectors at 2048 Hz of synthetic coloured noise with 5 lines, drawn on the GPU; both
+banks, 4,612 circular + 1,268 eccentric = 5,880 templates; chunks on 512 s and 128 s segment grids. The job is cut
+into 20 slices of 10 chunks per detector: one step filters one slice (conditioning of every chunk from its raw strain
+resident in HBM: float64 median-Welch PSD, truncation, whitening; template norms; per tile the basis transforms and
+the fused GEMM + window max; candidate selection; the exact stage with +-3-sample re-centring: edges, chi-squared
+sums, float64 finalization), so the 20 default steps filter every chunk once. Warm-up steps run on separate chunks.
+The bank products (templates, float64 SVD bases) are built once before timing, as a search builds them once per run;
+config.value_with_setup counts them into the whole job.
 
-value = 5,880 x analysed seconds (both detectors) / seconds per step, for the BF16 form. The FP32 plain form of the
-same step is timed beside it, the trigger lists of the two are compared, and every exact trigger's SNR is compared
-with a float64 direct matched filter of the same conditioned data (computed with torch complex128 FFTs).
+value = 5,880 x analysed seconds of the timed slices (both detectors) / their wall time, for the BF16 form. The
+FP32 plain form (and the FP8 form) are timed on the first slices beside it, the trigger lists compared with the
+BF16 form's, and every exact trigger's SNR of the loudest templates compared with a float64 direct matched filter of
+the same conditioned data (torch complex128 FFTs). The production workload (2 detectors x 512 s with lines,
+glitches and 20 injections) follows as a secondary object with its reports (consolidation at the production cuts,
+overlap-save chunks, refinement, time slides).
 
 Prints one JSON line. Progress goes to stderr.
 """
@@ -104,29 +110,31 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(streams, banks, cond, ntemp=512):
-    """The oracle's plain method (numpy / scipy.fft / OpenBLAS float32) on a bounded sample: H1 only, the first
-    `ntemp` circular templates in chirp-time order as one tile; conditioning plus the per-segment work (basis
-    transforms, GEMM and window maxima, exact stage) is timed, the tile's templates and SVD are not (as on the GPU)."""
+def cpu_baseline(x, sr, gps_start, cond, bank, label, ntemp=512):
+    """The oracle's plain method (numpy / scipy.fft / OpenBLAS float32) on a bounded sample: one detector stretch x
+    (float32 strain, host), the first `ntemp` circular templates in chirp-time order as one tile; conditioning plus
+    the per-segment work (basis transforms, GEMM and window maxima, exact stage) is timed, the tile's templates and SVD
+    are not (as on the GPU, where the bank products are made once per run)."""
     import numpy as np
     from oracle import core, pipeline
     cores = min(len(os.sched_getaffinity(0)), int(os.environ.get('OMP_NUM_THREADS', '16')))
-    st = streams[0]
-    bank = banks[0]
     ids = pipeline.tiles_by_tau0(bank.m1, bank.m2, 15.0, ntemp)[0]
     t0 = time.perf_counter()
-    d = core.condition(st.x, st.sr, st.gps_start, cond.seg_len, cond.start_pad, cond.end_pad, cond.psd_seg_len,
+    d = core.condition(x, sr, gps_start, cond.seg_len, cond.start_pad, cond.end_pad, cond.psd_seg_len,
                        cond.psd_stride, cond.psd_nseg, cond.psd_inv_len, 15.0)
     t_cond = time.perf_counter() - t0
     T = {}
     pipeline.filter_plain([d], bank.m1[ids], bank.m2[ids], None, tile=ntemp, dtype=np.float32, workers=cores,
                           timing=T)
     t = t_cond + T['basis_fft_s'] + T['gemm_s'] + T['exact_s']
-    an = sum((a1 - a0) // 2048 * 2048 / st.sr for a0, a1 in d['analyze'])
+    an = sum((a1 - a0) // 2048 * 2048 / sr for a0, a1 in d['analyze'])
     return dict(value=ntemp * an / t, unit='template*s/CPU-s', cores=cores, kind='port',
-                sample=f'H1 512 s, {ntemp} circular templates as one tile, float32 plain method '
-                       f'(oracle/pipeline.py), conditioning + per-segment work timed ({t:.1f} s), '
-                       f'templates/SVD excluded ({T["templates_s"] + T["svd_s"]:.1f} s)')
+                per_core=ntemp * an / t / cores,
+                sample=f'{label}: {an:.0f} analysed s, {ntemp} circular templates as one tile, float32 plain method '
+                       f'(oracle/pipeline.py), conditioning + per-segment work timed ({t:.1f} s: conditioning '
+                       f'{t_cond:.1f}, basis transforms {T["basis_fft_s"]:.1f}, GEMM and window maxima '
+                       f'{T["gemm_s"]:.1f}, exact stage {T["exact_s"]:.1f}), templates/SVD excluded '
+                       f'({T["templates_s"] + T["svd_s"]:.1f} s)')
 
 
 def refinement_report(plans, streams, ds, outs, top, neighbours=8, steps=20):
@@ -240,49 +248,62 @@ class ProductionWorkload:
 
 
 class BatchedWorkload:
-    """Uneven chunks (64-1024 s in 16 s steps, 2 detectors, noise with lines), both banks, grouped by segment grid
-    (batched.BatchedSearch). Datasets and trigger lists are flattened over grids in grid order."""
+    """The batched configuration: per detector n_chunks chunks of 64-1024 s in 16 s steps (segment gaps), 2 detectors
+    at 2048 Hz of noise with the production lines drawn on the GPU, grouped on 512 s and 128 s segment grids; both
+    banks. The job is cut into nslice slices of n_chunks / nslice chunks per detector; one step filters one slice, so
+    nslice steps filter every chunk once. Warm-up steps filter a separate slice of warm chunks (other lengths and
+    noise), whose conditioned data also give each grid's SVD metric. Datasets and trigger lists of a step are
+    flattened over grids in grid order."""
 
-    def __init__(self, n_chunks, opts):
-        import torch
+    def __init__(self, n_chunks, opts, nslice=20, n_warm=10):
         import batched, workloads
-        chunks, self.banks = workloads.batched(n_chunks)
-        x_dev = [torch.from_numpy(st.x).cuda() for st, _ in chunks]
-        self.n_chunks = n_chunks
-        self.search = batched.BatchedSearch(chunks, self.banks, x_dev, opts)
-        self.plans = None
+        chunks, x_dev, self.banks = workloads.batched_device(n_chunks)
+        wch, wx, _ = workloads.batched_device(n_warm, seed=97, noise_seed=500000, n_circ=0, n_ecc=0)
+        self.n_chunks, self.nslice = n_chunks, nslice
+        self.slices = batched.slices(n_chunks, 2, nslice)
+        self.warm = list(range(len(chunks), len(chunks) + len(wch)))
+        self.search = batched.BatchedSearch(chunks + wch, self.banks, x_dev + wx, opts, ref=self.warm)
+        self._last_ds = None
 
-    def step(self, prec, check=False):
-        ds, outs, filters = self.search.step(prec, self_check=check)
+    def step(self, prec, k=None, check=False):
+        """One step: slice k (mod nslice), or the warm slice for k None."""
+        sel = self.warm if k is None else self.slices[k % self.nslice]
+        ds, outs, filters = self.search.step(prec, sel, self_check=check)
         g = self.search.grids
         self._last_ds = ds
-        flat_ds = [d for k in g for d in ds[k]]
-        flat = [[t for k in g for t in outs[k][b]] for b in range(len(self.banks))]
+        flat_ds = [d for x in g for d in ds[x]]
+        flat = [[t for x in g for t in outs[x][b]] for b in range(len(self.banks))]
         return flat_ds, filters, flat
 
     def ranks(self):
         return self.search.ranks()
 
     def describe(self, tile):
-        return ('batched: %d chunks per detector, 64-1024 s in 16 s steps, 2 detectors at 2048 Hz (noise with lines), '
+        return ('batched: %d chunks per detector of 64-1024 s in 16 s steps, 2 detectors at 2048 Hz (noise with lines), '
                 'grouped on 512 s and 128 s segment grids; 4,612 circular + 1,268 eccentric SPA templates, tiles of '
                 '%d, SVD worst lost power 1e-4, 16 chi-squared bins, SNR threshold 5.5, 1 s windows; BF16 GEMM, FP32 '
-                'exact stage' % (self.n_chunks, tile))
+                'exact stage; one step = one of %d slices of %d chunks per detector' %
+                (self.n_chunks, tile, self.nslice, self.n_chunks // self.nslice))
 
     def compare(self, a, b):
         # per bank, the lists of every dataset merged (keys are unique: each dataset has its own end times)
-        out = []
-        for i in range(len(a)):
-            out.append(compare_triggers(_merge(a[i]), _merge(b[i])))
-        return out
+        return [compare_triggers(_merge(a[i]), _merge(b[i])) for i in range(len(a))]
 
-    def direct_check(self, ds, outs):
-        # the float64 direct filter on the datasets of the first grid (the flat lists start with them; that grid's
-        # plans hold the dense templates on its frequency bins)
-        g = self.search.grids[0]
-        nd = len(self._last_ds[g])
-        part = [outs[b][:nd] for b in range(len(outs))]
-        return direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=12)
+    def direct_check(self, ds, outs, max_templates=12):
+        """The float64 direct filter on the last step's datasets, grid by grid (each grid's plans hold the dense
+        templates on its frequency bins)."""
+        errs, n = [], 0
+        off = 0
+        for g in self.search.grids:
+            nd = len(self._last_ds[g])
+            if nd:
+                part = [outs[b][off:off + nd] for b in range(len(outs))]
+                e, k = direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=max_templates)
+                if e is not None:
+                    errs.append(e)
+                n += k
+            off += nd
+        return (max(errs) if errs else None), n
 
 
 SG_LOCATIONS = ['mtotal>30:20-15,20-30,20-45,20-60,20-75,20-90,20-105,20-120']
@@ -394,28 +415,175 @@ def _merge(dicts):
     return {k: np.concatenate([d[k] for d in dicts]) for k in keys}
 
 
+def _kernel_rates(ms, nsteps, dt):
+    """HBM-bound kernels' rates from the meter: name -> achieved GB/s on their algorithmic bytes."""
+    out = {}
+    for name, (n_, t_, by) in ms.items():
+        if not name.startswith('gemm'):
+            out[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 1e9 / 8000.0,
+                             launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
+                             share_of_step=t_ / nsteps / dt)
+    return out
+
+
+def _diff_totals(diffs):
+    keys = ('n_a', 'n_b', 'only_a', 'only_b', 'windows_only_a', 'windows_only_b')
+    tot = {k: int(sum(d[k] for d in diffs)) for k in keys}
+    tot['same_sample_min'] = min(d['same_sample'] for d in diffs) if diffs else None
+    return tot
+
+
+def run_batched(a, opts, cfg, tuned, t_setup):
+    """The headline: the batched configuration, one slice per step, BF16; FP32 and FP8 forms timed on the first
+    a.fp32_slices slices; trigger lists compared; exact SNRs checked against the float64 direct filter."""
+    import numpy as np
+    import torch
+    import _dev
+    wl = BatchedWorkload(a.chunks, opts, nslice=a.slices)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    log('setup', round(setup_s, 1), 's; ranks', wl.ranks())
+    ntemp = sum(len(b) for b in wl.banks)
+    log('warmup')
+    for _ in range(a.warmup):
+        wl.step('bf16', None, check=True)
+    meter = _dev.METER
+    meter.reset(); meter.on = True
+    torch.cuda.synchronize()
+    an_s, per_step, outs_bf = 0.0, [], {}
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        t1 = time.perf_counter()
+        ds, filters, outs = wl.step('bf16', k)
+        an_s += wl.search.analysed_seconds(wl._last_ds)
+        torch.cuda.synchronize()
+        per_step.append(time.perf_counter() - t1)
+        if k < a.fp32_slices:
+            outs_bf[k] = outs
+        log(f'step {k + 1}/{a.steps} {per_step[-1] * 1e3:.0f} ms')
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    meter.on = False
+    ms = meter.stats()
+    dt = total / a.steps
+    value = ntemp * an_s / total
+    n_l, t_k, fl = ms['gemm_bf16']
+    res = dict(value=value, ms_per_step=dt * 1e3, analysed_s=an_s, setup_s=setup_s, ms=ms, n_launches=n_l,
+               t_gemm=t_k, flops=fl, hbm=_kernel_rates(ms, a.steps, dt), per_step_ms=[x * 1e3 for x in per_step],
+               ntemp=ntemp, wl=wl)
+    F = min(a.fp32_slices, a.steps)
+    forms = {}
+    for prec in ('fp32', 'fp8'):
+        if F <= 0 or (prec == 'fp8' and not a.fp8_steps):
+            continue
+        log(f'{prec} form on {F} slices')
+        wl.step(prec, None)
+        meter.reset(); meter.on = True
+        t_f, outs_f = 0.0, {}
+        for k in reversed(range(F)):            # slice 0 last: its datasets stay for the direct-filter check
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, _, outs_f[k] = wl.step(prec, k)
+            torch.cuda.synchronize()
+            t_f += time.perf_counter() - t1
+        meter.on = False
+        st = meter.stats()
+        n_, t_, f_ = st['gemm_' + prec]
+        err_f = nchk = None
+        if not a.no_check:
+            err_f, nchk = wl.direct_check(None, outs_f[0])
+        forms[prec] = dict(seconds=t_f, bf16_seconds=sum(per_step[:F]), speedup_bf16=t_f / sum(per_step[:F]),
+                           gemm_tflops=f_ / t_ / 1e12, gemm_frac=f_ / t_ / 1e12 / PEAK[prec],
+                           gemm_avg_launch_ms=t_ / max(n_, 1) * 1e3, snr_rel_err_vs_fp64=err_f, n_checked=nchk,
+                           outs=outs_f)
+    err_bf = nchk_bf = None
+    if not a.no_check and F > 0:
+        _, _, o0 = wl.step('bf16', 0, check=True)
+        err_bf, nchk_bf = wl.direct_check(None, o0)
+    res.update(forms=forms, err_bf=err_bf, nchk_bf=nchk_bf, outs_bf=outs_bf, F=F)
+    return res
+
+
+def run_production(a, opts, cfg):
+    """The production workload (2 detectors x 512 s, lines, glitches, 20 injections; both banks), reported beside
+    the headline: BF16 / FP32 / FP8 step times, the production cuts with consolidation, the overlap-save chunked forms,
+    the refinement of the loudest triggers and time slides."""
+    import torch
+    import _dev, workloads
+    t0 = time.perf_counter()
+    wl = ProductionWorkload(opts)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    wl.step('bf16', check=True)
+    ntemp = sum(len(b) for b in wl.banks)
+    meter = _dev.METER
+    res = dict(workload=wl.describe(a.tile), setup_s=setup, ranks=wl.ranks())
+    outs = {}
+    for prec, nst in (('bf16', a.production_steps), ('fp32', 1), ('fp8', a.fp8_steps)):
+        if nst <= 0:
+            continue
+        wl.step(prec)
+        meter.reset(); meter.on = True
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(nst):
+            ds, _, outs[prec] = wl.step(prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / nst
+        meter.on = False
+        st = meter.stats()
+        n_, t_, f_ = st['gemm_' + prec]
+        an_s = workloads.analysed_seconds(ds, 2048)
+        res[prec] = dict(value=ntemp * an_s / dt, ms_per_step=dt * 1e3, steps=nst, gemm_tflops=f_ / t_ / 1e12,
+                         gemm_frac=f_ / t_ / 1e12 / PEAK[prec], gemm_avg_launch_ms=t_ / max(n_, 1) * 1e3,
+                         gemm_share_of_step=t_ / nst / dt)
+        if prec == 'bf16':
+            res['hbm_kernels'] = _kernel_rates(st, nst, dt)
+        if not a.no_check:
+            res[prec]['snr_rel_err_vs_fp64'], res[prec]['n_checked'] = wl.direct_check(ds, outs[prec])
+    if 'fp32' in outs:
+        res['speedup_bf16_vs_fp32'] = res['fp32']['ms_per_step'] / res['bf16']['ms_per_step']
+        res['trigger_diff_bf16_vs_fp32'] = wl.compare(outs['bf16'], outs['fp32'])
+        if 'fp8' in outs:
+            res['trigger_diff_fp8_vs_fp32'] = wl.compare(outs['fp8'], outs['fp32'])
+    if a.consolidate_steps > 0:
+        log('production cuts with consolidation')
+        res['consolidation'] = consolidation_report(wl.step, a.consolidate_steps,
+                                                    ntemp * workloads.analysed_seconds(ds, 2048), outs['bf16'])
+    if a.chunked > 0:
+        log('overlap-save chunks')
+        res['chunked'] = chunked_report(wl, cfg, a.tile)
+    if a.refine_top > 0:
+        log('refinement of the loudest triggers')
+        res['refinement'] = refinement_report(wl.plans, wl.streams, ds, outs['bf16'], a.refine_top)
+    if a.slides > 0:
+        log('time slides')
+        res['time_slides'] = slides_report(ds, outs['bf16'], [b.name for b in wl.banks], K=a.slides)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
-    ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--fp32-steps', type=int, default=1)
-    ap.add_argument('--fp8-steps', type=int, default=1, help='timed steps of the FP8 form (production workload)')
-    ap.add_argument('--fp8-margin', type=float, default=0.04, help='candidate margin of the FP8 form')
-    ap.add_argument('--fp8-refine', type=int, default=3, help='exact-stage re-centring half width of the FP8 form')
+    ap.add_argument('--steps', type=int, default=20, help='timed steps; one step filters one slice of the batched '
+                    'workload, so --slices steps filter every chunk once')
+    ap.add_argument('--warmup', type=int, default=2, help='untimed steps on the warm chunks')
+    ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
+    ap.add_argument('--slices', type=int, default=20, help='slices of the batched workload (one per step)')
+    ap.add_argument('--fp32-slices', type=int, default=2, help='slices timed in the FP32 (and FP8) forms and compared '
+                    'with the BF16 form')
+    ap.add_argument('--fp8-steps', type=int, default=1, help='0: no FP8 form')
     ap.add_argument('--tile', type=int, default=512)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-check', action='store_true', help='skip the float64 direct-filter SNR check')
     ap.add_argument('--gemm-cfg', default=None, help='tiles_per_block[,variant]')
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
-    ap.add_argument('--timing', action='store_true', help='one extra step with stage timing (synchronised) to stderr')
-    ap.add_argument('--workload', choices=('production', 'batched'), default='production',
-                    help='production: 2 detectors x 512 s with lines, glitches and injections; batched: uneven chunks')
-    ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
-    ap.add_argument('--graphs', action='store_true', help='capture stage A of every batch in HIP graphs (replayed '
-                    'from the second step on; production workload)')
+    ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
+    ap.add_argument('--production', type=int, default=1, help='also run the production workload (2 detectors x 512 s '
+                    'with lines, glitches and injections) and its reports; 0: skip')
+    ap.add_argument('--production-steps', type=int, default=3, help='timed BF16 steps of the production workload')
     ap.add_argument('--refine-top', type=int, default=10000, help='refine this many of the loudest exact triggers '
-                    'off the bank after the timed steps (production workload; 0: skip)')
+                    'off the bank (production workload; 0: skip)')
     ap.add_argument('--consolidate-steps', type=int, default=1,
                     help='timed steps of the production cuts (SNR threshold 4, exact band from 5.5, keep the 30 '
                     'loudest per 1.072 s, exempt from SNR 5, sine-Gaussian chi-squared from newsnr 6) with host '
@@ -423,7 +591,7 @@ def main():
     ap.add_argument('--chunked', type=int, default=1, help='run the overlap-save chunked forms beside whole segments '
                     '(production workload; 0: skip)')
     ap.add_argument('--slides', type=int, default=500, help='time slides |k| <= K over the two detectors\' '
-                    'trigger lists after the timed steps (production workload; 0: skip)')
+                    'trigger lists (production workload; 0: skip)')
     a = ap.parse_args()
     if a.gpus != 1:
         print(f'bench.py runs on one GPU: there is no multi-GPU path here (asked for --gpus {a.gpus}); the '
@@ -434,7 +602,7 @@ def main():
     import _lib
     if not os.path.exists(_lib.LIB_PATH):
         _lib.build()
-    import _dev, fp_filter, workloads
+    import fp_filter
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     t_setup = time.perf_counter()
     import fp_gemm
@@ -446,154 +614,71 @@ def main():
         cfg, tuned = fp_gemm.pick_cfg(M=512, K=832, Tn=2 * 448 * 2048, dtype=torch.bfloat16)
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
-    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False,
-                                 refine=a.refine, graphs=a.graphs) for p in ('bf16', 'fp32', 'fp8')}
-    opts['fp8'].margin, opts['fp8'].refine = a.fp8_margin, a.fp8_refine
+    opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
+                                 segment_batch=a.segment_batch) for p in ('bf16', 'fp32', 'fp8')}
     import consolidate
     kl = consolidate.KeepLoudest(newsnr_threshold=4.0, interval=1.072, num=30, exempt_snr=5.0)
     for name, pre in (('cons', False), ('pre', True)):
         opts[name] = fp_filter.Options(tile=a.tile, precision='bf16', gemm_cfg=cfg, self_check=False,
                                        refine=a.refine, snr_threshold=4.0, exact_snr=5.5, consolidate=kl,
                                        pre_consolidate=pre, sgchisq_snr_threshold=6.0)
-    if a.workload == 'batched':
-        wl = BatchedWorkload(a.chunks, opts)
-    else:
-        wl = ProductionWorkload(opts)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t_setup
-    log('setup', round(setup_s, 1), 's; ranks', wl.ranks())
-    banks = wl.banks
-    cw = 2048
-    step = wl.step
-
-    log('warmup')
-    for _ in range(a.warmup):
-        ds, _, _ = step('bf16', check=True)
-    if a.timing:
-        for p in ('bf16', 'fp32'):
-            opts[p].timing = True
-            t0 = time.perf_counter()
-            _, fs, _ = step(p)
-            torch.cuda.synchronize()
-            log(f'timing {p}: step {time.perf_counter() - t0:.3f} s', json.dumps(
-                {k: round(sum(f.T[k] for f in fs), 4) for k in fs[0].T}))
-            opts[p].timing = False
-    an_s = workloads.analysed_seconds(ds, cw)
-    ntemp = sum(len(b) for b in banks)
-    plans = wl.plans
-    meter = _dev.METER
-    meter.reset(); meter.on = True
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        ds, filters, outs_bf = step('bf16')
-        log(f'step {k + 1}/{a.steps}')
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.steps
-    ms = meter.stats()
-    value = ntemp * an_s / dt
-    meter.on = False
-    def metered_plain(prec):
-        # kernels inside replayed graphs carry no events: one more step of the same work without graphs, metered
-        opts[prec].graphs = False
-        meter.reset(); meter.on = True
-        step(prec)
-        st = meter.stats()
-        meter.on = False
-        opts[prec].graphs = a.graphs
-        return st
-
-    if a.graphs:
-        ms = metered_plain('bf16')
-    n_l, t_k, fl = ms['gemm_bf16']
-    dt32 = outs_fp = None
-    if a.fp32_steps > 0:
-        log('fp32 form')
-        step('fp32')
-        meter.reset(); meter.on = True
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(a.fp32_steps):
-            _, f32f, outs_fp = step('fp32')
-        torch.cuda.synchronize()
-        dt32 = (time.perf_counter() - t1) / a.fp32_steps
-        meter.on = False
-        n32, t32, fl32 = (metered_plain('fp32') if a.graphs else meter.stats())['gemm_fp32']
-    dt8 = outs_f8 = None
-    if a.fp8_steps > 0 and a.workload == 'production':
-        # the FP8 e4m3 form of the same step (per-basis and per-template scales), reported beside the headline
-        log('fp8 form')
-        step('fp8')
-        meter.reset(); meter.on = True
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        for _ in range(a.fp8_steps):
-            _, _, outs_f8 = step('fp8')
-        torch.cuda.synchronize()
-        dt8 = (time.perf_counter() - t2) / a.fp8_steps
-        meter.on = False
-        n8, t8, fl8 = (metered_plain('fp8') if a.graphs else meter.stats())['gemm_fp8']
-    hbm = {}
-    msteps = 1 if a.graphs else a.steps
-    for name, (n_, t_, by) in ms.items():
-        if not name.startswith('gemm'):
-            hbm[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 1e9 / 8000.0,
-                             launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
-                             share_of_step=t_ / msteps / dt)
-    diffs = None
-    if outs_fp is not None:
-        diffs = wl.compare(outs_bf, outs_fp)
-    err = err32 = err8 = None
-    nchk = 0
-    if not a.no_check:
-        err, nchk = wl.direct_check(ds, outs_bf)
-        if outs_fp is not None:
-            err32, _ = wl.direct_check(ds, outs_fp)
-        if outs_f8 is not None:
-            err8, _ = wl.direct_check(ds, outs_f8)
+    b = run_batched(a, opts, cfg, tuned, t_setup)
+    wl = b['wl']
+    F = b['F']
+    diffs = {}
+    for prec, f in b['forms'].items():
+        diffs[prec] = [wl.compare(b['outs_bf'][k] if prec == 'fp32' else f['outs'][k],
+                                  b['forms']['fp32']['outs'][k]) for k in range(F)] if 'fp32' in b['forms'] else None
+    dflat = {p: [x for per in d for x in per] for p, d in diffs.items() if d is not None}
+    fp32 = b['forms'].get('fp32')
+    fp8 = b['forms'].get('fp8')
+    err_max = max([e for e in (b['err_bf'],) if e is not None], default=None)
+    # the whole job (every slice once) with the bank set-up (templates, SVD, autotune) counted: per-slice time
+    # from the timed steps, scaled to all slices
+    slices_s = sum(b['per_step_ms'][:a.slices]) / 1e3 * (a.slices / min(a.steps, a.slices))
+    an_job = b['analysed_s'] * a.slices / a.steps
     res = dict(
-        metric=METRIC, value=value, unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
-        ms_per_step=dt * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
-        data='synthetic',
-        config=dict(workload=wl.describe(a.tile), templates=ntemp, analysed_s=an_s, tile=a.tile,
-                    ranks=wl.ranks(), setup_s=round(setup_s, 1), gemm_cfg=list(cfg) if cfg else None,
+        metric=METRIC, value=b['value'], unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
+        ms_per_step=b['ms_per_step'], higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
+        data='synthetic (noise drawn on the GPU, seeded)',
+        config=dict(workload=wl.describe(a.tile), templates=b['ntemp'], analysed_s=b['analysed_s'],
+                    slices=a.slices, tile=a.tile, segment_batch=a.segment_batch, ranks=wl.ranks(),
+                    setup_s=round(b['setup_s'], 1),
+                    value_with_setup=b['ntemp'] * an_job / (slices_s + b['setup_s']),
+                    speedup_bf16_vs_fp32=None if fp32 is None else fp32['speedup_bf16'],
+                    speedup_fp32_slices=F,
+                    snr_rel_err_vs_fp64=err_max, snr_rel_err_n_checked=b['nchk_bf'],
+                    snr_rel_err_vs_fp64_fp32=None if fp32 is None else fp32['snr_rel_err_vs_fp64'],
+                    trigger_diff_bf16_vs_fp32=_diff_totals(dflat['fp32']) if 'fp32' in dflat else None,
+                    gemm_cfg=list(cfg) if cfg else None,
                     autotune_ms={k: round(v * 1e3, 3) for k, v in (tuned or {}).items() if isinstance(v, float)}),
-        roofline=dict(bound='mfma', achieved=fl / t_k / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
-                      frac=fl / t_k / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
-                      kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=n_l,
-                      avg_launch_ms=t_k / max(n_l, 1) * 1e3, gemm_share_of_step=t_k / msteps / dt),
-        fp32=None if dt32 is None else dict(
-            value=ntemp * an_s / dt32, ms_per_step=dt32 * 1e3, gemm_tflops=fl32 / t32 / 1e12,
-            gemm_frac=fl32 / t32 / 1e12 / PEAK['fp32'], gemm_share_of_step=t32 / dt32),
-        hbm_kernels=hbm,
-        speedup_bf16_vs_fp32=None if dt32 is None else dt32 / dt,
-        snr_rel_err_vs_fp64=dict(bf16=err, fp32=err32, n_checked=nchk),
-        trigger_diff_bf16_vs_fp32=diffs,
-        fp8=None if dt8 is None else dict(
-            value=ntemp * an_s / dt8, ms_per_step=dt8 * 1e3, gemm_tflops=fl8 / t8 / 1e12,
-            gemm_frac=fl8 / t8 / 1e12 / PEAK['fp8'], gemm_avg_launch_ms=t8 / max(n8, 1) * 1e3,
-            speedup_vs_fp32=None if dt32 is None else dt32 / dt8, snr_rel_err_vs_fp64=err8,
-            trigger_diff_fp8_vs_fp32=None if outs_fp is None else wl.compare(outs_f8, outs_fp)),
+        roofline=dict(bound='mfma', achieved=b['flops'] / b['t_gemm'] / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
+                      frac=b['flops'] / b['t_gemm'] / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
+                      kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=b['n_launches'],
+                      avg_launch_ms=b['t_gemm'] / max(b['n_launches'], 1) * 1e3,
+                      gemm_share_of_step=b['t_gemm'] / a.steps / (b['ms_per_step'] / 1e3)),
+        per_step_ms=[round(x, 1) for x in b['per_step_ms']],
+        hbm_kernels=b['hbm'],
+        fp32=None if fp32 is None else {k: v for k, v in fp32.items() if k != 'outs'},
+        fp8=None if fp8 is None else dict({k: v for k, v in fp8.items() if k != 'outs'},
+                                          trigger_diff_fp8_vs_fp32=_diff_totals(dflat['fp8']) if 'fp8' in dflat
+                                          else None),
+        trigger_diff_bf16_vs_fp32=dflat.get('fp32'),
     )
-    if a.workload == 'production' and a.consolidate_steps > 0:
-        log('production cuts with consolidation')
-        res['consolidation'] = consolidation_report(step, a.consolidate_steps, ntemp * an_s, outs_bf)
-    if a.workload == 'production' and a.chunked > 0:
-        log('overlap-save chunks')
-        res['chunked'] = chunked_report(wl, cfg, a.tile)
-    if a.workload == 'production' and a.refine_top > 0:
-        log('refinement of the loudest triggers')
-        res['refinement'] = refinement_report(wl.plans, wl.streams, ds, outs_bf, a.refine_top)
-    if a.workload == 'production' and a.slides > 0:
-        log('time slides')
-        res['time_slides'] = slides_report(ds, outs_bf, [b.name for b in banks], K=a.slides)
+    if a.production > 0:
+        log('production workload')
+        res['production'] = run_production(a, opts, cfg)
     if not a.no_cpu:
         log('cpu baseline')
-        streams, banks_p, cond = workloads.production()
-        res['cpu_baseline'] = cpu_baseline(streams, banks_p, cond)
+        import workloads
+        # one chunk of the first slice on the 512 s grid, H1
+        st, cond, x = next((st, c, x) for i in wl.slices[0] for st, c, x in [wl.search.chunks[i]]
+                           if c is not None and c.seg_len == 512 and st.ifo == 'H1')
+        res['cpu_baseline'] = cpu_baseline(x.cpu().numpy(), st.sr, st.gps_start, cond, wl.banks[0],
+                                           f'H1 chunk of {x.numel() / st.sr:.0f} s from the first slice')
     else:
         res['cpu_baseline'] = None
-    print(json.dumps(res), flush=True)
+    print(json.dumps(res, default=float), flush=True)
 
 
 if __name__ == '__main__':

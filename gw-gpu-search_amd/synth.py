@@ -52,6 +52,33 @@ def colored_noise(n, sr, seed, lines=()):
     return np.fft.irfft(X * sr, n)
 
 
+def colored_noise_device(n, sr, seed, lines=()):
+    """colored_noise generated on the GPU (torch's seeded Philox generator instead of numpy's; the same spectrum and
+    conventions), float64 tensor of n samples. For workloads too large to draw on the host in reasonable time."""
+    import torch
+    g = torch.Generator(device='cuda')
+    g.manual_seed(int(seed))
+    df = sr / n
+    flen = n // 2 + 1
+    S = torch.from_numpy(analytic_psd(np.arange(flen) * df, lines)).cuda()
+    sd = torch.sqrt(S / (4 * df))
+    re = torch.randn(flen, generator=g, device='cuda', dtype=torch.float64)
+    im = torch.randn(flen, generator=g, device='cuda', dtype=torch.float64)
+    X = torch.complex(sd * re, sd * im)
+    X[0] = 0.0
+    if n % 2 == 0:
+        X[-1] = X[-1].real * np.sqrt(2.0)
+    return torch.fft.irfft(X * sr, n)
+
+
+def device_stream(ifo, duration, sr, seed, lines=(), gps_start=1.0e9):
+    """A noise-only Stream whose strain lives on the GPU: (Stream with an empty host array, float32 tensor of
+    strain x DYN_RANGE_FAC)."""
+    n = int(round(duration * sr))
+    x = (colored_noise_device(n, sr, seed, lines) * DYN_RANGE_FAC).to(dtype=__import__('torch').float32)
+    return Stream(ifo, np.zeros(0, np.float32), float(sr), gps_start), x
+
+
 def mass_params(m1, m2):
     M = (m1 + m2) * MSUN_S
     eta = m1 * m2 / (m1 + m2) ** 2

@@ -85,6 +85,28 @@ def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4
     return chunks, banks
 
 
+def batched_device(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4612, n_ecc=1268,
+                   noise_seed=0):
+    """batched() with the chunks' noise drawn on the GPU (synth.device_stream): the same chunk lengths, GPS layout and
+    banks; per chunk the noise seed is noise_seed + 1000 (d + 1) + c. Returns (chunks, x_dev, banks)."""
+    rng = np.random.default_rng(seed)
+    lens = lmin + step * rng.integers(0, (lmax - lmin) // step + 1, (ndet, n_chunks))
+    lines = synth.default_lines()
+    chunks, x_dev = [], []
+    t0 = 1.0e9
+    for c in range(n_chunks):
+        for d, ifo in enumerate(('H1', 'L1')[:ndet]):
+            L = int(lens[d, c])
+            st, x = synth.device_stream(ifo, L, SR, noise_seed + 1000 * (d + 1) + c, lines=lines, gps_start=t0)
+            chunks.append((st, chunk_conditioning(L)))
+            x_dev.append(x)
+        t0 += int(lens[:, c].max()) + 8
+    banks = [synth.circular_bank(n_circ, seed=SEEDS['bank'])]
+    if n_ecc:
+        banks.append(synth.eccentric_bank(n_ecc, seed=SEEDS['bank_ecc']))
+    return chunks, x_dev, banks
+
+
 def condition_streams(streams, cond: Conditioning, x_dev=None):
     """GPU conditioning of each stream (autogating first when configured). x_dev: optional list of float32 strain
     tensors already on the GPU (they are gated in place)."""

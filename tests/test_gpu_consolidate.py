@@ -117,6 +117,24 @@ def test_pre_keep_crowded_window(gpu):
     assert np.bincount(wt).min() > 4096 and ((out > cut) & rb).sum() == 30 * len(np.bincount(wt))
 
 
+def test_pre_keep_staged_window(gpu):
+    """A keep-loudest window of 1,025 to 4,096 ranked candidates (above the rank kernel's LDS counting path of 1,024
+    keys, within its staged max-reduction path): 6,000 rows over one output window."""
+    import consolidate, fp_filter
+    rng = np.random.default_rng(12)
+    gps = 1380077576.0
+    segs = [dict(gps=gps, seg_start=0, a0=0, nw=1, gates=[], floor=None)]
+    vw, iw = _batch(rng, 6000, segs, 0.95)
+    kl = consolidate.KeepLoudest(4.0, 1.072, 30, None)
+    cut = fp_filter.candidate_cut(4.0, 0.04)
+    out = _run(vw, iw, segs, cut, 4.0, 5.5, 0.04, kl)
+    exp = _expected(vw, iw, segs, cut, 4.0, 5.5, 0.04, kl)
+    np.testing.assert_array_equal(out > cut, exp)
+    rb = (vw > cut) & (np.sqrt(vw) > 4.0) & (np.sqrt(vw) < np.float32(5.28))
+    cnt = np.bincount(oc.window_of(gps + iw[rb] / SR, gps, SR, 1.072))
+    assert 1024 < cnt.max() <= 4096 and ((out > cut) & rb).sum() == 30 * (cnt > 0).sum()
+
+
 @pytest.fixture(scope='module')
 def loud_data(gpu):
     """One detector, 64 s of seeded coloured noise with two injections, 64 templates: at SNR threshold 4 most
