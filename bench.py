@@ -265,15 +265,24 @@ class BatchedWorkload:
         self.search = batched.BatchedSearch(chunks + wch, self.banks, x_dev + wx, opts, ref=self.warm)
         self._last_ds = None
 
+    def _flat(self, ds, outs):
+        g = self.search.grids
+        return [d for x in g for d in ds[x]], [[t for x in g for t in outs[x][b]] for b in range(len(self.banks))]
+
     def step(self, prec, k=None, check=False):
-        """One step: slice k (mod nslice), or the warm slice for k None."""
+        """One step: slice k (mod nslice), or the warm slice for k None. Returns (datasets, filters, trigger lists)
+        flattened over grids."""
         sel = self.warm if k is None else self.slices[k % self.nslice]
         ds, outs, filters = self.search.step(prec, sel, self_check=check)
-        g = self.search.grids
         self._last_ds = ds
-        flat_ds = [d for x in g for d in ds[x]]
-        flat = [[t for x in g for t in outs[x][b]] for b in range(len(self.banks))]
-        return flat_ds, filters, flat
+        fd, fo = self._flat(ds, outs)
+        return fd, filters, fo
+
+    def run(self, prec, ks, check=False):
+        """Slices ks in one pipeline (the next slice conditioned while the previous one's last batches run). Returns
+        per slice (grouped datasets, filters, trigger lists flattened over grids)."""
+        res = self.search.run(prec, [self.slices[k % self.nslice] for k in ks], self_check=check)
+        return [(ds, filters, self._flat(ds, outs)[1]) for ds, outs, filters in res]
 
     def ranks(self):
         return self.search.ranks()
@@ -290,15 +299,16 @@ class BatchedWorkload:
         return [compare_triggers(_merge(a[i]), _merge(b[i])) for i in range(len(a))]
 
     def direct_check(self, ds, outs, max_templates=12):
-        """The float64 direct filter on the last step's datasets, grid by grid (each grid's plans hold the dense
-        templates on its frequency bins)."""
+        """The float64 direct filter on a step's datasets (grouped by grid; default: the last step's), grid by grid
+        (each grid's plans hold the dense templates on its frequency bins)."""
+        ds = self._last_ds if ds is None else ds
         errs, n = [], 0
         off = 0
         for g in self.search.grids:
-            nd = len(self._last_ds[g])
+            nd = len(ds[g])
             if nd:
                 part = [outs[b][off:off + nd] for b in range(len(outs))]
-                e, k = direct_snr_check(self._last_ds[g], self.search.plans[g], part, max_templates=max_templates)
+                e, k = direct_snr_check(ds[g], self.search.plans[g], part, max_templates=max_templates)
                 if e is not None:
                     errs.append(e)
                 n += k
@@ -450,56 +460,56 @@ def run_batched(a, opts, cfg, tuned, t_setup):
     meter = _dev.METER
     meter.reset(); meter.on = True
     torch.cuda.synchronize()
-    an_s, per_step, outs_bf = 0.0, [], {}
+    outs_bf = {}
     t0 = time.perf_counter()
-    for k in range(a.steps):
-        t1 = time.perf_counter()
-        ds, filters, outs = wl.step('bf16', k)
-        an_s += wl.search.analysed_seconds(wl._last_ds)
-        torch.cuda.synchronize()
-        per_step.append(time.perf_counter() - t1)
-        if k < a.fp32_slices:
-            outs_bf[k] = outs
-        log(f'step {k + 1}/{a.steps} {per_step[-1] * 1e3:.0f} ms')
+    res_steps = wl.run('bf16', list(range(a.steps)))
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     meter.on = False
     ms = meter.stats()
+    an_s = sum(wl.search.analysed_seconds(ds) for ds, _, _ in res_steps)
+    for k, (_, _, outs) in enumerate(res_steps):
+        if k < a.fp32_slices:
+            outs_bf[k] = outs
+    log(f'{a.steps} steps in {total:.3f} s')
+    del res_steps
     dt = total / a.steps
     value = ntemp * an_s / total
     n_l, t_k, fl = ms['gemm_bf16']
     res = dict(value=value, ms_per_step=dt * 1e3, analysed_s=an_s, setup_s=setup_s, ms=ms, n_launches=n_l,
-               t_gemm=t_k, flops=fl, hbm=_kernel_rates(ms, a.steps, dt), per_step_ms=[x * 1e3 for x in per_step],
-               ntemp=ntemp, wl=wl)
+               t_gemm=t_k, flops=fl, hbm=_kernel_rates(ms, a.steps, dt), ntemp=ntemp, wl=wl)
     F = min(a.fp32_slices, a.steps)
     forms = {}
-    for prec in ('fp32', 'fp8'):
+    t_bf = None
+    for prec in ('bf16', 'fp32', 'fp8'):
         if F <= 0 or (prec == 'fp8' and not a.fp8_steps):
             continue
         log(f'{prec} form on {F} slices')
         wl.step(prec, None)
         meter.reset(); meter.on = True
-        t_f, outs_f = 0.0, {}
-        for k in reversed(range(F)):            # slice 0 last: its datasets stay for the direct-filter check
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            _, _, outs_f[k] = wl.step(prec, k)
-            torch.cuda.synchronize()
-            t_f += time.perf_counter() - t1
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rs = wl.run(prec, list(range(F)))
+        torch.cuda.synchronize()
+        t_f = time.perf_counter() - t1
         meter.on = False
+        if prec == 'bf16':                       # the same slices in the BF16 form, timed alike
+            t_bf = t_f
+            continue
+        outs_f = {k: rs[k][2] for k in range(F)}
         st = meter.stats()
         n_, t_, f_ = st['gemm_' + prec]
         err_f = nchk = None
         if not a.no_check:
-            err_f, nchk = wl.direct_check(None, outs_f[0])
-        forms[prec] = dict(seconds=t_f, bf16_seconds=sum(per_step[:F]), speedup_bf16=t_f / sum(per_step[:F]),
+            err_f, nchk = wl.direct_check(rs[0][0], outs_f[0])
+        forms[prec] = dict(seconds=t_f, bf16_seconds=t_bf, speedup_bf16=t_f / t_bf,
                            gemm_tflops=f_ / t_ / 1e12, gemm_frac=f_ / t_ / 1e12 / PEAK[prec],
                            gemm_avg_launch_ms=t_ / max(n_, 1) * 1e3, snr_rel_err_vs_fp64=err_f, n_checked=nchk,
                            outs=outs_f)
     err_bf = nchk_bf = None
     if not a.no_check and F > 0:
-        _, _, o0 = wl.step('bf16', 0, check=True)
-        err_bf, nchk_bf = wl.direct_check(None, o0)
+        ds0, _, o0 = wl.run('bf16', [0], check=True)[0]
+        err_bf, nchk_bf = wl.direct_check(ds0, o0)
     res.update(forms=forms, err_bf=err_bf, nchk_bf=nchk_bf, outs_bf=outs_bf, F=F)
     return res
 
@@ -579,6 +589,7 @@ def main():
     ap.add_argument('--gemm-cfg', default=None, help='tiles_per_block[,variant]')
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
     ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
+    ap.add_argument('--overlap', type=int, default=0, help='1: basis transforms on their own stream beside the GEMM')
     ap.add_argument('--production', type=int, default=1, help='also run the production workload (2 detectors x 512 s '
                     'with lines, glitches and injections) and its reports; 0: skip')
     ap.add_argument('--production-steps', type=int, default=3, help='timed BF16 steps of the production workload')
@@ -615,7 +626,7 @@ def main():
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
-                                 segment_batch=a.segment_batch) for p in ('bf16', 'fp32', 'fp8')}
+                                 segment_batch=a.segment_batch, overlap=bool(a.overlap)) for p in ('bf16', 'fp32', 'fp8')}
     import consolidate
     kl = consolidate.KeepLoudest(newsnr_threshold=4.0, interval=1.072, num=30, exempt_snr=5.0)
     for name, pre in (('cons', False), ('pre', True)):
@@ -635,7 +646,7 @@ def main():
     err_max = max([e for e in (b['err_bf'],) if e is not None], default=None)
     # the whole job (every slice once) with the bank set-up (templates, SVD, autotune) counted: per-slice time
     # from the timed steps, scaled to all slices
-    slices_s = sum(b['per_step_ms'][:a.slices]) / 1e3 * (a.slices / min(a.steps, a.slices))
+    slices_s = b['ms_per_step'] / 1e3 * a.slices
     an_job = b['analysed_s'] * a.slices / a.steps
     res = dict(
         metric=METRIC, value=b['value'], unit='template*s/GPU-s', n_gpus=1, steps=a.steps, warmup=a.warmup,
@@ -657,7 +668,6 @@ def main():
                       kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=b['n_launches'],
                       avg_launch_ms=b['t_gemm'] / max(b['n_launches'], 1) * 1e3,
                       gemm_share_of_step=b['t_gemm'] / a.steps / (b['ms_per_step'] / 1e3)),
-        per_step_ms=[round(x, 1) for x in b['per_step_ms']],
         hbm_kernels=b['hbm'],
         fp32=None if fp32 is None else {k: v for k, v in fp32.items() if k != 'outs'},
         fp8=None if fp8 is None else dict({k: v for k, v in fp8.items() if k != 'outs'},

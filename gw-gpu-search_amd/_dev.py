@@ -64,6 +64,18 @@ def ready_stream():
     return _READY
 
 
+_BASIS = None
+
+
+def basis_stream():
+    """The stream the filter's basis transforms run on (default priority), so that a batch's transform overlaps the
+    previous batch's GEMM (instruction- and memory-bound work beside matrix-core work)."""
+    global _BASIS
+    if _BASIS is None:
+        _BASIS = torch.cuda.Stream()
+    return _BASIS
+
+
 def to_dev(a, dtype=None):
     """Host array -> device tensor without a host-side wait: staged in pinned memory and copied on the current
     stream (the pinned block is kept by torch's host allocator until the copy has run)."""

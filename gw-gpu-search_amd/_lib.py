@@ -42,6 +42,8 @@ SIGNATURES = {
     'gwgs_chisq_points_workspace': [_i, _i],
     'gwgs_chisq_peaks_per_group': [],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],
+    'gwgs_edges_fused_multi': [_p, _l, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p],
+    'gwgs_chisq_points_multi': [_p, _l, _p, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_form_q': [_p, _i, _l, _p, _l, _p, _p],
     'gwgs_pack_zt': [_i, _p, _l, _i, _l, _l, _l, _p, _p, _p],
     'gwgs_stack_a': [_i, _p, _i, _i, _p, _p, _i, _i, _p, _p, _p],

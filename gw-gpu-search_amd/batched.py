@@ -38,31 +38,63 @@ class BatchedSearch:
                                                 name=f'{b.name}@{g}s') for b in banks]
 
     def condition(self, sel):
-        """Condition the selected chunks (a fresh copy of each strain); datasets grouped by segment grid, in chunk
-        order."""
+        """Condition the selected chunks; datasets grouped by segment grid, in chunk order. Chunks without autogating
+        are conditioned together per grid (gpu_prep.condition_many: a few batched launches, no host synchronisation;
+        the strain is read, not modified); a chunk with autogating is gated on a fresh copy of its strain and
+        conditioned alone."""
+        import gpu_prep
         out = {g: [] for g in self.grids}
+        batch = {g: [] for g in self.grids}
         for i in sel:
             st, c, x = self.chunks[i]
-            if c is not None:
+            if c is None:
+                continue
+            if c.autogating_threshold is not None:
                 out[c.seg_len] += workloads.condition_streams([st], c, [x.clone()])
+            else:
+                batch[c.seg_len].append((st, c, x))
+        for g, items in batch.items():
+            if not items:
+                continue
+            c0 = items[0][1]
+            assert all((c.start_pad, c.end_pad, c.psd_seg_len, c.psd_stride, c.psd_inv_len) ==
+                       (c0.start_pad, c0.end_pad, c0.psd_seg_len, c0.psd_stride, c0.psd_inv_len) for _, c, _ in items)
+            out[g] += gpu_prep.condition_many([x for _, _, x in items], items[0][0].sr,
+                                              [st.gps_start for st, _, _ in items], g, c0.start_pad, c0.end_pad,
+                                              c0.psd_seg_len, c0.psd_stride, [c.psd_nseg for _, c, _ in items],
+                                              c0.psd_inv_len, workloads.FLOW, tags=[st.ifo for st, _, _ in items])
+        return out
+
+    def _jobs(self, prec, sels, self_check, rec):
+        """The (Filter, plans) of every (slice, grid) in order, set up lazily: a slice is conditioned when the
+        pipeline reaches it, while the previous slice's last batches are still in flight."""
+        o = self.opts[prec]
+        if self_check:
+            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
+        for k, sel in enumerate(sels):
+            ds = self.condition(sel)
+            for g in self.grids:
+                if ds[g]:
+                    f = fp_filter.Filter(ds[g], o)
+                    rec.append((k, g, ds[g], f))
+                    yield f, self.plans[g]
+
+    def run(self, prec, sels, self_check=False):
+        """Several slices (lists of chunk indices) in one software pipeline with both banks. Returns per slice
+        ({grid: datasets}, {grid: [per bank [per dataset trigger dict]]}, [Filter])."""
+        rec = []
+        res = fp_filter.run_jobs(self._jobs(prec, sels, self_check, rec))
+        out = [({g: [] for g in self.grids}, {g: [[] for _ in self.banks] for g in self.grids}, []) for _ in sels]
+        for (k, g, dsg, f), tr in zip(rec, res):
+            out[k][0][g] = dsg
+            out[k][1][g] = [[t.to_dict(gates=d.gates) for t, d in zip(tb, dsg)] for tb in tr]
+            out[k][2].append(f)
         return out
 
     def step(self, prec, sel=None, self_check=False):
         """One pass over the selected chunks (default: all) with both banks. Returns ({grid: datasets}, {grid: [per
         bank [per dataset trigger dict]]}, [Filter])."""
-        ds = self.condition(range(len(self.chunks)) if sel is None else sel)
-        o = self.opts[prec]
-        if self_check:
-            o = fp_filter.Options(**{**o.__dict__, 'self_check': True})
-        outs, filters = {}, []
-        for g in self.grids:
-            if not ds[g]:
-                outs[g] = [[] for _ in self.banks]
-                continue
-            f = fp_filter.Filter(ds[g], o)
-            outs[g] = [[t.to_dict(gates=d.gates) for t, d in zip(tr, ds[g])] for tr in f.run_banks(self.plans[g])]
-            filters.append(f)
-        return ds, outs, filters
+        return self.run(prec, [range(len(self.chunks)) if sel is None else sel], self_check)[0]
 
     def analysed_seconds(self, ds, cw=2048):
         return sum(workloads.analysed_seconds(ds[g], cw) for g in self.grids)

@@ -34,6 +34,25 @@ def edges_rows_dev(rows_dev_idx, nb_, kmin, k1, Hd, ips):
     return out
 
 
+def edges_rows_dev_multi(rows, rseg, nb_, kmin, k1, Hd, ip_tab):
+    """edges_rows_dev over requests of several segments: request g takes row rows[g] of Hd and 1/psd from the
+    segment rseg[g], whose float32 row pointer is ip_tab[rseg[g]] (int64 host array of device addresses). Enqueued
+    without waiting; returns the (n, B + 1) int32 device tensor."""
+    n = len(rows)
+    B = int(np.max(nb_)) if n else 0
+    out = torch.empty((n, B + 1), dtype=torch.int32, device=Hd.device)
+    if n == 0:
+        return out
+    assert Hd.dtype == torch.complex64 and Hd.is_contiguous()
+    r, sg, kk, nb, tab = _dev.to_dev_many(*(np.asarray(x, np.int32) for x in (rows, rseg, k1, nb_)),
+                                          np.asarray(ip_tab, np.int64))
+    e0 = _dev.METER.start()
+    _lib.call('gwgs_edges_fused_multi', Hd.data_ptr(), Hd.shape[1], r.data_ptr(), tab.data_ptr(), sg.data_ptr(),
+              kk.data_ptr(), nb.data_ptr(), B + 1, int(kmin), n, out.data_ptr(), _dev.stream())
+    _dev.METER.stop('edges', e0, 12.0 * float(np.asarray(k1, np.int64).sum()))
+    return out
+
+
 def _edges_rows(rows_dev_idx, nb_, kmin, k1, Hd, ips):
     out = edges_rows_dev(rows_dev_idx, nb_, kmin, k1, Hd, ips)
     return out.cpu().numpy().astype(np.int64)

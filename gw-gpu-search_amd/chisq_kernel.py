@@ -78,6 +78,34 @@ def launch_sorted(q, rows, times, E_dev, e_index, want_z=False, d=None, n_time=N
     return _launch(q, np.asarray(times, np.int64), grow, gstart, gcount, E_dev[gi], want_z, d, N, span)
 
 
+def launch_sorted_multi(h, keys, n_rows, times, E_dev, e_index, d_tab, n_time, row_span=None):
+    """launch_sorted in fused mode over peaks of several segments: keys = segment n_rows + row, sorted (any time
+    order within a key); the template rows are h's, the data row of segment s is at d_tab[s] (int64 host array of
+    device addresses of complex64 rows). Returns device (out, zo) in the given peak order."""
+    assert h.dtype == torch.complex64 and h.is_contiguous()
+    keys = np.asarray(keys, np.int64)
+    gkey, gstart, gcount = groups_of_sorted(keys)
+    npk = len(keys)
+    out = torch.empty(npk, dtype=torch.float32, device=h.device)
+    zo = torch.empty(npk, dtype=torch.complex64, device=h.device)
+    if len(gkey) == 0:
+        return out, zo
+    gi = np.asarray(e_index, np.int64)[gstart]
+    tdev, g_row, g_seg, g_start, g_cnt, g_i, tab = _dev.to_dev_many(
+        np.asarray(times, np.int64), (gkey % n_rows).astype(np.int32), (gkey // n_rows).astype(np.int32),
+        gstart.astype(np.int32), gcount.astype(np.int32), gi, np.asarray(d_tab, np.int64))
+    ge = E_dev[g_i].contiguous()
+    nwork = int(_lib.load().gwgs_chisq_points_workspace(len(gstart), ge.shape[1]))
+    work = torch.empty(max(nwork, 8), dtype=torch.uint8, device=h.device)
+    span = 0.0 if row_span is None else float(np.asarray(row_span)[gkey % n_rows].sum())
+    e0 = _dev.METER.start()
+    _lib.call('gwgs_chisq_points_multi', h.data_ptr(), h.shape[1], tab.data_ptr(), g_seg.data_ptr(), int(n_time),
+              g_row.data_ptr(), g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(), ge.shape[1],
+              len(gstart), out.data_ptr(), zo.data_ptr(), work.data_ptr(), _dev.stream())
+    _dev.METER.stop('chisq', e0, span * 16)
+    return out, zo
+
+
 def bin_power_sums(q, rows, times, edges_per_peak, want_z=False, d=None, n_time=None):
     """sum_j |Z_j|^2 for each peak; q is (nrows, >= N or flen) complex64 on the GPU. rows, times: int arrays per
     peak; edges_per_peak: (npeaks, nedge) int bin edges into q's row (bins with e_j >= e_j+1 are empty).

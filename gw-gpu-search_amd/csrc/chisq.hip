@@ -31,13 +31,14 @@ __device__ __forceinline__ float2 phasor(const unsigned long long k, const unsig
 // per step; a wave butterfly sum gives Z_j of every peak, written to zp[(g nbin + j) P + p]; chisq_finish_k sums
 // over j. Wave i = g nbin + j: a group's bins run on neighbouring waves (they share the row in L2). FUSE: the summand
 // is conj(h_k) d_k formed here.
-template <bool FUSE>
+template <bool FUSE, bool MULTI = false>
 __global__ void __launch_bounds__(NT) chisq_bins_k(const float2* __restrict__ q, const long ldq,
                                                    const float2* __restrict__ dv, const unsigned long long N,
                                                    const int* __restrict__ grow, const int* __restrict__ gstart,
                                                    const int* __restrict__ gcount, const long long* __restrict__ ptime,
                                                    const int* __restrict__ edges, const int nedge, const int ngroups,
-                                                   float2* __restrict__ zp)
+                                                   float2* __restrict__ zp, const float2* const* __restrict__ dtab,
+                                                   const int* __restrict__ gseg)
 {
     const int nbin = nedge - 1;
     const int lane = threadIdx.x & 63;
@@ -56,6 +57,7 @@ __global__ void __launch_bounds__(NT) chisq_bins_k(const float2* __restrict__ q,
     }
     const unsigned long long mask = (N & (N - 1)) == 0 && N <= (1ull << 24) ? N - 1 : 0;
     const float2* qr = q + (size_t)row * ldq;
+    if (MULTI) dv = dtab[gseg[g]];            // the group's segment (its data row)
     unsigned long long t[P];
     float2 rot[P];
 #pragma unroll
@@ -168,15 +170,19 @@ __device__ __forceinline__ double block_exscan_f64(const double x, double* sh, d
 // just short).
 constexpr int EMAXCH = 4096;
 
+template <bool MULTI = false>
 __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h, const long ldh,
                                                     const int* __restrict__ rows, const float* __restrict__ ips,
                                                     const int* __restrict__ k1s, const int* __restrict__ nbs,
-                                                    const int nedge, const int kmin, int* __restrict__ out)
+                                                    const int nedge, const int kmin, int* __restrict__ out,
+                                                    const float* const* __restrict__ iptab,
+                                                    const int* __restrict__ rseg)
 {
     __shared__ double ex[EMAXCH + 1];
     __shared__ double sh[NT / 64];
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (MULTI) ips = iptab[rseg[g]];          // the request's segment (its 1/psd row)
     const float2* __restrict__ hr = h + (size_t)rows[g] * ldh;
     const long k1 = k1s[g];
     const int nb = nbs[g];
@@ -279,10 +285,40 @@ GWGS_API int gwgs_chisq_points(const void* q, long ldq, const void* d, long long
     const dim3 grid((unsigned)(((long)ngroups * (nedge - 1) + NT / 64 - 1) / (NT / 64)));
     if (d)
         hipLaunchKernelGGL(chisq_bins_k<true>, grid, dim3(NT), 0, s, (const float2*)q, ldq, (const float2*)d,
-                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work);
+                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work,
+                           nullptr, nullptr);
     else
         hipLaunchKernelGGL(chisq_bins_k<false>, grid, dim3(NT), 0, s, (const float2*)q, ldq, nullptr,
-                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work);
+                           (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work,
+                           nullptr, nullptr);
+    hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const float2*)work, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
+    return gwgs_status();
+}
+
+// gwgs_chisq_points in fused mode over groups of several segments: group g reads its data row from
+// d_tab[gseg[g]] (device table of complex64 pointers, every row >= the largest edge long); everything else as
+// gwgs_chisq_points, whose workspace size applies.
+GWGS_API int gwgs_chisq_points_multi(const void* h, long ldh, const void* const* d_tab, const int* gseg, long long N,
+                                     const int* grow, const int* gstart, const int* gcount, const long long* ptime,
+                                     const int* edges, int nedge, int ngroups, float* out, void* zout, void* work,
+                                     void* stream)
+{
+    GWGS_CHECK(h && d_tab && gseg && grow && gstart && gcount && ptime && edges && out && N > 0 && nedge >= 1 &&
+               ngroups >= 0, GWGS_EINVAL);
+    if (ngroups == 0) return GWGS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (nedge == 1) {
+        hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                           (const float2*)nullptr, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
+        return gwgs_status();
+    }
+    GWGS_CHECK(work != nullptr, GWGS_EINVAL);
+    GWGS_CHECK((long)ngroups * (nedge - 1) < (1L << 33), GWGS_ESIZE);
+    const dim3 grid((unsigned)(((long)ngroups * (nedge - 1) + NT / 64 - 1) / (NT / 64)));
+    hipLaunchKernelGGL((chisq_bins_k<true, true>), grid, dim3(NT), 0, s, (const float2*)h, ldh, nullptr,
+                       (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge, ngroups, (float2*)work,
+                       (const float2* const*)d_tab, gseg);
     hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
                        (const float2*)work, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
     return gwgs_status();
@@ -297,7 +333,20 @@ GWGS_API int gwgs_edges_fused(const void* h, long ldh, const int* rows, const fl
 {
     GWGS_CHECK(h && rows && ips && k1s && nbs && out && nedge >= 2 && n >= 0, GWGS_EINVAL);
     if (n == 0) return GWGS_OK;
-    hipLaunchKernelGGL(edges_fused_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)h, ldh, rows, ips,
-                       k1s, nbs, nedge, kmin, out);
+    hipLaunchKernelGGL(edges_fused_k<false>, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)h, ldh, rows,
+                       ips, k1s, nbs, nedge, kmin, out, nullptr, nullptr);
+    return gwgs_status();
+}
+
+// gwgs_edges_fused over requests of several segments: request g reads 1/psd from ip_tab[rseg[g]] (device table of
+// float32 pointers, every row >= its k1 long).
+GWGS_API int gwgs_edges_fused_multi(const void* h, long ldh, const int* rows, const float* const* ip_tab,
+                                    const int* rseg, const int* k1s, const int* nbs, int nedge, int kmin, int n,
+                                    int* out, void* stream)
+{
+    GWGS_CHECK(h && rows && ip_tab && rseg && k1s && nbs && out && nedge >= 2 && n >= 0, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL(edges_fused_k<true>, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)h, ldh, rows,
+                       nullptr, k1s, nbs, nedge, kmin, out, (const float* const*)ip_tab, rseg);
     return gwgs_status();
 }

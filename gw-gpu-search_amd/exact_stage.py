@@ -21,7 +21,7 @@ import torch
 
 import _dev
 import chisq_kernel
-from chisq_edges import edges_blocked, edges_rows_dev
+from chisq_edges import edges_blocked, edges_rows_dev, edges_rows_dev_multi
 
 MAX_PEAKS = 8
 CHECK_ROWS = 16
@@ -150,6 +150,52 @@ class ExactStage:
         return PendingValues(self, seg, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
 
 
+    def evaluate_async_multi(self, segs, sidx, req: PeakRequest, refine=0, lo=None, hi=None) -> 'PendingValues':
+        """evaluate_async over the peaks of several segments of equal tlen and df in one edges launch and one sums
+        launch: segs is a list of SegmentData, sidx the segment of each peak. Results in request order; the first-call
+        self-check runs on the peaks of the first requested segment."""
+        rows, samples = req.rows, req.samples
+        sidx = np.asarray(sidx, np.int64)
+        npk = len(rows)
+        seg0 = segs[int(sidx[0])] if npk else segs[0]
+        if npk == 0:
+            return PendingValues(self, seg0, rows, samples[:, None], None, None, None)
+        assert all(sg.tlen == seg0.tlen and sg.df == seg0.df for sg in segs)
+        n = self.tile.H.shape[0]
+        assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg0.tlen
+        assert sidx.shape == rows.shape and 0 <= sidx.min() and sidx.max() < len(segs)
+        t = self.tile
+        key = sidx * n + rows
+        ob = np.argsort(key, kind='stable')
+        ks = key[ob]
+        first = np.r_[True, ks[1:] != ks[:-1]]
+        uk = ks[first]
+        inv = np.empty(npk, np.int64)
+        inv[ob] = np.cumsum(first) - 1
+        ur, us = uk % n, uk // n
+        E_dev = edges_rows_dev_multi(ur, us, self._nbe(ur), t.kmin, t.kmax[ur], t.H,
+                                     np.array([sg.IP.data_ptr() for sg in segs], np.int64))
+        zero = np.flatnonzero(t.nbins[ur] == 0)
+        if len(zero):
+            E_dev[_dev.to_dev(zero), 0] = t.kmin
+        if refine > 0:
+            lo = np.zeros(npk, np.int64) if lo is None else np.asarray(lo, np.int64)
+            hi = np.full(npk, seg0.tlen, np.int64) if hi is None else np.asarray(hi, np.int64)
+            off = np.arange(-refine, refine + 1)
+            cand = np.clip(samples[:, None] + off[None, :], lo[:, None], hi[:, None] - 1)
+        else:
+            cand = samples[:, None]
+        noff = cand.shape[1]
+        out, zo = chisq_kernel.launch_sorted_multi(t.H, np.repeat(ks, noff), n, cand[ob].ravel(), E_dev,
+                                                   np.repeat(inv[ob], noff),
+                                                   np.array([sg.D.data_ptr() for sg in segs], np.int64), seg0.tlen,
+                                                   row_span=np.maximum(t.kmax - t.kmin, 0))
+        pv = PendingValues(self, seg0, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
+        pv.check_mask = sidx == sidx[0]
+        pv.keep_alive = segs
+        return pv
+
+
 class PendingValues:
     """Exact values of one evaluate_async call; result() waits for the device work and builds the PeakValues (the
     refine choice, fpeak from the edges, and the first-call self-check)."""
@@ -177,7 +223,12 @@ class PendingValues:
         E = Eu.astype(np.int64)[self.inv]
         out = PeakValues(z, pw, _fpeak(E, st.tile.nbins[rows], self.seg.df), samples)
         if st.self_check and st.check is None:
-            st.check = _self_check(st, self.seg, PeakRequest(rows, samples), out, E)
+            m = getattr(self, 'check_mask', None)
+            if m is None:
+                st.check = _self_check(st, self.seg, PeakRequest(rows, samples), out, E)
+            else:
+                st.check = _self_check(st, self.seg, PeakRequest(rows[m], samples[m]),
+                                       PeakValues(z[m], pw[m], out.fpeak[m], samples[m]), E[m])
         return out
 
 

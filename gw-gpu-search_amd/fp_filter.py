@@ -76,6 +76,8 @@ class Options:
     sgchisq_snr_threshold: float = None    # sine-Gaussian chi-squared of exact peaks with newsnr >= this, for the
                                            # templates BankPlan(sg_locations=...) gives tiles (None: off)
     gate: object = None                    # Gate: filter a tile on a segment only near base-search triggers
+    overlap: bool = False                  # basis transforms on their own stream, overlapping the previous batch's
+                                           # GEMM (off with graphs: each batch is captured on one stream)
 
 
 @dataclass
@@ -261,6 +263,31 @@ class Triggers:
         return out
 
 
+class _SharedValues:
+    """One exact-stage launch over several segments: result() waits once and keeps the PeakValues."""
+
+    def __init__(self, pv):
+        self.pv, self.v = pv, None
+
+    def result(self):
+        if self.v is None:
+            self.v = self.pv.result()
+        return self.v
+
+
+class _PartValues:
+    """The peaks [off, off + n) of a shared exact-stage result (one segment's)."""
+
+    def __init__(self, shared, off, n):
+        self.shared, self.off, self.n = shared, off, n
+
+    def result(self):
+        from exact_stage import PeakValues
+        v = self.shared.result()
+        sl = slice(self.off, self.off + self.n)
+        return PeakValues(v.z[sl], v.pw[sl], v.fpeak[sl], v.samples[sl])
+
+
 def candidate_cut(thr, margin):
     """The candidate cut on |z|^2 of fp_filter.py:747, vw > (thr (1 - margin))^2: the square in float64, then
     float32, as torch compares a float32 tensor with a Python float."""
@@ -295,6 +322,39 @@ def _fp8_scales(pw):
     rms = torch.sqrt(acc / pw.shape[0]).to(torch.float32)
     zcs = torch.where(rms > 0, 1.0 / rms.clamp(min=1e-30), torch.zeros_like(rms)).contiguous()
     return zcs, rms.contiguous()
+
+
+def run_jobs(jobs):
+    """Filter runs one after another in one software pipeline: jobs is an iterable of (Filter, plans), which may be a
+    generator, so that a job is set up (its data conditioned, its Filter made) when the pipeline reaches it, while
+    the previous job's last batches are still in flight. Batches are launched `depth` ahead of the one being decoded
+    (the depth of the job being launched), decoded by their own Filter and finished one behind. Returns, per job, per
+    plan, one Triggers per dataset."""
+    done = []
+    launched, fin = deque(), deque()
+    for f, plans in jobs:
+        outs = f._begin(plans)
+        done.append((f, plans, outs))
+        for cur in f._batches(plans, outs):
+            if f.opts.timing:
+                f.decode_batch(cur); f.finish_batch(cur)
+                continue
+            launched.append(cur)
+            while len(launched) > f.opts.depth:
+                st = launched.popleft()
+                fin.append(st['filter'].decode_batch(st))
+            while len(fin) > 1:
+                st = fin.popleft()
+                st['filter'].finish_batch(st)
+    while launched:
+        st = launched.popleft()
+        fin.append(st['filter'].decode_batch(st))
+    while fin:
+        st = fin.popleft()
+        st['filter'].finish_batch(st)
+    for f, plans, outs in done:
+        f._end(plans, outs)
+    return [outs for _, _, outs in done]
 
 
 def svd_metric(datasets):
@@ -357,6 +417,9 @@ class Filter:
         assert not self._cons_jobs and not self._sg_pending, 'a previous run did not finish'
         self._floors = {}
 
+    def _overlap(self):
+        return self.opts.overlap and not self.opts.graphs
+
     def _sync(self):
         if self.opts.timing:
             torch.cuda.synchronize()
@@ -375,11 +438,10 @@ class Filter:
         isw2 = torch.where(self.IPall.double().mean(0) > 0, 1.0 / self.IPall.double().mean(0).clamp(min=1e-300),
                            torch.zeros((), dtype=torch.float64, device=self.IPall.device))[self.kmin:]
         n1 = 1 << ((int(self.tlen).bit_length() - 1) // 2)
-        vals = []
-        for m in self.data:
-            c = ((m.D[:, self.kmin:].abs().double() ** 2) * isw2).mean(1)
-            vals.append(torch.where(c > 0, torch.sqrt(n1 / c.clamp(min=1e-300)) / 64.0, torch.ones_like(c)))
-        v = torch.cat(vals).clamp(1e-30, 1e30).to(torch.float32)
+        Dall = self.data[0].D if len(self.data) == 1 else torch.cat([m.D for m in self.data], 0)
+        c = ((Dall[:, self.kmin:].abs().double() ** 2) * isw2).mean(1)
+        v = torch.where(c > 0, torch.sqrt(n1 / c.clamp(min=1e-300)) / 64.0, torch.ones_like(c))
+        v = v.clamp(1e-30, 1e30).to(torch.float32)
         if self._ysc is None:
             self._ysc = torch.empty((len(v), self.opts.tile + 16), dtype=torch.float32, device='cuda')
         self._ysc.copy_(v[:, None].expand_as(self._ysc))
@@ -466,14 +528,26 @@ class Filter:
     def run_banks(self, plans):
         """Every tile of every plan (bank) against every segment; returns, per plan, one Triggers per dataset.
 
-        Software pipeline on one stream: the transforms, GEMM and candidate selection of the next `depth` batches
-        are enqueued before a batch's candidates are read back, and a batch's exact stage is enqueued before the
-        previous batch's results are collected, so the GPU has queued work while the host decodes and finalises,
+        Software pipeline on one stream (run_jobs): the transforms, GEMM and candidate selection of the next `depth`
+        batches are enqueued before a batch's candidates are read back, and a batch's exact stage is enqueued before
+        the previous batch's results are collected, so the GPU has queued work while the host decodes and finalises,
         across tiles and banks. Copies to the host run on a side stream after the events they depend on. With
         opts.timing every stage is synchronised instead."""
+        return run_jobs([(self, plans)])[0]
+
+    def _begin(self, plans):
+        """Per-run set-up: the state one run must not hand to the next, the output lists, the segment list."""
         self._reset_run_state()
+        if self.IPall is None:                 # released at the end of a previous run's launches
+            self.IPall = torch.cat([m.IP for m in self.data], 0)
+        if self._overlap():
+            # the transform stream sees everything enqueued before the run (conditioning, the fp16 scales)
+            _dev.basis_stream().wait_stream(torch.cuda.current_stream())
         sg_on = self.opts.sgchisq_snr_threshold is not None
-        outs = [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
+        return [[Triggers(sg=sg_on) for _ in self.data] for _ in plans]
+
+    def _batches(self, plans, outs):
+        """Launch every batch of the run in order (tile norms at each tile's start), yielding each batch's state."""
         segs = []
         for di, m in enumerate(self.data):
             for s, (a0, a1) in enumerate(m.analyze):
@@ -483,7 +557,6 @@ class Filter:
         B = max(1, self.opts.segment_batch)
         gate = self.opts.gate
         gsegs = [gate.per_segment(m) for m in self.data] if gate is not None else None
-        launched, fin = deque(), deque()
         if self.opts.graphs and self._bufs is None:
             self._alloc_rotation(plans, segs, B)
         ordinal = 0
@@ -515,18 +588,16 @@ class Filter:
                     ordinal += 1
                     cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(tsegs), plan.sg_locs
                     cur['gate'] = gsel
-                    if self.opts.timing:
-                        self.decode_batch(cur); self.finish_batch(cur)
-                        continue
-                    launched.append(cur)
-                    while len(launched) > self.opts.depth:
-                        fin.append(self.decode_batch(launched.popleft()))
-                    while len(fin) > 1:
-                        self.finish_batch(fin.popleft())
-        while launched:
-            fin.append(self.decode_batch(launched.popleft()))
-        while fin:
-            self.finish_batch(fin.popleft())
+                    cur['filter'] = self
+                    yield cur
+        if not self.opts.graphs:
+            # every transform and norm of the run is enqueued: the workspaces go back to the allocator (reused in
+            # stream order by the next job), so a pipeline of many jobs holds one set at a time
+            self._work = self._q = None
+            self.IPall = None
+
+    def _end(self, plans, outs):
+        """After the run's last batch: the pending sine-Gaussian values and the final consolidations."""
         self._resolve_sg()
         if self.opts.consolidate is not None:
             # the parts added after each dataset's last consolidation (sine-Gaussian values pending then, or found
@@ -539,7 +610,6 @@ class Filter:
         if self._cons_pool is not None:
             self._cons_pool.shutdown(wait=True)
             self._cons_pool = None
-        return outs
 
     def finish_batch(self, st):
         """The sine-Gaussian values of earlier batches (their kernels ran right after this batch's exact stage, which
@@ -688,7 +758,20 @@ class Filter:
             vw, iw = bufs['vw'][:n * nwsum].view(n, nwsum), bufs['iw'][:n * nwsum].view(n, nwsum)
             sel = bufs['sel']
         t0 = self._sync()
-        Zt, acs = self.basis_operand(tb, segs, geo['Tns'], geo['row0'], Zt)
+        if self._overlap() and bufs is None:
+            # on the transform stream; the GEMM below waits for this batch's transforms only, so the next batch's
+            # transforms can run beside this batch's GEMM. Zt (and the fp8 scales) come from that stream's pool and
+            # are marked as read on the compute stream.
+            main = torch.cuda.current_stream()
+            with torch.cuda.stream(_dev.basis_stream()):
+                Zt, acs = self.basis_operand(tb, segs, geo['Tns'], geo['row0'], Zt)
+                done = _dev.mark()
+            main.wait_event(done)
+            Zt.record_stream(main)
+            if acs is not None:
+                acs.record_stream(main)
+        else:
+            Zt, acs = self.basis_operand(tb, segs, geo['Tns'], geo['row0'], Zt)
         t1 = self._sync(); self.T['basis_fft_s'] += t1 - t0
         if Ar is None:
             Ar = torch.empty((n, K), dtype=dt, device='cuda'); Ai = torch.empty_like(Ar)
@@ -760,7 +843,10 @@ class Filter:
             m.D.copy_(d.D)
             m.IP.copy_(d.IP)
             m.gates, m.gps_start, m.seg_start = list(d.gates), d.gps_start, list(d.seg_start)
-        torch.cat([m.IP for m in self.data], 0, out=self.IPall)
+        if self.IPall is None:
+            self.IPall = torch.cat([m.IP for m in self.data], 0)
+        else:
+            torch.cat([m.IP for m in self.data], 0, out=self.IPall)
         self._reset_run_state()
         self._y16_update()
         self.T = {k: 0 if isinstance(v, int) else 0.0 for k, v in self.T.items()}
@@ -790,6 +876,7 @@ class Filter:
                   zreb_d.data_ptr(), _dev.stream())
         st['zreb'] = _dev.Fetch(zreb_d)
         o = self.opts
+        xparts = []                           # (pend index, tile rows, samples, window starts) of exact candidates
         for i, (di, s, gs, a0, nw) in enumerate(segs):
             sel = np.flatnonzero(si == i)
             if not len(sel):
@@ -804,18 +891,30 @@ class Filter:
                     continue
             tb_, samp, ax = m_[sel], a0 + il[sel], approx[sel]
             ex = ax >= o.exact_snr * (1 - o.margin)
-            pv = None
             if ex.any():
-                tbx, sx = tb_[ex], samp[ex]
-                seg = SegmentData(m.D[s], m.IP[s], m.df, m.tlen)
-                if o.refine > 0:
-                    # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
-                    wlo = a0 + (sx - a0) // self.cw * self.cw
-                    pv = tb.stage.evaluate_async(seg, PeakRequest(tbx, sx), refine=o.refine, lo=wlo,
-                                                 hi=wlo + self.cw)
-                else:
-                    pv = tb.stage.evaluate_async(seg, PeakRequest(tbx, sx))
-            st['pend'].append((di, s, gs, sel, tb_, samp, ax, ex, pv))
+                sx = samp[ex]
+                # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
+                xparts.append((len(st['pend']), tb_[ex], sx, a0 + (sx - a0) // self.cw * self.cw))
+            st['pend'].append([di, s, gs, sel, tb_, samp, ax, ex, None])
+        if xparts:
+            # the exact stage of every segment of the batch in one edges launch and one sums launch
+            pend = st['pend']
+            seglist = [SegmentData(self.data[pend[k][0]].D[pend[k][1]], self.data[pend[k][0]].IP[pend[k][1]],
+                                   self.df, self.tlen) for k, _, _, _ in xparts]
+            rows = np.concatenate([x[1] for x in xparts])
+            smp = np.concatenate([x[2] for x in xparts])
+            sidx = np.repeat(np.arange(len(xparts)), [len(x[1]) for x in xparts])
+            if o.refine > 0:
+                wlo = np.concatenate([x[3] for x in xparts])
+                pv = tb.stage.evaluate_async_multi(seglist, sidx, PeakRequest(rows, smp), refine=o.refine, lo=wlo,
+                                                   hi=wlo + self.cw)
+            else:
+                pv = tb.stage.evaluate_async_multi(seglist, sidx, PeakRequest(rows, smp))
+            shared = _SharedValues(pv)
+            off = 0
+            for k, r_, _, _ in xparts:
+                pend[k][8] = _PartValues(shared, off, len(r_))
+                off += len(r_)
         # the batch's device buffers stay referenced by st until collect_batch
         self.T['rebuilt_z_s'] += time.perf_counter() - t0
         return st

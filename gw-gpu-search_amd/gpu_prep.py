@@ -20,6 +20,23 @@ import peak_kernel
 DYN_RANGE_FAC = 5.9029581035870565e+20
 
 
+def rows_fft(fn, x, **kw):
+    """fn (torch.fft.rfft or irfft) along the last dimension of 2-D x, over its rows in power-of-two blocks: hipFFT
+    plans are then made once per block size and reused, where a new row count (the segment or PSD-span count of a
+    batch) would make and upload a new plan each time."""
+    R = x.shape[0]
+    if R & (R - 1) == 0:
+        return fn(x, dim=1, **kw)
+    first = fn(x[:1], dim=1, **kw)
+    out = torch.empty((R,) + tuple(first.shape[1:]), dtype=first.dtype, device=x.device)
+    r0 = 0
+    for b in reversed(range(R.bit_length())):
+        if R >> b & 1:
+            fn(x[r0:r0 + (1 << b)], dim=1, out=out[r0:r0 + (1 << b)], **kw)
+            r0 += 1 << b
+    return out
+
+
 def median_bias(n):
     """gpu_prep.py:47-53."""
     if n >= 1000:
@@ -49,7 +66,7 @@ def welch(x, seg_len, seg_stride, dt, num_segments=None, avg='median'):
     segs = torch.empty((rows, num_segments, seg_len), dtype=torch.float64, device=X.device)
     _lib.call('gwgs_welch_segments', X.data_ptr(), X.stride(0), rows, start, num_segments, seg_len, seg_stride,
               segs.data_ptr(), _dev.stream())
-    st = torch.fft.rfft(segs, dim=2)
+    st = rows_fft(torch.fft.rfft, segs.view(rows * num_segments, seg_len)).view(rows, num_segments, -1)
     nf = seg_len // 2 + 1
     out = torch.empty((rows, nf), dtype=torch.float64, device=X.device)
     w = np.hanning(seg_len)
@@ -80,10 +97,10 @@ def inverse_spectrum_truncation(psd, df, max_filter_len, low_frequency_cutoff, t
     kmin = int(low_frequency_cutoff / df) if low_frequency_cutoff else 1
     inv = torch.empty((rows, flen), dtype=torch.complex128, device=P.device)
     _lib.call('gwgs_isqrt_band', P.data_ptr(), rows, flen, kmin, N // 2, inv.data_ptr(), _dev.stream())
-    q = torch.fft.irfft(inv, n=N, dim=1, norm='forward').contiguous()
+    q = rows_fft(torch.fft.irfft, inv, n=N, norm='forward').contiguous()
     _lib.call('gwgs_trunc', q.data_ptr(), rows, N, int(max_filter_len), int(trunc_method == 'hann'), float(df),
               _dev.stream())
-    pt = torch.fft.rfft(q, dim=1)
+    pt = rows_fft(torch.fft.rfft, q)
     out = torch.empty((rows, flen), dtype=torch.float64, device=P.device)
     _lib.call('gwgs_inv_abs2', pt.data_ptr(), rows, flen, 1.0 / (df * N), out.data_ptr(), _dev.stream())
     return out[0] if psd.ndim == 1 else out
@@ -244,7 +261,7 @@ def condition(s, sr, gps_start, seg_len, start_pad, end_pad, psd_seg_len, psd_st
     df = 1.0 / seg_len
     starts = [a for (a, _), _ in seg]
     x = torch.stack([s[a:a + tlen] for a in starts])
-    X = torch.fft.rfft(x, dim=1).contiguous()                 # complex64, scaled by 1/sr in the whitening pass
+    X = rows_fft(torch.fft.rfft, x).contiguous()               # complex64, scaled by 1/sr in the whitening pass
     pl, ps = int(psd_seg_len * sr), int(psd_stride * sr)
     spans = overlapping_psd_spans(n, pl, ps, psd_nseg)
     parts = torch.stack([s[a:b] for a, b in spans])
@@ -262,3 +279,62 @@ def condition(s, sr, gps_start, seg_len, start_pad, end_pad, psd_seg_len, psd_st
     _lib.call('gwgs_whiten', X.data_ptr(), float(np.float32(1.0 / sr)), psd.data_ptr(), pick.data_ptr(), len(seg),
               flen, kmin, D.data_ptr(), IP.data_ptr(), _dev.stream())
     return Dataset(tag, D, IP, flen, tlen, df, float(sr), float(gps_start), [a for _, a in seg], starts, list(gates))
+
+
+def condition_many(xs, sr, gps_starts, seg_len, start_pad, end_pad, psd_seg_len, psd_stride, psd_nsegs, psd_inv_len,
+                   flow, trunc_method=None, avg='median', tags=None, gates=None):
+    """condition() for several strains on one segment grid in a handful of batched launches: every segment of every
+    strain in one segment transform, the PSD spans of all strains with the same Welch segment count in one Welch
+    estimate, interpolation and truncation, and one whitening pass over all segments with each segment's PSD row
+    picked by largest overlap within its own strain (gpu_prep.py:405-454 per dataset). xs: float32 strain tensors
+    (x DYN_RANGE_FAC, on the GPU); psd_nsegs: the Welch segment count of each strain. No host synchronisation: the
+    index arithmetic is host code and the pick table goes over in a pinned copy. Returns one fp_filter.Dataset per
+    strain; their D and IP are row blocks of one tensor each."""
+    from fp_filter import Dataset
+    tlen = int(seg_len * sr)
+    flen = tlen // 2 + 1
+    df = 1.0 / seg_len
+    kmin = int(flow / df)
+    pl, ps = int(psd_seg_len * sr), int(psd_stride * sr)
+    per = []
+    for x, gps, nseg in zip(xs, gps_starts, psd_nsegs):
+        n = x.shape[0]
+        seg = segment_slices(n, sr, gps, seg_len, start_pad, end_pad)
+        per.append((seg, overlapping_psd_spans(n, pl, ps, nseg)))
+    xs = [x.to(torch.float32).contiguous() for x in xs]
+    X = rows_fft(torch.fft.rfft, torch.stack([x[a:a + tlen] for x, (seg, _) in zip(xs, per)
+                                              for (a, _), _ in seg])).contiguous()
+    groups = {}
+    for i, nseg in enumerate(psd_nsegs):
+        groups.setdefault(int(nseg), []).append(i)
+    row0 = [0] * len(xs)
+    psds, row = [], 0
+    for nseg, idx in groups.items():
+        parts = torch.stack([xs[i][a:b] for i in idx for a, b in per[i][1]])
+        psd, pdf = welch(parts, int(psd_seg_len * sr + 0.5), int(psd_stride * sr + 0.5), 1.0 / sr,
+                         num_segments=nseg, avg=avg)
+        if pdf != df:
+            psd = interpolate(psd, pdf, df, flen)
+        if psd_inv_len:
+            psd = inverse_spectrum_truncation(psd, df, int(psd_inv_len * sr), flow, trunc_method)
+        for i in idx:
+            row0[i] = row
+            row += len(per[i][1])
+        psds.append(psd)
+    P = torch.cat(psds).contiguous()
+    pick = np.concatenate([np.asarray(pick_psds(seg, spans), np.int64) + row0[i]
+                           for i, (seg, spans) in enumerate(per)]).astype(np.int32)
+    pick_d = _dev.to_dev(pick)
+    S = X.shape[0]
+    D = torch.empty((S, flen), dtype=torch.complex64, device=X.device)
+    IP = torch.empty((S, flen), dtype=torch.float32, device=X.device)
+    _lib.call('gwgs_whiten', X.data_ptr(), float(np.float32(1.0 / sr)), P.data_ptr(), pick_d.data_ptr(), S, flen, kmin,
+              D.data_ptr(), IP.data_ptr(), _dev.stream())
+    out, s0 = [], 0
+    for i, (seg, _) in enumerate(per):
+        s1 = s0 + len(seg)
+        out.append(Dataset(tags[i] if tags else '', D[s0:s1], IP[s0:s1], flen, tlen, df, float(sr),
+                           float(gps_starts[i]), [a for _, a in seg], [a for (a, _), _ in seg],
+                           list(gates[i]) if gates else []))
+        s0 = s1
+    return out

@@ -96,11 +96,22 @@ int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const
 /* bytes of the device workspace gwgs_chisq_points needs (the per-bin sums of every group; 0 when there is none) */
 long gwgs_chisq_points_workspace(int ngroups, int nedge);
 int gwgs_chisq_peaks_per_group(void);
+/* pipeline/exact_stage.py:173-191 ExactStage.evaluate for the peaks of several segments of one tile in one launch
+ * (the reference evaluates segment by segment): gwgs_chisq_points in fused mode where group g reads its data row
+ * from d_tab[gseg[g]] (device table of complex64 row addresses). Same workspace as gwgs_chisq_points. */
+int gwgs_chisq_points_multi(const void* h, long ldh, const void* const* d_tab, const int* gseg, long long N,
+                            const int* grow, const int* gstart, const int* gcount, const long long* ptime,
+                            const int* edges, int nedge, int ngroups, float* out, void* zout, void* work,
+                            void* stream);
 /* pipeline/chisq_edges.py:90-143 edges_fused (and edges_blocked, multi_filter.py:264-282 edges_per_template):
  * per request i, row rows[i] of h: edge j = first k with float64 cumsum(|h|^2 ips)[k] > j / nbs[i] * total over
  * [0, k1s[i]), clipped to [kmin, k1s[i]]; columns j >= nbs[i] and the last hold k1s[i]. out (n, nedge) int32. */
 int gwgs_edges_fused(const void* h, long ldh, const int* rows, const float* ips, const int* k1s, const int* nbs,
                      int nedge, int kmin, int n, int* out, void* stream);
+/* pipeline/chisq_edges.py:90-143 edges_fused over requests of several segments: request i reads 1/psd from
+ * ip_tab[rseg[i]] (device table of float32 row addresses); otherwise as gwgs_edges_fused. */
+int gwgs_edges_fused_multi(const void* h, long ldh, const int* rows, const float* const* ip_tab, const int* rseg,
+                           const int* k1s, const int* nbs, int nedge, int kmin, int n, int* out, void* stream);
 
 /* ---- basis series and GEMM operands (pipeline/fp_filter.py:597-621 basis_series, stacked_A) --------------------
  * q[j, k] = Vc[j, k] D[k] for k < flen, j < rp (q: (rp, ldq) complex64; columns >= flen are not written). */

@@ -343,3 +343,28 @@ def test_condition_vs_reference_stage2_golden(gpu):
         assert np.max(np.abs(D[:, ::st] - gD)) < 1e-5 * np.abs(gD).max(), name
         assert [list(a) for a in d.analyze] == m['analyze'] and list(d.seg_start) == m['seg_start']
         assert d.flen == m['flen'] and d.tlen == m['tlen'] and d.df == m['df']
+
+
+def test_condition_many_equals_condition(gpu):
+    """gpu_prep.condition_many (every strain of a segment grid in a few batched launches) gives each strain's
+    condition() result: the same segment layout, IP within 1e-6 and D within 1e-6 of its scale (the batched FFTs may
+    round differently), for strains of different lengths and Welch segment counts (one and several PSD spans)."""
+    import gpu_prep, synth, workloads
+    xs, conds, sts = [], [], []
+    for L, seed in ((192, 3), (320, 4), (560, 5), (448, 6)):
+        st, x = synth.device_stream('H1', L, SR, seed, lines=synth.default_lines(), gps_start=1e9 + 2000 * seed)
+        xs.append(x); conds.append(workloads.chunk_conditioning(L)); sts.append(st)
+    c0 = conds[0]
+    many = gpu_prep.condition_many(xs, SR, [s.gps_start for s in sts], 128, c0.start_pad, c0.end_pad, c0.psd_seg_len,
+                                   c0.psd_stride, [c.psd_nseg for c in conds], c0.psd_inv_len, 15.0,
+                                   tags=['H1'] * len(xs))
+    for x, st, c, m in zip(xs, sts, conds, many):
+        one = gpu_prep.condition(x.clone(), SR, st.gps_start, 128, c.start_pad, c.end_pad, c.psd_seg_len,
+                                 c.psd_stride, c.psd_nseg, c.psd_inv_len, 15.0, tag='H1')
+        assert m.analyze == one.analyze and m.seg_start == one.seg_start and m.gps_start == one.gps_start
+        ip1, ipm = one.IP.cpu().numpy(), m.IP.cpu().numpy()
+        nz = ip1 > 0
+        assert np.array_equal(ipm > 0, nz)
+        assert np.max(np.abs(ipm - ip1)[nz] / ip1[nz]) < 1e-6
+        d1, dm = one.D.cpu().numpy(), m.D.cpu().numpy()
+        assert np.max(np.abs(dm - d1)) < 1e-6 * np.abs(d1).max()
