@@ -590,6 +590,7 @@ def main():
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
     ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
     ap.add_argument('--overlap', type=int, default=0, help='1: basis transforms on their own stream beside the GEMM')
+    ap.add_argument('--depth', type=int, default=2, help='batches enqueued ahead of the one being decoded')
     ap.add_argument('--production', type=int, default=1, help='also run the production workload (2 detectors x 512 s '
                     'with lines, glitches and injections) and its reports; 0: skip')
     ap.add_argument('--production-steps', type=int, default=3, help='timed BF16 steps of the production workload')
@@ -626,7 +627,8 @@ def main():
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
-                                 segment_batch=a.segment_batch, overlap=bool(a.overlap)) for p in ('bf16', 'fp32', 'fp8')}
+                                 segment_batch=a.segment_batch, overlap=bool(a.overlap), depth=a.depth)
+            for p in ('bf16', 'fp32', 'fp8')}
     import consolidate
     kl = consolidate.KeepLoudest(newsnr_threshold=4.0, interval=1.072, num=30, exempt_snr=5.0)
     for name, pre in (('cons', False), ('pre', True)):

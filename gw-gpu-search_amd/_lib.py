@@ -66,6 +66,7 @@ SIGNATURES = {
     'gwgs_gate_whiten': [_p, _p, _l, _l, _d, _d, _p],
     'gwgs_gate_mag': [_p, _l, _l, _l, _d, _p, _p],
     'gwgs_gate_apply': [_p, _l, _l, _l, _l, _l, _l, _p],
+    'gwgs_fir_decimate': [_p, _l, _p, _i, _i, _p, _p],
     'gwgs_phasor_data': [_p, _l, _l, _p, _p, _ll, _i, _p, _p, _l, _p],
     'gwgs_snr_grad': [_p, _p, _l, _p, _l, _p, _l, _i, _i, _d, _p, _p, _p, _p, _p, _p, _l, _p],
     'gwgs_slide_count': [_p, _p, _i, _p, _p, _p, _l, _d, _d, _i, _d, _p, _p, _p],

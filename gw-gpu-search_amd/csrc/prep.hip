@@ -73,6 +73,56 @@ __global__ void __launch_bounds__(256) welch_reduce_k(const double2* __restrict_
     }
 }
 
+// The same reduction for more than 512 segments (the autogating Welch over hours of data): one workgroup per (bin,
+// row), the bin's nseg powers in LDS padded with +inf to NP, a bitonic sort over all NP (NP / 2 compare-exchanges
+// per stage over 1024 threads, one barrier per stage), then the same median (or the mean in segment order).
+template <int NP>
+__global__ void __launch_bounds__(1024) welch_reduce_big_k(const double2* __restrict__ X, const int nseg,
+                                                          const int nf, const double dt2, const int mean,
+                                                          const double bias, const double fac,
+                                                          double* __restrict__ out)
+{
+    __shared__ double v[NP];
+    const int b = blockIdx.x, r = blockIdx.y, tid = threadIdx.x;
+    const double2* __restrict__ xr = X + (size_t)r * nseg * nf;
+    for (int s = tid; s < NP; s += 1024) {
+        double p = __builtin_inf();
+        if (s < nseg) {
+            const double2 z = xr[(size_t)s * nf + b];
+            p = (z.x * z.x + z.y * z.y) * dt2;
+            if (b == 0 || b == nf - 1) p /= 2;
+        }
+        v[s] = p;
+    }
+    __syncthreads();
+    if (mean) {
+        if (tid == 0) {
+            double acc = 0.0;
+            for (int s = 0; s < nseg; s++) acc += v[s];
+            out[(size_t)r * nf + b] = acc / nseg * fac;
+        }
+        return;
+    }
+    for (int k = 2; k <= NP; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int pi = tid; pi < NP / 2; pi += 1024) {
+                const int i = 2 * j * (pi / j) + pi % j, l = i + j;
+                const double a = v[i], c = v[l];
+                if ((a > c) == ((i & k) == 0)) {
+                    v[i] = c;
+                    v[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (tid == 0) {
+        const int m = nseg / 2;
+        double res = (nseg & 1) ? v[m] : 0.5 * (v[m - 1] + v[m]);
+        res /= bias;
+        out[(size_t)r * nf + b] = res * fac;
+    }
+}
+
 // np.interp of P (rows, nold) sampled at k df_old onto i df_new, i < nnew (right end held)
 __global__ void interp_k(const double* __restrict__ P, const int nold, const double df_old, const double df_new,
                          const long nnew, double* __restrict__ out)
@@ -204,6 +254,24 @@ __global__ void gate_apply_k(float* __restrict__ x, const long offset, const lon
 inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 inline dim3 grid2(long n, int rows) { return dim3((unsigned)((n + NT - 1) / NT), rows); }
 
+// 'ldas' FIR decimation (pycbc resample_to_delta_t, gpu_prep.py:67-84): out[i] = sum_k c[k] x[n - k] with
+// n = factor i + shift, zero where n < 2 (ntaps / 2) or n >= L (the filter's start-up samples, and the wrapped
+// indices of the rolled series that land there); float64 taps and sum in tap order without fused multiply-add (the
+// reference's float64 numpy arithmetic, bit for bit), one thread per output sample.
+__global__ void fir_decimate_k(const double* __restrict__ x, const long L, const double* __restrict__ c,
+                               const int ntaps, const int factor, const long shift, const long nout,
+                               double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const long n = (long)factor * i + shift;
+    double acc = 0.0;
+    if (n >= 2L * (ntaps / 2) && n < L)
+        for (int k = 0; k < ntaps; k++) acc += c[k] * x[n - k];          // contraction off: as numpy
+    out[i] = acc;
+}
+
 } // namespace
 
 GWGS_API int gwgs_welch_segments(const float* x, long ldx, int rows, long start, int nseg, int L, int stride,
@@ -219,10 +287,25 @@ GWGS_API int gwgs_welch_reduce(const void* X, int rows, int nseg, int nf, double
                                double* out, void* stream)
 {
     GWGS_CHECK(X && out && rows > 0 && nseg > 0 && nf > 0, GWGS_EINVAL);
-    GWGS_CHECK(nseg <= 512, GWGS_ESIZE);
+    GWGS_CHECK(nseg <= 16384, GWGS_ESIZE);
     const double2* x = (const double2*)X;
     hipStream_t st = (hipStream_t)stream;
     const double dt2 = dt * dt;
+    if (nseg > 512) {
+        const dim3 g((unsigned)nf, (unsigned)rows);
+        if (nseg <= 1024)
+            hipLaunchKernelGGL(welch_reduce_big_k<1024>, g, dim3(1024), 0, st, x, nseg, nf, dt2, mean, bias, fac, out);
+        else if (nseg <= 2048)
+            hipLaunchKernelGGL(welch_reduce_big_k<2048>, g, dim3(1024), 0, st, x, nseg, nf, dt2, mean, bias, fac, out);
+        else if (nseg <= 4096)
+            hipLaunchKernelGGL(welch_reduce_big_k<4096>, g, dim3(1024), 0, st, x, nseg, nf, dt2, mean, bias, fac, out);
+        else if (nseg <= 8192)
+            hipLaunchKernelGGL(welch_reduce_big_k<8192>, g, dim3(1024), 0, st, x, nseg, nf, dt2, mean, bias, fac, out);
+        else
+            hipLaunchKernelGGL(welch_reduce_big_k<16384>, g, dim3(1024), 0, st, x, nseg, nf, dt2, mean, bias, fac,
+                               out);
+        return gwgs_status();
+    }
     if (nseg <= 64)
         hipLaunchKernelGGL(welch_reduce_k<64>, dim3((nf + 127) / 128, rows), dim3(256), 0, st, x, nseg, nf, dt2, mean,
                            bias, fac, out);
@@ -307,5 +390,18 @@ GWGS_API int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, lo
                GWGS_EINVAL);
     if (i2 <= i1) return GWGS_OK;
     hipLaunchKernelGGL(gate_apply_k, grid1(i2 - i1), dim3(NT), 0, (hipStream_t)stream, x, offset, i1, i2, pad, win);
+    return gwgs_status();
+}
+
+// Decimation by `factor` of x (L samples, float64) with the ntaps FIR taps c (float64, device): out has
+// ceil(L / factor) samples (gpu_prep.py:67-84 resample_ldas, the shift -ntaps // 2 rounded as Python does).
+GWGS_API int gwgs_fir_decimate(const double* x, long L, const double* c, int ntaps, int factor, double* out,
+                               void* stream)
+{
+    GWGS_CHECK(x && c && out && L > 0 && ntaps > 0 && factor > 0, GWGS_EINVAL);
+    const long nout = (L + factor - 1) / factor;
+    const long shift = (ntaps + 1) / 2;      // -(-ntaps // 2)
+    hipLaunchKernelGGL(fir_decimate_k, grid1(nout), dim3(NT), 0, (hipStream_t)stream, x, L, c, ntaps, factor, shift,
+                       nout, out);
     return gwgs_status();
 }

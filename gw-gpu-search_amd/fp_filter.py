@@ -95,6 +95,8 @@ class Dataset:
     analyze: list
     seg_start: list
     gates: list = field(default_factory=list)
+    first: list = None         # with overlap-save chunks: the first chunk of each original segment (the peak file's
+                               # sigmasq columns, fp_filter.py:878); None: every segment
 
 
 @dataclass
@@ -224,6 +226,7 @@ class Triggers:
         self.sg = sg                       # the peak file carries sg_chisq and sg_chisq_computed
         self.unready = []                  # sgc arrays of parts whose sine-Gaussian values are still on the device
         self.settled = 0                   # 1 when the first part is a consolidation's result
+        self.sigmasq = None                # (bank templates, first segments) float64: each template's sigma^2
 
     def ready_parts(self):
         """Number of leading parts whose fields are final (consolidation works on these)."""
@@ -589,6 +592,7 @@ class Filter:
                     cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(tsegs), plan.sg_locs
                     cur['gate'] = gsel
                     cur['filter'] = self
+                    cur['ntmp'] = len(plan.m1)
                     yield cur
         if not self.opts.graphs:
             # every transform and norm of the run is enqueued: the workspaces go back to the allocator (reused in
@@ -623,7 +627,20 @@ class Filter:
         self._apply_consolidated(block=False)
         self.collect_batch(st)
         if st.get('last'):
+            self._record_sigmasq(st)
             self._consolidate(st['pi'], st['outs'])
+
+    def _record_sigmasq(self, st):
+        """The tile's sigma^2 of every template in each dataset's first segments into the dataset's Triggers (the
+        peak file's 2-D sigmasq, fp_filter.py:878)."""
+        sig_h, _ = st['tile_host'].get()
+        tb = st['tb']
+        ntmp = int(st['ntmp'])
+        for di, (m, tr) in enumerate(zip(self.data, st['outs'])):
+            first = m.first if m.first is not None else list(range(len(m.analyze)))
+            if tr.sigmasq is None:
+                tr.sigmasq = np.zeros((ntmp, len(first)))
+            tr.sigmasq[tb.ids] = sig_h[:, self.seg0[di] + np.asarray(first, np.int64)]
 
     def _consolidate(self, pi, outs):
         """Start the consolidation of every dataset's finished parts; a list whose previous consolidation is still
@@ -990,3 +1007,302 @@ class Filter:
                 self._sg_pending.append((out, sgc, sgd, pend))
             self.T['n_chisq_points'] += int(k.sum())
         self.T['refine_chisq_s'] += time.perf_counter() - t0
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Peak files and the command line (fp_filter.py:47-160, 910-977)
+# ---------------------------------------------------------------------------------------------------------------------
+
+PEAK_FIELDS = ('template_id', 'template_hash', 'snr', 'end_time', 'sigmasq', 'coa_phase', 'peak_sigmasq', 'exact',
+               'chisq', 'chisq_dof', 'newsnr', 'sg_chisq', 'sg_chisq_computed')
+
+
+def peak_file_arrays(fields, template_hash, sigmasq, gates, provenance_json='{}', config_md5=''):
+    """The datasets and attributes of one peak file (fp_filter.py:910-937) from Triggers.to_dict fields (gate veto
+    and (template_id, end_time) order already applied): 1-D fields of equal length, template_hash from the bank,
+    the 2-D sigmasq (templates x first segments, float64), attrs n_gates / provenance / config_md5 and the gates
+    group time / width / taper. Keys are the HDF5 paths ('attrs/...' for attributes)."""
+    out = {'template_id': np.asarray(fields['template_id'], np.int64),
+           'template_hash': np.asarray(template_hash)[np.asarray(fields['template_id'], np.int64)],
+           'snr': fields['snr'], 'end_time': fields['end_time'],
+           'sigmasq': np.asarray(sigmasq if sigmasq is not None else np.zeros((len(template_hash), 0)), np.float64),
+           'coa_phase': fields['coa_phase'], 'peak_sigmasq': fields['peak_sigmasq'], 'exact': fields['exact']}
+    for k in ('chisq', 'chisq_dof', 'newsnr', 'sg_chisq', 'sg_chisq_computed'):
+        if k in fields:
+            out[k] = fields[k]
+    gt = np.array(list(gates), np.float64).reshape(-1, 3)
+    out.update({'attrs/n_gates': np.int64(len(gt)), 'attrs/provenance': np.array(provenance_json),
+                'attrs/config_md5': np.array(config_md5), 'gates/time': gt[:, 0], 'gates/width': gt[:, 1],
+                'gates/taper': gt[:, 2]})
+    return out
+
+
+def write_peaks(path, arrays):
+    """Write a peak file atomically (fp_filter.py:918-933): to path + '.part', then renamed, so a reader never sees a
+    partial file. h5py is not available here: the file is a numpy .npz archive whose keys are the HDF5 dataset paths
+    (read back with read_peaks)."""
+    import os
+    tmp = path + '.part'
+    try:
+        with open(tmp, 'wb') as f:
+            np.savez(f, **arrays)
+    except BaseException:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+        raise
+    os.replace(tmp, path)
+
+
+def read_peaks(path):
+    """A peak file written by write_peaks: dict of its datasets (attributes under 'attrs/...')."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def template_hashes(m1, m2, e0=None):
+    """Stable 64-bit template hashes for banks that carry none (the reference takes PyCBC's bank table column): the
+    first 8 bytes of md5 over the float64 parameters."""
+    import hashlib
+    e0 = np.zeros(len(m1)) if e0 is None else np.asarray(e0, np.float64)
+    out = np.empty(len(m1), np.int64)
+    for i, v in enumerate(np.stack([np.asarray(m1, np.float64), np.asarray(m2, np.float64), e0], 1)):
+        out[i] = np.frombuffer(hashlib.md5(v.tobytes()).digest()[:8], np.int64)[0]
+    return out
+
+
+def load_bank(path, approximant, max_templates=None):
+    """A template bank file (numpy .npz: mass1, mass2, optional template_hash, optional ecc / e0) for an approximant
+    this path generates: the closed-form 0PN+1PN stationary-phase chirp ('SPAtmplt', 'TaylorF2'), with the leading
+    eccentric phase term for an eccentric approximant ('SEOBNRv5EHM_ecc', 'TaylorF2Ecc'). Returns (m1, m2, e0,
+    template_hash). Other approximants need PyCBC's waveform generators, which are not available here."""
+    ecc_names = ('SEOBNRv5EHM_ecc', 'TaylorF2Ecc')
+    if approximant not in ('SPAtmplt', 'TaylorF2') + ecc_names:
+        raise NotImplementedError(f'approximant {approximant}: only the closed-form SPA chirps are generated here')
+    with np.load(path, allow_pickle=False) as z:
+        m1, m2 = np.asarray(z['mass1'], float), np.asarray(z['mass2'], float)
+        e0 = None
+        for k in ('ecc', 'e0', 'ecc_b'):
+            if k in z.files and approximant in ecc_names:
+                e0 = np.asarray(z[k], float)
+        th = np.asarray(z['template_hash']) if 'template_hash' in z.files else None
+    if max_templates is not None:
+        m1, m2 = m1[:max_templates], m2[:max_templates]
+        e0 = None if e0 is None else e0[:max_templates]
+        th = None if th is None else th[:max_templates]
+    if th is None:
+        th = template_hashes(m1, m2, e0)
+    return m1, m2, e0, th
+
+
+def prep_one(job):
+    """fp_filter.py:53-97: condition one dataset and save D and IP to an .npz, returning its metadata. job = (tag,
+    argv, outdir, cache, wait_s). With a cache directory a finished result is reused (key: the hash of the full argv;
+    written under a temporary name and renamed). The conditioning is gpu_prep.prep (the GPU stages in this process;
+    the reference runs PyCBC on CPU workers here)."""
+    import hashlib, json, os
+    import gpu_prep
+    tag, argv, outdir, cache, wait_s = job
+    key = hashlib.sha1(' '.join(argv).encode()).hexdigest()[:16]
+    if cache:
+        cj = os.path.join(cache, f'{tag}-{key}.json')
+        t_wait = time.time() + wait_s
+        while wait_s and not os.path.exists(cj) and time.time() < t_wait:
+            time.sleep(5)
+        if os.path.exists(cj):
+            meta = json.load(open(cj))
+            meta['prep_s'] = 0.0
+            meta['cached'] = True
+            return meta
+    t0 = time.perf_counter()
+    g = gpu_prep.prep(argv)
+    os.makedirs(cache or outdir, exist_ok=True)
+    fn = os.path.join(cache or outdir, f'{tag}-{key}.npz')
+    with open(fn + '.part', 'wb') as f:
+        np.savez(f, D=g['D'].cpu().numpy(), IP=g['IP'].cpu().numpy())
+    os.replace(fn + '.part', fn)
+    meta = dict(tag=tag, file=fn, flen=g['flen'], tlen=g['tlen'], df=g['df'], sr=g['sr'], gps_start=g['gps_start'],
+                analyze=[tuple(map(int, a)) for a in g['analyze']], seg_start=[int(x) for x in g['seg_start']],
+                gates=[tuple(map(float, x)) for x in g['gates']], prep_s=time.perf_counter() - t0,
+                injection_file=gpu_prep.options(argv).injection_file, cached=False)
+    if cache:
+        json.dump(meta, open(cj + '.part', 'w'))
+        os.replace(cj + '.part', cj)
+    return meta
+
+
+def _chisq_bins(spec):
+    spec = str(spec).strip('"\'')
+    try:
+        return int(spec)
+    except ValueError:
+        raise NotImplementedError(f'--chisq-bins {spec!r}: only a fixed bin count is supported (an expression needs '
+                                  f'PyCBC)') from None
+
+
+def main(argv=None):
+    """fp_filter.py:100-160 and run_group: the command line of the reference's filter.
+
+    fp_filter.py --datasets datasets.txt --bank BANK.npz --approximant A --argv inspiral_argv.txt --out OUTDIR
+        [--tile 1024] [--eps 1e-3] [--margin 0.04] [--exact-snr S] [--consolidate] [--pre-consolidate] ...
+        -- [pycbc_inspiral options]
+
+    datasets.txt: one 'IFO GPS_START GPS_END STORE [TRIG_START_OFFSET]' per line; writes OUTDIR/peaks/IFO-START-END.npz
+    per dataset (OUTDIR/peaks_b2/ for --bank2) and OUTDIR/timing.json. Flags the reference accepts for choices this
+    path does not have (template interpolation, edge algorithms, kernel variants, worker counts) are accepted and
+    ignored; --queue DIR filters the DIR/*.job groups one after another."""
+    import argparse, json, os, shlex, sys
+    import consolidate as cons
+    import gpu_prep
+    import window_filter
+    from multi_filter import pycbc_parser, setopt, dataset_argv
+    import provenance
+    args = list(sys.argv[1:] if argv is None else argv)
+    split = args.index('--') if '--' in args else len(args)
+    ap = argparse.ArgumentParser(prog='fp_filter')
+    ap.add_argument('--datasets'); ap.add_argument('--bank', required=True)
+    ap.add_argument('--approximant', required=True); ap.add_argument('--argv', required=True)
+    ap.add_argument('--out'); ap.add_argument('--queue')
+    ap.add_argument('--pre-consolidate', action='store_true')
+    ap.add_argument('--tile', type=int, default=1024); ap.add_argument('--eps', type=float, default=1e-3)
+    ap.add_argument('--margin', type=float, default=0.04)
+    ap.add_argument('--templates', choices=['interp', 'pycbc'], default='interp')
+    ap.add_argument('--prep-procs', type=int, default=8); ap.add_argument('--gen-threads', type=int, default=8)
+    ap.add_argument('--n-chirp', type=int, default=6000); ap.add_argument('--n-log', type=int, default=1500)
+    ap.add_argument('--max-templates', type=int)
+    ap.add_argument('--prep-cache'); ap.add_argument('--wait-prep', type=float, default=0)
+    ap.add_argument('--gemm-cfg', help='tiles_per_block[,variant] for the GEMM launch; skips the tuning sweep')
+    ap.add_argument('--prep-only', action='store_true')
+    ap.add_argument('--prep', choices=['cpu', 'gpu'], default='cpu',
+                    help='cpu: prep_one per dataset through an .npz (reused with --prep-cache); gpu: in memory')
+    ap.add_argument('--edges', choices=['blocked', 'full', 'fused'], default='blocked')
+    ap.add_argument('--exact-snr', type=float, default=0)
+    ap.add_argument('--consolidate', action='store_true')
+    ap.add_argument('--consolidate-exempt-snr', type=float, default=None)
+    ap.add_argument('--exact-stage', choices=['dense', 'regen'], default='dense')
+    ap.add_argument('--fuse-q', action='store_true'); ap.add_argument('--sg-kernel', action='store_true')
+    ap.add_argument('--sg-batch', action='store_true'); ap.add_argument('--legacy-qzero', action='store_true')
+    ap.add_argument('--chunk'); ap.add_argument('--chunk2'); ap.add_argument('--gate')
+    ap.add_argument('--gate-mc-ratio', type=float, default=1.25); ap.add_argument('--gate-window', type=float, default=1.0)
+    ap.add_argument('--bank2'); ap.add_argument('--approximant2', default='IMRPhenomD'); ap.add_argument('--eps2', type=float)
+    ap.add_argument('--history')
+    ap.add_argument('--precision', choices=list(PRECISIONS), default='bf16', help='GEMM operand type (this path)')
+    ap.add_argument('--refine', type=int, default=0, help='exact-stage re-centring half width (0: the reference rule)')
+    a = ap.parse_args(args[:split])
+    assert not a.gate or a.chunk, '--gate needs --chunk'
+    base = shlex.split(open(a.argv).read()) + args[split + 1:]
+    base = setopt(setopt(base, '--bank-file', a.bank), '--approximant', a.approximant)
+    assert bool(a.queue) != bool(a.datasets and a.out), 'give --datasets and --out, or --queue'
+    assert not a.pre_consolidate or (a.consolidate and not a.gate), '--pre-consolidate needs --consolidate and no --gate'
+    if a.prep_cache:
+        os.makedirs(a.prep_cache, exist_ok=True)
+
+    def ds_argv(i, s_, e_, st, *trig_start):
+        av = dataset_argv(base, i, int(s_), int(e_), st)
+        return setopt(av, '--trig-start-time', str(int(s_) + int(trig_start[0]))) if trig_start else av
+
+    def read_jobs(ds_file, out):
+        ds = [ln.split() for ln in open(ds_file) if ln.strip() and not ln.startswith('#')]
+        return [(f'{d[0]}-{d[1]}-{d[2]}', ds_argv(*d), os.path.join(out, 'prep'), a.prep_cache, a.wait_prep) for d in ds]
+
+    def run_group(datasets_file, out):
+        T = dict(prep_s=0.0, templates_s=0.0, svd_s=0.0)
+        t_start = time.perf_counter()
+        jobs = read_jobs(datasets_file, out)
+        if a.prep_only:
+            assert a.prep_cache, '--prep-only needs --prep-cache'
+            for j in jobs:
+                prep_one(j)
+            return
+        for d in ('peaks', 'prep'):
+            os.makedirs(os.path.join(out, d), exist_ok=True)
+        data = []
+        t0 = time.perf_counter()
+        for tag, av, pdir, cache, wait in jobs:
+            if a.prep == 'gpu':
+                g = gpu_prep.prep(av)
+                D, IP, meta = g['D'], g['IP'], g
+            else:
+                meta = prep_one((tag, av, pdir, cache, wait))
+                with np.load(meta['file']) as z:
+                    D, IP = torch.from_numpy(z['D']).cuda(), torch.from_numpy(z['IP']).cuda()
+                if not a.prep_cache:
+                    os.remove(meta['file'])
+            data.append(Dataset(tag, D, IP, int(meta['flen']), int(meta['tlen']), float(meta['df']),
+                                float(meta['sr']), float(meta['gps_start']), [tuple(x) for x in meta['analyze']],
+                                list(meta['seg_start']), [tuple(x) for x in meta['gates']]))
+        T['prep_s'] = time.perf_counter() - t0
+        opt, _ = pycbc_parser().parse_known_args(base)
+        nbins = _chisq_bins(opt.chisq_bins)
+        sgp = argparse.ArgumentParser(add_help=False)
+        sgp.add_argument('--sgchisq-snr-threshold', type=float); sgp.add_argument('--sgchisq-locations', nargs='+')
+        sgo, _ = sgp.parse_known_args(base)
+        kl = cons.KeepLoudest.from_pycbc(opt, a.consolidate_exempt_snr) if a.consolidate else None
+        gate = None
+        if a.gate:
+            with np.load(a.gate, allow_pickle=False) as z:
+                gate = Gate(np.asarray(z['ifo']).astype(str), z['time'], z['mchirp'], a.gate_mc_ratio, a.gate_window)
+        prov = provenance.record(dict(bank=a.bank, bank2=a.bank2, datasets=datasets_file, argv_file=a.argv,
+                                      gate=a.gate), argv=['fp_filter'] + args + ['[resolved]'] + base,
+                                 dq_files=[datasets_file],
+                                 dq_flags=[x for i, x in enumerate(base) if x.startswith('--autogating')
+                                           or (i and base[i - 1].startswith('--autogating'))])
+        T['provenance'] = prov
+        provenance.log(a.history, 'fp_filter', 'start', out, prov)
+        banks = [(a.bank, a.approximant, 'peaks', a.eps, a.chunk)]
+        if a.bank2:
+            banks.append((a.bank2, a.approximant2, 'peaks_b2', a.eps2 or a.eps, a.chunk2))
+        T['n_peaks'] = {}
+        whole = data
+        for bank_i, (path, appx, pdir, eps_b, chunk) in enumerate(banks):
+            ds = window_filter.to_chunks(whole, chunk)[0] if chunk else whole
+            m1, m2, e0, th = load_bank(path, appx, a.max_templates)
+            o = Options(snr_threshold=opt.snr_threshold, cluster_window=opt.cluster_window, chisq_bins=nbins,
+                        low_frequency_cutoff=opt.low_frequency_cutoff, tile=a.tile, eps=eps_b, margin=a.margin,
+                        exact_snr=a.exact_snr, precision=a.precision, refine=a.refine,
+                        gemm_cfg=tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None,
+                        consolidate=kl, pre_consolidate=a.pre_consolidate,
+                        sgchisq_snr_threshold=sgo.sgchisq_snr_threshold if sgo.sgchisq_locations else None,
+                        gate=gate if chunk else None)
+            IPall, sw, isw = svd_metric(ds)
+            plan = BankPlan(m1, m2, e0, ds[0].flen, ds[0].df, o, sw, isw, name=os.path.basename(path),
+                            sg_locations=sgo.sgchisq_locations if o.sgchisq_snr_threshold is not None else None)
+            T['templates_s'] += plan.T['templates_s']; T['svd_s'] += plan.T['svd_s']
+            f = Filter(ds, o)
+            trs = f.run(plan)
+            for k, v in f.T.items():
+                if isinstance(v, (int, float)):
+                    T[k + ('' if bank_i == 0 else f'_b{bank_i + 1}')] = v
+            os.makedirs(os.path.join(out, pdir), exist_ok=True)
+            for d, tr in zip(ds, trs):
+                fields = tr.to_dict(gates=d.gates, chisq=nbins > 0)
+                arr = peak_file_arrays(fields, th, tr.sigmasq, d.gates, json.dumps(prov, sort_keys=True),
+                                       prov['config_md5'])
+                write_peaks(os.path.join(out, pdir, d.tag + '.npz'), arr)
+                T['n_peaks'][d.tag + ('' if bank_i == 0 else f'_b{bank_i + 1}')] = int(len(fields['snr']))
+            T.setdefault('ranks', {})[os.path.basename(path)] = plan.ranks
+        T['total_s'] = time.perf_counter() - t_start
+        T['data_ks'] = sum((a1 - a0) / m.sr for m in data for a0, a1 in m.analyze) / 1000
+        json.dump(T, open(os.path.join(out, 'timing.json'), 'w'), indent=1, default=float)
+        provenance.log(a.history, 'fp_filter', 'done', out, prov, n_peaks=sum(T['n_peaks'].values()),
+                       total_s=T['total_s'])
+        return T
+
+    if not a.queue:
+        return run_group(a.datasets, a.out)
+    seen = set()
+    while True:
+        js = sorted(f_ for f_ in os.listdir(a.queue) if f_.endswith('.job') and f_ not in seen)
+        if not js:
+            if os.path.exists(os.path.join(a.queue, 'END')):
+                return None
+            time.sleep(2)
+            continue
+        seen.add(js[0])
+        ds_file, out = open(os.path.join(a.queue, js[0])).read().split()[:2]
+        run_group(ds_file, out)
+
+
+if __name__ == '__main__':
+    import os as _os, sys as _sys
+    _sys.path.insert(0, _os.path.dirname(_os.path.abspath(__file__)))
+    main()

@@ -249,13 +249,13 @@ def autogate(s, sr, start_gps, threshold, cluster=5.0, width=0.25, taper=0.25, p
 
 
 def condition(s, sr, gps_start, seg_len, start_pad, end_pad, psd_seg_len, psd_stride, psd_nseg, psd_inv_len, flow,
-              trunc_method=None, avg='median', tag='', gates=()):
+              trunc_method=None, avg='median', tag='', gates=(), trig_start=0, trig_end=0):
     """gpu_stage2 (gpu_prep.py:405-454) from the float32 strain tensor s (x DYN_RANGE_FAC, on the GPU): segment
     FFTs, PSD spans, float64 median Welch, interpolation, truncation, and D, IP. Returns an fp_filter.Dataset."""
     from fp_filter import Dataset
     s = s.to(torch.float32).contiguous()
     n = s.shape[0]
-    seg = segment_slices(n, sr, gps_start, seg_len, start_pad, end_pad)
+    seg = segment_slices(n, sr, gps_start, seg_len, start_pad, end_pad, trig_start, trig_end)
     tlen = int(seg_len * sr)
     flen = tlen // 2 + 1
     df = 1.0 / seg_len
@@ -337,4 +337,232 @@ def condition_many(xs, sr, gps_starts, seg_len, start_pad, end_pad, psd_seg_len,
                            float(gps_starts[i]), [a for _, a in seg], [a for (a, _), _ in seg],
                            list(gates[i]) if gates else []))
         s0 = s1
+    return out
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# The reference's stage API (gpu_prep.py:214-470): argv-driven conditioning of one dataset. The CPU stages read the
+# store and apply the IIR high-pass; the GPU stages resample, add injections, gate, and condition. xp (cupy in the
+# reference) is accepted for the signature and ignored: arrays are torch tensors on the GPU.
+# ---------------------------------------------------------------------------------------------------------------------
+
+UNSUPPORTED = ('--gating-file', '--zpk-z', '--normalize-strain', '--strain-low-pass', '--taper-data',
+               '--witness-frame-type', '--sgburst-injection-file', '--psd-model', '--psd-file', '--asd-file',
+               '--fake-strain', '--filter-inj-only', '--allow-zero-padding', '--frame-files', '--frame-cache',
+               '--frame-type')
+
+
+def parser():
+    """The pycbc_inspiral options the conditioning reads (gpu_prep.py:24-44); the options it does not implement are
+    accepted by the parser and refused by options()."""
+    import argparse
+    p = argparse.ArgumentParser(add_help=False)
+    a = p.add_argument
+    a('--hdf-store'); a('--channel-name'); a('--gps-start-time', type=int); a('--gps-end-time', type=int)
+    a('--pad-data', type=int, default=0); a('--strain-high-pass', type=float); a('--sample-rate', type=float)
+    a('--low-frequency-cutoff', type=float); a('--segment-length', type=int)
+    a('--segment-start-pad', type=int); a('--segment-end-pad', type=int)
+    a('--trig-start-time', type=int, default=0); a('--trig-end-time', type=int, default=0)
+    a('--psd-estimation'); a('--psd-segment-length', type=float); a('--psd-segment-stride', type=float)
+    a('--psd-num-segments', type=int); a('--psd-inverse-length', type=float); a('--invpsd-trunc-method')
+    a('--autogating-threshold', type=float); a('--autogating-cluster', type=float, default=5.0)
+    a('--autogating-width', type=float, default=0.25); a('--autogating-taper', type=float, default=0.25)
+    a('--autogating-pad', type=float, default=16); a('--autogating-max-iterations', type=int, default=1)
+    a('--gating-method', default='taper')
+    a('--injection-file'); a('--injection-scale-factor', type=float, default=1); a('--injection-sample-rate', type=float)
+    for k in UNSUPPORTED:
+        a(k, dest='unsupported_' + k[2:].replace('-', '_'))
+    return p
+
+
+def options(argv):
+    """gpu_prep.py:306-311: parsed options; AssertionError on an option this path does not implement."""
+    opt, _ = parser().parse_known_args(argv)
+    bad = [k for k, v in vars(opt).items() if k.startswith('unsupported_') and v]
+    assert not bad, f'gpu_prep does not implement {bad}'
+    assert opt.psd_estimation in ('median', 'mean') and opt.gating_method == 'taper' and opt.hdf_store
+    return opt
+
+
+class _Laps:
+    """Stage timings as the reference's timing dicts (gpu_prep.py:296-303): seconds since the previous lap, with the
+    device synchronised first."""
+
+    def __init__(self, gpu):
+        import time
+        self.gpu, self.T, self.t, self._time = gpu, {}, time.perf_counter(), time.perf_counter
+
+    def __call__(self, k):
+        if self.gpu:
+            torch.cuda.synchronize()
+        self.T[k] = round(self._time() - self.t, 4)
+        self.t = self._time()
+
+
+def write_store(path, channel, segments):
+    """A strain store in the layout the stage API reads (the reference's HDF5 store, gpu_prep.py:321-330, as a numpy
+    .npz archive: h5py is not available here): per channel the arrays '{channel}/segments/start' and '.../end' (GPS
+    seconds) and '{channel}/{i}' (float64 strain of segment i). segments: [(gps_start, gps_end, strain)]. Written to
+    a temporary name and renamed."""
+    arrs = {f'{channel}/segments/start': np.array([a for a, _, _ in segments], np.float64),
+            f'{channel}/segments/end': np.array([b for _, b, _ in segments], np.float64)}
+    for i, (_, _, x) in enumerate(segments):
+        arrs[f'{channel}/{i}'] = np.asarray(x, np.float64)
+    tmp = path + '.part'
+    with open(tmp, 'wb') as f:
+        np.savez(f, **arrs)
+    import os
+    os.replace(tmp, path)
+
+
+def highpass_cpu(x, sr, epoch, freq, filter_order=8, attenuation=0.1):
+    """The IIR high-pass of pycbc.filter.highpass (gpu_prep.py:56-64 calls lal's HighPassREAL8/REAL4TimeSeries with
+    1 - attenuation at freq and order filter_order). lal is not available here: this is a Butterworth high-pass of
+    that order applied forward and backward (zero phase), whose two-pass amplitude at freq is 1 - attenuation, in the
+    input's precision. Parity with lal's design is unpinned."""
+    import scipy.signal
+    x = np.asarray(x)
+    # two passes: |H(f)|^2 = 1 / (1 + (fc / f)^(2 order)) at f = freq equals 1 - attenuation
+    a2 = 1.0 - attenuation
+    fc = freq * (1.0 / a2 - 1.0) ** (1.0 / (2 * filter_order))
+    sos = scipy.signal.butter(filter_order, fc, btype='highpass', fs=sr, output='sos')
+    return scipy.signal.sosfiltfilt(sos, x.astype(np.float64)).astype(x.dtype)
+
+
+def cpu_stage1(argv):
+    """gpu_prep.py:314-337: read the dataset's span (with padding) from the store and apply the first (float64)
+    high-pass. Returns (x float64 host array at the store's rate, timing)."""
+    L = _Laps(False)
+    opt = options(argv)
+    pad = opt.pad_data
+    t0 = opt.gps_start_time - pad
+    with np.load(opt.hdf_store) as f:
+        ch = opt.channel_name
+        starts, ends = f[f'{ch}/segments/start'], f[f'{ch}/segments/end']
+        diff = t0 - starts
+        loc = np.where(diff >= 0)[0]
+        assert len(loc), 'no store segment starts at or before the dataset'
+        sidx = loc[diff[loc].argmin()]
+        assert ends[sidx] >= opt.gps_end_time + pad, 'the store segment ends before the dataset'
+        d = f[f'{ch}/{sidx}']
+        sr0 = len(d) / (ends[sidx] - starts[sidx])
+        x = d[int((t0 - starts[sidx]) * sr0):int((opt.gps_end_time + pad - starts[sidx]) * sr0)]
+    L('read')
+    if opt.strain_high_pass:
+        x = highpass_cpu(x, sr0, t0, opt.strain_high_pass)
+    L('highpass1_cpu')
+    return x, L.T
+
+
+def n_resampled(argv):
+    o = options(argv)
+    return int(round((o.gps_end_time - o.gps_start_time + 2 * o.pad_data) * o.sample_rate))
+
+
+def injection_runs(argv):
+    """gpu_prep.py:345-356. Injections from a file need PyCBC's InjectionSet, which is not available here: raises
+    NotImplementedError when --injection-file is given (pass a dense injection signal to gpu_stage1 instead)."""
+    if options(argv).injection_file:
+        raise NotImplementedError('--injection-file needs PyCBC (absent); pass inj= to gpu_stage1 / prep')
+    return [], 0.0
+
+
+def injection_dense(runs, n):
+    x = np.zeros(n)
+    for a, v in runs:
+        x[a:a + len(v)] = v
+    return x
+
+
+def resample_ldas(x, factor):
+    """gpu_prep.py:67-84 on the GPU (gwgs_fir_decimate): x float64 tensor; the FIR taps are scipy's firwin, as the
+    reference's."""
+    import scipy.signal
+    ntaps = factor * 20 + 1
+    c = torch.from_numpy(scipy.signal.firwin(ntaps, 1.0 / factor, window=('kaiser', 5))).to(x.device)
+    x = x.to(torch.float64).contiguous()
+    out = torch.empty((x.shape[0] + factor - 1) // factor, dtype=torch.float64, device=x.device)
+    _lib.call('gwgs_fir_decimate', x.data_ptr(), x.shape[0], c.data_ptr(), ntaps, factor, out.data_ptr(),
+              _dev.stream())
+    return out
+
+
+def gpu_stage1(xp, x, argv, inj=None):
+    """gpu_prep.py:359-388: resample to --sample-rate, add the injection signal (dense float64 at the new rate; from
+    --injection-file only through PyCBC, see injection_runs), x DYN_RANGE_FAC to float32, autogating. Returns (gated
+    float32 strain on the GPU, gates [(time, width, taper)], timing)."""
+    L = _Laps(True)
+    opt = options(argv)
+    t0 = opt.gps_start_time - opt.pad_data
+    sr = opt.sample_rate
+    sr0 = len(x) / (opt.gps_end_time - opt.gps_start_time + 2 * opt.pad_data)
+    factor = int(round(sr0 / sr))
+    xg = torch.as_tensor(np.asarray(x, np.float64)).cuda() if not isinstance(x, torch.Tensor) else x.to(torch.float64)
+    if factor != 1:
+        xg = resample_ldas(xg, factor)
+    L('resample')
+    if inj is None and opt.injection_file:
+        injection_runs(argv)                      # raises: PyCBC's injection set is not available
+    if inj is not None:
+        xg = xg + torch.as_tensor(np.asarray(inj, np.float64)).to(xg.device)
+        L('injections')
+    s = (xg * DYN_RANGE_FAC).to(torch.float32)
+    del xg
+    gates = []
+    if opt.autogating_threshold is not None:
+        for _ in range(opt.autogating_max_iterations):
+            gt = detect_loud_glitches(s, sr, t0, opt.autogating_threshold, opt.autogating_cluster,
+                                      opt.strain_high_pass or 0.0, opt.pad_data + opt.autogating_pad)
+            gp = [(g_, opt.autogating_width, opt.autogating_taper) for g_ in gt]
+            gates += gp
+            gate_taper(s, sr, t0, gp)
+            if not gt:
+                break
+    L('autogating')
+    return s, [(float(g), w, tp) for g, w, tp in gates], L.T
+
+
+def cpu_stage2(h, argv):
+    """gpu_prep.py:391-397: the second (float32) high-pass, after gating, on a host array."""
+    L = _Laps(False)
+    opt = options(argv)
+    if opt.strain_high_pass:
+        h = highpass_cpu(h, opt.sample_rate, opt.gps_start_time - opt.pad_data, opt.strain_high_pass)
+    L('highpass2_cpu')
+    return h, L.T
+
+
+def gpu_stage2(xp, h, argv):
+    """gpu_prep.py:405-454: remove the padding, segment FFTs, overlapping PSDs (float64 median Welch here),
+    interpolation, truncation, D and IP. h: float32 strain (host array or GPU tensor) with its padding. Returns the
+    reference's dict: D, IP (GPU tensors), flen, tlen, df, sr, gps_start, analyze, seg_start, timing."""
+    L = _Laps(True)
+    opt = options(argv)
+    sr, pad = opt.sample_rate, opt.pad_data
+    s = torch.as_tensor(np.asarray(h, np.float32)).cuda() if not isinstance(h, torch.Tensor) else h
+    if pad:
+        s = s[int(pad * sr):int(s.shape[0] - sr * pad)]
+    d = condition(s, sr, float(opt.gps_start_time), opt.segment_length, opt.segment_start_pad, opt.segment_end_pad,
+                  opt.psd_segment_length, opt.psd_segment_stride, opt.psd_num_segments, opt.psd_inverse_length,
+                  opt.low_frequency_cutoff, opt.invpsd_trunc_method, opt.psd_estimation, trig_start=opt.trig_start_time,
+                  trig_end=opt.trig_end_time)
+    L('output')
+    return dict(D=d.D, IP=d.IP, flen=d.flen, tlen=d.tlen, df=d.df, sr=float(sr), gps_start=float(opt.gps_start_time),
+                analyze=d.analyze, seg_start=d.seg_start, timing=L.T)
+
+
+def prep(argv, xp=None, inj=None):
+    """gpu_prep.py:457-470: condition one dataset, all stages in this process. Returns gpu_stage2's dict plus
+    'gates' and the merged timing."""
+    x, T = cpu_stage1(argv)
+    s, gates, T1 = gpu_stage1(xp, x, argv, inj)
+    import time
+    t = time.perf_counter()
+    h = s.cpu().numpy()
+    del s
+    h, T2 = cpu_stage2(h, argv)
+    T2['highpass2_cpu'] = round(time.perf_counter() - t, 4)        # includes the device-to-host copy
+    out = gpu_stage2(xp, h, argv)
+    out['gates'] = gates
+    out['timing'] = {**T, **T1, **T2, **out['timing']}
     return out

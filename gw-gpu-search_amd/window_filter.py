@@ -62,8 +62,9 @@ def to_chunks(datasets, spec):
         dec = int(round(sr / (2 * v[3]))) if len(v) > 3 else 1
         assert dec >= 1 and Lc % (2 * dec) == 0 and hist % dec == 0 and fut % dec == 0
         nb = Lc // (2 * dec) + 1
-        Ds, IPs, an, ss = [], [], [], []
+        Ds, IPs, an, ss, first = [], [], [], [], []
         for s, (a0, a1) in enumerate(m.analyze):
+            first.append(len(an))
             st = chunk_starts(m.tlen, Lc, hist, fut, a0, a1)
             va, vb = valid(st, Lc, hist, fut, a1)
             assert np.all(st % dec == 0) and np.all(va % dec == 0) and np.all(vb % dec == 0)
@@ -74,5 +75,5 @@ def to_chunks(datasets, spec):
             an += [(int(x - y) // dec, int(z - y) // dec) for x, z, y in zip(va, vb, st)]
             ss += [int(m.seg_start[s] + y) // dec for y in st]
         out.append(fp_filter.Dataset(m.tag, torch.cat(Ds), torch.cat(IPs), nb, Lc // dec, sr / Lc, sr / dec,
-                                     m.gps_start, an, ss, list(m.gates)))
+                                     m.gps_start, an, ss, list(m.gates), first))
     return out, c_hist

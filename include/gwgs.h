@@ -191,6 +191,10 @@ int gwgs_gate_whiten(void* st, const double* psd, long flen, long kmin, double d
 int gwgs_gate_mag(const double* wt, long pad_start, long n, long cl, double df, float* mag, void* stream);
 /* gate_taper (gpu_prep.py:212-231): x[offset + i] *= inverted Tukey window(i), i in [i1, i2) */
 int gwgs_gate_apply(float* x, long n, long offset, long i1, long i2, long pad, long win, void* stream);
+/* resample_ldas (gpu_prep.py:67-84, pycbc's 'ldas' resampling): out[i] = sum_k c[k] x[factor i + ceil(ntaps / 2) - k]
+ * in float64 without fused multiply-add, zero where that index is below 2 (ntaps / 2) or at or past L;
+ * ceil(L / factor) outputs. */
+int gwgs_fir_decimate(const double* x, long L, const double* c, int ntaps, int factor, double* out, void* stream);
 
 /* ---- differentiable refinement (no reference function: the backward kernel of the peak SNR; its forward value
  * is pipeline/fp_filter.py:759-764's rebuilt SNR, evaluated off the sample grid and off the bank) -------------------

@@ -184,6 +184,26 @@ def condition(s, sr, gps_start, seg_len, start_pad, end_pad, psd_seg_len, psd_st
                 analyze=[a for _, a in seg], seg_start=[s0 for (s0, _), _ in seg], psd=psd, stilde=stilde)
 
 
+def resample_ldas(x, factor):
+    """gpu_prep.py:67-84 (pycbc resample_to_delta_t, method 'ldas'): causal FIR with scipy's firwin(20 factor + 1,
+    1 / factor, Kaiser beta 5) taps, the first 2 (ntaps // 2) outputs zeroed, rolled left by ceil(ntaps / 2),
+    decimated: out[i] = sum_k c[k] x[factor i + ceil(ntaps / 2) - k], float64, zero where that index is below
+    2 (ntaps // 2) or past the end."""
+    import scipy.signal
+    x = np.asarray(x, np.float64)
+    ntaps = factor * 20 + 1
+    c = scipy.signal.firwin(ntaps, 1.0 / factor, window=('kaiser', 5))
+    L = len(x)
+    n = np.arange(0, L, factor) + (ntaps + 1) // 2
+    ok = (n >= 2 * (ntaps // 2)) & (n < L)
+    out = np.zeros(len(n))
+    acc = np.zeros(int(ok.sum()))
+    for k in range(ntaps):
+        acc += c[k] * x[n[ok] - k]
+    out[ok] = acc
+    return out
+
+
 def cluster_over_window(idx, vals, window):
     """gpu_prep.py:161-172 (findchirp_cluster_over_window): keep the loudest of each run closer than window to the
     run's current keeper."""

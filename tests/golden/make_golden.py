@@ -1,7 +1,8 @@
 """Writes golden vectors from the reference's own numpy-only functions (run only where the reference's sources are
 mounted; the tests read the written vectors, never the reference). Only functions whose integer/index conventions no closed form
 pins are sampled: gpu_prep.segment_slices, overlapping_psd_spans, inverse_spectrum_truncation (edge bins, Hann
-halves), welch (float32 path), median_bias, interpolate, cluster_over_window, window_filter.chunk_starts/valid, and
+halves), welch (float32 path), median_bias, interpolate, cluster_over_window, window_filter.chunk_starts/valid,
+resample_ldas (the 'ldas' FIR decimation), and
 gpu_prep.gpu_stage2 (numpy backend) end to end on a seeded strain: one PSD span, and several spans with per-segment
 PSD picks (stage2_strain() below rebuilds the inputs from their seeds; D and IP are stored on a bin subsample).
 
@@ -94,6 +95,10 @@ def main():
         a, b = w.valid(st, Lc, hist, fut, hi)
         chunks.append(dict(case=[tlen, Lc, hist, fut, lo, hi], starts=st.tolist(), a=a.tolist(), b=b.tolist()))
     out['chunk_json'] = np.array(json.dumps(chunks))
+    xr = rng.standard_normal(6001)
+    out['resample_x'] = xr
+    out['resample_4'] = g.resample_ldas(np, xr.copy(), 4)
+    out['resample_2'] = g.resample_ldas(np, xr.copy(), 2)
     meta = []
     for name, seed, sr, seconds, pad, opts in STAGE2:
         h = stage2_strain(seed, sr, seconds, pad)

@@ -23,15 +23,21 @@ def test_welch_median_and_mean(gpu):
     np.testing.assert_allclose(p.cpu().numpy(), pr, rtol=1e-9)
 
 
-@pytest.mark.parametrize('nseg', [63, 64, 65, 200, 301, 512])
+@pytest.mark.parametrize('nseg', [63, 64, 65, 200, 301, 512, 513, 1500, 6000])
 def test_welch_median_many_segments(gpu, nseg):
-    """The median over up to 512 segments (odd and even counts, every staging size) against the oracle."""
+    """The median over up to 512 segments (odd and even counts, every staging size) and, through the one-bin-per-
+    workgroup path, over hours of data (the autogating PSD of a 12,272 s dataset has about 6,000) against the
+    oracle; the mean too for the largest."""
     import gpu_prep
     rng = np.random.default_rng(nseg)
     x = (rng.standard_normal((2, 128 * (nseg + 1))) * 3).astype(np.float32)
     p, _ = gpu_prep.welch(torch.from_numpy(x).cuda(), 256, 128, 1 / 2048., num_segments=nseg)
     pr, _ = core.welch(x, 256, 128, 1 / 2048., num_segments=nseg)
     np.testing.assert_allclose(p.cpu().numpy(), pr, rtol=1e-9)
+    if nseg == 6000:
+        p, _ = gpu_prep.welch(torch.from_numpy(x).cuda(), 256, 128, 1 / 2048., num_segments=nseg, avg='mean')
+        pr, _ = core.welch(x, 256, 128, 1 / 2048., num_segments=nseg, avg='mean')
+        np.testing.assert_allclose(p.cpu().numpy(), pr, rtol=1e-9)
 
 
 def test_interpolate_and_truncation(gpu):

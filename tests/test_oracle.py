@@ -377,3 +377,13 @@ def test_white_noise_snr_normalisation():
         acc.append((np.abs(z) ** 2 * norm[:, 0][:, None] ** 2).mean(1))
     m = np.mean(acc, 0)
     assert np.all(np.abs(m - 2.0) < 0.06), m
+
+
+def test_resample_ldas_golden():
+    """The 'ldas' FIR decimation by 2 and 4 equals the reference's resample_ldas bit for bit (float64, same tap
+    order), including the zeroed start-up samples and the tail past the data."""
+    for f in (2, 4):
+        got = core.resample_ldas(GOLD['resample_x'], f)
+        np.testing.assert_array_equal(got, GOLD[f'resample_{f}'])
+        ntaps = 20 * f + 1
+        assert np.all(got[:(2 * (ntaps // 2) - (ntaps + 1) // 2 + f - 1) // f] == 0) and got[-1] == 0
