@@ -48,6 +48,7 @@ SIGNATURES = {
     'gwgs_pack_zt': [_i, _p, _l, _i, _l, _l, _l, _p, _p, _p],
     'gwgs_stack_a': [_i, _p, _i, _i, _p, _p, _i, _i, _p, _p, _p],
     'gwgs_stack_a_fp8': [_p, _i, _i, _p, _p, _i, _p, _p, _p, _p],
+    'gwgs_stack_a_f16': [_p, _i, _i, _p, _p, _i, _p, _p, _p, _p],
     'gwgs_basis_power': [_p, _l, _i, _l, _l, _p, _p],
     'gwgs_basis_series_workspace': [_i, _l],
     'gwgs_basis_series': [_i, _p, _l, _i, _i, _p, _l, _l, _l, _l, _l, _p, _i, _p, _p, _p, _l, _p],

@@ -34,6 +34,8 @@ template <>
 __device__ __forceinline__ uint16_t cvt_out<uint16_t>(const float x) { return f32_to_bf16(x); }
 template <>
 __device__ __forceinline__ uint8_t cvt_out<uint8_t>(const float x) { return f32_to_fp8(x); }
+template <>
+__device__ __forceinline__ f16s cvt_out<f16s>(const float x) { return f16s{f32_to_f16(x)}; }
 
 template <typename T>
 __global__ void __launch_bounds__(NT) pack_zt_k(const float2* __restrict__ Zc, const long ldc, const int rp,
@@ -74,6 +76,8 @@ template <>
 __device__ __forceinline__ float from_f32<float>(const float x) { return x; }
 template <>
 __device__ __forceinline__ uint16_t from_f32<uint16_t>(const float x) { return f32_to_bf16(x); }
+template <>
+__device__ __forceinline__ f16s from_f32<f16s>(const float x) { return f16s{f32_to_f16(x)}; }
 
 template <typename T>
 __global__ void stack_a_k(const float2* __restrict__ A, const int n, const int r, const float* __restrict__ rs,
@@ -95,13 +99,14 @@ __global__ void stack_a_k(const float2* __restrict__ A, const int n, const int r
     Ai[(size_t)m * K + 2 * j + 1] = from_f32<T>(re);
 }
 
-// fp8 operands for one template row per block: values As = A[m, j] rs[m] cs[j] (Re/Im stacked as stack_a_k), a row
-// scale ra = 224 / max |As| so the row spans half the e4m3 range, bytes fp8(As ra), and pw_scale[m] = 1 / ra^2,
-// which the GEMM applies to the row's |z|^2.
-__global__ void __launch_bounds__(NT) stack_a_fp8_k(const float2* __restrict__ A, const int n, const int r,
-                                                    const float* __restrict__ rs, const float* __restrict__ cs,
-                                                    const int K, uint8_t* __restrict__ Ar, uint8_t* __restrict__ Ai,
-                                                    float* __restrict__ pw_scale)
+// fp8 / fp16 operands for one template row per block: values As = A[m, j] rs[m] cs[j] (Re/Im stacked as stack_a_k),
+// a row scale ra = TGT / max |As| (224: half the e4m3 range; 1024 for fp16), operands As ra in the output type, and
+// pw_scale[m] = 1 / ra^2, which the GEMM applies to the row's |z|^2.
+template <typename T, int TGT>
+__global__ void __launch_bounds__(NT) stack_a_scaled_k(const float2* __restrict__ A, const int n, const int r,
+                                                       const float* __restrict__ rs, const float* __restrict__ cs,
+                                                       const int K, T* __restrict__ Ar, T* __restrict__ Ai,
+                                                       float* __restrict__ pw_scale)
 {
     const int m = blockIdx.x;
     float mx = 0.f;
@@ -116,7 +121,7 @@ __global__ void __launch_bounds__(NT) stack_a_fp8_k(const float2* __restrict__ A
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
     __syncthreads();
     mx = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
-    const float ra = mx > 0.f ? 224.f / mx : 1.f;
+    const float ra = mx > 0.f ? (float)TGT / mx : 1.f;
     if (threadIdx.x == 0) pw_scale[m] = 1.f / (ra * ra);
     for (int j = threadIdx.x; j < K / 2; j += NT) {
         float re = 0.f, im = 0.f;
@@ -125,10 +130,10 @@ __global__ void __launch_bounds__(NT) stack_a_fp8_k(const float2* __restrict__ A
             const float s = rs[m] * (cs ? cs[j] : 1.f) * ra;
             re = a.x * s; im = a.y * s;
         }
-        Ar[(size_t)m * K + 2 * j] = f32_to_fp8(re);
-        Ar[(size_t)m * K + 2 * j + 1] = f32_to_fp8(-im);
-        Ai[(size_t)m * K + 2 * j] = f32_to_fp8(im);
-        Ai[(size_t)m * K + 2 * j + 1] = f32_to_fp8(re);
+        Ar[(size_t)m * K + 2 * j] = cvt_out<T>(re);
+        Ar[(size_t)m * K + 2 * j + 1] = cvt_out<T>(-im);
+        Ai[(size_t)m * K + 2 * j] = cvt_out<T>(im);
+        Ai[(size_t)m * K + 2 * j + 1] = cvt_out<T>(re);
     }
 }
 
@@ -160,7 +165,8 @@ GWGS_API int gwgs_form_q(const void* Vc, int rp, long flen, const void* D, long 
     return gwgs_status();
 }
 
-// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (values clamped to +-448). Zc (rp, ldc) complex64; Zt (Tpad, 2 rp).
+// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (values clamped to +-448), 3: fp16. Zc (rp, ldc) complex64; Zt
+// (Tpad, 2 rp).
 // colscale: per basis vector or null.
 GWGS_API int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, long Tn, long Tpad,
                           const float* colscale, void* Zt, void* stream)
@@ -169,7 +175,10 @@ GWGS_API int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, 
                GWGS_EINVAL);
     if (Tpad == 0) return GWGS_OK;
     const dim3 grid((unsigned)((Tpad + 63) / 64), rp / 16);
-    if (dtype == 2)
+    if (dtype == 3)
+        hipLaunchKernelGGL(pack_zt_k<f16s>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)Zc, ldc, rp, a0,
+                           Tn, Tpad, colscale, (f16s*)Zt);
+    else if (dtype == 2)
         hipLaunchKernelGGL(pack_zt_k<uint8_t>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)Zc, ldc, rp, a0,
                            Tn, Tpad, colscale, (uint8_t*)Zt);
     else if (dtype == 1)
@@ -190,7 +199,10 @@ GWGS_API int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* r
     const long tot = (long)Mpad * (K / 2);
     if (tot == 0) return GWGS_OK;
     const dim3 grid((unsigned)((tot + NT - 1) / NT));
-    if (dtype == 1)
+    if (dtype == 3)
+        hipLaunchKernelGGL(stack_a_k<f16s>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)A, n, r,
+                           rowscale, colscale, Mpad, K, (f16s*)Ar, (f16s*)Ai);
+    else if (dtype == 1)
         hipLaunchKernelGGL(stack_a_k<uint16_t>, grid, dim3(NT), 0, (hipStream_t)stream, (const float2*)A, n, r,
                            rowscale, colscale, Mpad, K, (uint16_t*)Ar, (uint16_t*)Ai);
     else if (dtype == 0)
@@ -201,14 +213,25 @@ GWGS_API int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* r
     return gwgs_status();
 }
 
-// fp8 operands: Ar, Ai (n, K) bytes and pw_scale (n floats); see stack_a_fp8_k.
+// fp8 operands: Ar, Ai (n, K) bytes and pw_scale (n floats); see stack_a_scaled_k.
 GWGS_API int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const float* colscale, int K,
                               void* Ar, void* Ai, float* pw_scale, void* stream)
 {
     GWGS_CHECK(A && rowscale && Ar && Ai && pw_scale && n >= 0 && r > 0 && K >= 2 * r && K % 2 == 0, GWGS_EINVAL);
     if (n == 0) return GWGS_OK;
-    hipLaunchKernelGGL(stack_a_fp8_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)A, n, r, rowscale,
-                       colscale, K, (uint8_t*)Ar, (uint8_t*)Ai, pw_scale);
+    hipLaunchKernelGGL((stack_a_scaled_k<uint8_t, 224>), dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)A,
+                       n, r, rowscale, colscale, K, (uint8_t*)Ar, (uint8_t*)Ai, pw_scale);
+    return gwgs_status();
+}
+
+// fp16 operands with a per-template scale to max |As| = 1024 (the GEMM's fp16 form): as gwgs_stack_a_fp8.
+GWGS_API int gwgs_stack_a_f16(const void* A, int n, int r, const float* rowscale, const float* colscale, int K,
+                              void* Ar, void* Ai, float* pw_scale, void* stream)
+{
+    GWGS_CHECK(A && rowscale && Ar && Ai && pw_scale && n >= 0 && r > 0 && K >= 2 * r && K % 2 == 0, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL((stack_a_scaled_k<f16s, 1024>), dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)A,
+                       n, r, rowscale, colscale, K, (f16s*)Ar, (f16s*)Ai, pw_scale);
     return gwgs_status();
 }
 

@@ -342,6 +342,14 @@ struct Out<uint16_t> {
     }
 };
 template <>
+struct Out<f16s> {
+    static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
+                                                 const float im)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_f16x2(re, im), rs, (int)off, 0, 0);
+    }
+};
+template <>
 struct Out<uint8_t> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
@@ -511,6 +519,7 @@ int series(int dtype, int lg, const float2* vc, long ldv, int r, int rp, const f
     if (e) return e;
     if (dtype == 0) return rows_dispatch<float, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (float*)Zt, ldz, s);
     if (dtype == 1) return rows_dispatch<uint16_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint16_t*)Zt, ldz, s);
+    if (dtype == 3) return rows_dispatch<f16s, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (f16s*)Zt, ldz, s);
     return rows_dispatch<uint8_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint8_t*)Zt, ldz, s);
 }
 
@@ -533,7 +542,7 @@ GWGS_API long gwgs_basis_series_workspace(int r, long n)
     return (long)r * n * 8;
 }
 
-// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (clamped to +-448). Vc (>= r rows, row pitch ldv >= kend) and D
+// dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (clamped to +-448), 3: fp16. Vc (>= r rows, row pitch ldv >= kend) and D
 // (>= kend) complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt (Tn rows, ldz >= 2 rp elements, even);
 // colscale: per basis row, or null. ytype 0: the intermediate in complex64; 1: in fp16 pairs scaled by yscale[j]
 // (r floats, e.g. sqrt(n1 / sum_k |x_jk|^2); null = 1), divided out again in the row pass.
@@ -546,9 +555,9 @@ GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int r
     const int lg = log2_exact(n);
     if (lg < 13 || lg > 22) return GWGS_EUNSUP;
     // the row kernel addresses Zt as one buffer with 32-bit byte offsets up to (Tn + 1) rows
-    const double esz = dtype == 0 ? 4.0 : dtype == 1 ? 2.0 : 1.0;
+    const double esz = dtype == 0 ? 4.0 : dtype == 2 ? 1.0 : 2.0;
     if ((double)(Tn + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;
-    if (dtype < 0 || dtype > 2 || ytype < 0 || ytype > 1) return GWGS_EUNSUP;
+    if (dtype < 0 || dtype > 3 || ytype < 0 || ytype > 1) return GWGS_EUNSUP;
     hipStream_t s = (hipStream_t)stream;
     const int e = ytype == 0 ? series<YF32>(dtype, lg, (const float2*)Vc, ldv, r, rp, (const float2*)D, kmin, kend, a0,
                                             Tn, colscale, yscale, work, Zt, ldz, s)

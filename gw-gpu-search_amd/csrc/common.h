@@ -49,6 +49,18 @@ __device__ __forceinline__ float wave_sum_f32(float x)
     return x;
 }
 
+// fp16 (IEEE binary16) operand storage: a 2-byte type of its own, so that templates over the operand type take the
+// f16 MFMA and conversions where they take the bf16 ones for uint16_t
+struct f16s { uint16_t v; };
+
+// float -> fp16 bits, round to nearest even (v_cvt_f16_f32); overflow goes to +-inf
+__device__ __forceinline__ uint16_t f32_to_f16(const float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__device__ __forceinline__ uint32_t f32x2_to_f16x2(const float a, const float b)
+{
+    return (uint32_t)f32_to_f16(a) | ((uint32_t)f32_to_f16(b) << 16);
+}
+__device__ __forceinline__ float f16_to_f32(const uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
 // two floats -> a packed bfloat16 pair (a in the low half), round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
 __device__ __forceinline__ uint32_t f32x2_to_bf16x2(const float a, const float b)
 {

@@ -26,6 +26,19 @@
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// The 32x32x16 MFMA of a 2-byte operand type: bf16 for uint16_t, fp16 for f16s; operands are passed as the 8
+// elements' bits (a bf16x8 value) in either case.
+template <typename T>
+__device__ __forceinline__ f32x16 mfma16(const bf16x8 a, const bf16x8 b, const f32x16 c, int, int, int)
+{
+    if constexpr (__is_same(T, f16s))
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 // One item of a launch: tiles [t_first, t_end) of one segment (tile = 128 samples).
@@ -324,6 +337,7 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
 // time tiles, and the epilogue after each tile's last chunk is the same in-register scan as above.
 template <typename T> struct KloopTraits;
 template <> struct KloopTraits<uint16_t> { static constexpr int KC = 64, EPV = 8, PAD = 8; };
+template <> struct KloopTraits<f16s> { static constexpr int KC = 64, EPV = 8, PAD = 8; };
 template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, PAD = 4; };
 template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16; };
 
@@ -432,12 +446,12 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
                 for (int ts = 0; ts < 4; ts++) {
                     const uint4 zv = *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]);
                     const bf16x8 z = __builtin_bit_cast(bf16x8, zv);
-                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_r, cr[ts], 0, 0, 0);
+                    cr[ts] = mfma16<T>(z, a_r, cr[ts], 0, 0, 0);
                     if constexpr (ROT)
-                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
+                        ci[ts] = mfma16<T>(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
                                                                          ci[ts], 0, 0, 0);
                     else
-                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
+                        ci[ts] = mfma16<T>(z, a_i, ci[ts], 0, 0, 0);
                 }
             }
         } else {
@@ -566,9 +580,9 @@ gemm_peaks_kloop_ra_k(const T* __restrict__ Zt, const long ldz, const T* __restr
 #pragma unroll
                 for (int ts = 0; ts < 4; ts++) {
                     const uint4 zv = *(const uint4*)(&zs[(ts * 32 + col) * PITCH + ks * 16 + half * 8]);
-                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, zv), a_r, cr[ts], 0, 0,
+                    cr[ts] = mfma16<T>(__builtin_bit_cast(bf16x8, zv), a_r, cr[ts], 0, 0,
                                                                      0);
-                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
+                    ci[ts] = mfma16<T>(__builtin_bit_cast(bf16x8, rot_bf16(zv)), a_r,
                                                                      ci[ts], 0, 0, 0);
                 }
             }
@@ -675,8 +689,8 @@ gemm_peaks_ring_k(const T* __restrict__ Zt, const long ldz, const T* __restrict_
 #pragma unroll
                 for (int ts = 0; ts < 4; ts++) {
                     const bf16x8 zf = __builtin_bit_cast(bf16x8, *(const uint4*)(&z[(ts * 32 + col) * PITCH + ks * 16 + half * 8]));
-                    cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_r, cr[ts], 0, 0, 0);
-                    ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_i, ci[ts], 0, 0, 0);
+                    cr[ts] = mfma16<T>(zf, a_r, cr[ts], 0, 0, 0);
+                    ci[ts] = mfma16<T>(zf, a_i, ci[ts], 0, 0, 0);
                 }
             }
         }
@@ -813,8 +827,8 @@ gemm_peaks_dma_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__
 #pragma unroll
                     for (int ts = 0; ts < 4; ts++) {
                         const bf16x8 zf = __builtin_bit_cast(bf16x8, *(const uint4*)(z + (ts * 32 + col) * ROWB + so));
-                        cr[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_r, cr[ts], 0, 0, 0);
-                        ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zf, a_i, ci[ts], 0, 0, 0);
+                        cr[ts] = mfma16<T>(zf, a_r, cr[ts], 0, 0, 0);
+                        ci[ts] = mfma16<T>(zf, a_i, ci[ts], 0, 0, 0);
                     }
                 }
             }
@@ -947,6 +961,11 @@ int launch(int dtype, int variant, const void* Zt, long ldz, const void* Ar, con
         GWGS_CHECK(aligned && ldz % 16 == 0 && lda % 16 == 0 && K % 16 == 0, GWGS_EINVAL);
         launch_kloop_any<uint8_t>(variant, rot, mode, grid, block, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
                                   (const uint8_t*)Ai, lda, K, J, V, I, ldv);
+    } else if (dtype == 3) {
+        // fp16 operands (the reference's type): the K loops only, with v_mfma_f32_32x32x16_f16
+        GWGS_CHECK(aligned && ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0, GWGS_EINVAL);
+        launch_kloop_any<f16s>(variant, rot, mode, grid, block, s, (const f16s*)Zt, ldz, (const f16s*)Ar,
+                               (const f16s*)Ai, lda, K, J, V, I, ldv);
     } else {
         return GWGS_EUNSUP;
     }
@@ -955,7 +974,7 @@ int launch(int dtype, int variant, const void* Zt, long ldz, const void* Ar, con
 
 } // namespace
 
-// One segment. dtype 0 = fp32, 1 = bf16, 2 = fp8 e4m3 (OCP) operands. variant: 0 = pick by K (register-resident
+// One segment. dtype 0 = fp32, 1 = bf16, 2 = fp8 e4m3 (OCP), 3 = fp16 operands. variant: 0 = pick by K (register-resident
 // template operands up to K = 256 bf16 / 128 fp32, else the K loop), 1 = always the K loop (fp8 always).
 // rowscale (M floats, may be null) multiplies each template's reported |z|^2.
 GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
@@ -1002,5 +1021,5 @@ GWGS_API int gwgs_gemm_item_bytes(void) { return (int)sizeof(GemmItem); }
 
 GWGS_API int gwgs_gemm_max_k(int dtype)
 {
-    return dtype >= 0 && dtype <= 2 ? (1 << 20) : 0;
+    return dtype >= 0 && dtype <= 3 ? (1 << 20) : 0;
 }

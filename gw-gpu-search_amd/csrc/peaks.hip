@@ -138,6 +138,8 @@ template <>
 __device__ __forceinline__ float to_f32<uint16_t>(const uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
 template <>
 __device__ __forceinline__ float to_f32<uint8_t>(const uint8_t x) { return fp8_to_f32(x); }
+template <>
+__device__ __forceinline__ float to_f32<f16s>(const f16s x) { return f16_to_f32(x.v); }
 
 template <typename T>
 __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar,
@@ -282,7 +284,10 @@ GWGS_API int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar,
     if (n == 0) return GWGS_OK;
     const dim3 grid((n + NT / 64 - 1) / (NT / 64));
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == 2)
+    if (dtype == 3)
+        hipLaunchKernelGGL(rebuilt_z_k<f16s>, grid, dim3(NT), 0, s, (const f16s*)Zt, ldz, (const f16s*)Ar,
+                           (const f16s*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
+    else if (dtype == 2)
         hipLaunchKernelGGL(rebuilt_z_k<uint8_t>, grid, dim3(NT), 0, s, (const uint8_t*)Zt, ldz, (const uint8_t*)Ar,
                            (const uint8_t*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
     else if (dtype == 1)

@@ -45,8 +45,8 @@ from exact_stage import ExactStage, PeakRequest, SegmentData, TemplateTile
 
 CH = 1 << 16          # frequency chunk for the Gram and basis products
 BN = fp_gemm.BN
-PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp8': 2}
-OPERAND_DTYPES = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8}
+PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp8': 2, 'fp16': 3}
+OPERAND_DTYPES = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8, 3: torch.float16}
 
 
 @dataclass
@@ -488,7 +488,7 @@ class Filter:
             # the transform's intermediate: complex64 for the fp32 form; two fp16 scaled to unit rms per basis row
             # for the bf16 / fp8 forms (half the bytes; its 11-bit rounding is well below theirs)
             ytype = 0 if self.code == 0 else 1
-            if self.code == 2:
+            if self.code in (2, 3):
                 pw = torch.zeros((nb, rp), dtype=torch.float64, device='cuda')
                 for i, (di, s, gs, a0, nw) in enumerate(segs):
                     _lib.call('gwgs_vd_power', tb.Vc.data_ptr(), self.flen, tb.r, self.data[di].D[s].data_ptr(),
@@ -512,7 +512,7 @@ class Filter:
             _lib.call('gwgs_form_q', tb.Vc.data_ptr(), rp, self.flen, self.data[di].D[s].data_ptr(), self.tlen,
                       q[i].data_ptr(), _dev.stream())
         Zc = torch.fft.ifft(q, dim=2, norm='forward')                 # (nb, rp, tlen): basis series
-        if self.code == 2:
+        if self.code in (2, 3):
             pw = torch.empty((nb, rp), dtype=torch.float64, device='cuda')
             for i in range(nb):
                 _lib.call('gwgs_basis_power', q[i].data_ptr(), self.tlen, rp, 0, self.flen, pw[i].data_ptr(),
@@ -796,10 +796,12 @@ class Filter:
             iw = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
         ones = self._ones(n)
         rows2 = (norm[:, geo['gsl']] ** 2).T.to(torch.float32)         # (nb, n): norm^2 per (segment, template)
-        if self.code == 2:
+        if self.code in (2, 3):
+            # per-basis scales (unit-rms Z, rms on A) and a per-template scale to the operand range, whose inverse
+            # square the GEMM applies to |z|^2
             pws = torch.empty(n, dtype=torch.float32, device='cuda')
-            _lib.call('gwgs_stack_a_fp8', tb.A.data_ptr(), n, tb.r, ones.data_ptr(), acs.data_ptr(), K,
-                      Ar.data_ptr(), Ai.data_ptr(), pws.data_ptr(), _dev.stream())
+            _lib.call('gwgs_stack_a_fp8' if self.code == 2 else 'gwgs_stack_a_f16', tb.A.data_ptr(), n, tb.r,
+                      ones.data_ptr(), acs.data_ptr(), K, Ar.data_ptr(), Ai.data_ptr(), pws.data_ptr(), _dev.stream())
             rows2 = rows2 * pws[None, :]
         else:
             _lib.call('gwgs_stack_a', self.code, tb.A.data_ptr(), n, tb.r, ones.data_ptr(), None, n, K,

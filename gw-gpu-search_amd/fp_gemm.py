@@ -3,7 +3,8 @@
 z_re = Are @ Z, z_im = Aim @ Z; each launch keeps only the max |z|^2 and its sample per BN = 128 samples (mode 0,
 fused_peaks) or per cluster window of cw samples (mode 1, fused_window_peaks, the window max of fp_filter.py:745-746
 done in the epilogue). The kernel is hand-written HIP for gfx950 (csrc/gemm_peaks.hip): fp32 operands on
-v_mfma_f32_32x32x2_f32 (the plain form), bf16 operands on v_mfma_f32_32x32x16_bf16, or fp8 e4m3 operands on
+v_mfma_f32_32x32x2_f32 (the plain form), bf16 operands on v_mfma_f32_32x32x16_bf16, fp16 operands (the
+reference's type) on v_mfma_f32_32x32x16_f16, or fp8 e4m3 operands on
 v_mfma_scale_f32_32x32x64_f8f6f4 (unit block scales; per-template factors applied to |z|^2 in the epilogue), fp32
 accumulation.
 
@@ -21,7 +22,7 @@ import _dev
 import _lib
 
 BN = 128
-DTYPES = {torch.float32: 0, torch.bfloat16: 1, torch.float8_e4m3fn: 2}
+DTYPES = {torch.float32: 0, torch.bfloat16: 1, torch.float8_e4m3fn: 2, torch.float16: 3}
 
 
 def _kpad(K):
@@ -32,8 +33,6 @@ def _prep_operands(Are, Aim, Z, Tn, zt):
     """Pad K to a multiple of 16 and rows of Zt to a multiple of BN; return (Zt, Ar, Ai, K, dtype code). float8_e4m3fn
     operands are passed to the kernel as their bytes."""
     dt = Are.dtype
-    if dt == torch.float16:
-        dt = torch.bfloat16
     assert dt in DTYPES, f'unsupported operand dtype {Are.dtype}'
     K = Are.shape[1]
     Kp = _kpad(K)
@@ -117,8 +116,8 @@ def gemm_peaks_multi(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, mode, per, c
 
 
 def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False, rowscale=None):
-    """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32 / bfloat16 (float16 is taken as
-    bfloat16). Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are
+    """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32, bfloat16, float16 (the reference's
+    operand type, on the fp16 MFMA) or float8_e4m3fn. Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are
     masked to -1 as in fp_gemm.py:29. cfg: (tiles_per_block,) from pick_cfg, or None."""
     Zt, Ar, Ai, K, code = _prep_operands(Are, Aim, Z, Tn, zt)
     nblk = -(-Tn // BN)

@@ -24,7 +24,8 @@ extern "C" {
 /* ---- fused SNR-reconstruction GEMM (pipeline/fp_gemm.py:14-44 gemm_absmax / fused_peaks; with mode 1 also the
  * cluster-window max of pipeline/fp_filter.py:745-746) ------------------------------------------------------------
  * z_re[m, t] = sum_k Ar[m, k] Zt[t, k], z_im[m, t] = sum_k Ai[m, k] Zt[t, k], pw = z_re^2 + z_im^2.
- * dtype: 0 = float32 operands, 1 = bfloat16 (bits in uint16), 2 = fp8 e4m3 bytes; fp32 accumulation.
+ * dtype: 0 = float32 operands, 1 = bfloat16 (bits in uint16), 2 = fp8 e4m3 bytes, 3 = float16 (the reference's
+ * operand type, v_mfma_f32_32x32x16_f16; K loops only); fp32 accumulation.
  * rowscale (M floats or NULL) multiplies template m's reported |z|^2.
  * variant: 0 = choose by K (template operands held in registers up to K = 256 bf16 / 128 fp32, else the K loop).
  * Zt: (>= ceil(Tn/128)*128 rows, ldz) time-major; Ar, Ai: (M, lda).
@@ -117,17 +118,21 @@ int gwgs_edges_fused_multi(const void* h, long ldh, const int* rows, const float
  * q[j, k] = Vc[j, k] D[k] for k < flen, j < rp (q: (rp, ldq) complex64; columns >= flen are not written). */
 int gwgs_form_q(const void* Vc, int rp, long flen, const void* D, long ldq, void* q, void* stream);
 /* Zt[t, 2j] = Re Zc[j, a0 + t] cs[j], Zt[t, 2j+1] = Im Zc[j, a0 + t] cs[j] for t < Tn, zero for Tn <= t < Tpad;
- * Zc (rp, ldc) complex64, rp % 16 == 0; Zt (Tpad, 2 rp) float32 (dtype 0), bfloat16 (1) or fp8 e4m3 bytes (2,
- * saturated to +-448); cs may be NULL. */
+ * Zc (rp, ldc) complex64, rp % 16 == 0; Zt (Tpad, 2 rp) float32 (dtype 0), bfloat16 (1), fp8 e4m3 bytes (2,
+ * saturated to +-448) or float16 (3); cs may be NULL. */
 int gwgs_pack_zt(int dtype, const void* Zc, long ldc, int rp, long a0, long Tn, long Tpad, const float* colscale,
                  void* Zt, void* stream);
 /* As = A[m, j] rowscale[m] colscale[j] (A (n, r) complex64); Ar[m, 2j] = Re As, Ar[m, 2j+1] = -Im As,
- * Ai[m, 2j] = Im As, Ai[m, 2j+1] = Re As; zero for j >= r, m >= n. Ar, Ai (Mpad, K). */
+ * Ai[m, 2j] = Im As, Ai[m, 2j+1] = Re As; zero for j >= r, m >= n. Ar, Ai (Mpad, K); dtype 0, 1 or 3. */
 int gwgs_stack_a(int dtype, const void* A, int n, int r, const float* rowscale, const float* colscale, int Mpad,
                  int K, void* Ar, void* Ai, void* stream);
 /* fp8 operands: per template row the scale ra = 224 / max |As| (As as gwgs_stack_a), Ar/Ai = e4m3(As ra) bytes
  * (n, K), pw_scale[m] = 1 / ra^2 (to multiply |z|^2). */
 int gwgs_stack_a_fp8(const void* A, int n, int r, const float* rowscale, const float* colscale, int K, void* Ar,
+                     void* Ai, float* pw_scale, void* stream);
+/* pipeline/fp_filter.py:615-621 stacked_A with fp16 operands as the reference's (whose single global scale is here a
+ * per-template ra = 1024 / max |As|): Ar/Ai = fp16(As ra), pw_scale[m] = 1 / ra^2. */
+int gwgs_stack_a_f16(const void* A, int n, int r, const float* rowscale, const float* colscale, int K, void* Ar,
                      void* Ai, float* pw_scale, void* stream);
 /* out[j] = sum_{a0 <= t < a0 + Tn} |Zc[j, t]|^2 (double), the rms scale of pipeline/fp_filter.py:609-610 */
 int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double* out, void* stream);
@@ -136,7 +141,8 @@ int gwgs_basis_power(const void* Zc, long ldc, int rp, long a0, long Tn, double*
  * torch.fft.ifft(q) * n and the fp16 stacked Z), as a four-step FFT of two passes for power-of-two n from 2^13 to
  * 2^22: x_j[k] = Vc[j, k] D[k] on [kmin, kend) (0 elsewhere), z_j[t] = sum_k x_j[k] e^(2 pi i k t / n), written as
  * Zt[t - a0, 2j] = Re z_j[t] cs[j], Zt[t - a0, 2j + 1] = Im z_j[t] cs[j] for a0 <= t < a0 + Tn, j < r, and zeros
- * for r <= j < rp. dtype 0 float32, 1 bfloat16 (nearest even), 2 fp8 e4m3 (saturated to +-448); colscale may be
+ * for r <= j < rp. dtype 0 float32, 1 bfloat16 (nearest even), 2 fp8 e4m3 (saturated to +-448), 3 float16
+ * (nearest even); colscale may be
  * NULL. Vc: >= r rows of pitch ldv complex64; D complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt: Tn
  * rows of ldz >= 2 rp elements (ldz even). Returns GWGS_EUNSUP for other n, GWGS_ESIZE when Zt exceeds 4 GiB. */
 long gwgs_basis_series_workspace(int r, long n);

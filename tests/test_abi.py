@@ -40,7 +40,7 @@ def test_bad_arguments_rejected(lib):
     assert lib.gwgs_chisq_points_workspace(3, 17) == 3 * 16 * pg * 8
     assert lib.gwgs_chisq_points_workspace(0, 17) == 0 and lib.gwgs_chisq_points_workspace(3, 1) == 0
     assert lib.gwgs_edges_fused(None, 10, None, None, None, None, 2, 0, 1, None, None) == -1
-    assert lib.gwgs_welch_reduce(ctypes.c_void_p(16), 1, 600, 8, 1.0, 0, 1.0, 1.0, ctypes.c_void_p(16), None) == -2
+    assert lib.gwgs_welch_reduce(ctypes.c_void_p(16), 1, 20000, 8, 1.0, 0, 1.0, 1.0, ctypes.c_void_p(16), None) == -2
     assert lib.gwgs_gemm_max_k(7) == 0
     assert lib.gwgs_select_workspace(4097) == 3
     # the fused basis transform: power-of-two lengths 2^13..2^22 only

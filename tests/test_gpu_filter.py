@@ -40,7 +40,7 @@ def _direct_snr(bank, ds, tids, samps):
     return out
 
 
-@pytest.mark.parametrize('precision', ['fp32', 'bf16', 'fp8'])
+@pytest.mark.parametrize('precision', ['fp32', 'bf16', 'fp8', 'fp16'])
 def test_c1_triggers_vs_oracle(c1_data, precision):
     """Same above-threshold (template, window) list as the float64 oracle, the same sample in each window, and exact
     SNR within 1e-4 of the float64 direct filter (exact_stage.py:35 snr_rel), chi-squared within 1e-3."""
@@ -71,7 +71,7 @@ def test_c1_triggers_vs_oracle(c1_data, precision):
           zip(trig['template_id'], trig['sample'], trig['snr'], trig['chisq'])}
     same_sample = np.mean([ko[k][0] == kg[k][0] for k in both])
     # the rebuilt argmax in fp32 is the float64 one; bf16 / fp8 may pick a neighbouring sample of a flat peak
-    assert same_sample > {'fp32': 0.98, 'bf16': 0.7, 'fp8': 0.3}[precision]
+    assert same_sample > {'fp32': 0.98, 'bf16': 0.7, 'fp8': 0.3, 'fp16': 0.9}[precision]
     d = _direct_snr(bank, ds, trig['template_id'], trig['sample'])
     assert np.max(np.abs(trig['snr'] - d) / d) < 1e-4
     chi_o = np.array([ko[k][2] for k in both if ko[k][0] == kg[k][0]])
@@ -80,7 +80,7 @@ def test_c1_triggers_vs_oracle(c1_data, precision):
     assert trig['snr'].max() > 20                        # the SNR 25 injection is found
 
 
-@pytest.mark.parametrize('precision', ['bf16', 'fp8'])
+@pytest.mark.parametrize('precision', ['bf16', 'fp8', 'fp16'])
 def test_c1_refined_triggers_identical_to_fp32(c1_data, precision):
     """With the exact stage re-centring each peak on the best exact |z| within +-3 samples of the rebuilt argmax
     (inside its window): the bf16 form reports the same (template, sample) triggers as the fp32 form; the fp8 form
@@ -96,11 +96,11 @@ def test_c1_refined_triggers_identical_to_fp32(c1_data, precision):
         plan = fp_filter.BankPlan(bank.m1, bank.m2, None, ds.flen, ds.df, opts, sw, isw)
         out[p] = fp_filter.Filter([ds], opts).run(plan)[0].to_dict()
     a, b = out['fp32'], out[precision]
-    key = (lambda d: zip(d['template_id'].tolist(), d['sample'].tolist())) if precision == 'bf16' else \
+    key = (lambda d: zip(d['template_id'].tolist(), d['sample'].tolist())) if precision != 'fp8' else \
         (lambda d: zip(d['template_id'].tolist(), (d['sample'] // 2048).tolist()))
     ka, kb = set(key(a)), set(key(b))
     for t, s in ka ^ kb:          # only near-threshold windows may differ
-        if precision == 'bf16':
+        if precision != 'fp8':
             v = a['snr'][(a['template_id'] == t) & (a['sample'] == s)].tolist() + \
                 b['snr'][(b['template_id'] == t) & (b['sample'] == s)].tolist()
         else:

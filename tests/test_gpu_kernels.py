@@ -16,12 +16,13 @@ def _operands(rng, M, K, Tn, dtype):
     Ar = rng.standard_normal((M, K)).astype(np.float32)
     Ai = rng.standard_normal((M, K)).astype(np.float32)
     Z = rng.standard_normal((K, Tn)).astype(np.float32)
-    if dtype == torch.bfloat16:
-        Ar, Ai, Z = _bf16_round(Ar), _bf16_round(Ai), _bf16_round(Z)
+    if dtype in (torch.bfloat16, torch.float16):
+        rd = lambda x: torch.from_numpy(x).to(dtype).to(torch.float32).numpy()
+        Ar, Ai, Z = rd(Ar), rd(Ai), rd(Z)
     return Ar, Ai, Z
 
 
-@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize('dtype', [torch.float32, torch.bfloat16, torch.float16])
 @pytest.mark.parametrize('M,K,Tn,variant', [(45, 64, 5 * 128 + 37, 0), (128, 40, 1024, 0), (300, 256, 2048, 0),
                                              (77, 520, 1000, 0), (130, 64, 1024, 1), (1, 16, 128, 1),
                                              (77, 520, 1000, 2), (200, 840, 1500, 3), (33, 72, 300, 3)])
@@ -46,7 +47,7 @@ def test_fused_peaks_vs_oracle(gpu, dtype, M, K, Tn, variant):
 
 
 @pytest.mark.parametrize('code,variant', [(1, 0), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5), (2, 0), (2, 1), (2, 3),
-                                          (2, 4), (2, 5)])
+                                          (2, 4), (2, 5), (3, 0), (3, 1), (3, 4), (3, 5)])
 def test_stacked_operands_rotated_path(gpu, code, variant):
     """Complex-stacked operands (Ai derived from Ar, as stack_a writes them) through the rotated K loops
     (mode | STACKED, Ai never read: a garbage Ai is passed) against the float64 GEMM of the same values, and equal
@@ -64,7 +65,7 @@ def test_stacked_operands_rotated_path(gpu, code, variant):
     Ai[:, 0:2 * r:2] = A.imag; Ai[:, 1:2 * r:2] = A.real
     Zt[:, 0:2 * r:2] = Zc.real; Zt[:, 1:2 * r:2] = Zc.imag
     cast = (lambda x: _fp8(np.clip(x, -448, 448))) if code == 2 else \
-        (lambda x: torch.from_numpy(x).to(torch.bfloat16))
+        (lambda x: torch.from_numpy(x).to(torch.float16 if code == 3 else torch.bfloat16))
     qa, qi, qz = cast(Ar), cast(Ai), cast(Zt)
     da, di, dz = (q.to(torch.float32).numpy().astype(np.float64) for q in (qa, qi, qz))
     view = (lambda t: t.view(torch.uint8)) if code == 2 else (lambda t: t)
