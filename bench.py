@@ -556,6 +556,9 @@ def run_production(a, opts, cfg):
         res['trigger_diff_bf16_vs_fp32'] = wl.compare(outs['bf16'], outs['fp32'])
         if 'fp8' in outs:
             res['trigger_diff_fp8_vs_fp32'] = wl.compare(outs['fp8'], outs['fp32'])
+        # the reference's rule: exact values at each form's rebuilt argmax
+        r0 = {p: wl.step(p + '_r0')[2] for p in ('bf16', 'fp32')}
+        res['trigger_diff_bf16_vs_fp32_refine0'] = wl.compare(r0['bf16'], r0['fp32'])
     if a.consolidate_steps > 0:
         log('production cuts with consolidation')
         res['consolidation'] = consolidation_report(wl.step, a.consolidate_steps,
@@ -588,6 +591,8 @@ def main():
     ap.add_argument('--no-check', action='store_true', help='skip the float64 direct-filter SNR check')
     ap.add_argument('--gemm-cfg', default=None, help='tiles_per_block[,variant]')
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
+    ap.add_argument('--alternates', type=int, default=3, help='with --refine: other local maxima per candidate window '
+                    'evaluated exactly (0 to 3)')
     ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
     ap.add_argument('--overlap', type=int, default=0, help='1: basis transforms on their own stream beside the GEMM')
     ap.add_argument('--depth', type=int, default=2, help='batches enqueued ahead of the one being decoded')
@@ -627,13 +632,18 @@ def main():
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
-                                 segment_batch=a.segment_batch, overlap=bool(a.overlap), depth=a.depth)
+                                 alternates=a.alternates, segment_batch=a.segment_batch, overlap=bool(a.overlap),
+                                 depth=a.depth)
             for p in ('bf16', 'fp32', 'fp8')}
+    # the reference's rule (exact values at the rebuilt argmax), for the trigger comparison beside the refined one
+    for p in ('bf16', 'fp32'):
+        opts[p + '_r0'] = fp_filter.Options(**{**opts[p].__dict__, 'refine': 0})
     import consolidate
     kl = consolidate.KeepLoudest(newsnr_threshold=4.0, interval=1.072, num=30, exempt_snr=5.0)
     for name, pre in (('cons', False), ('pre', True)):
         opts[name] = fp_filter.Options(tile=a.tile, precision='bf16', gemm_cfg=cfg, self_check=False,
-                                       refine=a.refine, snr_threshold=4.0, exact_snr=5.5, consolidate=kl,
+                                       refine=a.refine, alternates=a.alternates, snr_threshold=4.0, exact_snr=5.5,
+                                       consolidate=kl,
                                        pre_consolidate=pre, sgchisq_snr_threshold=6.0)
     b = run_batched(a, opts, cfg, tuned, t_setup)
     wl = b['wl']
@@ -679,7 +689,16 @@ def main():
     )
     if a.production > 0:
         log('production workload')
-        res['production'] = run_production(a, opts, cfg)
+        res['production'] = pr = run_production(a, opts, cfg)
+        # the target's fields for the production workload, in config where the record keeps them
+        res['config']['production'] = dict(
+            bf16_ms_per_step=pr['bf16']['ms_per_step'], bf16_value=pr['bf16']['value'],
+            speedup_bf16_vs_fp32=pr.get('speedup_bf16_vs_fp32'),
+            snr_rel_err_vs_fp64=pr['bf16'].get('snr_rel_err_vs_fp64'),
+            trigger_diff_bf16_vs_fp32=_diff_totals(pr['trigger_diff_bf16_vs_fp32'])
+            if 'trigger_diff_bf16_vs_fp32' in pr else None,
+            trigger_diff_bf16_vs_fp32_refine0=_diff_totals(pr['trigger_diff_bf16_vs_fp32_refine0'])
+            if 'trigger_diff_bf16_vs_fp32_refine0' in pr else None)
     if not a.no_cpu:
         log('cpu baseline')
         import workloads

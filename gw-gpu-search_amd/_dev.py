@@ -65,6 +65,16 @@ def ready_stream():
 
 
 _BASIS = None
+_CLIMB = None
+
+
+def climb_stream():
+    """The stream of the exact stage's follow-up passes (default priority): they are launched after later batches'
+    work is already queued on the compute stream, and need none of it."""
+    global _CLIMB
+    if _CLIMB is None:
+        _CLIMB = torch.cuda.Stream()
+    return _CLIMB
 
 
 def basis_stream():
@@ -122,8 +132,10 @@ class Fetch:
     copies do not queue behind the side stream's waits. get() waits for these copies only and returns numpy
     arrays."""
 
-    def __init__(self, *tensors, after=None):
-        if after is not None and after.query():
+    def __init__(self, *tensors, after=None, here=False):
+        if here:
+            s = torch.cuda.current_stream()      # after the current stream's own work, in order
+        elif after is not None and after.query():
             s = ready_stream()
         else:
             ev = after if after is not None else mark()

@@ -30,6 +30,9 @@ SIGNATURES = {
     'gwgs_gemm_peaks': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _l, _p, _p],
     'gwgs_gemm_max_k': [_i],
     'gwgs_gemm_peaks_multi': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _p, _i, _i, _i, _p, _p, _l, _p, _p],
+    'gwgs_gemm_peaks_multi_lm': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _p, _i, _i, _i, _i, _f, _f, _p, _p, _l, _p,
+                                 _p, _l, _p, _p],
+    'gwgs_gemm_lm_count': [],
     'gwgs_gemm_item_bytes': [],
     'gwgs_window_peaks': [_p, _l, _i, _i, _i, _i, _p, _p, _p],
     'gwgs_block_window_max': [_p, _p, _l, _i, _i, _i, _p, _p, _p],
@@ -38,9 +41,11 @@ SIGNATURES = {
     'gwgs_select_peaks': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_select_peaks_noalloc': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p],
     'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
+    'gwgs_window_alternates': [_p, _p, _l, _i, _p, _p, _p, _p, _i, _i, _i, _f, _f, _i, _i, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_chisq_points_workspace': [_i, _i],
     'gwgs_chisq_peaks_per_group': [],
+    'gwgs_climb_points': [_p, _l, _p, _ll, _p, _p, _p, _i, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_edges_fused_multi': [_p, _l, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_chisq_points_multi': [_p, _l, _p, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],

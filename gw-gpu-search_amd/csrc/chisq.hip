@@ -255,6 +255,102 @@ __global__ void __launch_bounds__(NT) edges_fused_k(const float2* __restrict__ h
     }
 }
 
+// Hill climbing on the exact |z|, z(t) = sum over k0 <= k < k1 of conj(h_k) d_k exp(2 pi i k t / N), for peaks whose
+// re-centring range ended on its edge: one workgroup per peak r, from start[r]: z at the 2 W + 1 samples c - W ..
+// c + W clipped to [lo, hi - 1]; while the largest |z| (first sample on ties) lies on the range's edge inside the
+// bounds, the range re-centres on it; out[r] = the range's largest sample at the end (after at most max_iter
+// ranges). k0, k1: the first and last entries of the peak's chi-squared edge row (the z the bin sums add up to).
+// Lanes take bins k0 + tid + NT s, each point's phasor reseeded from the exact integer phase every CR steps.
+constexpr int CW = 3;            // largest half width
+constexpr int CP = 2 * CW + 1;
+constexpr int CR = 32;
+__global__ void __launch_bounds__(NT) climb_k(const float2* __restrict__ h, const long ldh,
+                                              const float2* const* __restrict__ dtab, const int* __restrict__ rows,
+                                              const int* __restrict__ segs, const int* __restrict__ edges,
+                                              const int nedge, const int* __restrict__ eidx,
+                                              const long long* __restrict__ start, const long long* __restrict__ lo,
+                                              const long long* __restrict__ hi, const unsigned long long N,
+                                              const int W, const int max_iter, const int n,
+                                              long long* __restrict__ out)
+{
+    __shared__ float2 red[NT / 64][CP];
+    __shared__ long long sc;
+    __shared__ int sdone;
+    const int r = blockIdx.x;
+    if (r >= n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int* e = edges + (size_t)eidx[r] * nedge;
+    const int k0 = e[0], k1 = e[nedge - 1];
+    long long c = start[r];
+    const long long L = lo[r], Hb = hi[r] - 1;
+    if (k1 <= k0) {                               // no bins: z = 0, nothing to climb
+        if (tid == 0) out[r] = c;
+        return;
+    }
+    const unsigned long long mask = (N & (N - 1)) == 0 && N <= (1ull << 24) ? N - 1 : 0;
+    const float2* hr = h + (size_t)rows[r] * ldh;
+    const float2* dv = dtab[segs[r]];
+    const __amdgpu_buffer_rsrc_t rh = buffer_of(hr, (long)k1 * 8), rd = buffer_of(dv, (long)k1 * 8);
+    const int np_ = 2 * W + 1;
+    for (int it = 0; it < max_iter; it++) {
+        unsigned long long t[CP];
+        long long ts[CP];
+#pragma unroll
+        for (int p = 0; p < CP; p++) {
+            ts[p] = min(max(c - W + min(p, np_ - 1), L), Hb);
+            t[p] = (unsigned long long)ts[p] % N;
+        }
+        float2 acc[CP], rot[CP];
+#pragma unroll
+        for (int p = 0; p < CP; p++) { acc[p] = make_float2(0.f, 0.f); rot[p] = phasor(NT, t[p], N, mask); }
+        for (int ra = k0; ra < k1; ra += NT * CR) {
+            const int ka = ra + tid;
+            float2 w[CP];
+#pragma unroll
+            for (int p = 0; p < CP; p++) w[p] = phasor((unsigned long long)ka, t[p], N, mask);
+            const int ns = (min(ra + NT * CR, k1) - ra + NT - 1) / NT;   // the same for every lane
+            for (int st = 0; st < ns; st++) {
+                const int kk = ka + NT * st;
+                // past k1 the buffers read zeros: exact zero terms
+                const float2 qv = load_f2(rh, (unsigned)kk * 8u), dk = load_f2(rd, (unsigned)kk * 8u);
+                const float2 v = make_float2(qv.x * dk.x + qv.y * dk.y, qv.x * dk.y - qv.y * dk.x);
+#pragma unroll
+                for (int p = 0; p < CP; p++) {
+                    acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
+                    acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
+                    const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
+                    w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
+                    w[p].x = wx;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < CP; p++) {
+            const float ax = wave_sum_f32(acc[p].x), ay = wave_sum_f32(acc[p].y);
+            if (lane == 0) red[wave][p] = make_float2(ax, ay);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int j = 0;
+            double bv = -1.0;
+            for (int p = 0; p < np_; p++) {
+                double zx = 0.0, zy = 0.0;
+                for (int w2 = 0; w2 < NT / 64; w2++) { zx += red[w2][p].x; zy += red[w2][p].y; }
+                const double v2 = zx * zx + zy * zy;
+                if (v2 > bv) { bv = v2; j = p; }
+            }
+            const bool edge = (j == 0 && ts[0] > L) || (j == np_ - 1 && ts[np_ - 1] < Hb);
+            sc = ts[j];
+            sdone = !edge;
+        }
+        __syncthreads();
+        c = sc;
+        if (sdone) break;
+        __syncthreads();                          // every thread has read sc before thread 0 writes it again
+    }
+    if (tid == 0) out[r] = c;
+}
+
 } // namespace
 
 // Groups: for g in [0, ngroups): row grow[g] of q (or h in fused mode), peaks gstart[g] .. gstart[g] + gcount[g]
@@ -325,6 +421,23 @@ GWGS_API int gwgs_chisq_points_multi(const void* h, long ldh, const void* const*
 }
 
 GWGS_API int gwgs_chisq_peaks_per_group(void) { return P; }
+
+// Hill climbing of n peaks on the exact |z| (see climb_k): rows, segs (int32) the peak's template row of h and data
+// row d_tab[segs[r]]; edges (rows of nedge int32) with eidx[r] the peak's edge row; start, lo, hi (int64); W in
+// [1, 3] the range half width; out (int64) the final samples.
+GWGS_API int gwgs_climb_points(const void* h, long ldh, const void* const* d_tab, long long N, const int* rows,
+                               const int* segs, const int* edges, int nedge, const int* eidx, const long long* start,
+                               const long long* lo, const long long* hi, int W, int max_iter, int n, long long* out,
+                               void* stream)
+{
+    GWGS_CHECK(h && d_tab && rows && segs && edges && eidx && start && lo && hi && out && N > 0 && nedge >= 2 &&
+               W >= 1 && W <= CW && max_iter >= 1 && n >= 0, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL(climb_k, dim3((unsigned)n), dim3(NT), 0, (hipStream_t)stream, (const float2*)h, ldh,
+                       (const float2* const*)d_tab, rows, segs, edges, nedge, eidx, start, lo, hi,
+                       (unsigned long long)N, W, max_iter, n, out);
+    return gwgs_status();
+}
 
 // Edges for n requested rows: rows index h (ldh complex per row); ips: 1/psd (float32, >= max k1 entries);
 // k1s, nbs per request; out (n, nedge) int32.

@@ -113,6 +113,11 @@ struct Job {
     const GemmItem* items;   // null: one segment at row 0, items derived from tpb, ntile, Tn
     int nitems, nmb, tpb, ntile, Tn, M, per;
     const float* rowscale;
+    float* vb;               // mode 2: each window's LM largest separated local maxima of |z|^2 (row scaled, -1
+    int* ib;                 // for none) at vb[m ldb + col LM + k], their samples (-1 for none) in ib; maxima
+    long ldb;                // closer than sep + 1 samples count once; those below gate x the window's maximum
+    int sep;                 // need not be listed, nor those below gate x floor (row scaled)
+    float gate, floor;
 };
 
 // logical block -> (template block, item); false for the grid's padding blocks
@@ -167,12 +172,210 @@ __device__ __forceinline__ void tile_done(const Job& J, const GemmItem& it, cons
                                           float& best, int& bi, float* __restrict__ V, int* __restrict__ I,
                                           const long ldv)
 {
-    const bool out = MODE == 0 || (tt + 1) % J.per == 0;
+    const bool out = MODE == 0 || (tt + 1) % J.per == 0;   // modes 1 and 2: at each window end
     if (!out) return;
     const float sc = (J.rowscale && m < J.M) ? J.rowscale[(size_t)it.seg * J.M + m] : 1.f;
     emit(best, bi, half, m, J.M, sc, V, I, ldv, it.col0 + (MODE == 0 ? tt : (tt + 1) / J.per - 1));
     best = -1.f;
     bi = (tt + 1) * BT;
+}
+
+// Mode 2 also keeps, per (template, window), the LM largest separated local maxima of |z|^2 (larger value first,
+// then smaller sample): sample t of the window is a local maximum when p[t] >= p[t - 1] and p[t] > p[t + 1], the
+// window's edges counting as open, and of local maxima closer than sep + 1 samples only the larger is kept (the
+// earlier on ties), so that the ripples rounding leaves on a broad peak's top do not crowd the list. A lane holds
+// the samples tb0 + 32 ts + 8 q + 4 half + j (run u = 4 ts + q, j = 0..3) of its template, so a run's neighbours
+// lie in the other half's lane: half 0's run u sits between half 1's runs u - 1 and u, half 1's run u between half
+// 0's runs u and u + 1. Half 0 carries half 1's last sample of the previous run as its left neighbour; half 1 holds
+// its run's last sample until half 0's first sample of the next run (the next tile's, across a tile edge) arrives.
+// A lane meets its samples in increasing order, so only its last accepted local maximum can lie within sep of a
+// new one; the two halves' lists merge at the window's end with the full separation test.
+constexpr int LM = 4;
+
+struct LocalMax {
+    float v[LM];
+    int i[LM];
+    float carry;   // half 0: the left neighbour of its next run (-1 at the window's left edge)
+    float pv;      // half 1: its last sample, waiting for its right neighbour
+    int pt;
+    bool pok;      // ... that sample is >= its left neighbour
+    float lv;      // the lane's last accepted local maximum
+    int lt;
+};
+
+__device__ __forceinline__ void lm_reset(LocalMax& L)
+{
+#pragma unroll
+    for (int k = 0; k < LM; k++) { L.v[k] = -1.f; L.i[k] = 0x7fffffff; }
+    L.carry = -1.f; L.pv = -1.f; L.pt = 0; L.pok = false; L.lv = -1.f; L.lt = -(1 << 30);
+}
+
+__device__ __forceinline__ bool beats(const float x, const int t, const float v, const int i)
+{
+    return x > v || (x == v && t < i);
+}
+
+// (x, t) into the sorted list, the smallest entry dropping out
+__device__ __forceinline__ void lm_insert(LocalMax& L, float x, int t)
+{
+#pragma unroll
+    for (int k = 0; k < LM; k++) {
+        const bool take = beats(x, t, L.v[k], L.i[k]);
+        const float ov = L.v[k];
+        const int oi = L.i[k];
+        if (take) { L.v[k] = x; L.i[k] = t; x = ov; t = oi; }
+    }
+}
+
+// a local maximum (x, t) met in increasing t: within sep of the last accepted one it replaces it if larger (in the
+// list, moving up; or entering it), else it is dropped; farther, it is a new one
+__device__ __forceinline__ void lm_try(LocalMax& L, const bool lm, const float x, const int t, const int sep)
+{
+    if (!lm) return;
+    if (t - L.lt <= sep) {
+        if (x > L.lv) {
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k < LM; k++)
+                if (L.i[k] == L.lt) { L.v[k] = x; L.i[k] = t; found = true; }
+            if (found) {
+#pragma unroll
+                for (int k = LM - 1; k > 0; k--) {
+                    if (beats(L.v[k], L.i[k], L.v[k - 1], L.i[k - 1])) {
+                        const float tv = L.v[k]; L.v[k] = L.v[k - 1]; L.v[k - 1] = tv;
+                        const int ti = L.i[k]; L.i[k] = L.i[k - 1]; L.i[k - 1] = ti;
+                    }
+                }
+            } else if (beats(x, t, L.v[LM - 1], L.i[LM - 1])) {
+                lm_insert(L, x, t);
+            }
+            L.lv = x; L.lt = t;
+        }
+        return;
+    }
+    L.lv = x; L.lt = t;
+    if (beats(x, t, L.v[LM - 1], L.i[LM - 1])) lm_insert(L, x, t);
+}
+
+__device__ __forceinline__ void cswap(LocalMax& L, const int a, const int b)
+{
+    if (beats(L.v[b], L.i[b], L.v[a], L.i[a])) {
+        const float tv = L.v[a]; L.v[a] = L.v[b]; L.v[b] = tv;
+        const int ti = L.i[a]; L.i[a] = L.i[b]; L.i[b] = ti;
+    }
+}
+
+// (x, t) from the other half into the list, in any order: dropped if an entry within sep beats it, else it
+// removes the entries within sep (which it beats) and enters
+__device__ __forceinline__ void lm_merge_one(LocalMax& L, const float x, const int t, const int sep)
+{
+    bool dom = false;
+#pragma unroll
+    for (int k = 0; k < LM; k++) dom |= abs(L.i[k] - t) <= sep && !beats(x, t, L.v[k], L.i[k]);
+    if (dom) return;
+#pragma unroll
+    for (int k = 0; k < LM; k++)
+        if (abs(L.i[k] - t) <= sep) { L.v[k] = -1.f; L.i[k] = 0x7fffffff; }
+    static_assert(LM == 4, "the sorting network below is for four entries");
+    cswap(L, 0, 1); cswap(L, 2, 3); cswap(L, 0, 2); cswap(L, 1, 3); cswap(L, 1, 2);
+    if (beats(x, t, L.v[LM - 1], L.i[LM - 1])) lm_insert(L, x, t);
+}
+
+// The tile's local maxima at or above thr into the lists; p: |z|^2 in place of cr (-1 past the segment's end, which
+// also makes the last valid sample's right edge open). thr is uniform over a template's two lanes, so a run in
+// which no lane of the wave holds a sample at or above its thr is skipped whole: its samples are all below thr, so
+// the next run's first sample, if at or above thr, is >= its left neighbour (carry -1 says so), and the held sample
+// cannot qualify.
+__device__ __forceinline__ void lm_tile(LocalMax& L, const f32x16 (&p)[4], const int tb0, const int half,
+                                        const int sep, const float thr)
+{
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int ts = u >> 2, q = u & 3;
+        const float x0 = p[ts][4 * q], x1 = p[ts][4 * q + 1], x2 = p[ts][4 * q + 2], x3 = p[ts][4 * q + 3];
+        const int t0 = tb0 + 32 * ts + 8 * q + 4 * half;
+        const bool hot = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) >= thr || (half && L.pok && L.pv >= thr);
+        if (__ballot(hot) == 0) {
+            L.carry = -1.f;
+            L.pok = false;
+            continue;
+        }
+        const float r0 = __shfl_xor(x0, 32, 64), r3 = __shfl_xor(x3, 32, 64);
+        const float prev = half ? r3 : L.carry;
+        lm_try(L, half && L.pok && L.pv > r0 && L.pv >= thr, L.pv, L.pt, sep);
+        lm_try(L, x0 >= thr && x0 >= prev && x0 > x1, x0, t0, sep);
+        lm_try(L, x1 >= thr && x1 >= x0 && x1 > x2, x1, t0 + 1, sep);
+        lm_try(L, x2 >= thr && x2 >= x1 && x2 > x3, x2, t0 + 2, sep);
+        lm_try(L, !half && x3 >= thr && x3 >= x2 && x3 > r0, x3, t0 + 3, sep);
+        L.carry = r3;
+        L.pv = x3; L.pt = t0 + 3; L.pok = half && x3 >= x2 && x3 >= 0.f;
+    }
+}
+
+// window end: half 1's held sample has an open right edge; the halves' lists merged; half 0 writes them row scaled
+__device__ __forceinline__ void lm_emit(LocalMax& L, const int half, const int m, const int M, const float sc,
+                                        const int sep, float* __restrict__ VL, int* __restrict__ IL, const long ldl,
+                                        const int col)
+{
+    lm_try(L, half && L.pok, L.pv, L.pt, sep);
+    float ov[LM];
+    int oi[LM];
+#pragma unroll
+    for (int k = 0; k < LM; k++) { ov[k] = __shfl_xor(L.v[k], 32, 64); oi[k] = __shfl_xor(L.i[k], 32, 64); }
+#pragma unroll
+    for (int k = 0; k < LM; k++)
+        if (ov[k] >= 0.f) lm_merge_one(L, ov[k], oi[k], sep);
+    if (half == 0 && m < M) {
+#pragma unroll
+        for (int k = 0; k < LM; k++) {
+            VL[(size_t)m * ldl + (size_t)col * LM + k] = L.v[k] >= 0.f ? L.v[k] * sc : -1.f;
+            IL[(size_t)m * ldl + (size_t)col * LM + k] = L.v[k] >= 0.f ? L.i[k] : -1;
+        }
+    }
+}
+
+// The epilogue of time tile tt: |z|^2 into the running (best, bi) (first index on ties), in mode 2 also into the
+// local-maximum lists; then the window (or block) outputs.
+template <int MODE>
+__device__ __forceinline__ void epilogue(const Job& J, const GemmItem& it, const int tt, const int half, const int m,
+                                         f32x16 (&cr)[4], const f32x16 (&ci)[4], float& best, int& bi, LocalMax& L,
+                                         float* __restrict__ V, int* __restrict__ I, const long ldv)
+{
+    if constexpr (MODE == 2) {
+        if (tt % J.per == 0) lm_reset(L);
+        float tmax = -1.f;
+#pragma unroll
+        for (int ts = 0; ts < 4; ts++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int t = tt * BT + ts * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                const float re = cr[ts][r], im = ci[ts][r];
+                const float pp = re * re + im * im;
+                if (t < it.Tn && pp > best) { best = pp; bi = t; }
+                cr[ts][r] = t < it.Tn ? pp : -1.f;
+                tmax = fmaxf(tmax, cr[ts][r]);
+            }
+        }
+        // local maxima below gate x the window's running maximum (so far, this tile included, over both halves)
+        // are below gate x its final maximum, and those below gate x floor (row scaled) below gate x any window
+        // maximum that reaches the floor: they need not be listed
+        const float sc = (J.rowscale && m < J.M) ? J.rowscale[(size_t)it.seg * J.M + m] : 1.f;
+        const float fl = sc > 0.f ? J.floor / sc : 0.f;
+        const float thr = J.gate * fmaxf(fmaxf(best, __shfl_xor(best, 32, 64)), fl);
+        // a tile with no sample at or above thr in any lane of the wave (nor a held one) is skipped whole, as a run
+        if (__ballot(tmax >= thr || (half && L.pok && L.pv >= thr)) != 0) {
+            lm_tile(L, cr, tt * BT, half, J.sep, thr);
+        } else {
+            L.carry = -1.f;
+            L.pok = false;
+        }
+        if ((tt + 1) % J.per == 0) {
+            lm_emit(L, half, m, J.M, sc, J.sep, J.vb, J.ib, J.ldb, it.col0 + (tt + 1) / J.per - 1);
+        }
+    } else {
+        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
+    }
+    tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
 }
 
 template <int KP, int MODE>
@@ -226,6 +429,7 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
     };
 
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     gload(it.t_first);
     sstore(0);
@@ -251,8 +455,7 @@ gemm_peaks_bf16_k(const uint16_t* __restrict__ Zt, const long ldz, const uint16_
                 ci[ts] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(z, a_i, ci[ts], 0, 0, 0);
             }
         }
-        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-        tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+        epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
         if (tt + 1 < it.t_end) sstore(buf ^ 1);
         __syncthreads();
     }
@@ -304,6 +507,7 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
     };
 
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     gload(it.t_first);
     for (int tt = it.t_first; tt < it.t_end; tt++) {
@@ -326,8 +530,7 @@ gemm_peaks_f32_k(const float* __restrict__ Zt, const long ldz, const float* __re
                 ci[ts] = __builtin_amdgcn_mfma_f32_32x32x2f32(z, ai[ks], ci[ts], 0, 0, 0);
             }
         }
-        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-        tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+        epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
     }
 }
 
@@ -395,6 +598,7 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
 
     f32x16 cr[4], ci[4];
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     gload(0);
     for (int step = 0; step < nsteps; step++) {
@@ -468,8 +672,7 @@ gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict
             }
         }
         if (kcix == nkc - 1) {
-            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+            epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
         }
     }
 }
@@ -535,6 +738,7 @@ gemm_peaks_kloop_ra_k(const T* __restrict__ Zt, const long ldz, const T* __restr
     f32x16 cr[4], ci[4];
     uint4 fr[NA], nr[NA];
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     gload(0);
     if (DB) aload(0, nr);
@@ -588,8 +792,7 @@ gemm_peaks_kloop_ra_k(const T* __restrict__ Zt, const long ldz, const T* __restr
             }
         }
         if (kcix == nkc - 1) {
-            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+            epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
         }
     }
 }
@@ -651,6 +854,7 @@ gemm_peaks_ring_k(const T* __restrict__ Zt, const long ldz, const T* __restrict_
 
     f32x16 cr[4], ci[4];
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     gload(0);
     sstore(0);
@@ -699,8 +903,7 @@ gemm_peaks_ring_k(const T* __restrict__ Zt, const long ldz, const T* __restrict_
             if (step + 2 < nsteps) gload(step + 2);
         }
         if (kcix == nkc - 1) {
-            scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-            tile_done<MODE>(J, it, tt, half, m, best, bi, V, I, ldv);
+            epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
         }
         __syncthreads();
     }
@@ -782,6 +985,7 @@ gemm_peaks_dma_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__
     const int fs = dma_swizzle(col);               // rows ts 32 + col and wave 32 + col share col's swizzle
     f32x16 cr[4], ci[4];
     float best = -1.f;
+    LocalMax L;
     int bi = it.t_first * BT;
     const int nsteps = ntl * nkc;
 #pragma unroll
@@ -835,12 +1039,7 @@ gemm_peaks_dma_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__
             if (++cst == S) cst = 0;
             left--;
         }
-        scan_tile(cr, ci, tt * BT, half, it.Tn, best, bi);
-        if (MODE == 0 || (tt + 1) % J.per == 0) {
-            emit(best, bi, half, m, M, sc, V, I, ldv, it.col0 + (MODE == 0 ? tt : (tt + 1) / J.per - 1));
-            best = -1.f;
-            bi = (tt + 1) * BT;
-        }
+        epilogue<MODE>(J, it, tt, half, m, cr, ci, best, bi, L, V, I, ldv);
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): no DMA in flight when the workgroup ends
 }
@@ -883,8 +1082,9 @@ template <typename T, bool ROT>
 void launch_kloop(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, const T* Ai,
                   long lda, int K, const Job& J, float* V, int* I, long ldv)
 {
-    hipLaunchKernelGGL((mode ? gemm_peaks_kloop_k<T, 1, ROT> : gemm_peaks_kloop_k<T, 0, ROT>), grid, block, 0, s, Zt,
-                       ldz, Ar, Ai, lda, K, J, V, I, ldv);
+    hipLaunchKernelGGL((mode == 2 ? gemm_peaks_kloop_k<T, 2, ROT> : mode ? gemm_peaks_kloop_k<T, 1, ROT>
+                                                                         : gemm_peaks_kloop_k<T, 0, ROT>),
+                       grid, block, 0, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
 }
 
 template <typename T, bool DB>
@@ -907,8 +1107,8 @@ template <typename T>
 void launch_dma(int mode, dim3 grid, dim3 block, hipStream_t s, const T* Zt, long ldz, const T* Ar, long lda, int K,
                 const Job& J, float* V, int* I, long ldv)
 {
-    hipLaunchKernelGGL((mode ? gemm_peaks_dma_k<T, 1> : gemm_peaks_dma_k<T, 0>), grid, block, 0, s, Zt, ldz, Ar, lda,
-                       K, J, V, I, ldv);
+    hipLaunchKernelGGL((mode == 2 ? gemm_peaks_dma_k<T, 2> : mode ? gemm_peaks_dma_k<T, 1> : gemm_peaks_dma_k<T, 0>),
+                       grid, block, 0, s, Zt, ldz, Ar, lda, K, J, V, I, ldv);
 }
 
 // bf16 / fp8 K loops for complex-stacked operands: variant 0 or 5 the LDS-DMA loop (when K is a multiple of its
@@ -919,6 +1119,13 @@ void launch_kloop_any(int variant, bool rot, int mode, dim3 grid, dim3 block, hi
                       const T* Ar, const T* Ai, long lda, int K, const Job& J, float* V, int* I, long ldv)
 {
     const bool dma_ok = K % (64 / (int)sizeof(T)) == 0;
+    if (mode == 2) {
+        // the local-maximum lists: the LDS-DMA loop where K allows, else the LDS-staged loop
+        if (rot && dma_ok) launch_dma<T>(mode, grid, block, s, Zt, ldz, Ar, lda, K, J, V, I, ldv);
+        else if (rot) launch_kloop<T, true>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+        else launch_kloop<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
+        return;
+    }
     if (!rot) launch_kloop<T, false>(mode, grid, block, s, Zt, ldz, Ar, Ai, lda, K, J, V, I, ldv);
     else if ((variant == 0 || variant == 5) && dma_ok) launch_dma<T>(mode, grid, block, s, Zt, ldz, Ar, lda, K, J, V,
                                                                      I, ldv);
@@ -940,7 +1147,8 @@ int launch(int dtype, int variant, const void* Zt, long ldz, const void* Ar, con
     if (nblocks == 0) return GWGS_OK;
     const dim3 grid((unsigned)((nblocks + 7) / 8 * 8)), block(NT);
     const bool aligned = ((uintptr_t)Zt & 15) == 0 && ((uintptr_t)Ar & 15) == 0 && ((uintptr_t)Ai & 15) == 0;
-    const bool kloop = variant >= 1 || dtype == 2 || (dtype == 1 ? K > 256 : K > 128) || K % 32;
+    // mode 2 (local-maximum lists) runs on the K loops only
+    const bool kloop = variant >= 1 || dtype == 2 || (dtype == 1 ? K > 256 : K > 128) || K % 32 || mode == 2;
     if (dtype == 1) {
         GWGS_CHECK(aligned && ldz % 8 == 0 && lda % 8 == 0 && K % 8 == 0, GWGS_EINVAL);
         if (kloop)
@@ -996,7 +1204,8 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
     if ((mode & 3) == 1 && tpb % per) tpb = (tpb / per + 1) * per;
     Job J;
     J.items = nullptr; J.nmb = (M + BMT - 1) / BMT; J.tpb = tpb; J.ntile = ntile; J.Tn = Tn; J.M = M; J.per = per;
-    J.nitems = (ntile + tpb - 1) / tpb; J.rowscale = rowscale;
+    J.nitems = (ntile + tpb - 1) / tpb; J.rowscale = rowscale; J.vb = nullptr; J.ib = nullptr; J.ldb = 0;
+    J.sep = 0; J.gate = 0.f; J.floor = 0.f;
     return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, mode, J, V, I, ldv, (hipStream_t)stream);
 }
 
@@ -1012,10 +1221,35 @@ GWGS_API int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long 
     GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
     Job J;
     J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
-    J.Tn = 0; J.M = M; J.per = (mode & 3) == 1 ? per : 1; J.rowscale = rowscale;
+    J.Tn = 0; J.M = M; J.per = (mode & 3) == 1 ? per : 1; J.rowscale = rowscale; J.vb = nullptr; J.ib = nullptr;
+    J.ldb = 0; J.sep = 0; J.gate = 0.f; J.floor = 0.f;
     GWGS_CHECK(J.per > 0, GWGS_EINVAL);
     return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, mode, J, V, I, ldv, (hipStream_t)stream);
 }
+
+// As gwgs_gemm_peaks_multi in mode 1 (windows of `per` tiles), and for every (template, window) also its LM = 4
+// largest separated local maxima of |z|^2 (p[t] >= p[t - 1] and p[t] > p[t + 1], open at the window's edges; of
+// maxima within sep samples the larger, see LocalMax), larger value first, then smaller sample: VL[m ldl + col LM +
+// k] the row-scaled value (-1 where the window has fewer), IL the segment-relative sample (-1 likewise). ldl >= LM x
+// the window columns. Local maxima below gate x max(the window's maximum, floor) (0 <= gate <= 1; floor on the
+// row-scaled |z|^2) need not be listed; the largest ones at or above it are.
+GWGS_API int gwgs_gemm_peaks_multi_lm(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai,
+                                      long lda, int M, int K, const void* items, int nitems, int stacked, int per,
+                                      int sep, float gate, float floor, float* V, int* I, long ldv, float* VL, int* IL,
+                                      long ldl, const float* rowscale, void* stream)
+{
+    GWGS_CHECK(Zt && Ar && Ai && V && I && VL && IL && items && nitems >= 0, GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && K > 0 && (stacked == 0 || stacked == 1) && per > 0 && sep >= 0 && sep < per * BT &&
+               gate >= 0.f && gate <= 1.f && floor >= 0.f &&
+               ldz >= K && lda >= K && ldv > 0 && ldl >= (long)LM * ldv, GWGS_EINVAL);
+    GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
+    Job J;
+    J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
+    J.Tn = 0; J.M = M; J.per = per; J.rowscale = rowscale; J.vb = VL; J.ib = IL; J.ldb = ldl; J.sep = sep; J.gate = gate; J.floor = floor;
+    return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, 2 | (stacked ? 4 : 0), J, V, I, ldv, (hipStream_t)stream);
+}
+
+GWGS_API int gwgs_gemm_lm_count(void) { return LM; }
 
 GWGS_API int gwgs_gemm_item_bytes(void) { return (int)sizeof(GemmItem); }
 

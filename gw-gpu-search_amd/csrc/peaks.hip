@@ -165,6 +165,38 @@ __global__ void __launch_bounds__(NT) rebuilt_z_k(const T* __restrict__ Zt, cons
     if (lane == 0) out[c] = make_float2(sr * sc, si * sc);
 }
 
+// The alternates of every exact-band candidate of a compacted list, from its window's local-maximum list (LM
+// entries per window as gwgs_gemm_peaks_multi_lm writes them, larger |z|^2 first, then the earlier sample): the
+// entries farther than R samples from the candidate's window maximum samp[c] whose |z|^2 is at least ratio val[c],
+// in list order, up to na of them; out[c na + a] = the entry's offset in its window (sample mod cw), -1 past the
+// last. Candidates with val[c] < exact_cut get none. One thread per candidate.
+__global__ void __launch_bounds__(NT) window_alternates_k(const float* __restrict__ VL, const int* __restrict__ IL,
+                                                          const long ldl, const int lm, const int* __restrict__ idx,
+                                                          const int* __restrict__ samp,
+                                                          const float* __restrict__ val,
+                                                          const int* __restrict__ count, const int cap,
+                                                          const int nwsum, const int cw, const float exact_cut,
+                                                          const float ratio, const int R, const int na,
+                                                          int* __restrict__ out)
+{
+    const int n = min(*count, cap);
+    for (int c = blockIdx.x * NT + threadIdx.x; c < n; c += gridDim.x * NT) {
+        int a = 0;
+        const float v0 = val[c];
+        if (v0 >= exact_cut) {
+            const long g = idx[c];
+            const size_t base = (size_t)(g / nwsum) * ldl + (size_t)(g % nwsum) * lm;
+            const int t0 = samp[c];
+            for (int k = 0; k < lm && a < na; k++) {
+                const float v = VL[base + k];
+                const int t = IL[base + k];
+                if (t >= 0 && v >= ratio * v0 && abs(t - t0) > R) out[(size_t)c * na + a++] = t % cw;
+            }
+        }
+        for (; a < na; a++) out[(size_t)c * na + a] = -1;
+    }
+}
+
 } // namespace
 
 GWGS_API int gwgs_window_peaks(const void* z, long ldz, int rows, int a0, int cw, int nwin, float* pmax, int* pidx,
@@ -298,5 +330,20 @@ GWGS_API int gwgs_rebuilt_z(int dtype, const void* Zt, long ldz, const void* Ar,
                            (const float*)Ai, lda, K, rows, tcol, sidx, n, pw_scale, (float2*)out);
     else
         return GWGS_EUNSUP;
+    return gwgs_status();
+}
+
+// The alternates of a compacted candidate list (see window_alternates_k); cap: the list's capacity (entries past
+// it are not read), out (cap, na) int32.
+GWGS_API int gwgs_window_alternates(const float* VL, const int* IL, long ldl, int lm, const int* idx, const int* samp,
+                                    const float* val, const int* count, int cap, int nwsum, int cw, float exact_cut,
+                                    float ratio, int R, int na, int* out, void* stream)
+{
+    GWGS_CHECK(VL && IL && idx && samp && val && count && out && lm > 0 && cap >= 0 && nwsum > 0 && cw > 0 &&
+               ldl >= (long)lm * nwsum && R >= 0 && na > 0, GWGS_EINVAL);
+    if (cap == 0) return GWGS_OK;
+    const int nb = (int)std::min<long>(((long)cap + NT - 1) / NT, 1024);
+    hipLaunchKernelGGL(window_alternates_k, dim3(nb), dim3(NT), 0, (hipStream_t)stream, VL, IL, ldl, lm, idx, samp,
+                       val, count, cap, nwsum, cw, exact_cut, ratio, R, na, out);
     return gwgs_status();
 }

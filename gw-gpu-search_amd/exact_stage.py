@@ -20,10 +20,12 @@ import numpy as np
 import torch
 
 import _dev
+import _lib
 import chisq_kernel
 from chisq_edges import edges_blocked, edges_rows_dev, edges_rows_dev_multi
 
 MAX_PEAKS = 8
+CLIMB_MAX_RANGES = 1024     # bound on a climb's re-centred ranges (a cluster window of 2048 samples needs < 700)
 CHECK_ROWS = 16
 CHECK_PEAKS = 64
 CHECK_TOL = dict(edge_bins=1, edge_frac=1e-2, snr_rel=1e-4, phase_rad=1e-4, chisq_rel=1e-3)
@@ -111,8 +113,10 @@ class ExactStage:
     def evaluate(self, seg: SegmentData, req: PeakRequest, refine=0, lo=None, hi=None) -> PeakValues:
         """Exact values at the requested peaks. refine = w > 0: each peak is re-centred on the largest exact |z| among
         the samples t - w .. t + w that lie in [lo, hi) (per-peak bounds, e.g. its cluster window; first sample on
-        ties), all candidates evaluated in the same pass; PeakValues.samples holds the chosen samples. refine = 0 is
-        the reference's rule: values at the requested sample."""
+        ties), all candidates evaluated in the same pass; a peak whose best lies on the edge of that range (inside
+        its bounds) is evaluated again around that sample, until every best is interior or on a bound, so each peak
+        climbs to a local maximum of the exact |z| whichever sample of its slope it started from. PeakValues.samples
+        holds the chosen samples. refine = 0 is the reference's rule: values at the requested sample."""
         return self.evaluate_async(seg, req, refine, lo, hi).result()
 
     def evaluate_async(self, seg: SegmentData, req: PeakRequest, refine=0, lo=None, hi=None) -> 'PendingValues':
@@ -147,13 +151,41 @@ class ExactStage:
         out, zo = chisq_kernel.launch_sorted(t.H, np.repeat(rs, noff), cand[ob].ravel(), E_dev,
                                              np.repeat(inv[ob], noff), want_z=True, d=seg.D, n_time=seg.tlen,
                                              row_span=np.maximum(t.kmax - t.kmin, 0))
-        return PendingValues(self, seg, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
+        pv = PendingValues(self, seg, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
+        if refine > 0:
+            pv.climb = (lo, hi, self._follow([seg], np.zeros(npk, np.int64), rows, lo, hi, refine, E_dev, ur, inv))
+        return pv
 
+    def _follow(self, segs, sidx, rows, lo, hi, refine, E_dev, uk, inv):
+        """The climb of peaks e from samples c (gwgs_climb_points on the device, on the current stream), then their
+        exact values re-centred once more within +-refine of the samples reached, so that the sample reported is
+        chosen by the same sums as for peaks that did not climb; returns that PendingValues (copies on the current
+        stream)."""
+        def follow(e, c):
+            n = len(e)
+            rows_d, seg_d, ei_d, st_d, lo_d, hi_d, tab = _dev.to_dev_many(
+                rows[e].astype(np.int32), sidx[e].astype(np.int32), inv[e].astype(np.int32),
+                np.asarray(c, np.int64), lo[e].astype(np.int64), hi[e].astype(np.int64),
+                np.array([sg.D.data_ptr() for sg in segs], np.int64))
+            fin = torch.empty(n, dtype=torch.int64, device=rows_d.device)
+            E = E_dev.contiguous()
+            _lib.call('gwgs_climb_points', self.tile.H.data_ptr(), self.tile.H.shape[1], tab.data_ptr(),
+                      int(segs[0].tlen), rows_d.data_ptr(), seg_d.data_ptr(), E.data_ptr(), E.shape[1],
+                      ei_d.data_ptr(), st_d.data_ptr(), lo_d.data_ptr(), hi_d.data_ptr(), min(int(refine), 3),
+                      CLIMB_MAX_RANGES, n, fin.data_ptr(), _dev.stream())
+            fin_h = _dev.Fetch(fin, here=True).get()[0]
+            return self.evaluate_async_multi(segs, sidx[e], PeakRequest(rows[e], fin_h), refine, lo[e], hi[e],
+                                             edges=(E, uk), here=True, climb=False)
+        return follow
 
-    def evaluate_async_multi(self, segs, sidx, req: PeakRequest, refine=0, lo=None, hi=None) -> 'PendingValues':
+    def evaluate_async_multi(self, segs, sidx, req: PeakRequest, refine=0, lo=None, hi=None, edges=None,
+                             here=False, climb=True) -> 'PendingValues':
         """evaluate_async over the peaks of several segments of equal tlen and df in one edges launch and one sums
         launch: segs is a list of SegmentData, sidx the segment of each peak. Results in request order; the first-call
-        self-check runs on the peaks of the first requested segment."""
+        self-check runs on the peaks of the first requested segment. edges: (device edges, their sorted keys
+        segment n + row) of an earlier call on the same segments holding every (segment, row) requested here, used
+        instead of recomputing them; here: the copies to the host run on the current stream; climb: peaks whose best
+        lies on the edge of their range climb on (with refine > 0)."""
         rows, samples = req.rows, req.samples
         sidx = np.asarray(sidx, np.int64)
         npk = len(rows)
@@ -173,11 +205,15 @@ class ExactStage:
         inv = np.empty(npk, np.int64)
         inv[ob] = np.cumsum(first) - 1
         ur, us = uk % n, uk // n
-        E_dev = edges_rows_dev_multi(ur, us, self._nbe(ur), t.kmin, t.kmax[ur], t.H,
-                                     np.array([sg.IP.data_ptr() for sg in segs], np.int64))
-        zero = np.flatnonzero(t.nbins[ur] == 0)
-        if len(zero):
-            E_dev[_dev.to_dev(zero), 0] = t.kmin
+        if edges is not None:
+            E_prev, k_prev = edges
+            E_dev = E_prev[_dev.to_dev(np.searchsorted(k_prev, uk))]
+        else:
+            E_dev = edges_rows_dev_multi(ur, us, self._nbe(ur), t.kmin, t.kmax[ur], t.H,
+                                         np.array([sg.IP.data_ptr() for sg in segs], np.int64))
+            zero = np.flatnonzero(t.nbins[ur] == 0)
+            if len(zero):
+                E_dev[_dev.to_dev(zero), 0] = t.kmin
         if refine > 0:
             lo = np.zeros(npk, np.int64) if lo is None else np.asarray(lo, np.int64)
             hi = np.full(npk, seg0.tlen, np.int64) if hi is None else np.asarray(hi, np.int64)
@@ -190,15 +226,19 @@ class ExactStage:
                                                    np.repeat(inv[ob], noff),
                                                    np.array([sg.D.data_ptr() for sg in segs], np.int64), seg0.tlen,
                                                    row_span=np.maximum(t.kmax - t.kmin, 0))
-        pv = PendingValues(self, seg0, rows, cand, ob, _dev.Fetch(out, zo, E_dev), inv)
+        pv = PendingValues(self, seg0, rows, cand, ob, _dev.Fetch(out, zo, E_dev, here=here), inv)
         pv.check_mask = sidx == sidx[0]
         pv.keep_alive = segs
+        if refine > 0 and climb:
+            pv.climb = (lo, hi, self._follow(segs, sidx, rows, lo, hi, refine, E_dev, uk, inv))
         return pv
 
 
 class PendingValues:
     """Exact values of one evaluate_async call; result() waits for the device work and builds the PeakValues (the
     refine choice, fpeak from the edges, and the first-call self-check)."""
+
+    climb = None   # (lo, hi, evaluate_async of (peak indices, new centres)) with refine > 0
 
     def __init__(self, stage, seg, rows, cand, order, fetch, inv):
         self.stage, self.seg, self.rows, self.cand = stage, seg, rows, cand
@@ -218,6 +258,18 @@ class PendingValues:
             j = np.argmax(np.abs(z_all.astype(np.complex128)), axis=1)  # first index: the earliest sample on ties
             r = np.arange(npk)
             samples, pw, z = self.cand[r, j], pw_all[r, j], z_all[r, j]
+            if self.climb is not None:
+                # a best on the edge of its range, inside its bounds, climbs on from there (the new range holds
+                # it, so the result is at least as large)
+                lo, hi, again = self.climb
+                e = np.flatnonzero(((j == 0) & (self.cand[:, 0] > lo)) |
+                                   ((j == noff - 1) & (self.cand[:, -1] < hi - 1)))
+                if len(e):
+                    # climbed on, on a stream of its own: the compute stream already holds later batches' work
+                    with torch.cuda.stream(_dev.climb_stream()):
+                        sub = again(e, samples[e]).result()
+                    samples, pw, z = samples.copy(), pw.copy(), z.copy()
+                    samples[e], pw[e], z[e] = sub.samples, sub.pw, sub.z
         else:
             samples, pw, z = self.cand[:, 0], pw_all[:, 0], z_all[:, 0]
         E = Eu.astype(np.int64)[self.inv]

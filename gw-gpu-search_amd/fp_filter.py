@@ -69,7 +69,13 @@ class Options:
                                            # the first run and replay them on later runs (same segmentation; new
                                            # data through rebind)
     refine: int = 0                        # exact stage re-centres each peak on the best exact |z| within +-refine
-                                           # samples inside its window (0: the reference's rebuilt-argmax sample)
+                                           # samples inside its window, climbing on from a best on the range's edge
+                                           # (0: the reference's rebuilt-argmax sample)
+    alternates: int = 3                    # with refine: up to this many other local maxima of each candidate's
+                                           # window evaluated too (those with |z| >= (1 - alt_margin) its maximum),
+                                           # the largest exact |z| kept (0 to 3)
+    alt_margin: float = 0.01
+    alt_sep: int = 10                      # local maxima closer than alt_sep + 1 samples count once (the larger)
     consolidate: object = None             # consolidate.KeepLoudest: consolidate every dataset's peaks after each
                                            # tile (None: keep every peak)
     pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
@@ -279,16 +285,25 @@ class _SharedValues:
 
 
 class _PartValues:
-    """The peaks [off, off + n) of a shared exact-stage result (one segment's)."""
+    """The peaks [off, off + n) of a shared exact-stage result (one segment's). alt: (candidate i in [0, n), request
+    j) pairs, the same candidate's requests at its window's alternates; each candidate keeps, of its own request and
+    its alternates', the one with the largest exact |z| (the smaller sample on equal |z|, so the choice does not
+    depend on which peak the GEMM's rounding made the window maximum)."""
 
-    def __init__(self, shared, off, n):
-        self.shared, self.off, self.n = shared, off, n
+    def __init__(self, shared, off, n, alt=None):
+        self.shared, self.off, self.n, self.alt = shared, off, n, alt
 
     def result(self):
         from exact_stage import PeakValues
         v = self.shared.result()
-        sl = slice(self.off, self.off + self.n)
-        return PeakValues(v.z[sl], v.pw[sl], v.fpeak[sl], v.samples[sl])
+        j = np.arange(self.off, self.off + self.n)
+        if self.alt is not None and len(self.alt[0]):
+            ci, aj = self.alt
+            cand, req = np.r_[np.arange(self.n), ci], np.r_[j, aj]
+            o = np.lexsort((v.samples[req], -np.abs(v.z[req].astype(np.complex128)), cand))
+            first = np.r_[True, cand[o][1:] != cand[o][:-1]]
+            j = req[o][first]
+        return PeakValues(v.z[j], v.pw[j], v.fpeak[j], v.samples[j])
 
 
 def candidate_cut(thr, margin):
@@ -750,7 +765,12 @@ class Filter:
                            sel=(torch.empty(w, dtype=torch.int32, device='cuda'),
                                 torch.empty(w, dtype=torch.int32, device='cuda'),
                                 torch.empty(w, dtype=torch.float32, device='cuda'),
-                                torch.empty(nwork, dtype=torch.int32, device='cuda')))
+                                torch.empty(nwork, dtype=torch.int32, device='cuda')),
+                           **(dict(vl=torch.empty(w * fp_gemm.LM, dtype=torch.float32, device='cuda'),
+                                   il=torch.empty(w * fp_gemm.LM, dtype=torch.int32, device='cuda'),
+                                   alt=torch.empty(w * max(self.opts.alternates, 1), dtype=torch.int32,
+                                                   device='cuda'))
+                              if self.opts.refine > 0 and self.opts.alternates > 0 else {}))
                       for _ in range(self._nsets())]
         ws = max((_lib.load().gwgs_basis_series_workspace(tb.r, self.tlen) for plan in plans for tb in plan.tiles),
                  default=0)
@@ -767,13 +787,22 @@ class Filter:
         n, rp, K = len(tb.ids), tb.rp, geo['K']
         dt = OPERAND_DTYPES[self.code]
         T, nwsum = int(geo['row0'][-1]), geo['nwsum']
+        LM = fp_gemm.LM
+        lists = o.refine > 0 and o.alternates > 0
+        assert 0 <= o.alternates < LM
         if bufs is None:
             Zt = Ar = Ai = vw = iw = sel = None
+            if lists:
+                vl = torch.empty((n, nwsum * LM), dtype=torch.float32, device='cuda')
+                il = torch.empty((n, nwsum * LM), dtype=torch.int32, device='cuda')
         else:
             Zt = bufs['zt'][:T * K].view(T, K)
             Ar, Ai = bufs['ar'][:n * K].view(n, K), bufs['ai'][:n * K].view(n, K)
             vw, iw = bufs['vw'][:n * nwsum].view(n, nwsum), bufs['iw'][:n * nwsum].view(n, nwsum)
             sel = bufs['sel']
+            if lists:
+                vl = bufs['vl'][:n * nwsum * LM].view(n, nwsum * LM)
+                il = bufs['il'][:n * nwsum * LM].view(n, nwsum * LM)
         t0 = self._sync()
         if self._overlap() and bufs is None:
             # on the transform stream; the GEMM below waits for this batch's transforms only, so the next batch's
@@ -808,19 +837,39 @@ class Filter:
                       Ar.data_ptr(), Ai.data_ptr(), _dev.stream())
         rows2 = rows2.contiguous()
         e0 = _dev.METER.start()
-        fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, geo['items'], 1 | fp_gemm.STACKED, geo['per'],
-                                 o.gemm_cfg, vw, iw, rows2)
+        ratio = float(np.float32((1 - o.alt_margin) ** 2))
+        cut = candidate_cut(o.snr_threshold, o.margin)
+        if lists:
+            # with each window's largest local maxima that can be alternates: within the margin of the window's
+            # maximum, in windows that reach the candidate cut (the gate a hair lower than the alternates' ratio and
+            # the floor than the cut, so that rounding cannot drop one)
+            fp_gemm.gemm_peaks_multi_lm(self.code, Zt, K, Ar, Ai, K, n, K, geo['items'], True, geo['per'],
+                                        o.alt_sep, o.gemm_cfg, vw, iw, vl, il, rows2, gate=ratio * (1 - 1e-5),
+                                        floor=float(cut) * (1 - 1e-5))
+        else:
+            fp_gemm.gemm_peaks_multi(self.code, Zt, K, Ar, Ai, K, n, K, geo['items'], 1 | fp_gemm.STACKED,
+                                     geo['per'], o.gemm_cfg, vw, iw, rows2)
         # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
         _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(geo['Tns'])))
-        cut = candidate_cut(o.snr_threshold, o.margin)
         if pk is not None:
             e0 = _dev.METER.start()
             self._prekeep.launch(vw, iw, cut, pk)
             # algorithmic bytes: one read of the window maxima (the candidates' samples and drops are a few percent)
             _dev.METER.stop('pre_keep', e0, 4.0 * vw.numel())
         idx, samp, val, cnt = peak_kernel.select_launch(vw, iw, float(cut), sel)
+        alt = None
+        if lists:
+            # every exact-band candidate's alternates: its window's other local maxima within the margin of its
+            # window maximum, farther than the re-centring half width from it
+            na, w = o.alternates, idx.numel()
+            alt = (bufs['alt'][:w * na] if bufs is not None else
+                   torch.empty(w * na, dtype=torch.int32, device='cuda')).view(w, na)
+            xcut = float(np.float32((o.exact_snr * (1 - o.margin)) ** 2))
+            _lib.call('gwgs_window_alternates', vl.data_ptr(), il.data_ptr(), vl.stride(0), LM, idx.data_ptr(),
+                      samp.data_ptr(), val.data_ptr(), cnt.data_ptr(), w, nwsum, self.cw, xcut, ratio,
+                      max(int(o.refine), int(o.alt_sep)), na, alt.data_ptr(), _dev.stream())
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
-        return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt))
+        return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt), alt=alt)
 
     def launch_batch(self, tb: TileBasis, segs, norm, tile_host, key=None, ordinal=0, pi=0):
         """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
@@ -848,7 +897,7 @@ class Filter:
                       for i, (di, s, gs, a0, nw) in enumerate(segs)]
             dev = self._batch_device(tb, segs, norm, geo, pk=pk)
         idx, samp, val, cnt = dev['sel']
-        sel = peak_kernel.Selection(idx, samp, val, cnt, _dev.mark())
+        sel = peak_kernel.Selection(idx, samp, val, cnt, _dev.mark(), extra=dev.get('alt'))
         return dict(tb=tb, segs=segs, Zt=dev['Zt'], Ar=dev['Ar'], Ai=dev['Ai'], rows2=dev['rows2'],
                     nwsum=geo['nwsum'], col0=geo['col0'], row0=geo['row0'], K=geo['K'], sel=sel,
                     tile_host=tile_host)
@@ -875,7 +924,10 @@ class Filter:
         """Stage B1: read the batch's candidate list back (waiting only for its own stage A), split it by segment,
         and enqueue the rebuilt phases (fp_filter.py:759-764) and each segment's exact stage."""
         t0 = time.perf_counter()
-        g, il, val = st['sel'].result()
+        res = st['sel'].result()
+        g, il, val = res[:3]
+        # (candidates, LM - 1) window offsets of each candidate's alternates, -1 past the last
+        offa = res[3] if len(res) > 3 else None
         st['pend'] = []
         self.T['n_candidates'] += len(g)
         if len(g) == 0:
@@ -913,26 +965,38 @@ class Filter:
             if ex.any():
                 sx = samp[ex]
                 # the cluster window of each candidate bounds its re-centring (a0 + window index * cw)
-                xparts.append((len(st['pend']), tb_[ex], sx, a0 + (sx - a0) // self.cw * self.cw))
+                wlo = a0 + (sx - a0) // self.cw * self.cw
+                s2 = offa[sel][ex] if offa is not None else np.full((len(sx), 1), -1)
+                xparts.append((len(st['pend']), tb_[ex], sx, wlo, s2))
             st['pend'].append([di, s, gs, sel, tb_, samp, ax, ex, None])
         if xparts:
             # the exact stage of every segment of the batch in one edges launch and one sums launch
             pend = st['pend']
             seglist = [SegmentData(self.data[pend[k][0]].D[pend[k][1]], self.data[pend[k][0]].IP[pend[k][1]],
-                                   self.df, self.tlen) for k, _, _, _ in xparts]
+                                   self.df, self.tlen) for k, _, _, _, _ in xparts]
             rows = np.concatenate([x[1] for x in xparts])
             smp = np.concatenate([x[2] for x in xparts])
             sidx = np.repeat(np.arange(len(xparts)), [len(x[1]) for x in xparts])
             if o.refine > 0:
                 wlo = np.concatenate([x[3] for x in xparts])
-                pv = tb.stage.evaluate_async_multi(seglist, sidx, PeakRequest(rows, smp), refine=o.refine, lo=wlo,
-                                                   hi=wlo + self.cw)
+                s2 = np.concatenate([x[4] for x in xparts])
+                # one more request per alternate: every request re-centred in its window, each candidate keeping
+                # the largest exact |z| (_PartValues)
+                ci, ai = np.nonzero(s2 >= 0)
+                req_rows, req_smp = np.r_[rows, rows[ci]], np.r_[smp, wlo[ci] + s2[ci, ai]]
+                req_lo = np.r_[wlo, wlo[ci]]
+                pv = tb.stage.evaluate_async_multi(seglist, np.r_[sidx, sidx[ci]], PeakRequest(req_rows, req_smp),
+                                                   refine=o.refine, lo=req_lo, hi=req_lo + self.cw)
             else:
                 pv = tb.stage.evaluate_async_multi(seglist, sidx, PeakRequest(rows, smp))
+                ci = np.zeros(0, np.int64)
+            aj = len(rows) + np.arange(len(ci))
+            self.T['n_alternates'] = self.T.get('n_alternates', 0) + len(ci)
             shared = _SharedValues(pv)
             off = 0
-            for k, r_, _, _ in xparts:
-                pend[k][8] = _PartValues(shared, off, len(r_))
+            for k, r_, _, _, _ in xparts:
+                m_ = (ci >= off) & (ci < off + len(r_))
+                pend[k][8] = _PartValues(shared, off, len(r_), (ci[m_] - off, aj[m_]))
                 off += len(r_)
         # the batch's device buffers stay referenced by st until collect_batch
         self.T['rebuilt_z_s'] += time.perf_counter() - t0

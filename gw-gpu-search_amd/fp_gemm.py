@@ -115,6 +115,22 @@ def gemm_peaks_multi(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, mode, per, c
               _dev.stream())
 
 
+LM = 4   # local maxima per (template, window) in gemm_peaks_multi_lm
+
+
+def gemm_peaks_multi_lm(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, stacked, per, sep, cfg, V, I, VL, IL,
+                        rowscale=None, gate=0.0, floor=0.0):
+    """gemm_peaks_multi in window mode, also writing each window's LM largest local maxima of |z|^2 separated by
+    more than sep samples (VL, row scaled, -1 for none) and their samples (IL) as (M, nwin * LM); local maxima below
+    gate x max(window maximum, floor) (row scaled) need not be listed."""
+    _, variant = _cfg(cfg)
+    tab, nitems = items
+    _lib.call('gwgs_gemm_peaks_multi_lm', dtype_code, variant, Zt.data_ptr(), ldz, Ar.data_ptr(), Ai.data_ptr(),
+              lda, M, K, tab.data_ptr(), nitems, int(bool(stacked)), per, int(sep), float(gate), float(floor),
+              V.data_ptr(), I.data_ptr(), V.stride(0), VL.data_ptr(), IL.data_ptr(), VL.stride(0),
+              _dev.ptr(rowscale), _dev.stream())
+
+
 def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False, rowscale=None):
     """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32, bfloat16, float16 (the reference's
     operand type, on the fp16 MFMA) or float8_e4m3fn. Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are

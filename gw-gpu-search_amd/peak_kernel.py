@@ -55,21 +55,26 @@ def select_above(vw, cut):
 
 
 class Selection:
-    """Candidate list being compacted on the device (select_peaks); result() reads it back."""
+    """Candidate list being compacted on the device (select_peaks); result() reads it back. extra: a per-candidate
+    device array written in the same pass (e.g. each candidate's alternate peaks), read back with the list."""
 
-    def __init__(self, idx, samp, val, count, after):
-        self.idx, self.samp, self.val, self.after = idx, samp, val, after
+    def __init__(self, idx, samp, val, count, after, extra=None):
+        self.idx, self.samp, self.val, self.after, self.extra = idx, samp, val, after, extra
         self.count = _dev.Fetch(count, after=after)
 
     def result(self):
-        """(flat index g, window sample iw[g], vw[g]) of every vw > cut, in increasing g, as numpy arrays. Waits only
-        for the selection's own work (and the copies), not for work enqueued after it."""
+        """(flat index g, window sample iw[g], vw[g]) of every vw > cut, in increasing g, as numpy arrays (and the
+        extra array's entries, or None, when made with extra). Waits only for the selection's own work (and the
+        copies), not for work enqueued after it."""
         import numpy as np
         n = int(self.count.get()[0][0])
         if n == 0:
-            return np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32)
-        g, s, v = _dev.Fetch(self.idx[:n], self.samp[:n], self.val[:n], after=self.after).get()
-        return g.astype(np.int64), s, v
+            out = (np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
+            return out if self.extra is None else out + (np.zeros((0,) + tuple(self.extra.shape[1:]), np.int32),)
+        g, s, v, e = _dev.Fetch(self.idx[:n], self.samp[:n], self.val[:n],
+                                None if self.extra is None else self.extra[:n], after=self.after).get()
+        out = (g.astype(np.int64), s, v)
+        return out if self.extra is None else out + (e,)
 
 
 def select_launch(vw, iw, cut, out=None):

@@ -49,6 +49,21 @@ int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, const void
 int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai, long lda,
                           int M, int K, const void* items, int nitems, int mode, int per, float* V, int* I, long ldv,
                           const float* rowscale, void* stream);
+/* No reference counterpart (the reference keeps one maximum per window, fp_gemm.py:21-33): gwgs_gemm_peaks_multi in
+ * mode 1 (stacked: 1 for the rotated-operand stacking of mode bit 4), and per (template, window) also its
+ * gwgs_gemm_lm_count() = 4 largest separated local maxima of |z|^2 (p[t] >= p[t - 1] and p[t] > p[t + 1], the
+ * window's edges open; of local maxima within sep samples of each other only the larger, the earlier on ties),
+ * larger value first, then earlier sample: VL[m ldl + col LM + k] row scaled (-1 where there are fewer), IL their
+ * segment-relative samples (-1 likewise); ldl >= LM ldv; 0 <= sep < per x 128. Local maxima below gate x
+ * max(window maximum, floor) (0 <= gate <= 1, floor >= 0 on the row-scaled |z|^2) need not be listed: the list holds
+ * the largest of those at or above it (a window below the floor may list none). The
+ * exact stage evaluates a candidate's window maximum together with the loudest other local maxima, so that two
+ * operand precisions report the same sample. */
+int gwgs_gemm_peaks_multi_lm(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai,
+                             long lda, int M, int K, const void* items, int nitems, int stacked, int per, int sep,
+                             float gate, float floor, float* V, int* I, long ldv, float* VL, int* IL, long ldl,
+                             const float* rowscale, void* stream);
+int gwgs_gemm_lm_count(void);
 int gwgs_gemm_item_bytes(void);
 /* largest K the GEMM accepts for dtype (0 if the dtype is unsupported) */
 int gwgs_gemm_max_k(int dtype);
@@ -77,6 +92,15 @@ int gwgs_select_peaks(const float* vw, const int* iw, long total, float cut, int
  * share the array and race; calls on one stream, or ordered by events, do not. */
 int gwgs_select_peaks_noalloc(const float* vw, const int* iw, long total, float cut, int* out, int* out_iw,
                               float* out_vw, int cap, int* count_dev, void* stream);
+/* No reference counterpart (the reference evaluates the exact stage at the rebuilt argmax only, fp_filter.py:759-
+ * 783): for every candidate of a compacted list (idx / samp / val / count as gwgs_select_peaks wrote them, capacity
+ * cap) whose val >= exact_cut, the entries of its window's local-maximum list (VL / IL / ldl / lm as
+ * gwgs_gemm_peaks_multi_lm wrote them, nwsum window columns) farther than R samples from samp[c] with |z|^2 >=
+ * ratio val[c], in list order, up to na: out[c na + a] = the entry's offset in its window of cw samples, -1 past
+ * the last. */
+int gwgs_window_alternates(const float* VL, const int* IL, long ldl, int lm, const int* idx, const int* samp,
+                           const float* val, const int* count, int cap, int nwsum, int cw, float exact_cut,
+                           float ratio, int R, int na, int* out, void* stream);
 /* pipeline/fp_filter.py:759-764: rebuilt complex SNR at candidates, out[c] = (Ar[rows[c]] . Zt[tcol[c]],
  * Ai[rows[c]] . Zt[tcol[c]]) sqrt(pw_scale[sidx[c]]) in fp32 (complex64 out); sidx NULL indexes pw_scale by rows[c];
  * pw_scale NULL: no scale. dtype as gwgs_gemm_peaks. */
@@ -97,6 +121,16 @@ int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const
 /* bytes of the device workspace gwgs_chisq_points needs (the per-bin sums of every group; 0 when there is none) */
 long gwgs_chisq_points_workspace(int ngroups, int nedge);
 int gwgs_chisq_peaks_per_group(void);
+/* No reference counterpart (the reference evaluates the exact stage at the rebuilt argmax, exact_stage.py:704-731):
+ * hill climbing of n peaks on the exact |z| = |sum over e0 <= k < eB of conj(h_k) d_k exp(2 pi i k t / N)|, e0 and eB
+ * the first and last entries of edge row eidx[r] (edges: rows of nedge int32): from start[r], z at the 2 W + 1 samples
+ * around c clipped to [lo[r], hi[r] - 1]; while the largest |z| (first on ties) lies on the range's edge inside the
+ * bounds the range re-centres on it, at most max_iter times; out[r] = the final largest sample. rows, segs int32:
+ * the template row of h (ldh complex64 per row) and the data row d_tab[segs[r]]; 1 <= W <= 3. One workgroup per
+ * peak. The exact stage uses it for peaks whose re-centring range ended on its edge. */
+int gwgs_climb_points(const void* h, long ldh, const void* const* d_tab, long long N, const int* rows, const int* segs,
+                      const int* edges, int nedge, const int* eidx, const long long* start, const long long* lo,
+                      const long long* hi, int W, int max_iter, int n, long long* out, void* stream);
 /* pipeline/exact_stage.py:173-191 ExactStage.evaluate for the peaks of several segments of one tile in one launch
  * (the reference evaluates segment by segment): gwgs_chisq_points in fused mode where group g reads its data row
  * from d_tab[gseg[g]] (device table of complex64 row addresses). Same workspace as gwgs_chisq_points. */

@@ -510,3 +510,191 @@ def test_autotuner_picks_a_running_config(gpu):
     for V, I in outs[1:]:
         np.testing.assert_allclose(V, outs[0][0], rtol=1e-5)
         assert (I == outs[0][1]).mean() > 0.99
+
+
+def _window_local_maxima(p, Tn, cw, nw, lm):
+    """Per (row, window) the lm largest local maxima of p over the window's valid samples (t < Tn): p[t] >= p[t - 1]
+    and p[t] > p[t + 1], the window's first sample and its last valid sample open on that side; larger value first,
+    then smaller sample; (-1, -1) padding."""
+    M = p.shape[0]
+    VL = np.full((M, nw * lm), -1.0, np.float32)
+    IL = np.full((M, nw * lm), -1, np.int32)
+    for w in range(nw):
+        lo, hi = w * cw, min((w + 1) * cw, Tn)
+        if hi <= lo:
+            continue
+        q = p[:, lo:hi]
+        left = np.ones_like(q, bool); left[:, 1:] = q[:, 1:] >= q[:, :-1]
+        right = np.ones_like(q, bool); right[:, :-1] = q[:, :-1] > q[:, 1:]
+        for m in range(M):
+            t = np.flatnonzero(left[m] & right[m])
+            o = np.lexsort((t, -q[m, t]))[:lm]
+            VL[m, w * lm:w * lm + len(o)] = q[m, t[o]]
+            IL[m, w * lm:w * lm + len(o)] = lo + t[o]
+    return VL, IL
+
+
+@pytest.mark.parametrize('code,stacked,r,sep,gate', [(1, True, 32, 0, 0), (1, True, 24, 0, 0), (1, False, 32, 0, 0),
+                                                     (0, False, 32, 0, 0), (2, True, 32, 0, 0), (3, True, 32, 0, 0),
+                                                     (3, True, 24, 0, 0), (1, True, 32, 16, 0), (0, False, 32, 5, 0),
+                                                     (3, True, 24, 40, 0), (1, True, 32, 0, 0.6),
+                                                     (0, False, 32, 0, 0.8), (1, True, 32, 0, -0.7)])
+def test_local_maxima_lists(gpu, code, stacked, r, sep, gate):
+    """gemm_peaks_multi_lm over three segments (one with a partial last window, rows past its end holding large
+    values that must be masked): integer operands make every |z|^2 exact in fp32 whatever the summation order, so
+    the window maxima equal the mode-1 launch's bit for bit, and with sep = 0 the local-maximum lists equal their
+    definition bit for bit, ties (frequent with integers) included; with a gate, so do the entries at or above gate x
+    the window maximum (gate < 0: gate |gate| with a floor at the windows' median maximum, entries at or above
+    |gate| x max(window maximum, floor)). With sep > 0: every entry is a local maximum, entries are more than sep apart and sorted, and
+    the first is the largest local maximum (earliest on ties). The alternates kernel then equals its rule on the same
+    lists."""
+    import fp_gemm
+    import peak_kernel
+    import _lib, _dev
+    rng = np.random.default_rng(40 + code + 7 * r + stacked + sep + int(10 * abs(gate)))
+    M, cw, LM = 70, 1024, fp_gemm.LM
+    K = 2 * r
+    nws, Tns = [3, 1, 2], [3 * cw, cw, 2 * cw - 300]
+    per = cw // 128
+    A = rng.integers(-3, 4, (M, r)) + 1j * rng.integers(-3, 4, (M, r))
+    Ar = np.zeros((M, K), np.float32); Ai = np.zeros((M, K), np.float32)
+    Ar[:, 0::2] = A.real; Ar[:, 1::2] = -A.imag
+    Ai[:, 0::2] = A.imag; Ai[:, 1::2] = A.real
+    Zs = []
+    for nw, Tn in zip(nws, Tns):
+        Zc = rng.integers(-3, 4, (nw * cw, r)) + 1j * rng.integers(-3, 4, (nw * cw, r))
+        Zc[Tn:] = 100 + 100j                         # past the segment's end: never reported
+        z = np.zeros((nw * cw, K), np.float32); z[:, 0::2] = Zc.real; z[:, 1::2] = Zc.imag
+        Zs.append(z)
+    Zt = np.concatenate(Zs)
+    scales = (2.0 ** rng.integers(-2, 3, (3, M))).astype(np.float32)
+    cast = {0: lambda x: torch.from_numpy(x), 1: lambda x: torch.from_numpy(x).to(torch.bfloat16),
+            2: lambda x: _fp8(x).view(torch.uint8), 3: lambda x: torch.from_numpy(x).to(torch.float16)}[code]
+    qa, qi, qz = (cast(x).cuda() for x in (Ar, Ai, Zt))
+    row0 = np.r_[0, np.cumsum([nw * cw for nw in nws])]; col0 = np.r_[0, np.cumsum(nws)]
+    items = fp_gemm.items_table([(int(row0[i]), nws[i] * per, int(col0[i]), i, Tns[i]) for i in range(3)], 2 * per)
+    nwsum = int(col0[-1])
+    V = torch.empty((M, nwsum), device='cuda'); I = torch.empty((M, nwsum), dtype=torch.int32, device='cuda')
+    VL = torch.empty((M, nwsum * LM), device='cuda'); IL = torch.empty((M, nwsum * LM), dtype=torch.int32,
+                                                                         device='cuda')
+    sc = torch.from_numpy(scales).cuda()
+    ai = torch.full_like(qi, 0x7b) if stacked and code != 0 else qi
+    V1 = torch.empty_like(V); I1 = torch.empty_like(I)
+    fp_gemm.gemm_peaks_multi(code, qz, K, qa, qi, K, M, K, items, 1, per, (2 * per, 2), V1, I1, sc)
+    floor = float(np.median(V1.cpu().numpy())) if gate < 0 else 0.0
+    gate = abs(gate)
+    fp_gemm.gemm_peaks_multi_lm(code, qz, K, qa, ai, K, M, K, items, stacked, per, sep, (2 * per, 0), V, I, VL, IL,
+                                sc, gate=gate, floor=floor)
+    torch.cuda.synchronize()
+    assert np.array_equal(V.cpu().numpy(), V1.cpu().numpy())
+    assert np.array_equal(I.cpu().numpy(), I1.cpu().numpy())
+    vl, il = VL.cpu().numpy(), IL.cpu().numpy()
+    for i in range(3):
+        z = Zt[row0[i]:row0[i + 1]].astype(np.float64)
+        p = ((Ar.astype(np.float64) @ z.T) ** 2 + (Ai.astype(np.float64) @ z.T) ** 2) * scales[i][:, None]
+        p = p.astype(np.float32)
+        sl = slice(col0[i] * LM, col0[i + 1] * LM)
+        if sep == 0:
+            ev, ei = _window_local_maxima(p, Tns[i], cw, nws[i], LM)
+            if gate > 0:
+                # entries at or above gate x the window maximum (its first entry) must match; others may be absent
+                top = np.repeat(np.maximum(ev.reshape(M, nws[i], LM)[:, :, 0], np.float32(floor)), LM,
+                                1).reshape(ev.shape)
+                keep = ev >= np.float32(gate) * top
+                g_keep = vl[:, sl] >= np.float32(gate) * top
+                assert np.array_equal(keep, g_keep), f'segment {i} gated entries'
+                assert np.array_equal(np.where(keep, vl[:, sl], -1), np.where(keep, ev, -1))
+                assert np.array_equal(np.where(keep, il[:, sl], -1), np.where(keep, ei, -1))
+                continue
+            assert np.array_equal(vl[:, sl], ev), f'segment {i} values'
+            assert np.array_equal(il[:, sl], ei), f'segment {i} samples'
+            continue
+        ev, ei = _window_local_maxima(p, Tns[i], cw, nws[i], cw)     # every local maximum, sorted
+        gv, gi = vl[:, sl].reshape(M, nws[i], LM), il[:, sl].reshape(M, nws[i], LM)
+        ev, ei = ev.reshape(M, nws[i], cw), ei.reshape(M, nws[i], cw)
+        for m in range(M):
+            for w in range(nws[i]):
+                ok = gi[m, w] >= 0
+                lmset = dict(zip(ei[m, w][ei[m, w] >= 0].tolist(), ev[m, w][ei[m, w] >= 0].tolist()))
+                assert ok[0] and (gv[m, w, 0], gi[m, w, 0]) == (ev[m, w, 0], ei[m, w, 0])
+                assert all(lmset.get(int(t)) == v for t, v in zip(gi[m, w][ok], gv[m, w][ok]))
+                t_ok = np.sort(gi[m, w][ok])
+                assert np.all(np.diff(t_ok) > sep)
+                vv = gv[m, w][ok]
+                assert np.all((vv[:-1] > vv[1:]) | ((vv[:-1] == vv[1:]) & (gi[m, w][ok][:-1] < gi[m, w][ok][1:])))
+    # the alternates of a candidate list on these lists
+    cut = float(np.quantile(V.cpu().numpy(), 0.7))
+    idx, samp, val, cnt = peak_kernel.select_launch(V, I, cut)
+    w, na, R, ratio, xcut = idx.numel(), LM - 1, 3, 0.9, float(np.quantile(V.cpu().numpy(), 0.8))
+    alt = torch.empty((w, na), dtype=torch.int32, device='cuda')
+    _lib.call('gwgs_window_alternates', VL.data_ptr(), IL.data_ptr(), VL.stride(0), LM, idx.data_ptr(),
+              samp.data_ptr(), val.data_ptr(), cnt.data_ptr(), w, nwsum, cw, float(np.float32(xcut)),
+              float(np.float32(ratio)), R,
+              na, alt.data_ptr(), _dev.stream())
+    torch.cuda.synchronize()
+    n = int(cnt.cpu()[0])
+    assert n > 10
+    g, s0, v0, a = (x.cpu().numpy()[:n] for x in (idx, samp, val, alt))
+    for c in range(n):
+        exp = []
+        if v0[c] >= np.float32(xcut):
+            m, col = divmod(int(g[c]), nwsum)
+            for k in range(LM):
+                t, v = il[m, col * LM + k], vl[m, col * LM + k]
+                if t >= 0 and v >= np.float32(ratio) * v0[c] and abs(t - s0[c]) > R and len(exp) < na:
+                    exp.append(t % cw)
+        assert list(a[c]) == exp + [-1] * (na - len(exp)), c
+
+
+def test_climb_points_vs_float64(gpu):
+    """gwgs_climb_points against the same climb on float64 exact sums: data holding a template at a known sample
+    plus noise; starts on either slope of a broad peak (reached from far: several re-centred ranges), at the peak,
+    with bounds that cut the climb off before the peak (the climb stops on the bound), and on a wide band (two
+    phasor runs per lane) with a sharp peak."""
+    import _lib, _dev
+    rng = np.random.default_rng(5)
+    N, nseg, nedge, W = 32768, 3, 17, 3
+    flen = N // 2 + 1
+    k = np.arange(flen)
+    bands = [(40, 400), (40, 400), (40, 16000)]
+    amp = np.stack([rng.random(flen) * ((k >= a) & (k < b)) for a, b in bands])
+    H = (amp * np.exp(1j * rng.uniform(0, 2 * np.pi, (nseg, flen)))).astype(np.complex64)
+    t0 = np.array([1000, 20500, 30000])
+    D = np.stack([H[s] * np.exp(-2j * np.pi * k * t0[s] / N) +
+                  0.05 * (rng.standard_normal(flen) + 1j * rng.standard_normal(flen)) for s in range(nseg)])
+    D = D.astype(np.complex64)
+    E = np.stack([np.linspace(a, b, nedge) for a, b in bands]).astype(np.int32)
+    rows, segs, start, lo, hi = [], [], [], [], []
+    for s in range(nseg):
+        for off, (a, b) in ((-25, (-100, 100)), (-7, (-100, 100)), (0, (-100, 100)), (4, (-100, 100)),
+                            (30, (-100, 100)), (-40, (-100, -10)), (40, (12, 100))):
+            rows.append(s); segs.append(s); start.append(t0[s] + off); lo.append(t0[s] + a); hi.append(t0[s] + b)
+    rows, segs, start, lo, hi = (np.array(x) for x in (rows, segs, start, lo, hi))
+    Hd, Dd = torch.from_numpy(H).cuda(), torch.from_numpy(D).cuda()
+    tab = np.array([Dd[s].data_ptr() for s in range(nseg)], np.int64)
+    r_d, s_d, e_d, st_d, lo_d, hi_d, tab_d, E_d = _dev.to_dev_many(rows.astype(np.int32), segs.astype(np.int32),
+                                                                   segs.astype(np.int32), start, lo, hi, tab, E)
+    out = torch.empty(len(rows), dtype=torch.int64, device='cuda')
+    _lib.call('gwgs_climb_points', Hd.data_ptr(), flen, tab_d.data_ptr(), N, r_d.data_ptr(), s_d.data_ptr(),
+              E_d.data_ptr(), nedge, e_d.data_ptr(), st_d.data_ptr(), lo_d.data_ptr(), hi_d.data_ptr(), W, 1024,
+              len(rows), out.data_ptr(), _dev.stream())
+    got = out.cpu().numpy()
+    prod = np.conj(H.astype(np.complex128)) * D.astype(np.complex128)
+
+    def climb(s, c, l, h):
+        a, b = bands[s]
+        kk = k[a:b]
+        while True:
+            ts = np.clip(c + np.arange(-W, W + 1), l, h - 1)
+            z = np.abs(prod[s, a:b] @ np.exp(2j * np.pi * np.outer(kk, ts) / N)) ** 2
+            j = int(np.argmax(z))
+            if (j == 0 and ts[0] > l) or (j == 2 * W and ts[-1] < h - 1):
+                c = ts[j]
+                continue
+            return ts[j]
+
+    exp = np.array([climb(s, c, l, h) for s, c, l, h in zip(segs, start, lo, hi)])
+    assert np.array_equal(got, exp), (got, exp)
+    for s in (0, 1):                      # the broad peaks: every start on the hill reaches the top
+        g = got[segs == s]
+        assert np.array_equal(g[:5], [t0[s]] * 5) and g[5] == t0[s] - 11 and g[6] == t0[s] + 12, (s, g)
