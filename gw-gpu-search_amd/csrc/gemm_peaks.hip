@@ -545,7 +545,7 @@ template <> struct KloopTraits<float> { static constexpr int KC = 32, EPV = 4, P
 template <> struct KloopTraits<uint8_t> { static constexpr int KC = 128, EPV = 16, PAD = 16; };
 
 template <typename T, int MODE, bool ROT_>
-__global__ void __launch_bounds__(NT, 2)
+__global__ void __launch_bounds__(NT, MODE == 2 ? 1 : 2)
 gemm_peaks_kloop_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const T* __restrict__ Ai,
                    const long lda, const int K, const Job J, float* __restrict__ V, int* __restrict__ I,
                    const long ldv)
