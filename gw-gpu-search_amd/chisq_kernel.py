@@ -78,10 +78,11 @@ def launch_sorted(q, rows, times, E_dev, e_index, want_z=False, d=None, n_time=N
     return _launch(q, np.asarray(times, np.int64), grow, gstart, gcount, E_dev[gi], want_z, d, N, span)
 
 
-def launch_sorted_multi(h, keys, n_rows, times, E_dev, e_index, d_tab, n_time, row_span=None):
+def launch_sorted_multi(h, keys, n_rows, times, E_dev, e_index, d_tab, n_time, row_span=None, regen=None):
     """launch_sorted in fused mode over peaks of several segments: keys = segment n_rows + row, sorted (any time
     order within a key); the template rows are h's, the data row of segment s is at d_tab[s] (int64 host array of
-    device addresses of complex64 rows). Returns device (out, zo) in the given peak order."""
+    device addresses of complex64 rows). regen (an exact_stage.Coarse): the rows are regenerated from their nodes
+    instead of read from h (gwgs_regen_points_multi). Returns device (out, zo) in the given peak order."""
     assert h.dtype == torch.complex64 and h.is_contiguous()
     keys = np.asarray(keys, np.int64)
     gkey, gstart, gcount = groups_of_sorted(keys)
@@ -99,10 +100,19 @@ def launch_sorted_multi(h, keys, n_rows, times, E_dev, e_index, d_tab, n_time, r
     work = torch.empty(max(nwork, 8), dtype=torch.uint8, device=h.device)
     span = 0.0 if row_span is None else float(np.asarray(row_span)[gkey % n_rows].sum())
     e0 = _dev.METER.start()
-    _lib.call('gwgs_chisq_points_multi', h.data_ptr(), h.shape[1], tab.data_ptr(), g_seg.data_ptr(), int(n_time),
-              g_row.data_ptr(), g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(), ge.shape[1],
-              len(gstart), out.data_ptr(), zo.data_ptr(), work.data_ptr(), _dev.stream())
-    _dev.METER.stop('chisq', e0, span * 16)
+    if regen is not None:
+        g = regen.grid
+        _lib.call('gwgs_regen_points_multi', regen.coef32.data_ptr(), regen.amp2.data_ptr(), g.nx,
+                  g.node_d.data_ptr(), g.d32.data_ptr(), g.t32.data_ptr(), g.kmin, tab.data_ptr(), g_seg.data_ptr(),
+                  int(n_time), g_row.data_ptr(), g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(),
+                  ge.shape[1], len(gstart), out.data_ptr(), zo.data_ptr(), work.data_ptr(), _dev.stream())
+        # HBM bytes: the data per bin (the grid and the node tables are shared by the rows and stay in cache)
+        _dev.METER.stop('chisq_regen', e0, span * 8)
+    else:
+        _lib.call('gwgs_chisq_points_multi', h.data_ptr(), h.shape[1], tab.data_ptr(), g_seg.data_ptr(), int(n_time),
+                  g_row.data_ptr(), g_start.data_ptr(), g_cnt.data_ptr(), tdev.data_ptr(), ge.data_ptr(),
+                  ge.shape[1], len(gstart), out.data_ptr(), zo.data_ptr(), work.data_ptr(), _dev.stream())
+        _dev.METER.stop('chisq', e0, span * 16)
     return out, zo
 
 

@@ -351,6 +351,155 @@ __global__ void __launch_bounds__(NT) climb_k(const float2* __restrict__ h, cons
     if (tid == 0) out[r] = c;
 }
 
+// ---- the regenerated exact stage (the reference's --exact-stage regen, exact_stage.py:472-647) --------------------
+// Templates given by coarse nodes (templates.hip spa_nodes_k): on the fine bins of [kmin, kend), bin q = k - kmin lies
+// in interval node[q] with d[q] = f - x_j and t[q] = d / (x_j+1 - x_j); h_k = (A_j (1 - t) + A_j+1 t) exp(i p_j(d)).
+
+// a^2 p0 + 2 a b p1 + b^2 p2 in a fixed association without contraction, so an interval's weight from its totals
+// equals its in-interval prefix at its last bin bit for bit
+__device__ __forceinline__ double quad3(const double a, const double b, const double* m)
+{
+    return __dadd_rn(__dadd_rn(__dmul_rn(__dmul_rn(a, a), m[0]), __dmul_rn(__dmul_rn(__dmul_rn(2.0, a), b), m[1])),
+                     __dmul_rn(__dmul_rn(b, b), m[2]));
+}
+
+// Chi-squared edges of the interpolated templates from the segment's moments of 1/psd: Mi (J, 3) the interval totals
+// of ((1-t)^2, t(1-t), t^2) IP and Pk (K, 3) their in-interval inclusive prefixes at each bin, so that the weight
+// sum of |h|^2 IP over interval j is quad3(A_j, A_j+1, Mi[j]) and up to bin k of it quad3(A_j, A_j+1, Pk[k]). One
+// block per requested row: each thread sums a contiguous run of intervals, a block scan of the runs, then per
+// target j / nb * total the run (binary search), the interval (a walk from the run's start) and the bin (binary
+// search on the prefixes): edge = first bin whose running sum exceeds the target. Columns >= nb hold kend.
+__global__ void __launch_bounds__(NT) edges_moments_k(const double* __restrict__ amp, const int nx,
+                                                      const double* __restrict__ Mi, const double* __restrict__ Pk,
+                                                      const int* __restrict__ kb, const int* __restrict__ rows,
+                                                      const int* __restrict__ nbs, const int nedge, const int kmin,
+                                                      const int kend, int* __restrict__ out)
+{
+    __shared__ double pre[NT + 1];
+    __shared__ double sh[NT / 64];
+    const int u = blockIdx.x, tid = threadIdx.x, J = nx - 1;
+    const double* A = amp + (size_t)rows[u] * nx;
+    const int nb = nbs[u];
+    const int per = (J + NT - 1) / NT, j0 = min(tid * per, J), j1 = min(j0 + per, J);
+    double s = 0.0;
+    for (int j = j0; j < j1; j++) s = __dadd_rn(s, quad3(A[j], A[j + 1], Mi + 3 * (size_t)j));
+    double tot;
+    const double ex = block_exscan_f64(s, sh, tot);
+    pre[tid] = ex;
+    if (tid == 0) pre[NT] = tot;
+    __syncthreads();
+    int* e = out + (size_t)u * nedge;
+    for (int i = tid; i < nedge; i += NT) {
+        if (i >= nb) { e[i] = kend; continue; }
+        const double T = ((double)i / (double)nb) * tot;
+        // the last run whose start lies at or below T
+        int lo = 0, hi = NT - 1;
+        while (lo < hi) { const int m = (lo + hi + 1) >> 1; if (pre[m] <= T) lo = m; else hi = m - 1; }
+        // the walk from the run's start (past its end if the scan's association left the run just short)
+        const int qa = min(lo * per, J);
+        double r = pre[lo];
+        int j = 0;
+        bool hit = false;
+        for (int jj = qa; jj < J; jj++) {
+            const double w = quad3(A[jj], A[jj + 1], Mi + 3 * (size_t)jj);
+            if (__dadd_rn(r, w) > T) { j = jj; hit = true; break; }
+            r = __dadd_rn(r, w);
+        }
+        if (!hit) { e[i] = kend; continue; }
+        const double a = A[j], b = A[j + 1];
+        int klo = kb[j] - kmin, khi = kb[j + 1] - kmin - 1;
+        while (klo < khi) {
+            const int m = (klo + khi) >> 1;
+            if (__dadd_rn(r, quad3(a, b, Pk + 3 * (size_t)m)) > T) khi = m; else klo = m + 1;
+        }
+        e[i] = min(max(klo + kmin, kmin), kend);
+    }
+}
+
+// chisq_bins_k with the template regenerated on the fly (no reads of H): per bin, the interval's float32 cubic
+// (c3, c2, c1, c0 with c0 reduced to [-pi, pi)) at d, reduced by 2 pi, __sincosf, the amplitude lerp; then
+// conj(h) D and the P peak phasors as in chisq_bins_k. coef32 (rows, J) float4, amp2 (rows, J) float2.
+template <bool MULTI>
+__global__ void __launch_bounds__(NT) regen_bins_k(const float4* __restrict__ coef32, const float2* __restrict__ amp2,
+                                                   const int J, const int* __restrict__ node,
+                                                   const float* __restrict__ dl, const float* __restrict__ tl,
+                                                   const int kmin, const float2* __restrict__ dv,
+                                                   const unsigned long long N, const int* __restrict__ grow,
+                                                   const int* __restrict__ gstart, const int* __restrict__ gcount,
+                                                   const long long* __restrict__ ptime, const int* __restrict__ edges,
+                                                   const int nedge, const int ngroups, float2* __restrict__ zp,
+                                                   const float2* const* __restrict__ dtab,
+                                                   const int* __restrict__ gseg)
+{
+    constexpr float TWO_PI = 6.283185307179586f, INV_TWO_PI = 0.15915494309189535f;
+    const int nbin = nedge - 1;
+    const int lane = threadIdx.x & 63;
+    const long wi = (long)blockIdx.x * (NT / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wi >= (long)ngroups * nbin) return;
+    const int g = (int)(wi / nbin), j = (int)(wi % nbin);
+    const int row = grow[g], p0 = gstart[g], np_ = gcount[g];
+    const int* e = edges + (size_t)g * nedge;
+    const int k0 = e[j], k1 = e[j + 1];
+    float2* zo = zp + (size_t)wi * P;
+    if (k0 >= k1) {
+        if (lane < P) zo[lane] = make_float2(0.f, 0.f);
+        return;
+    }
+    if (MULTI) dv = dtab[gseg[g]];
+    const unsigned long long mask = (N & (N - 1)) == 0 && N <= (1ull << 24) ? N - 1 : 0;
+    const float4* cr = coef32 + (size_t)row * J;
+    const float2* ar = amp2 + (size_t)row * J;
+    unsigned long long t[P];
+    float2 rot[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        t[p] = p < np_ ? (unsigned long long)ptime[p0 + p] % N : 0ull;
+        rot[p] = phasor(64ull, t[p], N, mask);
+    }
+    float2 acc[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) acc[p] = make_float2(0.f, 0.f);
+    constexpr int RW = 64 * R;
+    for (int ra = k0; ra < k1; ra += RW) {
+        const int ka = ra + lane;
+        const int ns = (min(ra + RW, k1) - ra + 63) >> 6;
+        float2 w[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) w[p] = p < np_ ? phasor((unsigned long long)ka, t[p], N, mask) : make_float2(0.f, 0.f);
+        for (int st = 0; st < ns; st++) {
+            const int k = ka + 64 * st;
+            float2 v = make_float2(0.f, 0.f);
+            if (k < k1) {
+                const int q = k - kmin, jn = node[q];
+                const float4 c = cr[jn];
+                const float2 am = ar[jn];
+                const float dd = dl[q];
+                float x = fmaf(fmaf(fmaf(c.x, dd, c.y), dd, c.z), dd, c.w);
+                x = fmaf(-TWO_PI, rintf(x * INV_TWO_PI), x);
+                const float a = fmaf(am.y - am.x, tl[q], am.x);
+                float sn, cs;
+                __sincosf(x, &sn, &cs);
+                const float hr = a * cs, hi = a * sn;
+                const float2 dk = dv[k];
+                v = make_float2(fmaf(hr, dk.x, hi * dk.y), fmaf(hr, dk.y, -hi * dk.x));   // conj(h) D
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                acc[p].x = fmaf(v.x, w[p].x, fmaf(-v.y, w[p].y, acc[p].x));
+                acc[p].y = fmaf(v.x, w[p].y, fmaf(v.y, w[p].x, acc[p].y));
+                const float wx = fmaf(w[p].x, rot[p].x, -w[p].y * rot[p].y);
+                w[p].y = fmaf(w[p].x, rot[p].y, w[p].y * rot[p].x);
+                w[p].x = wx;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const float ax = wave_sum_f32(acc[p].x), ay = wave_sum_f32(acc[p].y);
+        if (lane == p) zo[p] = make_float2(ax, ay);
+    }
+}
+
 } // namespace
 
 // Groups: for g in [0, ngroups): row grow[g] of q (or h in fused mode), peaks gstart[g] .. gstart[g] + gcount[g]
@@ -461,5 +610,49 @@ GWGS_API int gwgs_edges_fused_multi(const void* h, long ldh, const int* rows, co
     if (n == 0) return GWGS_OK;
     hipLaunchKernelGGL(edges_fused_k<true>, dim3(n), dim3(NT), 0, (hipStream_t)stream, (const float2*)h, ldh, rows,
                        nullptr, k1s, nbs, nedge, kmin, out, (const float* const*)ip_tab, rseg);
+    return gwgs_status();
+}
+
+// Chi-squared edges of n rows of interpolated templates from one segment's moments (see edges_moments_k): amp (rows,
+// nx) float64 node amplitudes; Mi (nx - 1, 3) and Pk (kend - kmin, 3) float64; kb (nx) int32 the first bin of each
+// interval (kb[nx - 1] = kend); out (n, nedge) int32.
+GWGS_API int gwgs_edges_moments(const double* amp, int nx, const double* Mi, const double* Pk, const int* kb,
+                                const int* rows, const int* nbs, int nedge, int kmin, int kend, int n, int* out,
+                                void* stream)
+{
+    GWGS_CHECK(amp && Mi && Pk && kb && rows && nbs && out && nx >= 2 && nedge >= 1 && kmin >= 0 && kend >= kmin &&
+               n >= 0, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    hipLaunchKernelGGL(edges_moments_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, amp, nx, Mi, Pk, kb, rows, nbs,
+                       nedge, kmin, kend, out);
+    return gwgs_status();
+}
+
+// gwgs_chisq_points_multi for templates regenerated from coarse nodes instead of read from H: coef32 (rows, nx - 1)
+// float4 (c3, c2, c1, c0 reduced), amp2 (rows, nx - 1) float2 (A_j, A_j+1); node, d, t over the kend - kmin fine bins
+// (int32, float32, float32); every edge within [kmin, kend]. Workspace as gwgs_chisq_points_workspace.
+GWGS_API int gwgs_regen_points_multi(const void* coef32, const void* amp2, int nx, const int* node, const float* d,
+                                     const float* t, int kmin, const void* const* d_tab, const int* gseg, long long N,
+                                     const int* grow, const int* gstart, const int* gcount, const long long* ptime,
+                                     const int* edges, int nedge, int ngroups, float* out, void* zout, void* work,
+                                     void* stream)
+{
+    GWGS_CHECK(coef32 && amp2 && node && d && t && d_tab && gseg && grow && gstart && gcount && ptime && edges && out &&
+               N > 0 && nx >= 2 && nedge >= 1 && ngroups >= 0, GWGS_EINVAL);
+    if (ngroups == 0) return GWGS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (nedge == 1) {
+        hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                           (const float2*)nullptr, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
+        return gwgs_status();
+    }
+    GWGS_CHECK(work != nullptr, GWGS_EINVAL);
+    GWGS_CHECK((long)ngroups * (nedge - 1) < (1L << 33), GWGS_ESIZE);
+    const dim3 grid((unsigned)(((long)ngroups * (nedge - 1) + NT / 64 - 1) / (NT / 64)));
+    hipLaunchKernelGGL(regen_bins_k<true>, grid, dim3(NT), 0, s, (const float4*)coef32, (const float2*)amp2, nx - 1,
+                       node, d, t, kmin, nullptr, (unsigned long long)N, grow, gstart, gcount, ptime, edges, nedge,
+                       ngroups, (float2*)work, (const float2* const*)d_tab, gseg);
+    hipLaunchKernelGGL(chisq_finish_k, dim3((unsigned)(((long)ngroups * P + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const float2*)work, gstart, gcount, edges, nedge, ngroups, out, (float2*)zout);
     return gwgs_status();
 }

@@ -108,6 +108,75 @@ __global__ void sigmasq_sum_k(const double* __restrict__ part, const int n, cons
     sig[(size_t)i * lds + s] = 4.0 * df * t;
 }
 
+// Coarse-node form of the same chirps (the interpolated templates of the reference's --templates interp,
+// imr_interp.py:53-85): at nodes x_0 < ... < x_J the amplitude f^(-7/6) (0 at or above f_isco) and, per interval
+// j, the phase -psi as a cubic in d = f - x_j, c3 d^3 + c2 d^2 + c1 d + c0, the Hermite cubic through -psi and its
+// analytic derivative at both ends (psi as in spa_templates_k). One thread per (template, node).
+__device__ __forceinline__ void spa_phase(const double f, const double M, const double pref, const double c1,
+                                          const double ke, const double f0, double& ph, double& dph)
+{
+    const double pmf = M_PI * M * f;
+    const double v5 = pow(pmf, -5.0 / 3.0), v2i = 1.0 / pmf;
+    const double fe = ke != 0.0 ? pow(f / f0, -19.0 / 9.0) : 0.0;
+    const double psi = pref * (v5 + c1 * v2i - ke * v5 * fe) - 0.25 * M_PI;
+    const double dpsi = pref * (-5.0 / 3.0 * v5 - c1 * v2i + (5.0 / 3.0 + 19.0 / 9.0) * ke * v5 * fe) / f;
+    ph = -psi;
+    dph = -dpsi;
+}
+
+__global__ void spa_nodes_k(const double* __restrict__ m1, const double* __restrict__ m2,
+                            const double* __restrict__ e0, const int n, const double* __restrict__ x, const int nx,
+                            const double f0, double* __restrict__ amp, double* __restrict__ coef)
+{
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || j >= nx) return;
+    const double M = (m1[i] + m2[i]) * MSUN_S;
+    const double eta = m1[i] * m2[i] / ((m1[i] + m2[i]) * (m1[i] + m2[i]));
+    const double fisco = 1.0 / (pow(6.0, 1.5) * M_PI * M);
+    const double ecc = e0 ? e0[i] : 0.0;
+    const double c1 = 3715.0 / 756.0 + 55.0 * eta / 9.0, pref = 3.0 / (128.0 * eta);
+    const double ke = (2355.0 / 1462.0) * ecc * ecc;
+    const double fa = x[j];
+    amp[(size_t)i * nx + j] = fa < fisco ? pow(fa, -7.0 / 6.0) : 0.0;
+    if (j == nx - 1) return;
+    const double fb = x[j + 1], h = fb - fa;
+    double pa, da, pb, db;
+    spa_phase(fa, M, pref, c1, ke, f0, pa, da);
+    spa_phase(fb, M, pref, c1, ke, f0, pb, db);
+    const double sl = (pb - pa) / h;
+    double* c = coef + ((size_t)i * (nx - 1) + j) * 4;
+    c[0] = (da + db - 2.0 * sl) / (h * h);
+    c[1] = (3.0 * sl - 2.0 * da - db) / h;
+    c[2] = da;
+    c[3] = pa;
+}
+
+// Dense rows of the interpolated templates on [kmin, kend): for local bin q, node[q] = its interval j, d[q] = f - x_j,
+// t[q] = d / (x_j+1 - x_j); h = (A_j (1 - t) + A_j+1 t) exp(i p(d)), evaluated in double, rounded to complex64
+// (imr_interp.py:63-72). Zeros elsewhere.
+__global__ void interp_eval_k(const double* __restrict__ amp, const double* __restrict__ coef, const int nx,
+                              const int* __restrict__ node, const double* __restrict__ d, const double* __restrict__ t,
+                              const long kmin, const long kend, const long flen, float2* __restrict__ H,
+                              const long ldh)
+{
+    const int i = blockIdx.y;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < flen; k += (long)gridDim.x * blockDim.x) {
+        float2 out = make_float2(0.f, 0.f);
+        if (k >= kmin && k < kend) {
+            const long q = k - kmin;
+            const int j = node[q];
+            const double* c = coef + ((size_t)i * (nx - 1) + j) * 4;
+            const double dd = d[q], tt = t[q];
+            const double p = ((c[0] * dd + c[1]) * dd + c[2]) * dd + c[3];
+            const double a = amp[(size_t)i * nx + j] * (1.0 - tt) + amp[(size_t)i * nx + j + 1] * tt;
+            double s, co;
+            sincos(p, &s, &co);
+            out = make_float2((float)(a * co), (float)(a * s));
+        }
+        H[(size_t)i * ldh + k] = out;
+    }
+}
+
 } // namespace
 
 GWGS_API int gwgs_spa_templates(const double* m1, const double* m2, const double* e0, int n, long flen, double df,
@@ -154,5 +223,34 @@ GWGS_API int gwgs_sigmasq(const void* H, long ldh, long kmin, long kend, int n, 
     const long tot = (long)n * nseg;
     hipLaunchKernelGGL(sigmasq_sum_k, dim3((unsigned)((tot + NT - 1) / NT)), dim3(NT), 0, st, work, n, nch, nseg, df,
                        sig, lds);
+    return gwgs_status();
+}
+
+// Coarse nodes of n chirps at the nx nodes x (ascending, device float64): amp (n, nx) and the phase cubics coef
+// (n, nx - 1, 4) as (c3, c2, c1, c0) in float64 (see spa_nodes_k).
+GWGS_API int gwgs_spa_nodes(const double* m1, const double* m2, const double* e0, int n, const double* x, int nx,
+                            double f0, double* amp, double* coef, void* stream)
+{
+    GWGS_CHECK(m1 && m2 && x && amp && coef && n >= 0 && nx >= 2, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    GWGS_CHECK(n < 65536, GWGS_ESIZE);
+    hipLaunchKernelGGL(spa_nodes_k, dim3((nx + NT - 1) / NT, n), dim3(NT), 0, (hipStream_t)stream, m1, m2, e0, n, x,
+                       nx, f0, amp, coef);
+    return gwgs_status();
+}
+
+// Dense interpolated templates (n, flen) complex64 from nodes (amp (n, nx), coef (n, nx - 1, 4) float64) and the
+// fine-bin grid of [kmin, kend) (node int32, d and t float64, kend - kmin entries each).
+GWGS_API int gwgs_interp_templates(const double* amp, const double* coef, int n, int nx, const int* node,
+                                   const double* d, const double* t, long kmin, long kend, long flen, void* H,
+                                   long ldh, void* stream)
+{
+    GWGS_CHECK(amp && coef && node && d && t && H && n >= 0 && nx >= 2 && kmin >= 0 && kend >= kmin &&
+               kend <= flen && ldh >= flen, GWGS_EINVAL);
+    if (n == 0) return GWGS_OK;
+    GWGS_CHECK(n < 65536, GWGS_ESIZE);
+    const unsigned bx = (unsigned)min((flen + NT - 1) / NT, 512L);
+    hipLaunchKernelGGL(interp_eval_k, dim3(bx, n), dim3(NT), 0, (hipStream_t)stream, amp, coef, nx, node, d, t, kmin,
+                       kend, flen, (float2*)H, ldh);
     return gwgs_status();
 }

@@ -34,7 +34,9 @@ CHECK_TOL = dict(edge_bins=1, edge_frac=1e-2, snr_rel=1e-4, phase_rad=1e-4, chis
 @dataclass(frozen=True, eq=False)
 class TemplateTile:
     """One tile. H (n, flen) complex64 on the GPU holds the dense rows. nbins per row (0 = no chi-squared, one bin).
-    Bins cover [kmin, kmax[row]). coarse is accepted for the reference's signature; the dense path ignores it."""
+    Bins cover [kmin, kmax[row]). coarse (a Coarse): the rows are the interpolated templates of those nodes, and the
+    exact stage regenerates them instead of reading H (exact_stage.py:53-65, 472-647); H then serves the climbs and
+    the first call's self-check."""
     H: object
     nbins: np.ndarray
     kmin: int
@@ -47,6 +49,26 @@ class TemplateTile:
         object.__setattr__(self, 'kmax', np.asarray(self.kmax, np.int64))
         assert self.H.ndim == 2 and self.H.dtype == torch.complex64 and self.H.is_contiguous()
         assert self.nbins.shape == (n,) and self.kmax.shape == (n,)
+        if self.coarse is not None:
+            g = self.coarse.grid
+            assert tuple(self.coarse.amp.shape) == (n, g.nx) and tuple(self.coarse.coef.shape) == (n, g.J, 4)
+            assert g.kmin == self.kmin and np.all(self.kmax == g.kend), 'interpolated rows cover [kmin, kend)'
+
+
+@dataclass(frozen=True, eq=False)
+class Coarse:
+    """The nodes a tile's rows were interpolated from (templates.nodes): grid (templates.Grid, shared by the bank),
+    amp (n, nx) and coef (n, J, 4) float64 on the GPU, the tile's rows in tile order; coef32 / amp2 the regenerated
+    exact stage's float32 tables (templates.regen_tables)."""
+    grid: object
+    amp: object
+    coef: object
+
+    def __post_init__(self):
+        import templates
+        c32, a2 = templates.regen_tables(self.amp, self.coef)
+        object.__setattr__(self, 'coef32', c32)
+        object.__setattr__(self, 'amp2', a2)
 
 
 @dataclass(frozen=True, eq=False)
@@ -103,6 +125,24 @@ class ExactStage:
     def _nbe(self, rows_u):
         return np.maximum(self.tile.nbins[rows_u], 1)
 
+    def _edges_moments(self, segs, rows, rseg):
+        """Edges of (segment rseg[i], row rows[i]) pairs from the segments' moments of 1/psd and the rows' node
+        amplitudes (gwgs_edges_moments, exact_stage.py:230-250), one launch per segment; (n, B + 1) int32."""
+        c, g = self.tile.coarse, self.tile.coarse.grid
+        nbe = self._nbe(rows)
+        B = int(nbe.max())
+        out = torch.empty((len(rows), B + 1), dtype=torch.int32, device='cuda')
+        for s in np.unique(rseg):
+            sel = np.flatnonzero(rseg == s)
+            Pk, Mi = g.moments(segs[int(s)].IP)
+            r_d, nb_d, sel_d = _dev.to_dev_many(rows[sel].astype(np.int32), nbe[sel].astype(np.int32), sel)
+            part = torch.empty((len(sel), B + 1), dtype=torch.int32, device='cuda')
+            _lib.call('gwgs_edges_moments', c.amp.data_ptr(), g.nx, Mi.data_ptr(), Pk.data_ptr(), g.kb_d.data_ptr(),
+                      r_d.data_ptr(), nb_d.data_ptr(), B + 1, g.kmin, g.kend, len(sel), part.data_ptr(),
+                      _dev.stream())
+            out[sel_d] = part
+        return out
+
     def edges(self, seg: SegmentData, rows) -> np.ndarray:
         """(len(rows), B + 1) chi-squared bin edges for the given tile rows in this segment."""
         rows = np.asarray(rows, np.int64)
@@ -126,6 +166,8 @@ class ExactStage:
         npk = len(rows)
         if npk == 0:
             return PendingValues(self, seg, rows, samples[:, None], None, None, None)
+        if self.tile.coarse is not None:
+            return self.evaluate_async_multi([seg], np.zeros(npk, np.int64), req, refine, lo, hi)
         n = self.tile.H.shape[0]
         assert 0 <= rows.min() and rows.max() < n and 0 <= samples.min() and samples.max() < seg.tlen
         t = self.tile
@@ -209,8 +251,11 @@ class ExactStage:
             E_prev, k_prev = edges
             E_dev = E_prev[_dev.to_dev(np.searchsorted(k_prev, uk))]
         else:
-            E_dev = edges_rows_dev_multi(ur, us, self._nbe(ur), t.kmin, t.kmax[ur], t.H,
-                                         np.array([sg.IP.data_ptr() for sg in segs], np.int64))
+            if t.coarse is not None:
+                E_dev = self._edges_moments(segs, ur, us)
+            else:
+                E_dev = edges_rows_dev_multi(ur, us, self._nbe(ur), t.kmin, t.kmax[ur], t.H,
+                                             np.array([sg.IP.data_ptr() for sg in segs], np.int64))
             zero = np.flatnonzero(t.nbins[ur] == 0)
             if len(zero):
                 E_dev[_dev.to_dev(zero), 0] = t.kmin
@@ -225,7 +270,7 @@ class ExactStage:
         out, zo = chisq_kernel.launch_sorted_multi(t.H, np.repeat(ks, noff), n, cand[ob].ravel(), E_dev,
                                                    np.repeat(inv[ob], noff),
                                                    np.array([sg.D.data_ptr() for sg in segs], np.int64), seg0.tlen,
-                                                   row_span=np.maximum(t.kmax - t.kmin, 0))
+                                                   row_span=np.maximum(t.kmax - t.kmin, 0), regen=t.coarse)
         pv = PendingValues(self, seg0, rows, cand, ob, _dev.Fetch(out, zo, E_dev, here=here), inv)
         pv.check_mask = sidx == sidx[0]
         pv.keep_alive = segs
@@ -305,6 +350,17 @@ def _self_check(stage, seg, req, out, E):
                **compare_values(out.z[sel], out.pw[sel], z_ref, pw_ref, stage.tile.nbins[req.rows[sel]]))
     t = CHECK_TOL
     ok = rep['snr_rel_max'] <= t['snr_rel'] and rep['phase_max'] <= t['phase_rad'] and rep['chisq_rel_max'] <= t['chisq_rel']
+    if stage.tile.coarse is not None:
+        # the edges from moments against the float64 cumsum of the dense rows' |h|^2 IP (exact_stage.py:283-287)
+        rs = np.asarray(req.rows[sel], np.int64)
+        nbe = stage._nbe(rs)
+        E_ref = edges_rows_dev_multi(rs, np.zeros(len(rs), np.int64), nbe, stage.tile.kmin, stage.tile.kmax[rs],
+                                     stage.tile.H, np.array([seg.IP.data_ptr()], np.int64)).cpu().numpy()
+        E_ref[stage.tile.nbins[rs] == 0, 0] = stage.tile.kmin
+        inside = np.arange(E_ref.shape[1])[None, :] <= nbe[:, None]
+        dif = np.abs(E[sel][:, :E_ref.shape[1]].astype(np.int64) - E_ref)[inside]
+        rep.update(edge_max_diff=int(dif.max()), edge_frac_diff=float((dif > 0).mean()))
+        ok = ok and rep['edge_max_diff'] <= t['edge_bins'] and rep['edge_frac_diff'] <= t['edge_frac']
     assert ok, f'exact stage self-check failed: {rep}'
     return rep
 

@@ -84,6 +84,12 @@ class Options:
     gate: object = None                    # Gate: filter a tile on a segment only near base-search triggers
     overlap: bool = False                  # basis transforms on their own stream, overlapping the previous batch's
                                            # GEMM (off with graphs: each batch is captured on one stream)
+    templates: str = 'closed'              # 'closed': the closed-form chirps on every bin; 'interp': the same
+                                           # chirps at coarse nodes, interpolated (fp_filter.py --templates interp)
+    exact_stage: str = 'dense'             # 'regen': the exact stage regenerates the interpolated templates from
+                                           # their nodes instead of reading H (--exact-stage regen; needs interp)
+    n_chirp: int = 6000                    # nodes uniform in f^(-5/3) and in log f (--n-chirp, --n-log)
+    n_log: int = 1500
 
 
 @dataclass
@@ -167,9 +173,26 @@ class BankPlan:
             (np.pi * (self.m1 + self.m2) * 4.925490947641267e-06 * fl) ** (-5.0 / 3.0)
         self.tiles = []
         self.T = dict(templates_s=0.0, svd_s=0.0)
+        assert opts.templates in ('closed', 'interp') and opts.exact_stage in ('dense', 'regen')
+        assert opts.exact_stage == 'dense' or opts.templates == 'interp', 'the regen exact stage needs interp templates'
+        self.grid = None
+        if opts.templates == 'interp':
+            # one node grid for the bank, over [flow, the bank's highest ISCO frequency] (fp_filter.py:518-524)
+            fisco = 1.0 / (6.0 ** 1.5 * np.pi * mt * 4.925490947641267e-06)
+            kend_b = min(int(fisco.max() / df), flen - 1)
+            self.grid = tmpl.Grid(tmpl.coarse_grid(fl, kend_b * df, opts.n_chirp, opts.n_log), df, self.kmin, kend_b)
         for ids in tmpl.tiles(tau0, opts.tile):
             t0 = time.perf_counter()
-            H, kmax = tmpl.generate(self.m1[ids], self.m2[ids], self.e0[ids], flen, df, fl)
+            coarse = None
+            if self.grid is not None:
+                amp, coef = tmpl.nodes(self.m1[ids], self.m2[ids], self.e0[ids], self.grid)
+                H = tmpl.interp_templates(amp, coef, self.grid, flen)
+                kmax = np.full(len(ids), self.grid.kend, np.int64)
+                if opts.exact_stage == 'regen':
+                    from exact_stage import Coarse
+                    coarse = Coarse(self.grid, amp, coef)
+            else:
+                H, kmax = tmpl.generate(self.m1[ids], self.m2[ids], self.e0[ids], flen, df, fl)
             torch.cuda.synchronize(); t1 = time.perf_counter()
             kend = int(kmax.max())
             r, rp, A, Vc, worst = svd_tile(H, sw, isw, self.kmin, kend, opts.eps)
@@ -177,7 +200,7 @@ class BankPlan:
             if 2 * rp > _lib.load().gwgs_gemm_max_k(PRECISIONS[opts.precision]):
                 raise AssertionError(f'tile rank {r} exceeds the GEMM kernel limit; use a smaller --tile')
             nb = np.full(len(ids), opts.chisq_bins, np.int64)
-            stage = ExactStage(TemplateTile(H, nb, self.kmin, kmax), self_check=opts.self_check)
+            stage = ExactStage(TemplateTile(H, nb, self.kmin, kmax, coarse), self_check=opts.self_check)
             self.tiles.append(TileBasis(ids, H, kmax, r, rp, A, Vc, worst, stage, kend))
             self.T['templates_s'] += t1 - t0; self.T['svd_s'] += t2 - t1
 
@@ -1213,9 +1236,11 @@ def main(argv=None):
         -- [pycbc_inspiral options]
 
     datasets.txt: one 'IFO GPS_START GPS_END STORE [TRIG_START_OFFSET]' per line; writes OUTDIR/peaks/IFO-START-END.npz
-    per dataset (OUTDIR/peaks_b2/ for --bank2) and OUTDIR/timing.json. Flags the reference accepts for choices this
-    path does not have (template interpolation, edge algorithms, kernel variants, worker counts) are accepted and
-    ignored; --queue DIR filters the DIR/*.job groups one after another."""
+    per dataset (OUTDIR/peaks_b2/ for --bank2) and OUTDIR/timing.json. --templates interp (the default) takes the
+    closed-form chirps at coarse nodes, interpolated; --templates pycbc takes them closed-form on every bin (PyCBC's
+    generators are absent); --exact-stage regen regenerates the interpolated templates in the exact stage. Flags the
+    reference accepts for choices this path does not have (edge algorithms, kernel variants, worker counts) are
+    accepted and ignored; --queue DIR filters the DIR/*.job groups one after another."""
     import argparse, json, os, shlex, sys
     import consolidate as cons
     import gpu_prep
@@ -1325,6 +1350,8 @@ def main(argv=None):
             o = Options(snr_threshold=opt.snr_threshold, cluster_window=opt.cluster_window, chisq_bins=nbins,
                         low_frequency_cutoff=opt.low_frequency_cutoff, tile=a.tile, eps=eps_b, margin=a.margin,
                         exact_snr=a.exact_snr, precision=a.precision, refine=a.refine,
+                        templates='interp' if a.templates == 'interp' else 'closed', exact_stage=a.exact_stage,
+                        n_chirp=a.n_chirp, n_log=a.n_log,
                         gemm_cfg=tuple(int(v) for v in a.gemm_cfg.split(',')) if a.gemm_cfg else None,
                         consolidate=kl, pre_consolidate=a.pre_consolidate,
                         sgchisq_snr_threshold=sgo.sgchisq_snr_threshold if sgo.sgchisq_locations else None,

@@ -121,6 +121,24 @@ int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const
 /* bytes of the device workspace gwgs_chisq_points needs (the per-bin sums of every group; 0 when there is none) */
 long gwgs_chisq_points_workspace(int ngroups, int nedge);
 int gwgs_chisq_peaks_per_group(void);
+/* pipeline/exact_stage.py:472-510 (edges_moments, the regen path's chi-squared edges): edges of n rows of
+ * interpolated templates in one segment without reading H, from the segment's moments of 1/psd over each interval
+ * (Mi (nx - 1, 3): sums of ((1-t)^2, t(1-t), t^2) IP) and their in-interval inclusive prefixes at each bin (Pk
+ * (kend - kmin, 3)), float64, and the rows' node amplitudes amp (rows, nx) float64: edge j = the first bin whose
+ * running sum of |h|^2 IP exceeds j / nb of the total, kb (nx int32) the first bin of each interval (kb[nx - 1] =
+ * kend); columns >= nb hold kend. out (n, nedge) int32. */
+int gwgs_edges_moments(const double* amp, int nx, const double* Mi, const double* Pk, const int* kb, const int* rows,
+                       const int* nbs, int nedge, int kmin, int kend, int n, int* out, void* stream);
+/* pipeline/exact_stage.py:512-647 (regen_bins + finalize): gwgs_chisq_points_multi with each template row
+ * regenerated from its nodes instead of read from H: coef32 (rows, nx - 1) float4 (c3, c2, c1, c0 with c0 reduced
+ * to [-pi, pi)), amp2 (rows, nx - 1) float2 (A_j, A_j+1), the grid's node (int32), d and t (float32) per bin of
+ * [kmin, kend); h_k = (A_j (1 - t) + A_j+1 t) exp(i p_j(d)) in float32, the phase reduced by 2 pi before the
+ * sincos. Every edge within [kmin, kend]. Workspace: gwgs_chisq_points_workspace(ngroups, nedge) bytes. */
+int gwgs_regen_points_multi(const void* coef32, const void* amp2, int nx, const int* node, const float* d,
+                            const float* t, int kmin, const void* const* d_tab, const int* gseg, long long N,
+                            const int* grow, const int* gstart, const int* gcount, const long long* ptime,
+                            const int* edges, int nedge, int ngroups, float* out, void* zout, void* work,
+                            void* stream);
 /* No reference counterpart (the reference evaluates the exact stage at the rebuilt argmax, exact_stage.py:704-731):
  * hill climbing of n peaks on the exact |z| = |sum over e0 <= k < eB of conj(h_k) d_k exp(2 pi i k t / N)|, e0 and eB
  * the first and last entries of edge row eidx[r] (edges: rows of nedge int32): from start[r], z at the 2 W + 1 samples
@@ -198,6 +216,18 @@ int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, lon
  * (e0 may be NULL); kmax: device int array (may be NULL). */
 int gwgs_spa_templates(const double* m1, const double* m2, const double* e0, int n, long flen, double df,
                        double flow, double f0, void* H, long ldh, int* kmax, void* stream);
+/* pipeline/imr_interp.py:18-40 (coarse_generate, the templates at coarse nodes for --templates interp): the same
+ * chirps at the nx ascending nodes x (device float64): amp (n, nx) = f^(-7/6) below f_isco, 0 above; coef (n, nx - 1,
+ * 4) = (c3, c2, c1, c0) of each interval's phase cubic in d = f - x_j, the Hermite cubic through -psi and its
+ * analytic derivative at both ends (the reference fits a spline to PyCBC's phase at the nodes). float64. */
+int gwgs_spa_nodes(const double* m1, const double* m2, const double* e0, int n, const double* x, int nx, double f0,
+                   double* amp, double* coef, void* stream);
+/* pipeline/imr_interp.py:43-85 (FineInterpolator): dense rows (n, flen) complex64 on [kmin, kend) from nodes (amp,
+ * coef as gwgs_spa_nodes writes them) and the fine-bin grid (node int32, d and t float64 per bin of [kmin, kend):
+ * its interval, f - x_j and the fraction (f - x_j) / (x_j+1 - x_j)): h = (A_j (1 - t) + A_j+1 t) exp(i p_j(d)) in
+ * double; zero outside [kmin, kend). */
+int gwgs_interp_templates(const double* amp, const double* coef, int n, int nx, const int* node, const double* d,
+                          const double* t, long kmin, long kend, long flen, void* H, long ldh, void* stream);
 /* sig[i, s] = 4 df sum_{kmin <= k < kend} |H[i, k]|^2 IP[s, k] (double accumulation, chunk partials summed in a
  * fixed order), s < nseg <= 16; work: gwgs_sigmasq_workspace(n, kmin, kend, nseg) doubles */
 long gwgs_sigmasq_workspace(int n, long kmin, long kend, int nseg);

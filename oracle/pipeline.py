@@ -25,6 +25,32 @@ def spa_phase(f, m1, m2, e0=0.0, f0=15.0):
     return 3.0 / (128.0 * eta * v ** 5) * pn - np.pi / 4.0
 
 
+def coarse_grid(flow, fhigh, n_chirp=6000, n_log=1500):
+    """imr_interp.py:11-15: nodes uniform in f^(-5/3) joined with a log grid and both ends, inside [flow, fhigh]."""
+    u = np.linspace(flow ** (-5 / 3), fhigh ** (-5 / 3), n_chirp) ** (-3 / 5)
+    g = np.geomspace(flow, fhigh, n_log)
+    x = np.unique(np.r_[u, g, flow, fhigh])
+    return x[(x >= flow) & (x <= fhigh)]
+
+
+def fine_interp(amp, coef, x, df, kmin, kend, flen):
+    """imr_interp.py:53-77 (FineInterpolator, numpy branch), float64: the bins k in [kmin, kend) at f = k df clipped
+    to [x_0, x_J]; interval i = the last node at or below f (side='right' search), clipped to [0, J - 1];
+    d = f - x_i, t = d / (x_i+1 - x_i); phase ((c3 d + c2) d + c1) d + c0 and amplitude amp_i (1 - t) + amp_i+1 t.
+    coef (n, J, 4) as (c3, c2, c1, c0) per interval (the reference keeps (n, 4, J), highest order first). Returns
+    (n, flen) complex128, zero outside [kmin, kend)."""
+    f = np.clip(np.arange(kmin, kend) * df, x[0], x[-1])
+    i = np.clip(np.searchsorted(x, f, side='right') - 1, 0, len(x) - 2)
+    d = f - x[i]
+    t = d / (x[i + 1] - x[i])
+    c = coef[:, i, :]
+    p = ((c[..., 0] * d + c[..., 1]) * d + c[..., 2]) * d + c[..., 3]
+    a = amp[:, i] * (1 - t) + amp[:, i + 1] * t
+    out = np.zeros((amp.shape[0], flen), np.complex128)
+    out[:, kmin:kend] = a * np.exp(1j * p)
+    return out
+
+
 def spa_bank(m1, m2, e0, flen, df, flow):
     """complex128 (n, flen) templates f^(-7/6) exp(-i psi) on [int(flow/df), min(int(f_isco/df), flen - 1)) and
     their end bins."""

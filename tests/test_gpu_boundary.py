@@ -60,7 +60,8 @@ REF_ARGV = ("--bank-file bank.hdf --hdf-store store.hdf --channel-name H1:GWOSC_
 
 def test_main_end_to_end(gpu, tmp_path):
     """fp_filter.main on one H1 dataset of 1,216 s stored at 4096 Hz (resampled to 2048 Hz, high-passed, autogated,
-    two 1024 s segments), 24 SPA templates, FP32 GEMM: the peak file's (template, window) list equals the oracle's
+    two 1024 s segments), 24 SPA templates (closed form on every bin, --templates pycbc), FP32 GEMM: the peak file's
+    (template, window) list equals the oracle's
     plain method on the dataset's conditioned data except within 1e-3 of the threshold, exact SNR within 1e-4,
     chi-squared within 1e-3 and phase within 1e-3 rad of it at the same sample, end times from the segment layout,
     the 2-D sigmasq equal to the oracle's per first segment, and template hashes from the bank."""
@@ -80,7 +81,7 @@ def test_main_end_to_end(gpu, tmp_path):
     out, cache = str(tmp_path / 'out'), str(tmp_path / 'cache')
     print('main', flush=True)
     fp_filter.main(['--datasets', dsf, '--bank', bankf, '--approximant', 'SPAtmplt', '--argv', argvf, '--out', out,
-                    '--precision', 'fp32', '--prep-cache', cache])
+                    '--precision', 'fp32', '--prep-cache', cache, '--templates', 'pycbc'])
     pk = fp_filter.read_peaks(os.path.join(out, 'peaks', f'H1-{gps0}-{gps0 + T}.npz'))
     assert os.path.exists(os.path.join(out, 'timing.json'))
     meta = json.load(open(glob.glob(os.path.join(cache, '*.json'))[0]))

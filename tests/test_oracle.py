@@ -387,3 +387,29 @@ def test_resample_ldas_golden():
         np.testing.assert_array_equal(got, GOLD[f'resample_{f}'])
         ntaps = 20 * f + 1
         assert np.all(got[:(2 * (ntaps // 2) - (ntaps + 1) // 2 + f - 1) // f] == 0) and got[-1] == 0
+
+
+def test_coarse_grid_layout():
+    """pipeline.coarse_grid: ascending unique nodes with both ends; its chirp part uniform in f^(-5/3)."""
+    x = pipeline.coarse_grid(15.0, 1000.0, n_chirp=300, n_log=100)
+    assert x[0] == 15.0 and x[-1] == 1000.0 and np.all(np.diff(x) > 0)
+    assert len(x) <= 402
+    u = np.linspace(15.0 ** (-5 / 3), 1000.0 ** (-5 / 3), 300) ** (-3 / 5)
+    assert np.isin(u[1:-1], x).mean() > 0.99
+
+
+def test_fine_interp_reproduces_cubic_phase_and_linear_amplitude():
+    """pipeline.fine_interp on nodes of a global cubic phase (expanded at each node: c0 = phi, c1 = phi', c2 =
+    phi''/2, c3 = phi'''/6) and a linear amplitude gives a(f) exp(i phi(f)) at every bin, clipped to the node span:
+    a wrong interval (search side, clip) or a wrong t or d fails it."""
+    x = np.array([20.0, 21.5, 23.0, 30.0, 31.0, 40.0])
+    al, be, ga, de = 3e-3, -0.2, 7.0, 0.4
+    ph = lambda f: ((al * f + be) * f + ga) * f + de
+    dph = lambda f: (3 * al * f + 2 * be) * f + ga
+    coef = np.stack([np.full(5, al), (3 * al * x[:-1] + be), dph(x[:-1]), ph(x[:-1])], -1)[None]
+    amp = (2.0 + 0.1 * x)[None]
+    df, kmin, kend, flen = 0.25, 76, 170, 200          # f from 19 Hz (below x_0: clipped) to 42.25 Hz (above x_J)
+    out = pipeline.fine_interp(amp, coef, x, df, kmin, kend, flen)
+    f = np.clip(np.arange(kmin, kend) * df, x[0], x[-1])
+    np.testing.assert_allclose(out[0, kmin:kend], (2.0 + 0.1 * f) * np.exp(1j * ph(f)), rtol=1e-12, atol=1e-12)
+    assert not out[0, :kmin].any() and not out[0, kend:].any()
