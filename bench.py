@@ -593,6 +593,10 @@ def main():
     ap.add_argument('--refine', type=int, default=3, help='exact-stage re-centring half width in samples (0: reference rule)')
     ap.add_argument('--alternates', type=int, default=3, help='with --refine: other local maxima per candidate window '
                     'evaluated exactly (0 to 3)')
+    ap.add_argument('--templates', choices=['closed', 'interp'], default='closed',
+                    help='interp: the chirps at coarse nodes, interpolated (the reference\'s --templates interp)')
+    ap.add_argument('--exact-stage', choices=['dense', 'regen'], default='dense',
+                    help='regen: the exact stage regenerates the interpolated templates instead of reading H')
     ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
     ap.add_argument('--overlap', type=int, default=0, help='1: basis transforms on their own stream beside the GEMM')
     ap.add_argument('--depth', type=int, default=2, help='batches enqueued ahead of the one being decoded')
@@ -633,7 +637,7 @@ def main():
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
                                  alternates=a.alternates, segment_batch=a.segment_batch, overlap=bool(a.overlap),
-                                 depth=a.depth)
+                                 depth=a.depth, templates=a.templates, exact_stage=a.exact_stage)
             for p in ('bf16', 'fp32', 'fp8')}
     # the reference's rule (exact values at the rebuilt argmax), for the trigger comparison beside the refined one
     for p in ('bf16', 'fp32'):
@@ -643,7 +647,7 @@ def main():
     for name, pre in (('cons', False), ('pre', True)):
         opts[name] = fp_filter.Options(tile=a.tile, precision='bf16', gemm_cfg=cfg, self_check=False,
                                        refine=a.refine, alternates=a.alternates, snr_threshold=4.0, exact_snr=5.5,
-                                       consolidate=kl,
+                                       templates=a.templates, exact_stage=a.exact_stage, consolidate=kl,
                                        pre_consolidate=pre, sgchisq_snr_threshold=6.0)
     b = run_batched(a, opts, cfg, tuned, t_setup)
     wl = b['wl']

@@ -47,7 +47,7 @@ SIGNATURES = {
     'gwgs_chisq_peaks_per_group': [],
     'gwgs_spa_nodes': [_p, _p, _p, _i, _p, _i, _d, _p, _p, _p],
     'gwgs_interp_templates': [_p, _p, _i, _i, _p, _p, _p, _l, _l, _l, _p, _l, _p],
-    'gwgs_edges_moments': [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p],
+    'gwgs_edges_moments': [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_regen_points_multi': [_p, _p, _i, _p, _p, _p, _i, _p, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_climb_points': [_p, _l, _p, _ll, _p, _p, _p, _i, _p, _p, _p, _p, _i, _i, _i, _p, _p],
     'gwgs_edges_fused': [_p, _l, _p, _p, _p, _p, _i, _i, _i, _p, _p],

@@ -368,18 +368,23 @@ __device__ __forceinline__ double quad3(const double a, const double b, const do
 // sum of |h|^2 IP over interval j is quad3(A_j, A_j+1, Mi[j]) and up to bin k of it quad3(A_j, A_j+1, Pk[k]). One
 // block per requested row: each thread sums a contiguous run of intervals, a block scan of the runs, then per
 // target j / nb * total the run (binary search), the interval (a walk from the run's start) and the bin (binary
-// search on the prefixes): edge = first bin whose running sum exceeds the target. Columns >= nb hold kend.
+// search on the prefixes): edge = first bin whose running sum exceeds the target. Columns >= nb hold the row's end
+// k1s[u] (<= kend; the row is zero from there on).
 __global__ void __launch_bounds__(NT) edges_moments_k(const double* __restrict__ amp, const int nx,
-                                                      const double* __restrict__ Mi, const double* __restrict__ Pk,
-                                                      const int* __restrict__ kb, const int* __restrict__ rows,
-                                                      const int* __restrict__ nbs, const int nedge, const int kmin,
-                                                      const int kend, int* __restrict__ out)
+                                                      const double* const* __restrict__ Mi_tab,
+                                                      const double* const* __restrict__ Pk_tab,
+                                                      const int* __restrict__ rseg, const int* __restrict__ kb,
+                                                      const int* __restrict__ rows, const int* __restrict__ nbs,
+                                                      const int* __restrict__ k1s, const int nedge, const int kmin,
+                                                      int* __restrict__ out)
 {
     __shared__ double pre[NT + 1];
     __shared__ double sh[NT / 64];
     const int u = blockIdx.x, tid = threadIdx.x, J = nx - 1;
+    const double* __restrict__ Mi = Mi_tab[rseg[u]];     // the request's segment
+    const double* __restrict__ Pk = Pk_tab[rseg[u]];
     const double* A = amp + (size_t)rows[u] * nx;
-    const int nb = nbs[u];
+    const int nb = nbs[u], kend = k1s[u];
     const int per = (J + NT - 1) / NT, j0 = min(tid * per, J), j1 = min(j0 + per, J);
     double s = 0.0;
     for (int j = j0; j < j1; j++) s = __dadd_rn(s, quad3(A[j], A[j + 1], Mi + 3 * (size_t)j));
@@ -613,18 +618,20 @@ GWGS_API int gwgs_edges_fused_multi(const void* h, long ldh, const int* rows, co
     return gwgs_status();
 }
 
-// Chi-squared edges of n rows of interpolated templates from one segment's moments (see edges_moments_k): amp (rows,
-// nx) float64 node amplitudes; Mi (nx - 1, 3) and Pk (kend - kmin, 3) float64; kb (nx) int32 the first bin of each
-// interval (kb[nx - 1] = kend); out (n, nedge) int32.
-GWGS_API int gwgs_edges_moments(const double* amp, int nx, const double* Mi, const double* Pk, const int* kb,
-                                const int* rows, const int* nbs, int nedge, int kmin, int kend, int n, int* out,
-                                void* stream)
+// Chi-squared edges of n (segment, row) requests of interpolated templates from the segments' moments (see
+// edges_moments_k): amp (rows, nx) float64 node amplitudes; per segment s Mi_tab[s] (nx - 1, 3) and Pk_tab[s]
+// (kend - kmin, 3) float64 (device tables of pointers), rseg (n) each request's segment; kb (nx) int32 the first bin
+// of each interval (kb[nx - 1] = kend); k1s (n) each row's end (<= kend, zero beyond); out (n, nedge) int32.
+GWGS_API int gwgs_edges_moments(const double* amp, int nx, const void* const* Mi_tab, const void* const* Pk_tab,
+                                const int* rseg, const int* kb, const int* rows, const int* nbs, const int* k1s,
+                                int nedge, int kmin, int n, int* out, void* stream)
 {
-    GWGS_CHECK(amp && Mi && Pk && kb && rows && nbs && out && nx >= 2 && nedge >= 1 && kmin >= 0 && kend >= kmin &&
-               n >= 0, GWGS_EINVAL);
+    GWGS_CHECK(amp && Mi_tab && Pk_tab && rseg && kb && rows && nbs && k1s && out && nx >= 2 && nedge >= 1 &&
+               kmin >= 0 && n >= 0, GWGS_EINVAL);
     if (n == 0) return GWGS_OK;
-    hipLaunchKernelGGL(edges_moments_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, amp, nx, Mi, Pk, kb, rows, nbs,
-                       nedge, kmin, kend, out);
+    hipLaunchKernelGGL(edges_moments_k, dim3(n), dim3(NT), 0, (hipStream_t)stream, amp, nx,
+                       (const double* const*)Mi_tab, (const double* const*)Pk_tab, rseg, kb, rows, nbs, k1s, nedge,
+                       kmin, out);
     return gwgs_status();
 }
 

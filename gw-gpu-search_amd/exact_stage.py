@@ -52,7 +52,7 @@ class TemplateTile:
         if self.coarse is not None:
             g = self.coarse.grid
             assert tuple(self.coarse.amp.shape) == (n, g.nx) and tuple(self.coarse.coef.shape) == (n, g.J, 4)
-            assert g.kmin == self.kmin and np.all(self.kmax == g.kend), 'interpolated rows cover [kmin, kend)'
+            assert g.kmin == self.kmin and np.all(self.kmax <= g.kend), 'interpolated rows lie in [kmin, kend)'
 
 
 @dataclass(frozen=True, eq=False)
@@ -127,20 +127,21 @@ class ExactStage:
 
     def _edges_moments(self, segs, rows, rseg):
         """Edges of (segment rseg[i], row rows[i]) pairs from the segments' moments of 1/psd and the rows' node
-        amplitudes (gwgs_edges_moments, exact_stage.py:230-250), one launch per segment; (n, B + 1) int32."""
+        amplitudes (gwgs_edges_moments, exact_stage.py:230-250), one launch; (n, B + 1) int32."""
         c, g = self.tile.coarse, self.tile.coarse.grid
         nbe = self._nbe(rows)
         B = int(nbe.max())
+        used = set(np.asarray(rseg).tolist())
+        mom = [g.moments(segs[s].IP) if s in used else None for s in range(len(segs))]
+        mi = np.array([m[1].data_ptr() if m is not None else 0 for m in mom], np.int64)
+        pk = np.array([m[0].data_ptr() if m is not None else 0 for m in mom], np.int64)
+        r_d, nb_d, k1_d, sg_d, mi_d, pk_d = _dev.to_dev_many(
+            rows.astype(np.int32), nbe.astype(np.int32), self.tile.kmax[rows].astype(np.int32),
+            np.asarray(rseg, np.int32), mi, pk)
         out = torch.empty((len(rows), B + 1), dtype=torch.int32, device='cuda')
-        for s in np.unique(rseg):
-            sel = np.flatnonzero(rseg == s)
-            Pk, Mi = g.moments(segs[int(s)].IP)
-            r_d, nb_d, sel_d = _dev.to_dev_many(rows[sel].astype(np.int32), nbe[sel].astype(np.int32), sel)
-            part = torch.empty((len(sel), B + 1), dtype=torch.int32, device='cuda')
-            _lib.call('gwgs_edges_moments', c.amp.data_ptr(), g.nx, Mi.data_ptr(), Pk.data_ptr(), g.kb_d.data_ptr(),
-                      r_d.data_ptr(), nb_d.data_ptr(), B + 1, g.kmin, g.kend, len(sel), part.data_ptr(),
-                      _dev.stream())
-            out[sel_d] = part
+        _lib.call('gwgs_edges_moments', c.amp.data_ptr(), g.nx, mi_d.data_ptr(), pk_d.data_ptr(), sg_d.data_ptr(),
+                  g.kb_d.data_ptr(), r_d.data_ptr(), nb_d.data_ptr(), k1_d.data_ptr(), B + 1, g.kmin, len(rows),
+                  out.data_ptr(), _dev.stream())
         return out
 
     def edges(self, seg: SegmentData, rows) -> np.ndarray:

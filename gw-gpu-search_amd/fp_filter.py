@@ -187,7 +187,7 @@ class BankPlan:
             if self.grid is not None:
                 amp, coef = tmpl.nodes(self.m1[ids], self.m2[ids], self.e0[ids], self.grid)
                 H = tmpl.interp_templates(amp, coef, self.grid, flen)
-                kmax = np.full(len(ids), self.grid.kend, np.int64)
+                kmax = tmpl.interp_end(self.m1[ids], self.m2[ids], self.grid)
                 if opts.exact_stage == 'regen':
                     from exact_stage import Coarse
                     coarse = Coarse(self.grid, amp, coef)

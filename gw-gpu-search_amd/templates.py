@@ -85,17 +85,27 @@ class Grid:
         self._last = torch.as_tensor(np.maximum(last, 0), dtype=torch.int64, device='cuda')
         self._nonempty = torch.as_tensor(last >= self.kb[:-1] - self.kmin, device='cuda')
         self._node64 = self.node_d.to(torch.int64)
+        self._mom = {}
 
     def moments(self, IP):
         """Per-segment moments of 1/psd in the (1-t)^2, t(1-t), t^2 basis (float64): Pk (kend - kmin, 3), each
         bin's in-interval inclusive prefix, and Mi (J, 3), each interval's total (equal to Pk at its last bin, 0 for
-        an empty interval): the weight of |h|^2 IP over an interval is A_j^2 M0 + 2 A_j A_j+1 M1 + A_j+1^2 M2."""
+        an empty interval): the weight of |h|^2 IP over an interval is A_j^2 M0 + 2 A_j A_j+1 M1 + A_j+1^2 M2.
+        Kept for the last few segments (keyed by the tensor, its address and its version counter, so new data in the
+        same buffer is seen)."""
+        key = (id(IP), IP.data_ptr(), IP._version)
+        hit = self._mom.get(key)
+        if hit is not None:
+            return hit
         w = IP[self.kmin:self.kend].to(torch.float64)
-        G = torch.cumsum(torch.stack([c * w for c in self._w], 1), 0)
-        Gb = torch.cat([torch.zeros((1, 3), dtype=torch.float64, device=G.device), G])
-        Pk = (G - Gb[self._start[self._node64]]).contiguous()
+        G = torch.cumsum(torch.stack([c * w for c in self._w]), 1)              # (3, K), scanned along rows
+        Gb = torch.cat([torch.zeros((3, 1), dtype=torch.float64, device=G.device), G], 1)
+        Pk = (G - Gb[:, self._start[self._node64]]).T.contiguous()
         Mi = torch.where(self._nonempty[:, None], Pk[self._last], torch.zeros_like(Pk[self._last])).contiguous()
-        return Pk, Mi
+        if len(self._mom) >= 32:
+            self._mom.pop(next(iter(self._mom)))
+        self._mom[key] = (Pk, Mi, IP)
+        return Pk, Mi, IP
 
 
 def nodes(m1, m2, e0, grid):
@@ -110,6 +120,16 @@ def nodes(m1, m2, e0, grid):
     _lib.call('gwgs_spa_nodes', a.data_ptr(), b.data_ptr(), _dev.ptr(e), n, grid.x_d.data_ptr(), grid.nx, F0_ECC,
               amp.data_ptr(), coef.data_ptr(), _dev.stream())
     return amp, coef
+
+
+def interp_end(m1, m2, grid):
+    """Each interpolated chirp's end bin: its amplitude is 0 from the first node at or above f_isco, so bins from
+    floor(that node / df) + 1 on are zero (at most kend). int64."""
+    fisco = 1.0 / (6.0 ** 1.5 * np.pi * (np.asarray(m1) + np.asarray(m2)) * 4.925490947641267e-06)
+    j = np.searchsorted(grid.x, fisco, side='left')
+    end = np.where(j < grid.nx, np.floor(grid.x[np.minimum(j, grid.nx - 1)] / grid.df).astype(np.int64) + 1,
+                   grid.kend)
+    return np.minimum(end, grid.kend).astype(np.int64)
 
 
 def interp_templates(amp, coef, grid, flen, out=None):

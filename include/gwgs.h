@@ -121,14 +121,16 @@ int gwgs_chisq_points(const void* q, long ldq, const void* d, long long N, const
 /* bytes of the device workspace gwgs_chisq_points needs (the per-bin sums of every group; 0 when there is none) */
 long gwgs_chisq_points_workspace(int ngroups, int nedge);
 int gwgs_chisq_peaks_per_group(void);
-/* pipeline/exact_stage.py:472-510 (edges_moments, the regen path's chi-squared edges): edges of n rows of
- * interpolated templates in one segment without reading H, from the segment's moments of 1/psd over each interval
- * (Mi (nx - 1, 3): sums of ((1-t)^2, t(1-t), t^2) IP) and their in-interval inclusive prefixes at each bin (Pk
- * (kend - kmin, 3)), float64, and the rows' node amplitudes amp (rows, nx) float64: edge j = the first bin whose
- * running sum of |h|^2 IP exceeds j / nb of the total, kb (nx int32) the first bin of each interval (kb[nx - 1] =
- * kend); columns >= nb hold kend. out (n, nedge) int32. */
-int gwgs_edges_moments(const double* amp, int nx, const double* Mi, const double* Pk, const int* kb, const int* rows,
-                       const int* nbs, int nedge, int kmin, int kend, int n, int* out, void* stream);
+/* pipeline/exact_stage.py:472-510 (edges_moments, the regen path's chi-squared edges): edges of n (segment, row)
+ * requests of interpolated templates without reading H, from each segment's moments of 1/psd over each interval
+ * (Mi_tab[s] (nx - 1, 3): sums of ((1-t)^2, t(1-t), t^2) IP) and their in-interval inclusive prefixes at each bin
+ * (Pk_tab[s] (kend - kmin, 3)), float64 (device tables of pointers; rseg[u] the request's segment), and the rows'
+ * node amplitudes amp (rows, nx) float64: edge j = the first bin whose running sum of |h|^2 IP exceeds j / nb of
+ * the total, kb (nx int32) the first bin of each interval (kb[nx - 1] = kend); columns >= nb hold the row's end
+ * k1s[u] (<= kend; the row is zero from there). out (n, nedge) int32. */
+int gwgs_edges_moments(const double* amp, int nx, const void* const* Mi_tab, const void* const* Pk_tab,
+                       const int* rseg, const int* kb, const int* rows, const int* nbs, const int* k1s, int nedge,
+                       int kmin, int n, int* out, void* stream);
 /* pipeline/exact_stage.py:512-647 (regen_bins + finalize): gwgs_chisq_points_multi with each template row
  * regenerated from its nodes instead of read from H: coef32 (rows, nx - 1) float4 (c3, c2, c1, c0 with c0 reduced
  * to [-pi, pi)), amp2 (rows, nx - 1) float2 (A_j, A_j+1), the grid's node (int32), d and t (float32) per bin of
