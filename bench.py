@@ -597,7 +597,9 @@ def main():
                     help='interp: the chirps at coarse nodes, interpolated (the reference\'s --templates interp)')
     ap.add_argument('--exact-stage', choices=['dense', 'regen'], default='dense',
                     help='regen: the exact stage regenerates the interpolated templates instead of reading H')
-    ap.add_argument('--segment-batch', type=int, default=4, help='segments per basis-transform batch and GEMM launch')
+    ap.add_argument('--segment-batch', type=int, default=32, help='at most this many segments per basis-transform '
+                    'batch and GEMM launch')
+    ap.add_argument('--batch-samples', type=int, default=1 << 22, help='and at most this many analysed samples')
     ap.add_argument('--overlap', type=int, default=0, help='1: basis transforms on their own stream beside the GEMM')
     ap.add_argument('--depth', type=int, default=2, help='batches enqueued ahead of the one being decoded')
     ap.add_argument('--production', type=int, default=1, help='also run the production workload (2 detectors x 512 s '
@@ -636,7 +638,8 @@ def main():
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
-                                 alternates=a.alternates, segment_batch=a.segment_batch, overlap=bool(a.overlap),
+                                 alternates=a.alternates, segment_batch=a.segment_batch,
+                                 batch_samples=a.batch_samples, overlap=bool(a.overlap),
                                  depth=a.depth, templates=a.templates, exact_stage=a.exact_stage)
             for p in ('bf16', 'fp32', 'fp8')}
     # the reference's rule (exact values at the rebuilt argmax), for the trigger comparison beside the refined one
@@ -669,7 +672,8 @@ def main():
         ms_per_step=b['ms_per_step'], higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
         data='synthetic (noise drawn on the GPU, seeded)',
         config=dict(workload=wl.describe(a.tile), templates=b['ntemp'], analysed_s=b['analysed_s'],
-                    slices=a.slices, tile=a.tile, segment_batch=a.segment_batch, ranks=wl.ranks(),
+                    slices=a.slices, tile=a.tile, segment_batch=a.segment_batch, batch_samples=a.batch_samples,
+                    ranks=wl.ranks(),
                     setup_s=round(b['setup_s'], 1),
                     value_with_setup=b['ntemp'] * an_job / (slices_s + b['setup_s']),
                     speedup_bf16_vs_fp32=None if fp32 is None else fp32['speedup_bf16'],

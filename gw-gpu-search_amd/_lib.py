@@ -61,6 +61,7 @@ SIGNATURES = {
     'gwgs_basis_power': [_p, _l, _i, _l, _l, _p, _p],
     'gwgs_basis_series_workspace': [_i, _l],
     'gwgs_basis_series': [_i, _p, _l, _i, _i, _p, _l, _l, _l, _l, _l, _p, _i, _p, _p, _p, _l, _p],
+    'gwgs_basis_series_batch': [_i, _p, _l, _i, _i, _i, _p, _l, _l, _l, _p, _p, _p, _p, _i, _p, _p, _p, _l, _p],
     'gwgs_vd_power': [_p, _l, _i, _p, _l, _l, _p, _p],
     'gwgs_spa_templates': [_p, _p, _p, _i, _l, _d, _d, _d, _p, _l, _p, _p],
     'gwgs_sigmasq_workspace': [_i, _l, _l, _i],

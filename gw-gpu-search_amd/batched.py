@@ -2,9 +2,10 @@
 few large launches instead of one pass per chunk.
 
 Chunks are grouped by segment grid: every chunk on the 512 s grid (tlen 2^20) and every chunk on the 128 s grid
-(tlen 2^18) of a step form one Filter each, whose segments are taken `segment_batch` at a time per basis-transform
-batch and GEMM launch (the item table of csrc/gemm_peaks.hip covers several segments of uneven analysed length in
-one launch). Each chunk's PSD is its own (fp_filter.py's prep per dataset). Bank products (dense templates and
+(tlen 2^18) of a step form one Filter each, whose segments are taken in batches of up to `segment_batch` segments
+and `batch_samples` analysed samples per basis-transform batch (one launch of each transform pass per 16 segments)
+and GEMM launch (the item table of csrc/gemm_peaks.hip covers several segments of uneven analysed length in one
+launch). Each chunk's PSD is its own (fp_filter.py's prep per dataset). Bank products (dense templates and
 float64 SVD bases) are built once per grid, since Vh and H live on the grid's frequency bins; the SVD metric of a
 grid is sqrt(mean IP) over the segments of the reference chunks (fp_filter.py:360-364 takes it over the group being
 filtered; it sets only how well the basis compresses, never the SNR: conj(h sw) (D / sw) = conj(h) D)."""

@@ -19,6 +19,8 @@
 // Inputs are read through buffer resources whose range check returns zeros outside the band (cols_k) or past the
 // last basis row (rows_k): every load of a thread is issued unconditionally, back to back, where a per-load branch
 // made the compiler wait for each load (or pair of loads) before issuing the next.
+// One launch of each pass covers every segment of a batch (blockIdx.z = segment, up to kMaxSeg): the segments share
+// the tile's Vc and band and have their own D, intermediate, analysed span and output rows.
 // HBM traffic per (tile, segment): 8 r (kend - kmin) bytes read of Vc, 8 r n written and read of Y, 2 Tn rp
 // elements of the output type written: two passes over Y instead of the batched FFT's several plus its input and
 // transpose passes.
@@ -209,16 +211,32 @@ __device__ __forceinline__ void twiddle_line(float2 (&v)[Q], const int p, const 
     }
 }
 
+// v[m2] *= sc w_L^(p m2) for 0 <= m2 < Q: a real scale carried by the rotation (seeded with magnitude sc)
+template <int Q, bool LP>
+__device__ __forceinline__ void twiddle_line_scaled(float2 (&v)[Q], const int p, const int L, const float sc)
+{
+    Rot<LP> w;
+    w.c = sc;
+    w.step(p, L);
+    v[0] = make_float2(v[0].x * sc, v[0].y * sc);
+#pragma unroll
+    for (int m2 = 1; m2 < Q; m2++) {
+        w.next();
+        v[m2] = w.apply(v[m2]);
+    }
+}
+
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
 // Storage of the intermediate Y: complex64, or two fp16 (round to nearest even) of the value times a per-row scale
 // (1 / rms of the row's Y, so the half range is never approached and its 11-bit precision applies relative to the
-// row's own level; used for the bf16 / fp8 operand forms, whose own rounding is coarser).
+// row's own level; used for the bf16 / fp8 operand forms, whose own rounding is coarser). The column pass applies
+// the scale through its output rotation, so put() only converts.
 typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
 struct YF32 {
     using E = float2;
-    static __device__ __forceinline__ E put(const float2 v, const float) { return v; }
+    static __device__ __forceinline__ E put(const float2 v) { return v; }
     static __device__ __forceinline__ float2 get(const E e) { return e; }
     static __device__ __forceinline__ float2 load(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
     {
@@ -227,11 +245,11 @@ struct YF32 {
 };
 struct YF16 {
     using E = h16x2;
-    static __device__ __forceinline__ E put(const float2 v, const float s)
+    static __device__ __forceinline__ E put(const float2 v)
     {
         E h;
-        h.x = (_Float16)(v.x * s);
-        h.y = (_Float16)(v.y * s);
+        h.x = (_Float16)v.x;
+        h.y = (_Float16)v.y;
         return h;
     }
     static __device__ __forceinline__ float2 get(const E e) { return make_float2((float)e.x, (float)e.y); }
@@ -241,12 +259,22 @@ struct YF16 {
     }
 };
 
-// Column transforms: block (k1 block, j); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1; inputs
-// k1 + n1 (p + P q) with q >= NZ are at or above kend (zero, not read)
+// Per segment of a launch: its data, the intermediate's scale (per basis row, or null), its analysed span and its first
+// row in Zt. Passed by value (kernel arguments) and indexed by the wave-uniform blockIdx.z.
+constexpr int kMaxSeg = 16;
+struct SegArgs {
+    const float2* D[kMaxSeg];
+    const float* ysc[kMaxSeg];
+    long a0[kMaxSeg];
+    long Tn[kMaxSeg];
+    long zrow[kMaxSeg];
+};
+
+// Column transforms: block (k1 block, j, segment); threads C x P, thread (c, p) = (tid % C, tid / C) in step 1;
+// inputs k1 + n1 (p + P q) with q >= NZ are at or above kend (zero, not read)
 template <int P, int Q, int C, int NZ, typename YS>
-__global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, const long ldv,
-                                                const float2* __restrict__ D, const long kmin, const long kend,
-                                                const int n1, const int lgn, const float* __restrict__ ysc,
+__global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, const long ldv, const SegArgs S,
+                                                const long kmin, const long kend, const int n1, const int lgn,
                                                 typename YS::E* __restrict__ Y)
 {
     constexpr int L = P * Q, ROW = pitch(P * C);
@@ -254,8 +282,11 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
     __shared__ float ex[Q * ROW];
     const int tid = threadIdx.x;
     const int c = tid % C, p = tid / C;
-    const int j = blockIdx.y;
+    const int j = blockIdx.y, seg = blockIdx.z;
     const long n = 1L << lgn;
+    const float2* __restrict__ D = S.D[seg];
+    const float* __restrict__ ysc = S.ysc[seg];
+    Y += (size_t)seg * gridDim.y * n;
     const int k1 = blockIdx.x * C + c;
     const float2* __restrict__ vr = Vc + (size_t)j * ldv;
     // per-lane 32-bit offset from wave-uniform bases (k < n <= 2^22)
@@ -304,15 +335,17 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
     for (int i = 0; i < Q / P; i++) {
         const int m2 = mb + P * i;
         dft_reg<P>(u[i]);
-        // w_n^(k1 t2), t2 = m2 + Q m1: Y[j, t2, k1] at the uniform base Q m1 n1 plus the lane offset m2 n1 + k1
+        // w_n^(k1 t2), t2 = m2 + Q m1: Y[j, t2, k1] at the uniform base Q m1 n1 plus the lane offset m2 n1 + k1. The
+        // fp16 form's scale rides on the rotation (its seed times ys; the unit steps keep that magnitude).
         Rot<LP> w;
         w.seed((long)k1 * m2, n);
+        if constexpr (LP) { w.c *= ys; w.s *= ys; }
         w.step((long)k1 * Q, n);
         const unsigned yo = (unsigned)(m2 * n1 + k1);
 #pragma unroll
         for (int m1 = 0; m1 < P; m1++) {
             if (m1) w.next();
-            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(w.apply(u[i][m1]), ys);
+            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(w.apply(u[i][m1]));
         }
     }
 }
@@ -358,22 +391,29 @@ struct Out<uint8_t> {
     }
 };
 
-// Row transforms: block (row block, t2), so that the blocks running together complete whole output rows; threads
-// JB x P, thread (p, jr) = (tid % P, tid / P) in step 1
+// Row transforms: block (row block, t2, segment), so that the blocks running together complete whole output rows;
+// threads JB x P, thread (p, jr) = (tid % P, tid / P) in step 1. The row scale 1 / yscale (and, for every output
+// type but fp8, the column scale) is applied with the step-1 twiddles; fp8's column scale multiplies the final
+// values, so that its bytes are the rounding of the float32 form's output times the scale.
 template <int P, int Q, int JB, typename T, typename YS>
 __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restrict__ Y, const int r, const int rp,
-                                                 const int lgn1, const int lgn, const long a0, const long Tn,
-                                                 const float* __restrict__ cs, const float* __restrict__ ysc,
-                                                 T* __restrict__ Zt, const long ldz)
+                                                 const int lgn1, const int lgn, const SegArgs S,
+                                                 const float* __restrict__ cs, T* __restrict__ Zt, const long ldz)
 {
     constexpr int L = P * Q, ROW = pitch(P * (JB + 1));
     constexpr bool LP = sizeof(typename YS::E) == 4;     // fp16 intermediate: float32 twiddles
+    constexpr bool LATE = sizeof(T) == 1;                // fp8: column scale on the final values
     __shared__ float ex[Q * ROW];
     const int tid = threadIdx.x;
     const int p = tid % P, jr = tid / P;
     const long n = 1L << lgn, n1 = 1L << lgn1, n2 = n >> lgn1;
     const long t2 = blockIdx.y;
     const int j0 = blockIdx.x * JB;
+    const int seg = blockIdx.z;
+    const long a0 = S.a0[seg], Tn = S.Tn[seg];
+    const float* __restrict__ ysc = S.ysc[seg];
+    Y += (size_t)seg * r * n;
+    Zt += (size_t)S.zrow[seg] * ldz;
 
     float2 v[Q];
     {
@@ -386,7 +426,15 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
         for (int q = 0; q < Q; q++) v[q] = YS::load(rs, yo + P * q * ES);
     }
     dft_reg<Q>(v);
-    twiddle_line<Q, LP>(v, p, L);
+    {
+        const int jj = j0 + jr;
+        float sc = 1.f;
+        if (jj < r) {
+            if (ysc) sc = 1.f / ysc[jj];
+            if (!LATE && cs) sc *= cs[jj];
+        }
+        twiddle_line_scaled<Q, LP>(v, p, L, sc);
+    }
     // exchange: U(jr, p, m2) at m2 ROW + p (JB + 1) + jr; step 2 takes tasks (jr2 = tid % JB, m2 = tid / JB + P i)
     const int jr2 = tid % JB, mb = tid / JB;
     float2 u[Q / P][P];
@@ -406,7 +454,7 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
     }
     const int j = j0 + jr2;
     if (j >= rp) return;
-    const float sc = j < r ? (cs ? cs[j] : 1.f) / (ysc ? ysc[j] : 1.f) : 1.f;
+    const float sc = LATE && j < r && cs ? cs[j] : 1.f;
     // Zt as a buffer of Tn rows: row t - a0 = tl + n2 Q m1; rows outside [0, Tn) go to offset Tn ldz (dropped)
     constexpr unsigned SZ = 2 * sizeof(T);
     const unsigned nbytes = (unsigned)(Tn * ldz * sizeof(T));
@@ -422,7 +470,8 @@ __global__ void __launch_bounds__(JB * P) rows_k(const typename YS::E* __restric
             const int t = tl + (int)n2 * Q * m1;
             const unsigned row = (unsigned)t < (unsigned)Tn ? (unsigned)t : (unsigned)Tn;
             const float2 z = u[i][m1];
-            Out<T>::store(rs, row * rowb + (unsigned)j * SZ, z.x * sc, z.y * sc);
+            if constexpr (LATE) Out<T>::store(rs, row * rowb + (unsigned)j * SZ, z.x * sc, z.y * sc);
+            else Out<T>::store(rs, row * rowb + (unsigned)j * SZ, z.x, z.y);
         }
     }
 }
@@ -447,80 +496,92 @@ __global__ void __launch_bounds__(256) vd_power_k(const float2* __restrict__ Vc,
 }
 
 template <int P, int Q, int NZ, typename YS>
-void launch_cols_nz(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
-                    const float* ysc, void* Y, hipStream_t s)
+void launch_cols_nz(const float2* Vc, long ldv, const SegArgs& S, int nseg, long kmin, long kend, int n1, int lgn,
+                    int r, void* Y, hipStream_t s)
 {
     // columns per block: 16 complex64 or 32 fp16 pairs give 128-byte runs per Y row (8 for the 2048-point lines)
     constexpr int C = Q > 32 ? 8 : sizeof(typename YS::E) == 8 ? 16 : 32;
-    hipLaunchKernelGGL((cols_k<P, Q, C, NZ, YS>), dim3((unsigned)(n1 / C), r), dim3(C * P), 0, s, Vc, ldv, D, kmin,
-                       kend, n1, lgn, ysc, (typename YS::E*)Y);
+    hipLaunchKernelGGL((cols_k<P, Q, C, NZ, YS>), dim3((unsigned)(n1 / C), r, nseg), dim3(C * P), 0, s, Vc, ldv, S,
+                       kmin, kend, n1, lgn, (typename YS::E*)Y);
 }
 
 // the thread of column k1 and lane p holds k = k1 + n1 (p + P q): all k >= kend for q >= NZ when n1 P NZ >= kend
 template <int P, int Q, typename YS>
-void launch_cols(const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn, int r,
-                 const float* ysc, void* Y, hipStream_t s)
+void launch_cols(const float2* Vc, long ldv, const SegArgs& S, int nseg, long kmin, long kend, int n1, int lgn, int r,
+                 void* Y, hipStream_t s)
 {
     const long span = (long)n1 * P;
-    if (kend <= span * (Q / 8)) launch_cols_nz<P, Q, Q / 8, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
-    else if (kend <= span * (Q / 4)) launch_cols_nz<P, Q, Q / 4, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
-    else if (kend <= span * (Q / 2)) launch_cols_nz<P, Q, Q / 2, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
-    else launch_cols_nz<P, Q, Q, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s);
+    if constexpr (Q >= 32) {
+        if (kend <= span * (Q / 32)) {
+            launch_cols_nz<P, Q, Q / 32, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
+            return;
+        }
+    }
+    if constexpr (Q >= 16) {
+        if (kend <= span * (Q / 16)) {
+            launch_cols_nz<P, Q, Q / 16, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
+            return;
+        }
+    }
+    if (kend <= span * (Q / 8)) launch_cols_nz<P, Q, Q / 8, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
+    else if (kend <= span * (Q / 4)) launch_cols_nz<P, Q, Q / 4, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
+    else if (kend <= span * (Q / 2)) launch_cols_nz<P, Q, Q / 2, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
+    else launch_cols_nz<P, Q, Q, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s);
 }
 
 template <typename YS>
-int cols_dispatch(const int lgn2, const float2* Vc, long ldv, const float2* D, long kmin, long kend, int n1, int lgn,
-                  int r, const float* ysc, void* Y, hipStream_t s)
+int cols_dispatch(const int lgn2, const float2* Vc, long ldv, const SegArgs& S, int nseg, long kmin, long kend,
+                  int n1, int lgn, int r, void* Y, hipStream_t s)
 {
     switch (lgn2) {
-    case 7: launch_cols<8, 16, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
-    case 8: launch_cols<16, 16, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
-    case 9: launch_cols<16, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
-    case 10: launch_cols<32, 32, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
-    case 11: launch_cols<32, 64, YS>(Vc, ldv, D, kmin, kend, n1, lgn, r, ysc, Y, s); break;
+    case 7: launch_cols<8, 16, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s); break;
+    case 8: launch_cols<16, 16, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s); break;
+    case 9: launch_cols<16, 32, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s); break;
+    case 10: launch_cols<32, 32, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s); break;
+    case 11: launch_cols<32, 64, YS>(Vc, ldv, S, nseg, kmin, kend, n1, lgn, r, Y, s); break;
     default: return GWGS_EUNSUP;
     }
     return GWGS_OK;
 }
 
 template <int P, int Q, typename T, typename YS>
-void launch_rows(const void* Y, int r, int rp, int lgn1, int lgn, long a0, long Tn, const float* cs, const float* ysc,
-                 T* Zt, long ldz, hipStream_t s)
+void launch_rows(const void* Y, int r, int rp, int lgn1, int lgn, const SegArgs& S, int nseg, const float* cs, T* Zt,
+                 long ldz, hipStream_t s)
 {
     // rows per block: 16 float32 pairs or 32 bf16 / fp8 pairs give 128- / 128- / 64-byte runs per output row
     constexpr int JB = Q > 32 ? 8 : sizeof(T) == 4 ? 16 : 32;
     const long n2 = 1L << (lgn - lgn1);
-    hipLaunchKernelGGL((rows_k<P, Q, JB, T, YS>), dim3((rp + JB - 1) / JB, (unsigned)n2), dim3(JB * P), 0, s,
-                       (const typename YS::E*)Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz);
+    hipLaunchKernelGGL((rows_k<P, Q, JB, T, YS>), dim3((rp + JB - 1) / JB, (unsigned)n2, nseg), dim3(JB * P), 0, s,
+                       (const typename YS::E*)Y, r, rp, lgn1, lgn, S, cs, Zt, ldz);
 }
 
 template <typename T, typename YS>
-int rows_dispatch(const int lgn1, const void* Y, int r, int rp, int lgn, long a0, long Tn, const float* cs,
-                  const float* ysc, T* Zt, long ldz, hipStream_t s)
+int rows_dispatch(const int lgn1, const void* Y, int r, int rp, int lgn, const SegArgs& S, int nseg, const float* cs,
+                  T* Zt, long ldz, hipStream_t s)
 {
     switch (lgn1) {
-    case 6: launch_rows<8, 8, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
-    case 7: launch_rows<8, 16, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
-    case 8: launch_rows<16, 16, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
-    case 9: launch_rows<16, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
-    case 10: launch_rows<32, 32, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
-    case 11: launch_rows<32, 64, T, YS>(Y, r, rp, lgn1, lgn, a0, Tn, cs, ysc, Zt, ldz, s); break;
+    case 6: launch_rows<8, 8, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
+    case 7: launch_rows<8, 16, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
+    case 8: launch_rows<16, 16, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
+    case 9: launch_rows<16, 32, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
+    case 10: launch_rows<32, 32, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
+    case 11: launch_rows<32, 64, T, YS>(Y, r, rp, lgn1, lgn, S, nseg, cs, Zt, ldz, s); break;
     default: return GWGS_EUNSUP;
     }
     return GWGS_OK;
 }
 
 template <typename YS>
-int series(int dtype, int lg, const float2* vc, long ldv, int r, int rp, const float2* d, long kmin, long kend,
-           long a0, long Tn, const float* cs, const float* ysc, void* Y, void* Zt, long ldz, hipStream_t s)
+int series(int dtype, int lg, const float2* vc, long ldv, int r, int rp, const SegArgs& S, int nseg, long kmin,
+           long kend, const float* cs, void* Y, void* Zt, long ldz, hipStream_t s)
 {
     const int lgn1 = lg / 2, lgn2 = lg - lg / 2;
-    int e = cols_dispatch<YS>(lgn2, vc, ldv, d, kmin, kend, 1 << lgn1, lg, r, ysc, Y, s);
+    int e = cols_dispatch<YS>(lgn2, vc, ldv, S, nseg, kmin, kend, 1 << lgn1, lg, r, Y, s);
     if (e) return e;
-    if (dtype == 0) return rows_dispatch<float, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (float*)Zt, ldz, s);
-    if (dtype == 1) return rows_dispatch<uint16_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint16_t*)Zt, ldz, s);
-    if (dtype == 3) return rows_dispatch<f16s, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (f16s*)Zt, ldz, s);
-    return rows_dispatch<uint8_t, YS>(lgn1, Y, r, rp, lg, a0, Tn, cs, ysc, (uint8_t*)Zt, ldz, s);
+    if (dtype == 0) return rows_dispatch<float, YS>(lgn1, Y, r, rp, lg, S, nseg, cs, (float*)Zt, ldz, s);
+    if (dtype == 1) return rows_dispatch<uint16_t, YS>(lgn1, Y, r, rp, lg, S, nseg, cs, (uint16_t*)Zt, ldz, s);
+    if (dtype == 3) return rows_dispatch<f16s, YS>(lgn1, Y, r, rp, lg, S, nseg, cs, (f16s*)Zt, ldz, s);
+    return rows_dispatch<uint8_t, YS>(lgn1, Y, r, rp, lg, S, nseg, cs, (uint8_t*)Zt, ldz, s);
 }
 
 int log2_exact(const long n)
@@ -542,6 +603,39 @@ GWGS_API long gwgs_basis_series_workspace(int r, long n)
     return (long)r * n * 8;
 }
 
+// The checks and launches of both entry points: segment i of nseg reads D[i], writes Zt rows zrow[i] .. zrow[i] +
+// Tn[i] - 1 and uses its own part of work (the intermediate of r rows each).
+static int series_call(int dtype, const void* Vc, long ldv, int r, int rp, int nseg, const void* const* D, long kmin,
+                       long kend, long n, const long* a0, const long* Tn, const long* zrow, const float* colscale,
+                       int ytype, const float* const* yscale, void* work, void* Zt, long ldz, void* stream)
+{
+    GWGS_CHECK(Vc && D && a0 && Tn && zrow && work && Zt && r > 0 && rp >= r && kmin >= 0 && kmin <= kend &&
+               kend <= ldv && kend <= n && ldz >= 2 * (long)rp && ldz % 2 == 0, GWGS_EINVAL);
+    if (nseg < 1 || nseg > kMaxSeg) return GWGS_ESIZE;
+    const int lg = log2_exact(n);
+    if (lg < 13 || lg > 22) return GWGS_EUNSUP;
+    if (dtype < 0 || dtype > 3 || ytype < 0 || ytype > 1) return GWGS_EUNSUP;
+    // the row kernel addresses a segment's Zt rows as one buffer with 32-bit byte offsets up to (Tn + 1) rows
+    const double esz = dtype == 0 ? 4.0 : dtype == 2 ? 1.0 : 2.0;
+    SegArgs S = {};
+    for (int i = 0; i < nseg; i++) {
+        GWGS_CHECK(D[i] && a0[i] >= 0 && Tn[i] >= 0 && a0[i] + Tn[i] <= n && zrow[i] >= 0, GWGS_EINVAL);
+        if ((double)(Tn[i] + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;
+        S.D[i] = (const float2*)D[i];
+        S.ysc[i] = ytype && yscale ? yscale[i] : nullptr;
+        S.a0[i] = a0[i];
+        S.Tn[i] = Tn[i];
+        S.zrow[i] = zrow[i];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int e = ytype == 0 ? series<YF32>(dtype, lg, (const float2*)Vc, ldv, r, rp, S, nseg, kmin, kend, colscale,
+                                            work, Zt, ldz, s)
+                             : series<YF16>(dtype, lg, (const float2*)Vc, ldv, r, rp, S, nseg, kmin, kend, colscale,
+                                            work, Zt, ldz, s);
+    if (e) return e;
+    return gwgs_status();
+}
+
 // dtype 0: float32 Zt, 1: bfloat16, 2: fp8 e4m3 (clamped to +-448), 3: fp16. Vc (>= r rows, row pitch ldv >= kend) and D
 // (>= kend) complex64; work: gwgs_basis_series_workspace(r, n) bytes; Zt (Tn rows, ldz >= 2 rp elements, even);
 // colscale: per basis row, or null. ytype 0: the intermediate in complex64; 1: in fp16 pairs scaled by yscale[j]
@@ -550,21 +644,21 @@ GWGS_API int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int r
                                long n, long a0, long Tn, const float* colscale, int ytype, const float* yscale,
                                void* work, void* Zt, long ldz, void* stream)
 {
-    GWGS_CHECK(Vc && D && work && Zt && r > 0 && rp >= r && kmin >= 0 && kmin <= kend && kend <= ldv && kend <= n &&
-               a0 >= 0 && Tn >= 0 && a0 + Tn <= n && ldz >= 2 * (long)rp && ldz % 2 == 0, GWGS_EINVAL);
-    const int lg = log2_exact(n);
-    if (lg < 13 || lg > 22) return GWGS_EUNSUP;
-    // the row kernel addresses Zt as one buffer with 32-bit byte offsets up to (Tn + 1) rows
-    const double esz = dtype == 0 ? 4.0 : dtype == 2 ? 1.0 : 2.0;
-    if ((double)(Tn + 1) * (double)ldz * esz >= 4294967295.0) return GWGS_ESIZE;
-    if (dtype < 0 || dtype > 3 || ytype < 0 || ytype > 1) return GWGS_EUNSUP;
-    hipStream_t s = (hipStream_t)stream;
-    const int e = ytype == 0 ? series<YF32>(dtype, lg, (const float2*)Vc, ldv, r, rp, (const float2*)D, kmin, kend, a0,
-                                            Tn, colscale, yscale, work, Zt, ldz, s)
-                             : series<YF16>(dtype, lg, (const float2*)Vc, ldv, r, rp, (const float2*)D, kmin, kend, a0,
-                                            Tn, colscale, yscale, work, Zt, ldz, s);
-    if (e) return e;
-    return gwgs_status();
+    const long z0 = 0;
+    return series_call(dtype, Vc, ldv, r, rp, 1, &D, kmin, kend, n, &a0, &Tn, &z0, colscale, ytype, &yscale, work,
+                       Zt, ldz, stream);
+}
+
+// nseg (1..16) segments in one launch of each pass: D[i], a0[i], Tn[i], zrow[i] (Zt row of the segment's first
+// sample), yscale[i] (host arrays of device addresses and sizes); work: nseg r n x 8 bytes (ytype 0) or x 4 (ytype
+// 1), segment i's intermediate at i r n elements; otherwise as gwgs_basis_series.
+GWGS_API int gwgs_basis_series_batch(int dtype, const void* Vc, long ldv, int r, int rp, int nseg, const void* const* D,
+                                     long kmin, long kend, long n, const long* a0, const long* Tn, const long* zrow,
+                                     const float* colscale, int ytype, const float* const* yscale, void* work, void* Zt,
+                                     long ldz, void* stream)
+{
+    return series_call(dtype, Vc, ldv, r, rp, nseg, D, kmin, kend, n, a0, Tn, zrow, colscale, ytype, yscale, work,
+                       Zt, ldz, stream);
 }
 
 // out[j] = sum over [kmin, kend) of |Vc[j, k] D[k]|^2 in double, j < r.

@@ -26,6 +26,7 @@ returns them (HDF5 writing is outside this path).
 """
 from __future__ import annotations
 
+import ctypes
 import time
 from collections import deque
 from concurrent.futures import ThreadPoolExecutor
@@ -44,6 +45,7 @@ import templates as tmpl
 from exact_stage import ExactStage, PeakRequest, SegmentData, TemplateTile
 
 CH = 1 << 16          # frequency chunk for the Gram and basis products
+SERIES_SEG = 16       # segments per launch of the fused basis transform (its kernels' limit)
 BN = fp_gemm.BN
 PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp8': 2, 'fp16': 3}
 OPERAND_DTYPES = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8, 3: torch.float16}
@@ -63,7 +65,9 @@ class Options:
     gemm_cfg: tuple | None = None          # (tiles_per_block,), from fp_gemm.pick_cfg
     self_check: bool = True                # the exact stage's first-call check
     timing: bool = False                   # synchronize around stages to fill the timing dict
-    segment_batch: int = 4                 # segments per batched basis transform and GEMM launch
+    segment_batch: int = 32                # at most this many segments per basis-transform batch and GEMM launch
+    batch_samples: int = 1 << 22           # and at most this many analysed samples per batch (a batch's operand Zt
+                                           # takes batch_samples x 2 rp x its element size; one segment always fits)
     depth: int = 2                         # batches enqueued ahead of the one being decoded
     graphs: bool = False                   # capture each tile's norms and each batch's stage A in a HIP graph on
                                            # the first run and replay them on later runs (same segmentation; new
@@ -533,17 +537,27 @@ class Filter:
                               self.kmin, tb.kend, pw[i].data_ptr(), _dev.stream())
                 zcs, acs = _fp8_scales(pw)
             assert not ytype or rp <= self._ysc.shape[1], 'tile larger than this Filter\'s opts.tile'
-            work = self._workbuf(ws)
             esz = Zt.element_size()
-            for i, (di, s, gs, a0, nw) in enumerate(segs):
+            # every segment of the batch in one launch of each pass, up to SERIES_SEG at a time (each with its own
+            # intermediate in the workspace)
+            wseg = tb.r * self.tlen * (4 if ytype else 8)       # one segment's intermediate
+            for c0 in range(0, nb, SERIES_SEG):
+                ch = range(c0, min(nb, c0 + SERIES_SEG))
+                work = self._workbuf(wseg * len(ch))
+                dptr = (ctypes.c_void_p * len(ch))(*[self.data[segs[i][0]].D[segs[i][1]].data_ptr() for i in ch])
+                yptr = (ctypes.c_void_p * len(ch))(*[self._ysc[self.seg0[segs[i][0]] + segs[i][1]].data_ptr()
+                                                     if ytype else None for i in ch])
+                a0s = (ctypes.c_long * len(ch))(*[segs[i][3] for i in ch])
+                tns = (ctypes.c_long * len(ch))(*[Tns[i] for i in ch])
+                zrs = (ctypes.c_long * len(ch))(*[int(row0[i]) for i in ch])
                 e0 = _dev.METER.start()
-                _lib.call('gwgs_basis_series', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp,
-                          self.data[di].D[s].data_ptr(), self.kmin, tb.kend, self.tlen, a0, Tns[i], _dev.ptr(zcs),
-                          ytype, self._ysc[self.seg0[di] + s].data_ptr() if ytype else None, work.data_ptr(),
-                          Zt[row0[i]].data_ptr(), K, _dev.stream())
-                # HBM bytes: Vc rows and D over [kmin, kend), Y written and read, Zt written
-                _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin) + 8 * (tb.kend - self.kmin)
-                                + (8 if ytype else 16) * tb.r * self.tlen + Tns[i] * K * esz)
+                _lib.call('gwgs_basis_series_batch', self.code, tb.Vc.data_ptr(), self.flen, tb.r, rp, len(ch), dptr,
+                          self.kmin, tb.kend, self.tlen, a0s, tns, zrs, _dev.ptr(zcs), ytype, yptr, work.data_ptr(),
+                          Zt.data_ptr(), K, _dev.stream())
+                # HBM bytes: Vc rows (once per launch) and D over [kmin, kend), Y written and read, Zt written
+                _dev.METER.stop('basis_series', e0, 8 * tb.r * (tb.kend - self.kmin)
+                                + sum(8 * (tb.kend - self.kmin) + (8 if ytype else 16) * tb.r * self.tlen
+                                      + Tns[i] * K * esz for i in ch))
             return Zt, acs
         q = self._qbuf(nb, rp)
         for i, (di, s, gs, a0, nw) in enumerate(segs):
@@ -595,11 +609,10 @@ class Filter:
                 nw = (a1 - a0) // self.cw
                 if nw > 0:
                     segs.append((di, s, self.seg0[di] + s, a0, nw))
-        B = max(1, self.opts.segment_batch)
         gate = self.opts.gate
         gsegs = [gate.per_segment(m) for m in self.data] if gate is not None else None
         if self.opts.graphs and self._bufs is None:
-            self._alloc_rotation(plans, segs, B)
+            self._alloc_rotation(plans, segs)
         ordinal = 0
         for pi, plan in enumerate(plans):
             for ti, tb in enumerate(plan.tiles):
@@ -623,11 +636,11 @@ class Filter:
                 sig, norm = self.tile_norms(tb, (id(plan), ti))
                 tile_host = _dev.Fetch(sig, norm)
                 t1 = self._sync(); self.T['norm_s'] += t1 - t0
-                for b0 in range(0, len(tsegs), B):
-                    cur = self.launch_batch(tb, tsegs[b0:b0 + B], norm, tile_host, key=(id(plan), ti, b0),
+                for b0, b1 in self._split(tsegs):
+                    cur = self.launch_batch(tb, tsegs[b0:b1], norm, tile_host, key=(id(plan), ti, b0),
                                             ordinal=ordinal, pi=pi)
                     ordinal += 1
-                    cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b0 + B >= len(tsegs), plan.sg_locs
+                    cur['outs'], cur['pi'], cur['last'], cur['sg_locs'] = outs[pi], pi, b1 >= len(tsegs), plan.sg_locs
                     cur['gate'] = gsel
                     cur['filter'] = self
                     cur['ntmp'] = len(plan.m1)
@@ -764,7 +777,21 @@ class Filter:
         gsl = _dev.to_dev(np.array([gs for (_, _, gs, _, _) in segs], np.int64))
         return dict(Tns=Tns, row0=row0, nwsum=sum(nws), col0=col0, items=items, gsl=gsl, K=2 * tb.rp, per=per)
 
-    def _alloc_rotation(self, plans, segs, B):
+    def _split(self, segs):
+        """Consecutive batches [b0, b1) of the segment list: up to opts.segment_batch segments and opts.batch_samples
+        analysed samples each (a longer segment alone)."""
+        out, b0, tot = [], 0, 0
+        for i, sg in enumerate(segs):
+            t = sg[4] * self.cw
+            if i > b0 and (i - b0 >= max(1, self.opts.segment_batch) or tot + t > self.opts.batch_samples):
+                out.append((b0, i))
+                b0, tot = i, 0
+            tot += t
+        if b0 < len(segs):
+            out.append((b0, len(segs)))
+        return out
+
+    def _alloc_rotation(self, plans, segs):
         """depth + 1 buffer sets, each sized for the largest batch of the run, for the captured stage-A graphs:
         batch i of a run writes set i mod (depth + 1). Invariant: run_banks decodes batch i (its candidate list read
         back and awaited, its rebuilt phases enqueued on the compute stream) once batches i + 1 .. i + depth are
@@ -774,8 +801,8 @@ class Filter:
         for plan in plans:
             for tb in plan.tiles:
                 n, K = len(tb.ids), 2 * tb.rp
-                for b0 in range(0, len(segs), B):
-                    sg = segs[b0:b0 + B]
+                for b0, b1 in self._split(segs):
+                    sg = segs[b0:b1]
                     T = sum(nw * self.cw for (_, _, _, _, nw) in sg)
                     nw = sum(x[4] for x in sg)
                     zt, a, w, nmax = max(zt, T * K), max(a, n * K), max(w, n * nw), max(nmax, n)
@@ -798,7 +825,8 @@ class Filter:
         ws = max((_lib.load().gwgs_basis_series_workspace(tb.r, self.tlen) for plan in plans for tb in plan.tiles),
                  default=0)
         if ws > 0:
-            self._workbuf(ws)
+            # the batched transform's intermediates for up to SERIES_SEG segments, so that no capture allocates
+            self._workbuf(ws * min(max(b1 - b0 for b0, b1 in self._split(segs)), SERIES_SEG))
         self._ones(nmax)
 
     def _batch_device(self, tb: TileBasis, segs, norm, geo, bufs=None, pk=None):

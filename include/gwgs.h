@@ -207,6 +207,15 @@ long gwgs_basis_series_workspace(int r, long n);
 int gwgs_basis_series(int dtype, const void* Vc, long ldv, int r, int rp, const void* D, long kmin, long kend, long n,
                       long a0, long Tn, const float* colscale, int ytype, const float* yscale, void* work, void* Zt,
                       long ldz, void* stream);
+/* pipeline/fp_filter.py:597-613 basis_series for every segment of one (tile, batch) in one launch of each pass:
+ * nseg (1..16) segments, segment i with data D[i], analysed span a0[i] .. a0[i] + Tn[i] - 1 written to Zt rows
+ * zrow[i] .. zrow[i] + Tn[i] - 1, and its own yscale[i] (ytype 1); D and yscale are host arrays of device addresses,
+ * a0, Tn and zrow host arrays. work: nseg r n bytes times 8 (ytype 0) or 4 (ytype 1). Otherwise as
+ * gwgs_basis_series; GWGS_ESIZE for nseg outside 1..16. */
+int gwgs_basis_series_batch(int dtype, const void* Vc, long ldv, int r, int rp, int nseg, const void* const* D,
+                            long kmin, long kend, long n, const long* a0, const long* Tn, const long* zrow,
+                            const float* colscale, int ytype, const float* const* yscale, void* work, void* Zt,
+                            long ldz, void* stream);
 /* out[j] = sum over kmin <= k < kend of |Vc[j, k] D[k]|^2 (double), j < r: the transform input's power by rows, from
  * which pipeline/fp_filter.py:609-610's rms scale follows by Parseval without forming q */
 int gwgs_vd_power(const void* Vc, long ldv, int r, const void* D, long kmin, long kend, double* out, void* stream);

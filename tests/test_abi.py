@@ -56,6 +56,14 @@ def test_bad_arguments_rejected(lib):
     assert lib.gwgs_basis_series(9, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 0, None, p, p, 32, None) == -3
     assert lib.gwgs_basis_series(1, p, 100, 10, 16, p, 0, 50, 1 << 16, 0, 10, None, 2, None, p, p, 32, None) == -3
     assert lib.gwgs_vd_power(None, 100, 10, p, 0, 50, p, None) == -1
+    # the batched form: 1..16 segments, each segment's span inside the transform length
+    arr = lambda ct, *v: (ct * len(v))(*v)
+    bs = lambda nseg, a0, tn: lib.gwgs_basis_series_batch(
+        1, p, 100, 10, 16, nseg, arr(ctypes.c_void_p, *([256] * max(nseg, 1))), 0, 50, 1 << 16,
+        arr(ctypes.c_long, *([a0] * max(nseg, 1))), arr(ctypes.c_long, *([tn] * max(nseg, 1))),
+        arr(ctypes.c_long, *([0] * max(nseg, 1))), None, 0, None, p, p, 32, None)
+    assert bs(0, 0, 10) == -2 and bs(17, 0, 10) == -2
+    assert bs(2, (1 << 16) - 5, 10) == -1 and bs(2, -1, 10) == -1
     # pre-consolidation: empty batches return at once; bad flags, no segment table, or ranking without a workspace
     assert lib.gwgs_pre_keep_workspace(100, 5) == 8 * 100 + 4 * 16
     pk = lambda n, nseg, flags, work, wb: lib.gwgs_pre_keep(p, p, n, 10, nseg, p, p, p, p, 2048.0, 2195.456, 3, 1.0,

@@ -469,6 +469,50 @@ def test_basis_series_padding_rows(gpu):
         assert torch.count_nonzero(outs[0][:, :2 * r]).item() > 0
 
 
+@pytest.mark.parametrize('n,ytype,dtype', [(1 << 18, 1, 1), (1 << 20, 1, 1), (1 << 18, 0, 0), (1 << 16, 1, 2)])
+def test_basis_series_batch_equals_per_segment(gpu, n, ytype, dtype):
+    """gwgs_basis_series_batch over segments of different data, analysed spans, output rows and fp16 scales (one
+    launch of each pass) equals one gwgs_basis_series call per segment, bit for bit, and leaves the Zt rows between
+    the segments untouched."""
+    import ctypes
+    import _dev, _lib
+    rng = np.random.default_rng(n + ytype)
+    flen, r, kmin = n // 2 + 1, 37, 60
+    kend = n // 20                                        # a band below n / 16: the column pass's pruned inputs
+    rp, K = 48, 96
+    Vc = np.zeros((rp, flen), np.complex64)
+    Vc[:r, kmin:kend] = rng.standard_normal((r, kend - kmin)) + 1j * rng.standard_normal((r, kend - kmin))
+    Vd = torch.from_numpy(Vc).cuda()
+    nseg = 5
+    Ds = [torch.from_numpy((rng.standard_normal(flen) + 1j * rng.standard_normal(flen)).astype(np.complex64)).cuda()
+          for _ in range(nseg)]
+    a0 = [n // 8, 0, n // 4 + 128, n // 2, 1000]
+    Tn = [n // 2, n // 8, n // 4, n // 2 - 256, 4096]
+    gap = 7
+    zrow = list(np.cumsum([0] + [t + gap for t in Tn])[:nseg])
+    rows = int(zrow[-1] + Tn[-1] + gap)
+    ysc = [torch.full((rp,), float(v), device='cuda') for v in rng.random(nseg) * 4 + 0.25]
+    cs = torch.from_numpy((rng.random(rp) * 0.02 + 0.01).astype(np.float32)).cuda() if dtype == 2 else None
+    dt = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8}[dtype]
+    lib = _lib.load()
+    one = torch.empty(lib.gwgs_basis_series_workspace(r, n), dtype=torch.uint8, device='cuda')
+    work = torch.empty(nseg * r * n * (4 if ytype else 8), dtype=torch.uint8, device='cuda')
+    ref = torch.full((rows, K), 7, dtype=dt, device='cuda')
+    for i in range(nseg):
+        _lib.call('gwgs_basis_series', dtype, Vd.data_ptr(), flen, r, rp, Ds[i].data_ptr(), kmin, kend, n, a0[i],
+                  Tn[i], _dev.ptr(cs), ytype, ysc[i].data_ptr() if ytype else None, one.data_ptr(),
+                  ref[zrow[i]].data_ptr(), K, _dev.stream())
+    got = torch.full((rows, K), 7, dtype=dt, device='cuda')
+    arr = lambda ct, v: (ct * nseg)(*v)
+    _lib.call('gwgs_basis_series_batch', dtype, Vd.data_ptr(), flen, r, rp, nseg,
+              arr(ctypes.c_void_p, [d.data_ptr() for d in Ds]), kmin, kend, n, arr(ctypes.c_long, a0),
+              arr(ctypes.c_long, Tn), arr(ctypes.c_long, [int(z) for z in zrow]), _dev.ptr(cs), ytype,
+              arr(ctypes.c_void_p, [y.data_ptr() if ytype else None for y in ysc]), work.data_ptr(), got.data_ptr(), K,
+              _dev.stream())
+    assert torch.equal(got.view(torch.uint8) if dtype == 2 else got, ref.view(torch.uint8) if dtype == 2 else ref)
+    assert torch.count_nonzero(got[:Tn[0]].float() if dtype != 2 else got[:Tn[0]]).item() > 0
+
+
 def test_vd_power_vs_float64(gpu):
     """gwgs_vd_power: sum over [kmin, kend) of |Vc D|^2 per row, against float64 (relative 1e-12)."""
     import _dev, _lib
