@@ -226,6 +226,11 @@ __device__ __forceinline__ void twiddle_line_scaled(float2 (&v)[Q], const int p,
     }
 }
 
+// Cache policy of the intermediate's and the operand's accesses: non-temporal (bit 1 of the buffer instructions'
+// cache-policy operand), since each is written or read once by a pass and their footprints (gigabytes per launch)
+// exceed the caches.
+constexpr int kStream = 2;
+
 // LDS row pitch (floats) per m2: congruent to 16 mod 32, so a wave's 4 rows of 16 lanes fall on distinct banks
 constexpr int pitch(int base) { return base + (16 - base % 32 + 32) % 32; }
 
@@ -237,6 +242,14 @@ typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
 struct YF32 {
     using E = float2;
     static __device__ __forceinline__ E put(const float2 v) { return v; }
+    static __device__ __forceinline__ void store(E* p, const float2 v)
+    {
+        typedef float f32x2_t __attribute__((ext_vector_type(2)));
+        f32x2_t w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, (f32x2_t*)p);
+    }
     static __device__ __forceinline__ float2 get(const E e) { return e; }
     static __device__ __forceinline__ float2 load(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
     {
@@ -252,10 +265,14 @@ struct YF16 {
         h.y = (_Float16)v.y;
         return h;
     }
+    static __device__ __forceinline__ void store(E* p, const float2 v)
+    {
+        __builtin_nontemporal_store(__builtin_bit_cast(uint32_t, put(v)), (uint32_t*)p);
+    }
     static __device__ __forceinline__ float2 get(const E e) { return make_float2((float)e.x, (float)e.y); }
     static __device__ __forceinline__ float2 load(const __amdgpu_buffer_rsrc_t rs, const unsigned off)
     {
-        return get(__builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0)));
+        return get(__builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, kStream)));
     }
 };
 
@@ -345,7 +362,7 @@ __global__ void __launch_bounds__(C * P) cols_k(const float2* __restrict__ Vc, c
 #pragma unroll
         for (int m1 = 0; m1 < P; m1++) {
             if (m1) w.next();
-            (yb + (size_t)Q * m1 * n1)[yo] = YS::put(w.apply(u[i][m1]));
+            YS::store(&(yb + (size_t)Q * m1 * n1)[yo], w.apply(u[i][m1]));
         }
     }
 }
@@ -363,7 +380,7 @@ struct Out<float> {
         u32x2 v;
         v.x = __float_as_uint(re);
         v.y = __float_as_uint(im);
-        __builtin_amdgcn_raw_buffer_store_b64(v, rs, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(v, rs, (int)off, 0, kStream);
     }
 };
 template <>
@@ -371,7 +388,7 @@ struct Out<uint16_t> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
     {
-        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_bf16x2(re, im), rs, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_bf16x2(re, im), rs, (int)off, 0, kStream);
     }
 };
 template <>
@@ -379,7 +396,7 @@ struct Out<f16s> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
     {
-        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_f16x2(re, im), rs, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(f32x2_to_f16x2(re, im), rs, (int)off, 0, kStream);
     }
 };
 template <>
@@ -387,7 +404,7 @@ struct Out<uint8_t> {
     static __device__ __forceinline__ void store(const __amdgpu_buffer_rsrc_t rs, const unsigned off, const float re,
                                                  const float im)
     {
-        __builtin_amdgcn_raw_buffer_store_b16(f32x2_to_fp8x2(re, im), rs, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(f32x2_to_fp8x2(re, im), rs, (int)off, 0, kStream);
     }
 };
 
