@@ -9,7 +9,8 @@ resident in HBM: float64 median-Welch PSD, truncation, whitening; template norms
 the fused GEMM + window max; candidate selection; the exact stage with +-3-sample re-centring: edges, chi-squared
 sums, float64 finalization), so the 20 default steps filter every chunk once. Warm-up steps run on separate chunks.
 The bank products (templates, float64 SVD bases) are built once before timing, as a search builds them once per run;
-config.value_with_setup counts them into the whole job.
+config.value_with_setup counts them, the autotuner and the SVD metric's conditioning into the whole job (not the
+synthetic data's generation, which stands in for reading strain), and config.setup_parts breaks that down.
 
 value = 5,880 x analysed seconds of the timed slices (both detectors) / their wall time, for the BF16 form. The
 FP32 plain form (and the FP8 form) are timed on the first slices beside it, the trigger lists compared with the
@@ -256,13 +257,25 @@ class BatchedWorkload:
     flattened over grids in grid order."""
 
     def __init__(self, n_chunks, opts, nslice=20, n_warm=10):
+        import torch
         import batched, workloads
+        t0 = time.perf_counter()
         chunks, x_dev, self.banks = workloads.batched_device(n_chunks)
         wch, wx, _ = workloads.batched_device(n_warm, seed=97, noise_seed=500000, n_circ=0, n_ecc=0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         self.n_chunks, self.nslice = n_chunks, nslice
         self.slices = batched.slices(n_chunks, 2, nslice)
         self.warm = list(range(len(chunks), len(chunks) + len(wch)))
         self.search = batched.BatchedSearch(chunks + wch, self.banks, x_dev + wx, opts, ref=self.warm)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        plans = [p for g in self.search.grids for p in self.search.plans[g]]
+        tm = sum(p.T['templates_s'] for p in plans)
+        sv = sum(p.T['svd_s'] for p in plans)
+        # synthetic data (not part of a search's set-up) apart from the bank products: templates and SVD bases of
+        # every grid, and the rest of the search set-up (the reference chunks' conditioning for the SVD metric)
+        self.setup = dict(data_s=t1 - t0, templates_s=tm, svd_s=sv, search_other_s=t2 - t1 - tm - sv)
         self._last_ds = None
 
     def _flat(self, ds, outs):
@@ -451,8 +464,10 @@ def run_batched(a, opts, cfg, tuned, t_setup):
     import _dev
     wl = BatchedWorkload(a.chunks, opts, nslice=a.slices)
     torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t_setup
-    log('setup', round(setup_s, 1), 's; ranks', wl.ranks())
+    # the search's set-up: autotuner, bank products (templates, SVD) and the SVD metric's conditioning; the
+    # synthetic data's generation is not counted
+    setup_s = time.perf_counter() - t_setup - wl.setup['data_s']
+    log('setup', round(setup_s, 1), 's', {k: round(v, 2) for k, v in wl.setup.items()}, '; ranks', wl.ranks())
     ntemp = sum(len(b) for b in wl.banks)
     log('warmup')
     for _ in range(a.warmup):
@@ -630,6 +645,7 @@ def main():
     t_setup = time.perf_counter()
     import fp_gemm
     tuned = None
+    t_tune = time.perf_counter()
     if a.gemm_cfg:
         cfg = tuple(int(v) for v in a.gemm_cfg.split(','))
     else:
@@ -637,6 +653,7 @@ def main():
         cfg, tuned = fp_gemm.pick_cfg(M=512, K=832, Tn=2 * 448 * 2048, dtype=torch.bfloat16)
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
+    t_tune = time.perf_counter() - t_tune
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
                                  alternates=a.alternates, segment_batch=a.segment_batch,
                                  batch_samples=a.batch_samples, overlap=bool(a.overlap),
@@ -675,6 +692,7 @@ def main():
                     slices=a.slices, tile=a.tile, segment_batch=a.segment_batch, batch_samples=a.batch_samples,
                     ranks=wl.ranks(),
                     setup_s=round(b['setup_s'], 1),
+                    setup_parts={k: round(v, 2) for k, v in dict(b['wl'].setup, autotune_s=t_tune).items()},
                     value_with_setup=b['ntemp'] * an_job / (slices_s + b['setup_s']),
                     speedup_bf16_vs_fp32=None if fp32 is None else fp32['speedup_bf16'],
                     speedup_fp32_slices=F,
