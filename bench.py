@@ -101,10 +101,10 @@ def compare_triggers(a, b):
 
 
 def measured_traffic():
-    """HBM bytes per GEMM launch from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a
-    plain copy of known size), or None when the summary is absent."""
+    """HBM bytes per GEMM launch of the headline workload from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by a plain copy of known size), or None when the summary is absent."""
     import re
-    p = os.path.join(ROOT, 'profiles', 'hbm_traffic_c2_pmc.md')
+    p = os.path.join(ROOT, 'profiles', 'hbm_traffic_batched_pmc.md')
     if not os.path.exists(p):
         return None
     m = re.search(r'traffic_bytes_per_launch \(gemm_peaks, bf16\) = ([0-9.eE+]+)', open(p).read())
@@ -491,7 +491,9 @@ def run_batched(a, opts, cfg, tuned, t_setup):
     dt = total / a.steps
     value = ntemp * an_s / total
     n_l, t_k, fl = ms['gemm_bf16']
+    nz, zb = meter.notes.get('gemm_bf16_zt_bytes', (0, 0.0))
     res = dict(value=value, ms_per_step=dt * 1e3, analysed_s=an_s, setup_s=setup_s, ms=ms, n_launches=n_l,
+               zt_bytes_per_launch=zb / max(nz, 1),
                t_gemm=t_k, flops=fl, hbm=_kernel_rates(ms, a.steps, dt), ntemp=ntemp, wl=wl)
     F = min(a.fp32_slices, a.steps)
     forms = {}
@@ -704,6 +706,7 @@ def main():
         roofline=dict(bound='mfma', achieved=b['flops'] / b['t_gemm'] / 1e12, peak=PEAK['bf16'], unit='TFLOP/s',
                       frac=b['flops'] / b['t_gemm'] / 1e12 / PEAK['bf16'], traffic=measured_traffic(),
                       kernel='gemm_peaks (bf16, fused |z|^2 + window max)', launches=b['n_launches'],
+                      algorithmic_bytes_per_launch=b['zt_bytes_per_launch'],
                       avg_launch_ms=b['t_gemm'] / max(b['n_launches'], 1) * 1e3,
                       gemm_share_of_step=b['t_gemm'] / a.steps / (b['ms_per_step'] / 1e3)),
         hbm_kernels=b['hbm'],

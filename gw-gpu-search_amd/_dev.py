@@ -174,6 +174,7 @@ class Meter:
     def __init__(self):
         self.on = False
         self.rec = {}
+        self.notes = {}
 
     def start(self):
         if not self.on or torch.cuda.is_current_stream_capturing():
@@ -189,8 +190,15 @@ class Meter:
         e1.record()
         self.rec.setdefault(name, []).append((e0, e1, float(work)))
 
+    def note(self, name, work):
+        """A quantity beside the timed launches (no events), e.g. the bytes a GEMM launch must read."""
+        if self.on and not torch.cuda.is_current_stream_capturing():
+            n, w = self.notes.get(name, (0, 0.0))
+            self.notes[name] = (n + 1, w + float(work))
+
     def reset(self):
         self.rec = {}
+        self.notes = {}
 
     def stats(self):
         """name -> (launches, kernel seconds, work)."""

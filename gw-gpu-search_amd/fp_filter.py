@@ -902,6 +902,8 @@ class Filter:
                                      geo['per'], o.gemm_cfg, vw, iw, rows2)
         # algorithmic flops: 4 n K_real Tn with K_real = 2 r (padding not counted)
         _dev.METER.stop('gemm_' + o.precision, e0, 4.0 * n * (2 * tb.r) * float(sum(geo['Tns'])))
+        # the operand Zt read once: the launch's algorithmic HBM bytes
+        _dev.METER.note('gemm_' + o.precision + '_zt_bytes', float(sum(geo['Tns'])) * K * Zt.element_size())
         if pk is not None:
             e0 = _dev.METER.start()
             self._prekeep.launch(vw, iw, cut, pk)
