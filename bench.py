@@ -614,6 +614,9 @@ def main():
                     help='interp: the chirps at coarse nodes, interpolated (the reference\'s --templates interp)')
     ap.add_argument('--exact-stage', choices=['dense', 'regen'], default='dense',
                     help='regen: the exact stage regenerates the interpolated templates instead of reading H')
+    ap.add_argument('--alt-second-pass', type=float, default=0.08, help='with --alternates: local-maximum lists in '
+                    'a second GEMM pass over the windows with exact-band candidates while those are under this '
+                    'fraction of a batch\'s windows (0: always in the first pass)')
     ap.add_argument('--segment-batch', type=int, default=32, help='at most this many segments per basis-transform '
                     'batch and GEMM launch')
     ap.add_argument('--batch-samples', type=int, default=1 << 22, help='and at most this many analysed samples')
@@ -659,6 +662,7 @@ def main():
     opts = {p: fp_filter.Options(tile=a.tile, precision=p, gemm_cfg=cfg, self_check=False, refine=a.refine,
                                  alternates=a.alternates, segment_batch=a.segment_batch,
                                  batch_samples=a.batch_samples, overlap=bool(a.overlap),
+                                 alt_second_pass=a.alt_second_pass,
                                  depth=a.depth, templates=a.templates, exact_stage=a.exact_stage)
             for p in ('bf16', 'fp32', 'fp8')}
     # the reference's rule (exact values at the rebuilt argmax), for the trigger comparison beside the refined one

@@ -148,7 +148,7 @@ __device__ __forceinline__ void scan_tile(const f32x16 (&cr)[4], const f32x16 (&
         for (int r = 0; r < 16; r++) {
             const int t = tb0 + ts * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
             const float re = cr[ts][r], im = ci[ts][r];
-            const float p = re * re + im * im;
+            const float p = __builtin_fmaf(re, re, im * im);     // one rounding order in every epilogue
             if (t < Tn && p > best) { best = p; bi = t; }
         }
     }
@@ -350,7 +350,7 @@ __device__ __forceinline__ void epilogue(const Job& J, const GemmItem& it, const
             for (int r = 0; r < 16; r++) {
                 const int t = tt * BT + ts * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
                 const float re = cr[ts][r], im = ci[ts][r];
-                const float pp = re * re + im * im;
+                const float pp = __builtin_fmaf(re, re, im * im);  // as scan_tile: the same window maxima
                 if (t < it.Tn && pp > best) { best = pp; bi = t; }
                 cr[ts][r] = t < it.Tn ? pp : -1.f;
                 tmax = fmaxf(tmax, cr[ts][r]);

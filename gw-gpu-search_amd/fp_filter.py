@@ -80,6 +80,12 @@ class Options:
                                            # the largest exact |z| kept (0 to 3)
     alt_margin: float = 0.01
     alt_sep: int = 10                      # local maxima closer than alt_sep + 1 samples count once (the larger)
+    alt_second_pass: float = 0.08          # with alternates: the GEMM keeps only window maxima, and a second GEMM
+                                           # pass over the windows holding exact-band candidates makes their
+                                           # local-maximum lists, while such windows are under this fraction of a
+                                           # batch's (measured on the batches decoded so far); above it the lists
+                                           # are kept in the first pass; 0: always in the first pass (the lists are
+                                           # the same either way)
     consolidate: object = None             # consolidate.KeepLoudest: consolidate every dataset's peaks after each
                                            # tile (None: keep every peak)
     pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
@@ -449,6 +455,7 @@ class Filter:
         self._cons_pool = None             # one worker thread for the consolidations (fp_filter.py:623, 888)
         self._cons_jobs = []               # (Triggers, plan index, dataset index, parts covered, future)
         self._ysc = None                   # bf16 / fp8 forms: the transform's fp16 intermediate scale per segment
+        self._hot = None                   # running fraction of a batch's windows holding exact-band candidates
         self._y16_update()
 
     def _nsets(self):
@@ -841,6 +848,10 @@ class Filter:
         LM = fp_gemm.LM
         lists = o.refine > 0 and o.alternates > 0
         assert 0 <= o.alternates < LM
+        # the lists in a second pass (decode_batch) while the candidates' windows are sparse
+        second = lists and bufs is None and o.alt_second_pass > 0 and (self._hot is None or
+                                                                       self._hot < o.alt_second_pass)
+        lists = lists and not second
         if bufs is None:
             Zt = Ar = Ai = vw = iw = sel = None
             if lists:
@@ -922,7 +933,7 @@ class Filter:
                       samp.data_ptr(), val.data_ptr(), cnt.data_ptr(), w, nwsum, self.cw, xcut, ratio,
                       max(int(o.refine), int(o.alt_sep)), na, alt.data_ptr(), _dev.stream())
         t2 = self._sync(); self.T['gemm_s'] += t2 - t1
-        return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt), alt=alt)
+        return dict(Zt=Zt, Ar=Ar, Ai=Ai, rows2=rows2, vw=vw, iw=iw, sel=(idx, samp, val, cnt), alt=alt, second=second)
 
     def launch_batch(self, tb: TileBasis, segs, norm, tile_host, key=None, ordinal=0, pi=0):
         """Stage A of one tile against a batch of segments, enqueued only: the basis series of every segment
@@ -953,7 +964,7 @@ class Filter:
         sel = peak_kernel.Selection(idx, samp, val, cnt, _dev.mark(), extra=dev.get('alt'))
         return dict(tb=tb, segs=segs, Zt=dev['Zt'], Ar=dev['Ar'], Ai=dev['Ai'], rows2=dev['rows2'],
                     nwsum=geo['nwsum'], col0=geo['col0'], row0=geo['row0'], K=geo['K'], sel=sel,
-                    tile_host=tile_host)
+                    tile_host=tile_host, second=dev.get('second', False), Tns=geo['Tns'], per=geo['per'])
 
     def rebind(self, datasets):
         """New data with the same segmentation: D and IP are copied into this Filter's tensors (their addresses are
@@ -981,6 +992,8 @@ class Filter:
         g, il, val = res[:3]
         # (candidates, LM - 1) window offsets of each candidate's alternates, -1 past the last
         offa = res[3] if len(res) > 3 else None
+        if st.get('second'):
+            offa = self._second_pass(st, g, val)
         st['pend'] = []
         self.T['n_candidates'] += len(g)
         if len(g) == 0:
@@ -1054,6 +1067,61 @@ class Filter:
         # the batch's device buffers stay referenced by st until collect_batch
         self.T['rebuilt_z_s'] += time.perf_counter() - t0
         return st
+
+    def _second_pass(self, st, g, val):
+        """The local-maximum lists of the windows that hold exact-band candidates, for a batch whose GEMM kept only
+        window maxima: the same GEMM kernel in list mode over just those windows (one work item each, every template
+        block; the same tiles and arithmetic as a first-pass list, so the same lists), then the candidates'
+        alternates. On the climb stream after the batch's own selection, so that it waits for none of the batches
+        launched since. Updates the running fraction of hot windows. Returns the alternates (candidates, na)."""
+        o = self.opts
+        na = o.alternates
+        nwsum = st['nwsum']
+        xcut = np.float32((o.exact_snr * (1 - o.margin)) ** 2)
+        cols = np.unique(g[val >= xcut] % nwsum) if len(g) else np.zeros(0, np.int64)
+        frac = len(cols) / max(nwsum, 1)
+        self._hot = frac if self._hot is None else 0.5 * (self._hot + frac)
+        self.T['second_pass_windows'] = self.T.get('second_pass_windows', 0) + len(cols)
+        if len(g) == 0:
+            return np.zeros((0, na), np.int32)
+        if len(cols) == 0:
+            return np.full((len(g), na), -1, np.int32)
+        tb, K, per = st['tb'], st['K'], st['per']
+        n = len(tb.ids)
+        col0, row0, Tns = st['col0'], st['row0'], st['Tns']
+        si = np.searchsorted(col0, cols, side='right') - 1
+        w = cols - col0[si]
+        tab = np.zeros(len(cols), dtype=fp_gemm.ITEM)
+        tab['zrow0'] = row0[si]
+        tab['t_first'] = w * per
+        tab['t_end'] = (w + 1) * per
+        tab['col0'] = col0[si]
+        tab['seg'] = si
+        tab['Tn'] = np.asarray(Tns)[si]
+        sel = st['sel']
+        LM = fp_gemm.LM
+        cs = _dev.climb_stream()
+        cs.wait_event(sel.after)
+        with torch.cuda.stream(cs):
+            items = (_dev.to_dev(tab.view(np.uint8)), len(cols))
+            vl = torch.empty((n, nwsum * LM), dtype=torch.float32, device='cuda')
+            il = torch.empty((n, nwsum * LM), dtype=torch.int32, device='cuda')
+            vw2 = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')     # the window maxima again (unused)
+            iw2 = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
+            for t in (st['Zt'], st['Ar'], st['Ai'], st['rows2'], sel.idx, sel.samp, sel.val):
+                t.record_stream(cs)
+            ratio = float(np.float32((1 - o.alt_margin) ** 2))
+            cut = candidate_cut(o.snr_threshold, o.margin)
+            fp_gemm.gemm_peaks_multi_lm(self.code, st['Zt'], K, st['Ar'], st['Ai'], K, n, K, items, True, per,
+                                        o.alt_sep, o.gemm_cfg, vw2, iw2, vl, il, st['rows2'],
+                                        gate=ratio * (1 - 1e-5), floor=float(cut) * (1 - 1e-5))
+            alt = torch.empty((len(g), na), dtype=torch.int32, device='cuda')
+            cnt = torch.tensor([len(g)], dtype=torch.int32).pin_memory().to('cuda', non_blocking=True)
+            _lib.call('gwgs_window_alternates', vl.data_ptr(), il.data_ptr(), vl.stride(0), LM, sel.idx.data_ptr(),
+                      sel.samp.data_ptr(), sel.val.data_ptr(), cnt.data_ptr(), len(g), nwsum, self.cw, float(xcut),
+                      ratio, max(int(o.refine), int(o.alt_sep)), na, alt.data_ptr(), _dev.stream())
+            out, = _dev.Fetch(alt, here=True).get()
+        return out
 
     def _sg_values(self, locs, m, s, tid, samples, snr, chi, dof, fpeak):
         """fp_filter.py:854-883: the sine-Gaussian chi-squared of exact peaks with newsnr >= the threshold whose

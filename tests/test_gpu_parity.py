@@ -173,6 +173,31 @@ def test_pipelined_equals_synchronised(dense_c1):
                 np.testing.assert_array_equal(a[k], b[k], err_msg=f'{rnd}: {k}')
 
 
+@pytest.mark.parametrize('precision', ['bf16', 'fp32'])
+def test_second_pass_lists_equal_first_pass(dense_c1, precision):
+    """Alternates from local-maximum lists made by a second GEMM pass over the windows that hold exact-band
+    candidates (Options.alt_second_pass above every batch's fraction) equal those kept by the first pass
+    (alt_second_pass 0): the same peak-file fields, with alternates evaluated and second-pass windows run."""
+    import fp_filter
+    bank, ds, _ = dense_c1
+    base = dict(tile=32, precision=precision, snr_threshold=4.0, exact_snr=5.5, refine=3, alternates=3,
+                alt_margin=0.05, segment_batch=1, self_check=False)
+    IPall, sw, isw = fp_filter.svd_metric([ds])
+    plan = fp_filter.BankPlan(bank.m1, bank.m2, None, ds.flen, ds.df, fp_filter.Options(**base), sw, isw)
+    out = {}
+    for name, frac in (('first', 0.0), ('second', 1.0)):
+        f = fp_filter.Filter([ds], fp_filter.Options(**base, alt_second_pass=frac))
+        out[name] = (f.run(plan)[0].to_dict(), dict(f.T))
+        _say(name, {k: out[name][1].get(k) for k in ('n_alternates', 'second_pass_windows', 'n_exact_candidates')})
+    a, b = out['first'][0], out['second'][0]
+    assert len(a['snr']) > 50
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    assert out['first'][1].get('n_alternates', 0) > 0 and out['first'][1].get('second_pass_windows', 0) == 0
+    assert out['second'][1].get('second_pass_windows', 0) > 0
+    assert out['second'][1]['n_alternates'] == out['first'][1]['n_alternates']
+
+
 def _production_tile(bank, n_tile, target_mc):
     """The 512-template chirp-time tile holding the template closest in chirp mass to target_mc."""
     import synth
