@@ -603,6 +603,8 @@ def main():
     ap.add_argument('--fp32-slices', type=int, default=2, help='slices timed in the FP32 (and FP8) forms and compared '
                     'with the BF16 form')
     ap.add_argument('--fp8-steps', type=int, default=1, help='0: no FP8 form')
+    ap.add_argument('--fp8-margin', type=float, default=0.08, help='candidate margin of the FP8 form')
+    ap.add_argument('--fp8-alt-margin', type=float, default=0.04, help='alternates\' margin of the FP8 form')
     ap.add_argument('--tile', type=int, default=512)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-check', action='store_true', help='skip the float64 direct-filter SNR check')
@@ -614,9 +616,10 @@ def main():
                     help='interp: the chirps at coarse nodes, interpolated (the reference\'s --templates interp)')
     ap.add_argument('--exact-stage', choices=['dense', 'regen'], default='dense',
                     help='regen: the exact stage regenerates the interpolated templates instead of reading H')
-    ap.add_argument('--alt-second-pass', type=float, default=0.08, help='with --alternates: local-maximum lists in '
+    ap.add_argument('--alt-second-pass', type=float, default=None, help='with --alternates: local-maximum lists in '
                     'a second GEMM pass over the windows with exact-band candidates while those are under this '
-                    'fraction of a batch\'s windows (0: always in the first pass)')
+                    'fraction of a batch\'s windows (0: always in the first pass; default: by precision, '
+                    'fp_filter.SECOND_PASS_DEFAULT)')
     ap.add_argument('--segment-batch', type=int, default=32, help='at most this many segments per basis-transform '
                     'batch and GEMM launch')
     ap.add_argument('--batch-samples', type=int, default=1 << 22, help='and at most this many analysed samples')
@@ -665,6 +668,10 @@ def main():
                                  alt_second_pass=a.alt_second_pass,
                                  depth=a.depth, templates=a.templates, exact_stage=a.exact_stage)
             for p in ('bf16', 'fp32', 'fp8')}
+    # e4m3's 3-bit mantissa moves rebuilt values by a few percent: the fp8 form widens the candidate margin (the
+    # reference's --margin, whose role is to absorb the compressed filter's error) and the alternates' margin
+    opts['fp8'] = fp_filter.Options(**{**opts['fp8'].__dict__, 'margin': a.fp8_margin,
+                                       'alt_margin': a.fp8_alt_margin})
     # the reference's rule (exact values at the rebuilt argmax), for the trigger comparison beside the refined one
     for p in ('bf16', 'fp32'):
         opts[p + '_r0'] = fp_filter.Options(**{**opts[p].__dict__, 'refine': 0})

@@ -41,6 +41,9 @@ SIGNATURES = {
     'gwgs_select_peaks': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p, _p],
     'gwgs_select_peaks_noalloc': [_p, _p, _l, _f, _p, _p, _p, _i, _p, _p],
     'gwgs_rebuilt_z': [_i, _p, _l, _p, _p, _l, _i, _p, _p, _p, _i, _p, _p, _p],
+    'gwgs_gemm_peaks_multi_lm_dyn': [_i, _i, _p, _l, _p, _p, _l, _i, _i, _p, _p, _i, _i, _i, _i, _f, _f, _p, _p, _l,
+                                     _p, _p, _l, _p, _p],
+    'gwgs_hot_items': [_p, _p, _p, _i, _i, _f, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_window_alternates': [_p, _p, _l, _i, _p, _p, _p, _p, _i, _i, _i, _f, _f, _i, _i, _p, _p],
     'gwgs_chisq_points': [_p, _l, _p, _ll, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p],
     'gwgs_chisq_points_workspace': [_i, _i],

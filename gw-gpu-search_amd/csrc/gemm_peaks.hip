@@ -23,6 +23,7 @@
 // Kernels: register-resident template operands for K <= 256 (bf16) / 128 (fp32), Zt tiles double buffered through
 // LDS; and a K-loop kernel (both operands staged through LDS in K chunks) for any K and for fp8 e4m3 operands.
 #include "common.h"
+#include <algorithm>
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -111,6 +112,7 @@ __device__ __forceinline__ uint4 arot_fp8(const uint4 v)
 
 struct Job {
     const GemmItem* items;   // null: one segment at row 0, items derived from tpb, ntile, Tn
+    const int* nitems_dev;   // non-null: the item count in device memory (at most nitems, the grid's size)
     int nitems, nmb, tpb, ntile, Tn, M, per;
     const float* rowscale;
     float* vb;               // mode 2: each window's LM largest separated local maxima of |z|^2 (row scaled, -1
@@ -124,10 +126,17 @@ struct Job {
 __device__ __forceinline__ bool block_item(const Job& J, int& mb, GemmItem& it)
 {
     // the hardware places workgroup b on XCD b % 8: consecutive logical blocks (the template blocks of one item)
-    // land on one XCD
-    const int b = blockIdx.x, n8 = gridDim.x >> 3;
+    // land on one XCD. With the item count in device memory the grid is sized for the maximum, and the mapping
+    // spreads the blocks of the actual count over all eight XCDs (the rest return).
+    const int b = blockIdx.x;
+    int n8 = gridDim.x >> 3, tot = J.nitems * J.nmb;
+    if (J.nitems_dev) {
+        tot = min(*J.nitems_dev, J.nitems) * J.nmb;
+        n8 = (tot + 7) >> 3;
+        if ((b >> 3) >= n8) return false;
+    }
     const int L = (b & 7) * n8 + (b >> 3);
-    if (L >= J.nitems * J.nmb) return false;
+    if (L >= tot) return false;
     mb = L % J.nmb;
     const int ix = L / J.nmb;
     if (J.items) {
@@ -1203,6 +1212,7 @@ GWGS_API int gwgs_gemm_peaks(int dtype, int variant, const void* Zt, long ldz, c
     int tpb = tiles_per_block > 0 ? tiles_per_block : ((mode & 3) == 1 ? per : 16);
     if ((mode & 3) == 1 && tpb % per) tpb = (tpb / per + 1) * per;
     Job J;
+    J.nitems_dev = nullptr;
     J.items = nullptr; J.nmb = (M + BMT - 1) / BMT; J.tpb = tpb; J.ntile = ntile; J.Tn = Tn; J.M = M; J.per = per;
     J.nitems = (ntile + tpb - 1) / tpb; J.rowscale = rowscale; J.vb = nullptr; J.ib = nullptr; J.ldb = 0;
     J.sep = 0; J.gate = 0.f; J.floor = 0.f;
@@ -1220,6 +1230,7 @@ GWGS_API int gwgs_gemm_peaks_multi(int dtype, int variant, const void* Zt, long 
                ldv > 0, GWGS_EINVAL);
     GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
     Job J;
+    J.nitems_dev = nullptr;
     J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
     J.Tn = 0; J.M = M; J.per = (mode & 3) == 1 ? per : 1; J.rowscale = rowscale; J.vb = nullptr; J.ib = nullptr;
     J.ldb = 0; J.sep = 0; J.gate = 0.f; J.floor = 0.f;
@@ -1244,9 +1255,98 @@ GWGS_API int gwgs_gemm_peaks_multi_lm(int dtype, int variant, const void* Zt, lo
                ldz >= K && lda >= K && ldv > 0 && ldl >= (long)LM * ldv, GWGS_EINVAL);
     GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
     Job J;
+    J.nitems_dev = nullptr;
     J.items = (const GemmItem*)items; J.nitems = nitems; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
     J.Tn = 0; J.M = M; J.per = per; J.rowscale = rowscale; J.vb = VL; J.ib = IL; J.ldb = ldl; J.sep = sep; J.gate = gate; J.floor = floor;
     return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, 2 | (stacked ? 4 : 0), J, V, I, ldv, (hipStream_t)stream);
+}
+
+// gwgs_gemm_peaks_multi_lm over an item table whose length is in device memory (*nitems_dev <= max_items), such as
+// gwgs_hot_items writes: the grid is sized for max_items and blocks past the count return at once.
+GWGS_API int gwgs_gemm_peaks_multi_lm_dyn(int dtype, int variant, const void* Zt, long ldz, const void* Ar,
+                                          const void* Ai, long lda, int M, int K, const void* items,
+                                          const int* nitems_dev, int max_items, int stacked, int per, int sep,
+                                          float gate, float floor, float* V, int* I, long ldv, float* VL, int* IL,
+                                          long ldl, const float* rowscale, void* stream)
+{
+    GWGS_CHECK(Zt && Ar && Ai && V && I && VL && IL && items && nitems_dev && max_items >= 0, GWGS_EINVAL);
+    GWGS_CHECK(M > 0 && K > 0 && (stacked == 0 || stacked == 1) && per > 0 && sep >= 0 && sep < per * BT &&
+               gate >= 0.f && gate <= 1.f && floor >= 0.f &&
+               ldz >= K && lda >= K && ldv > 0 && ldl >= (long)LM * ldv, GWGS_EINVAL);
+    GWGS_CHECK(((uintptr_t)items & 7) == 0, GWGS_EINVAL);
+    Job J;
+    J.nitems_dev = nitems_dev;
+    J.items = (const GemmItem*)items; J.nitems = max_items; J.nmb = (M + BMT - 1) / BMT; J.tpb = 0; J.ntile = 0;
+    J.Tn = 0; J.M = M; J.per = per; J.rowscale = rowscale; J.vb = VL; J.ib = IL; J.ldb = ldl; J.sep = sep; J.gate = gate; J.floor = floor;
+    return launch(dtype, variant, Zt, ldz, Ar, Ai, lda, K, 2 | (stacked ? 4 : 0), J, V, I, ldv, (hipStream_t)stream);
+}
+
+namespace {
+
+// One segment of a batch for gwgs_hot_items: its first Zt row, first window column, analysed samples.
+struct HotSeg {
+    long long zrow0;
+    int col0;
+    int Tn;
+};
+
+__global__ void __launch_bounds__(256) hot_zero_k(int* __restrict__ flags, const int n, int* __restrict__ nitems)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) flags[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nitems = 0;
+}
+
+__global__ void __launch_bounds__(256) hot_mark_k(const int* __restrict__ idx, const float* __restrict__ val,
+                                                  const int* __restrict__ count, const int cap, const int nwsum,
+                                                  const float cut, int* __restrict__ flags)
+{
+    const int n = min(*count, cap);
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256)
+        if (val[c] >= cut) flags[idx[c] % nwsum] = 1;
+}
+
+__global__ void __launch_bounds__(256) hot_items_k(const int* __restrict__ flags, const int nwsum,
+                                                   const HotSeg* __restrict__ segs, const int nseg, const int per,
+                                                   GemmItem* __restrict__ items, int* __restrict__ nitems)
+{
+    for (int col = blockIdx.x * 256 + threadIdx.x; col < nwsum; col += gridDim.x * 256) {
+        if (!flags[col]) continue;
+        int s = 0;
+        while (s + 1 < nseg && segs[s + 1].col0 <= col) s++;
+        GemmItem it;
+        it.zrow0 = segs[s].zrow0;
+        it.t_first = (col - segs[s].col0) * per;
+        it.t_end = it.t_first + per;
+        it.col0 = segs[s].col0;
+        it.seg = s;
+        it.Tn = segs[s].Tn;
+        it.pad = 0;
+        items[atomicAdd(nitems, 1)] = it;
+    }
+}
+
+} // namespace
+
+// The windows holding candidates at or above `cut` as one-window GEMM items (for gwgs_gemm_peaks_multi_lm_dyn): from
+// a compacted candidate list (idx: row-major index into (M, nwsum) window maxima; *count entries, at most cap), flag
+// each candidate window's column, then write one item per flagged column (its segment from segs: nseg rows of
+// {int64 first Zt row, int32 first column, int32 analysed samples}, columns ascending) in any order, and their count
+// to *nitems. flags: nwsum int32 of workspace; items: room for nwsum.
+GWGS_API int gwgs_hot_items(const int* idx, const float* val, const int* count, int cap, int nwsum, float cut,
+                            const void* segs, int nseg, int per, int* flags, void* items, int* nitems, void* stream)
+{
+    GWGS_CHECK(idx && val && count && segs && flags && items && nitems && cap >= 0 && nwsum > 0 && nseg > 0 &&
+               per > 0, GWGS_EINVAL);
+    GWGS_CHECK(((uintptr_t)items & 7) == 0 && ((uintptr_t)segs & 7) == 0, GWGS_EINVAL);
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = std::min((nwsum + 255) / 256, 1024);
+    hipLaunchKernelGGL(hot_zero_k, dim3(nb), dim3(256), 0, s, flags, nwsum, nitems);
+    if (cap > 0)
+        hipLaunchKernelGGL(hot_mark_k, dim3(std::min((cap + 255) / 256, 1024)), dim3(256), 0, s, idx, val, count, cap,
+                           nwsum, cut, flags);
+    hipLaunchKernelGGL(hot_items_k, dim3(nb), dim3(256), 0, s, flags, nwsum, (const HotSeg*)segs, nseg, per,
+                       (GemmItem*)items, nitems);
+    return gwgs_status();
 }
 
 GWGS_API int gwgs_gemm_lm_count(void) { return LM; }

@@ -49,6 +49,10 @@ SERIES_SEG = 16       # segments per launch of the fused basis transform (its ke
 BN = fp_gemm.BN
 PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp8': 2, 'fp16': 3}
 OPERAND_DTYPES = {0: torch.float32, 1: torch.bfloat16, 2: torch.uint8, 3: torch.float16}
+# Options.alt_second_pass by precision: where the list epilogue costs a large share of the GEMM (fp32: +25%, fp8:
+# +50% over the window-max epilogue on MI355X) a second pass over the hot windows pays while they are a small
+# fraction; for bf16 / fp16 (+9%) the first pass keeps the lists.
+SECOND_PASS_DEFAULT = {'fp32': 0.08, 'fp8': 0.2, 'bf16': 0.0, 'fp16': 0.0}
 
 
 @dataclass
@@ -80,12 +84,12 @@ class Options:
                                            # the largest exact |z| kept (0 to 3)
     alt_margin: float = 0.01
     alt_sep: int = 10                      # local maxima closer than alt_sep + 1 samples count once (the larger)
-    alt_second_pass: float = 0.08          # with alternates: the GEMM keeps only window maxima, and a second GEMM
+    alt_second_pass: float = None          # with alternates: the GEMM keeps only window maxima, and a second GEMM
                                            # pass over the windows holding exact-band candidates makes their
                                            # local-maximum lists, while such windows are under this fraction of a
                                            # batch's (measured on the batches decoded so far); above it the lists
                                            # are kept in the first pass; 0: always in the first pass (the lists are
-                                           # the same either way)
+                                           # the same either way); None: SECOND_PASS_DEFAULT by precision
     consolidate: object = None             # consolidate.KeepLoudest: consolidate every dataset's peaks after each
                                            # tile (None: keep every peak)
     pre_consolidate: bool = False          # also drop on the device the candidates consolidation would drop
@@ -782,7 +786,11 @@ class Filter:
         items = fp_gemm.items_table([(int(row0[i]), Tns[i] // BN, int(col0[i]), i, Tns[i]) for i in range(len(segs))],
                                     fp_gemm.tiles_per_item(self.opts.gemm_cfg, per))
         gsl = _dev.to_dev(np.array([gs for (_, _, gs, _, _) in segs], np.int64))
-        return dict(Tns=Tns, row0=row0, nwsum=sum(nws), col0=col0, items=items, gsl=gsl, K=2 * tb.rp, per=per)
+        o = self.opts
+        hseg = (fp_gemm.hot_segments(row0, col0, Tns) if o.refine > 0 and o.alternates > 0 and self._second_frac() > 0
+                else None)
+        return dict(Tns=Tns, row0=row0, nwsum=sum(nws), col0=col0, items=items, gsl=gsl, K=2 * tb.rp, per=per,
+                    hseg=hseg)
 
     def _split(self, segs):
         """Consecutive batches [b0, b1) of the segment list: up to opts.segment_batch segments and opts.batch_samples
@@ -848,9 +856,9 @@ class Filter:
         LM = fp_gemm.LM
         lists = o.refine > 0 and o.alternates > 0
         assert 0 <= o.alternates < LM
-        # the lists in a second pass (decode_batch) while the candidates' windows are sparse
-        second = lists and bufs is None and o.alt_second_pass > 0 and (self._hot is None or
-                                                                       self._hot < o.alt_second_pass)
+        # the lists in a second pass over the candidates' windows (below) while those are sparse
+        second = (lists and bufs is None and geo['hseg'] is not None and
+                  (self._hot is None or self._hot < self._second_frac()))
         lists = lists and not second
         if bufs is None:
             Zt = Ar = Ai = vw = iw = sel = None
@@ -922,13 +930,30 @@ class Filter:
             _dev.METER.stop('pre_keep', e0, 4.0 * vw.numel())
         idx, samp, val, cnt = peak_kernel.select_launch(vw, iw, float(cut), sel)
         alt = None
-        if lists:
-            # every exact-band candidate's alternates: its window's other local maxima within the margin of its
-            # window maximum, farther than the re-centring half width from it
+        if lists or second:
             na, w = o.alternates, idx.numel()
             alt = (bufs['alt'][:w * na] if bufs is not None else
                    torch.empty(w * na, dtype=torch.int32, device='cuda')).view(w, na)
             xcut = float(np.float32((o.exact_snr * (1 - o.margin)) ** 2))
+            if second:
+                # the local-maximum lists of only the windows that hold exact-band candidates: the same GEMM kernel
+                # in list mode over one-window items built on the device from the candidate list (the same tiles
+                # and arithmetic as a first-pass list, so the same lists); the window maxima it writes again go to
+                # scratch
+                e0 = _dev.METER.start()
+                flags = torch.empty(nwsum, dtype=torch.int32, device='cuda')
+                items2 = torch.empty(nwsum * fp_gemm.ITEM.itemsize, dtype=torch.uint8, device='cuda')
+                nit = torch.empty(1, dtype=torch.int32, device='cuda')
+                fp_gemm.hot_items(idx, val, cnt, nwsum, xcut, geo['hseg'], geo['per'], flags, items2, nit)
+                vl = torch.empty((n, nwsum * LM), dtype=torch.float32, device='cuda')
+                il = torch.empty((n, nwsum * LM), dtype=torch.int32, device='cuda')
+                fp_gemm.gemm_peaks_multi_lm_dyn(self.code, Zt, K, Ar, Ai, K, n, K, items2, nit, nwsum, True,
+                                                geo['per'], o.alt_sep, o.gemm_cfg, torch.empty_like(vw),
+                                                torch.empty_like(iw), vl, il, rows2, gate=ratio * (1 - 1e-5),
+                                                floor=float(cut) * (1 - 1e-5))
+                _dev.METER.stop('second_pass', e0, 0.0)
+            # every exact-band candidate's alternates: its window's other local maxima within the margin of its
+            # window maximum, farther than the re-centring half width from it
             _lib.call('gwgs_window_alternates', vl.data_ptr(), il.data_ptr(), vl.stride(0), LM, idx.data_ptr(),
                       samp.data_ptr(), val.data_ptr(), cnt.data_ptr(), w, nwsum, self.cw, xcut, ratio,
                       max(int(o.refine), int(o.alt_sep)), na, alt.data_ptr(), _dev.stream())
@@ -993,7 +1018,7 @@ class Filter:
         # (candidates, LM - 1) window offsets of each candidate's alternates, -1 past the last
         offa = res[3] if len(res) > 3 else None
         if st.get('second'):
-            offa = self._second_pass(st, g, val)
+            self._hot_update(st, g, val)
         st['pend'] = []
         self.T['n_candidates'] += len(g)
         if len(g) == 0:
@@ -1068,60 +1093,21 @@ class Filter:
         self.T['rebuilt_z_s'] += time.perf_counter() - t0
         return st
 
-    def _second_pass(self, st, g, val):
-        """The local-maximum lists of the windows that hold exact-band candidates, for a batch whose GEMM kept only
-        window maxima: the same GEMM kernel in list mode over just those windows (one work item each, every template
-        block; the same tiles and arithmetic as a first-pass list, so the same lists), then the candidates'
-        alternates. On the climb stream after the batch's own selection, so that it waits for none of the batches
-        launched since. Updates the running fraction of hot windows. Returns the alternates (candidates, na)."""
+    def _second_frac(self):
+        """The hot-window fraction below which the local-maximum lists come from a second GEMM pass."""
+        f = self.opts.alt_second_pass
+        return SECOND_PASS_DEFAULT[self.opts.precision] if f is None else f
+
+    def _hot_update(self, st, g, val):
+        """The running fraction of windows holding exact-band candidates, for a batch whose local-maximum lists came
+        from the second pass: while it stays below _second_frac() the next batches take that pass too, else
+        the lists return to the first GEMM."""
         o = self.opts
-        na = o.alternates
-        nwsum = st['nwsum']
         xcut = np.float32((o.exact_snr * (1 - o.margin)) ** 2)
-        cols = np.unique(g[val >= xcut] % nwsum) if len(g) else np.zeros(0, np.int64)
-        frac = len(cols) / max(nwsum, 1)
+        cols = np.unique(g[val >= xcut] % st['nwsum']) if len(g) else np.zeros(0, np.int64)
+        frac = len(cols) / max(st['nwsum'], 1)
         self._hot = frac if self._hot is None else 0.5 * (self._hot + frac)
         self.T['second_pass_windows'] = self.T.get('second_pass_windows', 0) + len(cols)
-        if len(g) == 0:
-            return np.zeros((0, na), np.int32)
-        if len(cols) == 0:
-            return np.full((len(g), na), -1, np.int32)
-        tb, K, per = st['tb'], st['K'], st['per']
-        n = len(tb.ids)
-        col0, row0, Tns = st['col0'], st['row0'], st['Tns']
-        si = np.searchsorted(col0, cols, side='right') - 1
-        w = cols - col0[si]
-        tab = np.zeros(len(cols), dtype=fp_gemm.ITEM)
-        tab['zrow0'] = row0[si]
-        tab['t_first'] = w * per
-        tab['t_end'] = (w + 1) * per
-        tab['col0'] = col0[si]
-        tab['seg'] = si
-        tab['Tn'] = np.asarray(Tns)[si]
-        sel = st['sel']
-        LM = fp_gemm.LM
-        cs = _dev.climb_stream()
-        cs.wait_event(sel.after)
-        with torch.cuda.stream(cs):
-            items = (_dev.to_dev(tab.view(np.uint8)), len(cols))
-            vl = torch.empty((n, nwsum * LM), dtype=torch.float32, device='cuda')
-            il = torch.empty((n, nwsum * LM), dtype=torch.int32, device='cuda')
-            vw2 = torch.empty((n, nwsum), dtype=torch.float32, device='cuda')     # the window maxima again (unused)
-            iw2 = torch.empty((n, nwsum), dtype=torch.int32, device='cuda')
-            for t in (st['Zt'], st['Ar'], st['Ai'], st['rows2'], sel.idx, sel.samp, sel.val):
-                t.record_stream(cs)
-            ratio = float(np.float32((1 - o.alt_margin) ** 2))
-            cut = candidate_cut(o.snr_threshold, o.margin)
-            fp_gemm.gemm_peaks_multi_lm(self.code, st['Zt'], K, st['Ar'], st['Ai'], K, n, K, items, True, per,
-                                        o.alt_sep, o.gemm_cfg, vw2, iw2, vl, il, st['rows2'],
-                                        gate=ratio * (1 - 1e-5), floor=float(cut) * (1 - 1e-5))
-            alt = torch.empty((len(g), na), dtype=torch.int32, device='cuda')
-            cnt = torch.tensor([len(g)], dtype=torch.int32).pin_memory().to('cuda', non_blocking=True)
-            _lib.call('gwgs_window_alternates', vl.data_ptr(), il.data_ptr(), vl.stride(0), LM, sel.idx.data_ptr(),
-                      sel.samp.data_ptr(), sel.val.data_ptr(), cnt.data_ptr(), len(g), nwsum, self.cw, float(xcut),
-                      ratio, max(int(o.refine), int(o.alt_sep)), na, alt.data_ptr(), _dev.stream())
-            out, = _dev.Fetch(alt, here=True).get()
-        return out
 
     def _sg_values(self, locs, m, s, tid, samples, snr, chi, dof, fpeak):
         """fp_filter.py:854-883: the sine-Gaussian chi-squared of exact peaks with newsnr >= the threshold whose

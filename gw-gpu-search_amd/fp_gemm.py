@@ -131,6 +131,37 @@ def gemm_peaks_multi_lm(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, stacked, 
               _dev.ptr(rowscale), _dev.stream())
 
 
+def gemm_peaks_multi_lm_dyn(dtype_code, Zt, ldz, Ar, Ai, lda, M, K, items, nitems_dev, max_items, stacked, per,
+                            sep, cfg, V, I, VL, IL, rowscale=None, gate=0.0, floor=0.0):
+    """gemm_peaks_multi_lm over an item table whose length is the int32 nitems_dev on the device (at most
+    max_items), as hot_items writes it."""
+    _, variant = _cfg(cfg)
+    _lib.call('gwgs_gemm_peaks_multi_lm_dyn', dtype_code, variant, Zt.data_ptr(), ldz, Ar.data_ptr(), Ai.data_ptr(),
+              lda, M, K, items.data_ptr(), nitems_dev.data_ptr(), int(max_items), int(bool(stacked)), per, int(sep),
+              float(gate), float(floor), V.data_ptr(), I.data_ptr(), V.stride(0), VL.data_ptr(), IL.data_ptr(),
+              VL.stride(0), _dev.ptr(rowscale), _dev.stream())
+
+
+# one segment of a batch for hot_items (csrc/gemm_peaks.hip HotSeg)
+HOTSEG = np.dtype([('zrow0', '<i8'), ('col0', '<i4'), ('Tn', '<i4')])
+
+
+def hot_segments(row0, col0, Tns):
+    """Device table of a batch's segments for hot_items: first Zt row, first window column, analysed samples."""
+    tab = np.zeros(len(Tns), dtype=HOTSEG)
+    tab['zrow0'], tab['col0'], tab['Tn'] = row0[:len(Tns)], col0[:len(Tns)], Tns
+    return _dev.to_dev(tab.view(np.uint8))
+
+
+def hot_items(idx, val, count, nwsum, cut, segs, per, flags, items, nitems):
+    """One-window GEMM items (ITEM rows in items, their count in the int32 nitems) for the windows holding a
+    candidate of a compacted list (idx, val, count as peak_kernel.select_launch writes them) with val >= cut. segs:
+    hot_segments of the batch; flags: nwsum int32 of workspace; items: room for nwsum ITEM rows."""
+    _lib.call('gwgs_hot_items', idx.data_ptr(), val.data_ptr(), count.data_ptr(), idx.numel(), nwsum, float(cut),
+              segs.data_ptr(), segs.numel() // HOTSEG.itemsize, per, flags.data_ptr(), items.data_ptr(),
+              nitems.data_ptr(), _dev.stream())
+
+
 def fused_peaks(Are, Aim, Z, M, Tn, cfg=None, zt=False, rowscale=None):
     """Are, Aim (M x K), Z (K x Tn) [or Zt (>= Tn x K) with zt=True], float32, bfloat16, float16 (the reference's
     operand type, on the fp16 MFMA) or float8_e4m3fn. Returns V, I: max |z|^2 and its sample per BN = 128 samples, (M, ceil(Tn/BN)); columns >= Tn are

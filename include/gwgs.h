@@ -63,6 +63,21 @@ int gwgs_gemm_peaks_multi_lm(int dtype, int variant, const void* Zt, long ldz, c
                              long lda, int M, int K, const void* items, int nitems, int stacked, int per, int sep,
                              float gate, float floor, float* V, int* I, long ldv, float* VL, int* IL, long ldl,
                              const float* rowscale, void* stream);
+/* No reference counterpart (the reference keeps no local-maximum lists): gwgs_gemm_peaks_multi_lm over an item
+ * table whose length is the int32 *nitems_dev in device memory, at most max_items (the grid is sized for max_items;
+ * blocks past the count return at once), as gwgs_hot_items writes it, so that a second list pass over the windows
+ * holding candidates needs no host round trip. */
+int gwgs_gemm_peaks_multi_lm_dyn(int dtype, int variant, const void* Zt, long ldz, const void* Ar, const void* Ai,
+                                 long lda, int M, int K, const void* items, const int* nitems_dev, int max_items,
+                                 int stacked, int per, int sep, float gate, float floor, float* V, int* I, long ldv,
+                                 float* VL, int* IL, long ldl, const float* rowscale, void* stream);
+/* No reference counterpart: the windows holding a candidate with val >= cut of a compacted list (idx / val / count
+ * as gwgs_select_peaks wrote them, capacity cap, idx a row-major index into (M, nwsum) window maxima) as one-window
+ * GEMM items (GemmItem rows, any order; their count to *nitems). segs: nseg rows of {int64 first Zt row, int32 first
+ * window column, int32 analysed samples}, columns ascending; per: 128-sample tiles per window; flags: nwsum int32
+ * of workspace; items: room for nwsum rows (8-byte aligned). */
+int gwgs_hot_items(const int* idx, const float* val, const int* count, int cap, int nwsum, float cut,
+                   const void* segs, int nseg, int per, int* flags, void* items, int* nitems, void* stream);
 int gwgs_gemm_lm_count(void);
 int gwgs_gemm_item_bytes(void);
 /* largest K the GEMM accepts for dtype (0 if the dtype is unsupported) */

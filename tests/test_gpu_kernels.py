@@ -688,6 +688,33 @@ def test_local_maxima_lists(gpu, code, stacked, r, sep, gate):
                 if t >= 0 and v >= np.float32(ratio) * v0[c] and abs(t - s0[c]) > R and len(exp) < na:
                     exp.append(t % cw)
         assert list(a[c]) == exp + [-1] * (na - len(exp)), c
+    # the second pass: one-window items for the windows holding candidates at or above xcut, built on the device,
+    # and the list-mode launch over them with the item count in device memory list those windows as above
+    hseg = fp_gemm.hot_segments(row0, col0, Tns)
+    flags = torch.empty(nwsum, dtype=torch.int32, device='cuda')
+    it2 = torch.empty(nwsum * fp_gemm.ITEM.itemsize, dtype=torch.uint8, device='cuda')
+    nit = torch.full((1,), 99, dtype=torch.int32, device='cuda')
+    hcut = np.float32(np.sort(V.cpu().numpy().max(0))[nwsum - 2])      # the two loudest windows hot
+    fp_gemm.hot_items(idx, val, cnt, nwsum, float(hcut), hseg, per, flags, it2, nit)
+    VL2 = torch.full_like(VL, -7.0); IL2 = torch.full_like(IL, -7)
+    fp_gemm.gemm_peaks_multi_lm_dyn(code, qz, K, qa, ai, K, M, K, it2, nit, nwsum, stacked, per, sep, (2 * per, 0),
+                                    torch.empty_like(V), torch.empty_like(I), VL2, IL2, sc, gate=gate, floor=floor)
+    torch.cuda.synchronize()
+    hot = np.unique(g[v0 >= hcut] % nwsum)
+    k = int(nit.cpu()[0])
+    assert k == len(hot) and 0 < k < nwsum
+    tab = it2.cpu().numpy()[:k * fp_gemm.ITEM.itemsize].view(fp_gemm.ITEM)
+    tab = tab[np.argsort(tab['col0'] * 1_000_000 + tab['t_first'])]
+    si = np.searchsorted(col0, hot, side='right') - 1
+    assert np.array_equal(tab['zrow0'], row0[si]) and np.array_equal(tab['seg'], si)
+    assert np.array_equal(tab['t_first'], (hot - col0[si]) * per)
+    assert np.array_equal(tab['t_end'] - tab['t_first'], np.full(k, per))
+    assert np.array_equal(tab['Tn'], np.asarray(Tns)[si])
+    vl2, il2 = VL2.cpu().numpy().reshape(M, nwsum, LM), IL2.cpu().numpy().reshape(M, nwsum, LM)
+    cold = np.setdiff1d(np.arange(nwsum), hot)
+    assert np.array_equal(vl2[:, hot], vl.reshape(M, nwsum, LM)[:, hot])
+    assert np.array_equal(il2[:, hot], il.reshape(M, nwsum, LM)[:, hot])
+    assert np.all(vl2[:, cold] == -7.0) and np.all(il2[:, cold] == -7)
 
 
 def test_climb_points_vs_float64(gpu):

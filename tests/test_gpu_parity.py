@@ -173,7 +173,7 @@ def test_pipelined_equals_synchronised(dense_c1):
                 np.testing.assert_array_equal(a[k], b[k], err_msg=f'{rnd}: {k}')
 
 
-@pytest.mark.parametrize('precision', ['bf16', 'fp32'])
+@pytest.mark.parametrize('precision', ['bf16', 'fp32', 'fp8', 'fp16'])
 def test_second_pass_lists_equal_first_pass(dense_c1, precision):
     """Alternates from local-maximum lists made by a second GEMM pass over the windows that hold exact-band
     candidates (Options.alt_second_pass above every batch's fraction) equal those kept by the first pass
