@@ -657,8 +657,11 @@ def main():
     if a.gemm_cfg:
         cfg = tuple(int(v) for v in a.gemm_cfg.split(','))
     else:
-        # the autotuner on the filter's shape: a 512-template tile, K = 2 x 416, two 448 s segments of windows
-        cfg, tuned = fp_gemm.pick_cfg(M=512, K=832, Tn=2 * 448 * 2048, dtype=torch.bfloat16)
+        # the autotuner on the filter's shape: a 512-template tile, K = 2 x 416, two 448 s segments of windows, in
+        # the mode the BF16 form's first pass runs (with the local-maximum lists unless a second pass makes them)
+        sp = fp_filter.SECOND_PASS_DEFAULT['bf16'] if a.alt_second_pass is None else a.alt_second_pass
+        cfg, tuned = fp_gemm.pick_cfg(M=512, K=832, Tn=2 * 448 * 2048, dtype=torch.bfloat16,
+                                      lists=a.refine > 0 and a.alternates > 0 and sp == 0)
         log('autotuned GEMM launch', cfg, {k: (round(v * 1e3, 3) if isinstance(v, float) else v)
                                           for k, v in tuned.items()})
     t_tune = time.perf_counter() - t_tune

@@ -536,6 +536,10 @@ def test_autotuner_picks_a_running_config(gpu):
     times = {k: v for k, v in res.items() if isinstance(v, float)}
     assert len(times) == len(res), res
     assert res[str(cfg)] == min(times.values())
+    # in list mode: tiles per item only
+    cfg2, res2 = fp_gemm.pick_cfg(M=96, K=160, Tn=8 * 2048, reps=1, lists=True)
+    assert all(isinstance(v, float) for v in res2.values()), res2
+    assert {eval(k)[1] for k in res2} == {fp_gemm.VARIANTS[-1]} and res2[str(cfg2)] == min(res2.values())
     rng = np.random.default_rng(3)
     M, r, Tn, per = 96, 80, 4 * 2048, 16
     K = 2 * r
