@@ -516,8 +516,10 @@ template <int P, int Q, int NZ, typename YS>
 void launch_cols_nz(const float2* Vc, long ldv, const SegArgs& S, int nseg, long kmin, long kend, int n1, int lgn,
                     int r, void* Y, hipStream_t s)
 {
-    // columns per block: 16 complex64 or 32 fp16 pairs give 128-byte runs per Y row (8 for the 2048-point lines)
-    constexpr int C = Q > 32 ? 8 : sizeof(typename YS::E) == 8 ? 16 : 32;
+    // columns per block: 16 (8 for the 2048-point lines), so that a block is 512 threads and two run on a CU (1,024
+    // threads need more than the 64 registers that two such blocks would leave each): 128-byte runs per Y row for
+    // complex64, 64-byte runs for fp16 pairs
+    constexpr int C = Q > 32 ? 8 : 16;
     hipLaunchKernelGGL((cols_k<P, Q, C, NZ, YS>), dim3((unsigned)(n1 / C), r, nseg), dim3(C * P), 0, s, Vc, ldv, S,
                        kmin, kend, n1, lgn, (typename YS::E*)Y);
 }
