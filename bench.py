@@ -3,7 +3,7 @@
 
 Headline workload (the batched configuration, workloads.batched_device): per detector 200 chunks of 64-1024 s in
 16 s steps (segment gaps), 2 detectors at 2048 Hz of synthetic coloured noise with 5 lines, drawn on the GPU; both
-banks, 4,612 circular + 1,268 eccentric = 5,880 templates; chunks on 512 s and 128 s segment grids. The job is cut
+banks, 4,612 circular + 1,268 eccentric = 5,880 templates; chunks on 512 s, 256 s and 128 s segment grids. The job is cut
 into 20 slices of 10 chunks per detector: one step filters one slice (conditioning of every chunk from its raw strain
 resident in HBM: float64 median-Welch PSD, truncation, whitening; template norms; per tile the basis transforms and
 the fused GEMM + window max; candidate selection; the exact stage with +-3-sample re-centring: edges, chi-squared
@@ -250,18 +250,19 @@ class ProductionWorkload:
 
 class BatchedWorkload:
     """The batched configuration: per detector n_chunks chunks of 64-1024 s in 16 s steps (segment gaps), 2 detectors
-    at 2048 Hz of noise with the production lines drawn on the GPU, grouped on 512 s and 128 s segment grids; both
+    at 2048 Hz of noise with the production lines drawn on the GPU, grouped on segment grids (512, 256 and 128 s); both
     banks. The job is cut into nslice slices of n_chunks / nslice chunks per detector; one step filters one slice, so
     nslice steps filter every chunk once. Warm-up steps filter a separate slice of warm chunks (other lengths and
     noise), whose conditioned data also give each grid's SVD metric. Datasets and trigger lists of a step are
     flattened over grids in grid order."""
 
-    def __init__(self, n_chunks, opts, nslice=20, n_warm=10):
+    def __init__(self, n_chunks, opts, nslice=20, n_warm=10, grids=None):
         import torch
         import batched, workloads
         t0 = time.perf_counter()
-        chunks, x_dev, self.banks = workloads.batched_device(n_chunks)
-        wch, wx, _ = workloads.batched_device(n_warm, seed=97, noise_seed=500000, n_circ=0, n_ecc=0)
+        self.grids = tuple(grids or workloads.GRIDS)
+        chunks, x_dev, self.banks = workloads.batched_device(n_chunks, grids=self.grids)
+        wch, wx, _ = workloads.batched_device(n_warm, seed=97, noise_seed=500000, n_circ=0, n_ecc=0, grids=self.grids)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         self.n_chunks, self.nslice = n_chunks, nslice
@@ -302,10 +303,11 @@ class BatchedWorkload:
 
     def describe(self, tile):
         return ('batched: %d chunks per detector of 64-1024 s in 16 s steps, 2 detectors at 2048 Hz (noise with lines), '
-                'grouped on 512 s and 128 s segment grids; 4,612 circular + 1,268 eccentric SPA templates, tiles of '
+                'grouped on %s s segment grids; 4,612 circular + 1,268 eccentric SPA templates, tiles of '
                 '%d, SVD worst lost power 1e-4, 16 chi-squared bins, SNR threshold 5.5, 1 s windows; BF16 GEMM, FP32 '
                 'exact stage; one step = one of %d slices of %d chunks per detector' %
-                (self.n_chunks, tile, self.nslice, self.n_chunks // self.nslice))
+                (self.n_chunks, ' / '.join(str(g) for g in sorted(self.grids, reverse=True)), tile, self.nslice,
+                 self.n_chunks // self.nslice))
 
     def compare(self, a, b):
         # per bank, the lists of every dataset merged (keys are unique: each dataset has its own end times)
@@ -462,7 +464,7 @@ def run_batched(a, opts, cfg, tuned, t_setup):
     import numpy as np
     import torch
     import _dev
-    wl = BatchedWorkload(a.chunks, opts, nslice=a.slices)
+    wl = BatchedWorkload(a.chunks, opts, nslice=a.slices, grids=[int(g) for g in a.grids.split(',')])
     torch.cuda.synchronize()
     # the search's set-up: autotuner, bank products (templates, SVD) and the SVD metric's conditioning; the
     # synthetic data's generation is not counted
@@ -600,6 +602,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2, help='untimed steps on the warm chunks')
     ap.add_argument('--chunks', type=int, default=200, help='chunks per detector of the batched workload')
     ap.add_argument('--slices', type=int, default=20, help='slices of the batched workload (one per step)')
+    ap.add_argument('--grids', default='512,256,128', help='segment lengths (s) of the batched workload\'s grids: a '
+                    'chunk takes the longest that leaves it a full segment beyond the 64 s of pads')
     ap.add_argument('--fp32-slices', type=int, default=2, help='slices timed in the FP32 (and FP8) forms and compared '
                     'with the BF16 form')
     ap.add_argument('--fp8-steps', type=int, default=1, help='0: no FP8 form')
@@ -705,6 +709,7 @@ def main():
         ms_per_step=b['ms_per_step'], higher_is_better=True, scaling='weak', vs_baseline=None, dtype='bf16',
         data='synthetic (noise drawn on the GPU, seeded)',
         config=dict(workload=wl.describe(a.tile), templates=b['ntemp'], analysed_s=b['analysed_s'],
+                    grids=list(wl.grids),
                     slices=a.slices, tile=a.tile, segment_batch=a.segment_batch, batch_samples=a.batch_samples,
                     ranks=wl.ranks(),
                     setup_s=round(b['setup_s'], 1),

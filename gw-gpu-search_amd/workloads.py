@@ -53,18 +53,25 @@ def production(duration=512, n_circ=4612, n_ecc=1268, n_inj=20, glitch_snr=300.0
     return streams, banks, Conditioning(duration, 48, 16, 16, 8, 63, 16, autogating_threshold=50.0)
 
 
-def chunk_conditioning(L):
-    """Segmenting of one uneven chunk of L seconds (the batched form): 512 s segments when L >= 576 s, 128 s
-    segments from 128 s up, with the production pads (48 s / 16 s); the 16 s / 8 s median-Welch PSD uses as many
-    segments as the chunk holds, up to 63; chunks under 128 s leave nothing to analyse (None)."""
-    if L < 128:
+# segment lengths of the batched form's grids: a chunk takes the longest that leaves it a full segment beyond the pads
+GRIDS = (512, 256, 128)
+
+
+def chunk_conditioning(L, grids=GRIDS):
+    """Segmenting of one uneven chunk of L seconds (the batched form), with the production pads (48 s / 16 s): the
+    longest segment length g of grids with L >= g + 64 s, else the shortest from L >= that length up (512 s segments
+    from 576 s, 256 s from 320 s, 128 s from 128 s with the default grids; a longer segment analyses a larger share
+    of what it transforms: 448 / 512, 192 / 256, 64 / 128). The 16 s / 8 s median-Welch PSD uses as many segments as
+    the chunk holds, up to 63; chunks shorter than the shortest segment leave nothing to analyse (None)."""
+    gs = sorted(grids, reverse=True)
+    if L < gs[-1]:
         return None
-    seg = 512 if L >= 576 else 128
+    seg = next((g for g in gs[:-1] if L >= g + 64), gs[-1])
     nseg = int(min(63, (L - 16) // 8 + 1))
     return Conditioning(seg, 48, 16, 16, 8, nseg, 16)
 
 
-def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4612, n_ecc=1268):
+def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4612, n_ecc=1268, grids=GRIDS):
     """The batched form: per detector n_chunks chunks with lengths lmin + step U{0 .. (lmax - lmin) / step} s (segment
     gaps), independent seeded noise with the production lines; both banks. Returns (chunks, banks): chunks is a list
     of (Stream, Conditioning or None)."""
@@ -77,7 +84,7 @@ def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4
         for d, ifo in enumerate(('H1', 'L1')[:ndet]):
             L = int(lens[d, c])
             st = synth.make_stream(ifo, L, SR, seed=1000 * (d + 1) + c, lines=lines, gps_start=t0)
-            chunks.append((st, chunk_conditioning(L)))
+            chunks.append((st, chunk_conditioning(L, grids)))
         t0 += int(lens[:, c].max()) + 8
     banks = [synth.circular_bank(n_circ, seed=SEEDS['bank'])]
     if n_ecc:
@@ -86,7 +93,7 @@ def batched(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4
 
 
 def batched_device(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n_circ=4612, n_ecc=1268,
-                   noise_seed=0):
+                   noise_seed=0, grids=GRIDS):
     """batched() with the chunks' noise drawn on the GPU (synth.device_stream): the same chunk lengths, GPS layout and
     banks; per chunk the noise seed is noise_seed + 1000 (d + 1) + c. Returns (chunks, x_dev, banks)."""
     rng = np.random.default_rng(seed)
@@ -98,7 +105,7 @@ def batched_device(n_chunks=200, ndet=2, seed=31, lmin=64, lmax=1024, step=16, n
         for d, ifo in enumerate(('H1', 'L1')[:ndet]):
             L = int(lens[d, c])
             st, x = synth.device_stream(ifo, L, SR, noise_seed + 1000 * (d + 1) + c, lines=lines, gps_start=t0)
-            chunks.append((st, chunk_conditioning(L)))
+            chunks.append((st, chunk_conditioning(L, grids)))
             x_dev.append(x)
         t0 += int(lens[:, c].max()) + 8
     banks = [synth.circular_bank(n_circ, seed=SEEDS['bank'])]

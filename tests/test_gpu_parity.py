@@ -293,8 +293,8 @@ def test_production_shape_tile(gpu):
 
 
 def test_batched_search_vs_single_filters(gpu):
-    """The batched form on six uneven chunks per detector (80 s: nothing to analyse, 192 / 320 / 448 s on the 128 s
-    grid, 640 / 1024 s on the 512 s grid), both banks (96 circular + 32 eccentric templates, tiles of 64), fp32: every
+    """The batched form on six uneven chunks per detector (80 s: nothing to analyse, 192 s on the 128 s grid, 320 /
+    448 s on the 256 s grid, 640 / 1024 s on the 512 s grid), both banks (96 circular + 32 eccentric templates, tiles of 64), fp32: every
     chunk's peak-file fields equal those of a Filter run on that chunk alone with the same bank products, and exact
     SNRs of the loudest templates equal the float64 direct filter to 1e-4."""
     import batched, fp_filter, synth, workloads
@@ -313,7 +313,7 @@ def test_batched_search_vs_single_filters(gpu):
     opts = {'fp32': fp_filter.Options(tile=64, precision='fp32', self_check=False, segment_batch=4)}
     _say('batched search')
     bs = batched.BatchedSearch(chunks, banks, x_dev, opts)
-    assert bs.grids == [128, 512]
+    assert bs.grids == [128, 256, 512]
     ds, outs, _ = bs.step('fp32')
     n_tr = 0
     for g in bs.grids:

@@ -94,3 +94,23 @@ def test_trigger_order_equals_lexsort():
 def test_gemm_cfg_parsing():
     import fp_gemm
     assert fp_gemm._cfg(None) == (0, 0) and fp_gemm._cfg(32) == (32, 0) and fp_gemm._cfg((16, 1)) == (16, 1)
+
+
+def test_batched_segment_grids():
+    """The batched form's segmenting: each chunk on the longest grid that leaves it a full segment beyond the 64 s of
+    pads (the shortest grid from its own length up); every chunk's segments analyse L - 64 s, whatever the grid."""
+    import gpu_prep, workloads
+    exp = {80: None, 127: None, 128: 128, 192: 128, 319: 128, 320: 256, 448: 256, 575: 256, 576: 512, 1024: 512}
+    for L, g in exp.items():
+        c = workloads.chunk_conditioning(L)
+        assert (c.seg_len if c else None) == g, L
+    # the two-grid segmenting: 128 s segments up to 575 s
+    assert [workloads.chunk_conditioning(L, (512, 128)).seg_len for L in (128, 320, 575, 576)] == [128, 128, 128, 512]
+    sr = 16
+    for L in range(128, 1025, 16):
+        c = workloads.chunk_conditioning(L)
+        seg = gpu_prep.segment_slices(L * sr, sr, 0.0, c.seg_len, c.start_pad, c.end_pad, 0, 0)
+        cov = np.zeros(L * sr, int)
+        for (a, _), (b0, b1) in seg:
+            cov[a + b0:a + b1] += 1
+        assert np.all(cov[c.start_pad * sr:(L - c.end_pad) * sr] == 1) and cov.sum() == (L - 64) * sr, L
