@@ -1,8 +1,8 @@
 """The batched form: detector chunks of uneven length (segment gaps), both detectors and both banks, filtered as a
 few large launches instead of one pass per chunk.
 
-Chunks are grouped by segment grid: every chunk on the 512 s grid (tlen 2^20) and every chunk on the 128 s grid
-(tlen 2^18) of a step form one Filter each, whose segments are taken in batches of up to `segment_batch` segments
+Chunks are grouped by segment grid (workloads.chunk_conditioning: 512 s, 256 s and 128 s segments, tlen 2^20, 2^19
+and 2^18 at 2048 Hz): the chunks of a step on one grid form one Filter, whose segments are taken in batches of up to `segment_batch` segments
 and `batch_samples` analysed samples per basis-transform batch (one launch of each transform pass per 16 segments)
 and GEMM launch (the item table of csrc/gemm_peaks.hip covers several segments of uneven analysed length in one
 launch). Each chunk's PSD is its own (fp_filter.py's prep per dataset). Bank products (dense templates and
