@@ -200,14 +200,15 @@ def pick_cfg(dev='cuda', M=1024, K=128, Tn=1 << 20, dtype=torch.bfloat16, cw=204
     the filter's own case. lists: time the launch with the local-maximum lists (gemm_peaks_multi_lm, whose kernel
     does not depend on the variant: tiles per item only), as the filter runs it when its first pass keeps them.
     Returns (cfg, {str(cfg): seconds or error}); raises RuntimeError if none runs."""
-    g = torch.Generator(device='cpu').manual_seed(0)
+    # seeded operands drawn on the device (a CPU draw of the Tn x K operand took seconds of the search's set-up)
+    g = torch.Generator(device=dev).manual_seed(0)
     K = _kpad(K) if K % 2 == 0 else _kpad(K + 1)
-    A = torch.randn((M, K), generator=g)
+    A = torch.randn((M, K), generator=g, device=dev)
     Ai = torch.empty_like(A)
     Ai[:, 0::2], Ai[:, 1::2] = -A[:, 1::2], A[:, 0::2]
     code = DTYPES[dtype]
     Tn = Tn // cw * cw
-    Zs = torch.randn((Tn, K), generator=g)
+    Zs = torch.randn((Tn, K), generator=g, device=dev)
     if code == 2:
         Ar_d, Ai_d, Z_d = (x.clamp(-448, 448).to(dev).to(torch.float8_e4m3fn).view(torch.uint8) for x in (A, Ai, Zs))
     else:
