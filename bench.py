@@ -1,16 +1,17 @@
 #!/usr/bin/env python
 """Throughput of the filtering path on one MI355X: template-seconds filtered per GPU-second.
 
-Headline workload (the batched configuration, workloads.batched_device): per detector 200 chunks of 64-1024 s in
-16 s steps (segment gaps), 2 detectors at 2048 Hz of synthetic coloured noise with 5 lines, drawn on the GPU; both
-banks, 4,612 circular + 1,268 eccentric = 5,880 templates; chunks on 512 s, 256 s and 128 s segment grids. The job is cut
-into 20 slices of 10 chunks per detector: one step filters one slice (conditioning of every chunk from its raw strain
-resident in HBM: float64 median-Welch PSD, truncation, whitening; template norms; per tile the basis transforms and
-the fused GEMM + window max; candidate selection; the exact stage with +-3-sample re-centring: edges, chi-squared
-sums, float64 finalization), so the 20 default steps filter every chunk once. Warm-up steps run on separate chunks.
-The bank products (templates, float64 SVD bases) are built once before timing, as a search builds them once per run;
-config.value_with_setup counts them, the autotuner and the SVD metric's conditioning into the whole job (not the
-synthetic data's generation, which stands in for reading strain), and config.setup_parts breaks that down.
+Headline workload (the batched configuration, workloads.batched_device): per detector 200 chunks of 64-1024 s in 16
+s steps (segment gaps), 2 detectors at 2048 Hz of synthetic coloured noise with 5 lines, drawn on the GPU; both
+banks, 4,612 circular + 1,268 eccentric = 5,880 templates; chunks on 512 s, 256 s and 128 s segment grids. The job
+is cut into 20 slices of 10 chunks per detector: one step filters one slice (conditioning of every chunk from its
+raw strain resident in HBM: float64 median-Welch PSD, truncation, whitening; template norms; per tile the basis
+transforms and the fused GEMM + window max; candidate selection; the exact stage with +-3-sample re-centring: edges,
+chi-squared sums, float64 finalization), so the 20 default steps filter every chunk once. Warm-up steps run on
+separate chunks. The bank products (templates, float64 SVD bases) are built once before timing, as a search builds
+them once per run; config.value_with_setup counts them, the autotuner and the SVD metric's conditioning into the
+whole job (not the synthetic data's generation, which stands in for reading strain), and config.setup_parts breaks
+that down.
 
 value = 5,880 x analysed seconds of the timed slices (both detectors) / their wall time, for the BF16 form. The
 FP32 plain form (and the FP8 form) are timed on the first slices beside it, the trigger lists compared with the

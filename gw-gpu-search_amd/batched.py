@@ -2,13 +2,14 @@
 few large launches instead of one pass per chunk.
 
 Chunks are grouped by segment grid (workloads.chunk_conditioning: 512 s, 256 s and 128 s segments, tlen 2^20, 2^19
-and 2^18 at 2048 Hz): the chunks of a step on one grid form one Filter, whose segments are taken in batches of up to `segment_batch` segments
-and `batch_samples` analysed samples per basis-transform batch (one launch of each transform pass per 16 segments)
-and GEMM launch (the item table of csrc/gemm_peaks.hip covers several segments of uneven analysed length in one
-launch). Each chunk's PSD is its own (fp_filter.py's prep per dataset). Bank products (dense templates and
-float64 SVD bases) are built once per grid, since Vh and H live on the grid's frequency bins; the SVD metric of a
-grid is sqrt(mean IP) over the segments of the reference chunks (fp_filter.py:360-364 takes it over the group being
-filtered; it sets only how well the basis compresses, never the SNR: conj(h sw) (D / sw) = conj(h) D)."""
+and 2^18 at 2048 Hz): the chunks of a step on one grid form one Filter, whose segments are taken in batches of up to
+`segment_batch` segments and `batch_samples` analysed samples per basis-transform batch (one launch of each
+transform pass per 16 segments) and GEMM launch (the item table of csrc/gemm_peaks.hip covers several segments of
+uneven analysed length in one launch). Each chunk's PSD is its own (fp_filter.py's prep per dataset). Bank products
+(dense templates and float64 SVD bases) are built once per grid, since Vh and H live on the grid's frequency bins;
+the SVD metric of a grid is sqrt(mean IP) over the segments of the reference chunks (fp_filter.py:360-364 takes it
+over the group being filtered; it sets only how well the basis compresses, never the SNR: conj(h sw) (D / sw) =
+conj(h) D)."""
 from __future__ import annotations
 
 import numpy as np
