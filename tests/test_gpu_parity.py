@@ -341,6 +341,20 @@ def test_batched_search_vs_single_filters(gpu):
                                          got['sample'][mm])
                         assert np.max(np.abs(got['snr'][mm] - snr) / snr) < 1e-4
     assert n_tr > 20
+    # several slices in one software pipeline (each slice conditioned when the pipeline reaches it, while the
+    # previous slice's last batches run) give each slice's own one-slice run
+    sels = [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11]]
+    _say('pipelined slices')
+    multi = bs.run('fp32', sels)
+    for k, sel in enumerate(sels):
+        ds1, outs1, _ = bs.step('fp32', sel)
+        dsm, outsm, _ = multi[k]
+        assert [g for g in bs.grids if ds1[g]] == [g for g in bs.grids if dsm[g]]
+        for g in bs.grids:
+            for bi in range(len(banks)):
+                for a, b in zip(outs1[g][bi], outsm[g][bi]):
+                    for key in a:
+                        np.testing.assert_array_equal(a[key], b[key], err_msg=f'slice {k} grid {g} bank {bi}: {key}')
 
 
 def test_condition_vs_reference_stage2_golden(gpu):
