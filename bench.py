@@ -418,12 +418,14 @@ def consolidation_report(step, nsteps, work, outs_headline):
                  sg_chisq_host_s=float(sum(f.T['sg_chisq_s'] for f in fs)))
         if 'pre_keep' in st:
             n_, t_, by = st['pre_keep']
-            r['pre_keep_kernels'] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 8e12,
+            # latency-bound: five dependent launches over small arrays
+            r['pre_keep_kernels'] = dict(achieved=by / t_ / 1e9, unit='GB/s', bound='latency', frac=None,
                                          launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
                                          share_of_step=t_ / nsteps / dt)
         if 'sg_tiles' in st:
             n_, t_, by = st['sg_tiles']
-            r['sg_tiles_kernel'] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 8e12,
+            # VALU-bound: its bytes are served from L2 / MALL, so GB/s over 8 TB/s would be no HBM fraction
+            r['sg_tiles_kernel'] = dict(achieved=by / t_ / 1e9, unit='GB/s', bound='valu', frac=None,
                                         launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
                                         share_of_step=t_ / nsteps / dt)
         res[name] = r
@@ -441,12 +443,23 @@ def _merge(dicts):
     return {k: np.concatenate([d[k] for d in dicts]) for k in keys}
 
 
+# what bounds each metered kernel (the counters' reading, profiles/hbm_traffic_batched_pmc.md): only for the
+# HBM-bound ones is algorithmic bytes / time over 8 TB/s a roofline fraction; the exact stage's and edges' short
+# launches are bound by their dependent chains, the sine-Gaussian tiles by the VALU (their D and IP stay in L2 / MALL),
+# the second list pass by its few workgroups
+KERNEL_BOUND = {'basis_series': 'hbm', 'sigmasq': 'hbm', 'edges': 'latency', 'chisq': 'latency', 'pre_keep': 'latency',
+                'sg_tiles': 'valu', 'second_pass': 'latency'}
+
+
 def _kernel_rates(ms, nsteps, dt):
-    """HBM-bound kernels' rates from the meter: name -> achieved GB/s on their algorithmic bytes."""
+    """Metered kernels' rates: name -> achieved GB/s on their algorithmic bytes, with the bound; frac (of 8 TB/s)
+    only for the HBM-bound ones."""
     out = {}
     for name, (n_, t_, by) in ms.items():
         if not name.startswith('gemm'):
-            out[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', frac=by / t_ / 1e9 / 8000.0,
+            bound = KERNEL_BOUND.get(name, 'hbm')
+            out[name] = dict(achieved=by / t_ / 1e9, peak=8000.0, unit='GB/s', bound=bound,
+                             frac=by / t_ / 1e9 / 8000.0 if bound == 'hbm' else None,
                              launches=n_, avg_launch_ms=t_ / n_ * 1e3, bytes_per_launch=by / n_,
                              share_of_step=t_ / nsteps / dt)
     return out
