@@ -964,7 +964,8 @@ gemm_peaks_ring_k(const T* __restrict__ Zt, const long ldz, const T* __restrict_
 // written out here. The kernel ends only after its last DMA has landed (an exiting workgroup's LDS may be reassigned).
 // The list mode (PARK) keeps the local-maximum state in LDS between tiles and runs two stages: 46 KiB of LDS and 168
 // registers, so that three workgroups share a CU instead of two (the compiler's few spills fall in the epilogue;
-// vector memory returns in order, so the waits above only grow stricter around them).
+// vector memory returns in order, so the waits above only grow stricter around them). The fp8 window-maximum
+// launches are held to the same 168 registers (5 spilled; 5-6% faster than at two workgroups per CU).
 constexpr int DMA_STAGES = 3;
 
 __device__ __forceinline__ int dma_swizzle(const int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
@@ -977,7 +978,7 @@ __device__ __forceinline__ void dma_load16(const uint32_t lds_addr, const uint32
 }
 
 template <typename T, int MODE, bool PARK = false>
-__global__ void __launch_bounds__(NT, PARK ? 3 : 2)
+__global__ void __launch_bounds__(NT, (PARK || sizeof(T) == 1) ? 3 : 2)
 gemm_peaks_dma_k(const T* __restrict__ Zt, const long ldz, const T* __restrict__ Ar, const long lda, const int K,
                  const Job J, float* __restrict__ V, int* __restrict__ I, const long ldv)
 {
