@@ -64,6 +64,16 @@ def test_bad_arguments_rejected(lib):
         arr(ctypes.c_long, *([0] * max(nseg, 1))), None, 0, None, p, p, 32, None)
     assert bs(0, 0, 10) == -2 and bs(17, 0, 10) == -2
     assert bs(2, (1 << 16) - 5, 10) == -1 and bs(2, -1, 10) == -1
+    # the second list pass: no buffers, a misaligned item table or segment table, and list-mode limits
+    q = ctypes.c_void_p(260)                                                    # 4-byte aligned only
+    assert lib.gwgs_hot_items(None, p, p, 10, 8, 1.0, p, 1, 16, p, p, p, None) == -1
+    assert lib.gwgs_hot_items(p, p, p, 10, 0, 1.0, p, 1, 16, p, p, p, None) == -1
+    assert lib.gwgs_hot_items(p, p, p, 10, 8, 1.0, q, 1, 16, p, p, p, None) == -1
+    assert lib.gwgs_hot_items(p, p, p, 10, 8, 1.0, p, 1, 16, p, q, p, None) == -1
+    dyn = lambda items, nit, sep, ldl: lib.gwgs_gemm_peaks_multi_lm_dyn(
+        1, 0, p, 64, p, p, 64, 96, 64, items, nit, 8, 1, 16, sep, 0.5, 0.0, p, p, 8, p, p, ldl, None, None)
+    assert dyn(None, p, 0, 32) == -1 and dyn(p, None, 0, 32) == -1 and dyn(q, p, 0, 32) == -1
+    assert dyn(p, p, 16 * 128, 32) == -1 and dyn(p, p, 0, 31) == -1     # sep beyond the window, VL rows too short
     # pre-consolidation: empty batches return at once; bad flags, no segment table, or ranking without a workspace
     assert lib.gwgs_pre_keep_workspace(100, 5) == 8 * 100 + 4 * 16
     pk = lambda n, nseg, flags, work, wb: lib.gwgs_pre_keep(p, p, n, 10, nseg, p, p, p, p, 2048.0, 2195.456, 3, 1.0,
