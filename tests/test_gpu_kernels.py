@@ -719,6 +719,14 @@ def test_local_maxima_lists(gpu, code, stacked, r, sep, gate):
     assert np.array_equal(vl2[:, hot], vl.reshape(M, nwsum, LM)[:, hot])
     assert np.array_equal(il2[:, hot], il.reshape(M, nwsum, LM)[:, hot])
     assert np.all(vl2[:, cold] == -7.0) and np.all(il2[:, cold] == -7)
+    # a device count above max_items is clamped to it: the first k items only
+    VL3 = torch.full_like(VL, -7.0); IL3 = torch.full_like(IL, -7)
+    big = torch.full((1,), k + 1000, dtype=torch.int32, device='cuda')
+    fp_gemm.gemm_peaks_multi_lm_dyn(code, qz, K, qa, ai, K, M, K, it2, big, k, stacked, per, sep, (2 * per, 0),
+                                    torch.empty_like(V), torch.empty_like(I), VL3, IL3, sc, gate=gate, floor=floor)
+    torch.cuda.synchronize()
+    assert np.array_equal(VL3.cpu().numpy(), VL2.cpu().numpy()) and np.array_equal(IL3.cpu().numpy(),
+                                                                                    IL2.cpu().numpy())
 
 
 def test_climb_points_vs_float64(gpu):
